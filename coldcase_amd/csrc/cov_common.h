@@ -1,0 +1,352 @@
+// cov_common.h -- shared definitions of the MI355X covering-code engine.
+//
+// Data layout in HBM, per cell (one (q,n,R,M) search state):
+//   cnt   uint16[N]   coverage multiplicity, MSB-first base-q word index;
+//                     allocated as uint32 pairs so ball updates can use 32-bit
+//                     atomics on the containing word (cnt <= M <= 65535 never
+//                     carries or borrows across the 16-bit lanes)
+//   zb,ob uint32[N/32] bit planes [cnt==0] and [cnt==1]: the evaluator reads
+//                     these (1 bit per word) instead of cnt, so a K6(9,4) plane
+//                     is 1.26 MB and stays resident in one XCD's 4 MB L2
+//   blk   uint32[N/4096] uncovered words per 4096-word block: turns the cyclic
+//                     "first uncovered at or after start" scan into a short
+//                     walk over block counts
+//   code  uint8[M*n], cpk uint64[M] (digits packed 4 bits per position, for
+//                     one-xor Hamming distances), cidx int64[M], best uint8[M*n]
+//   tabu  int32[M*n*q]
+//   ul_*  uncovered-word list (index + packed digits) with a membership plane,
+//         used for the entering side of the evaluator once it is short
+//   rbuf  uint64[RB] ring of xoshiro256++ draws; decisions consume draws by a
+//         cursor, generation runs ahead off the critical path
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace cov {
+
+constexpr int MAXN = 16;
+constexpr int MAXQ = 16;
+constexpr int MAXR = 8;
+constexpr int WG = 256;              // threads per workgroup in every kernel
+constexpr int WAVES = WG / 64;
+constexpr int BLK_SHIFT = 12;        // words per uncovered-count block = 4096
+constexpr int BLK_WORDS = 1 << BLK_SHIFT;
+constexpr int RB = 1 << 14;          // rng ring size (draws)
+constexpr int ACCW = 1 + 2 * MAXN;   // per-slot accumulators: t1, hit[16], neg[16]
+constexpr int LIST_CHUNK = 512;      // uncovered-list entries per evaluator unit
+constexpr int64_t DSEL_RANDOM = 1LL << 60;
+constexpr int TRW = 16;
+constexpr uint64_t NIB1 = 0x1111111111111111ULL;
+
+// Device-resident search state.  Scalars written by the narrow phases (one
+// workgroup) and read by the wide kernels of the same step.
+struct Ctl {
+    uint64_t rst[4];        // generator state after rgen draws
+    int64_t rgen;           // draws generated into the ring
+    int64_t ruse;           // draws consumed
+    int64_t rlow;           // oldest draw still to be read this step
+    int64_t it, it_stop, since, nkicks, unc, best_unc;
+    int64_t stall;
+    int64_t dunc;           // commit accumulator (device atomics)
+    int64_t ul_n;
+    int64_t u;
+    uint64_t upk;
+    int64_t sh_pos;         // ring position of the first shuffle draw
+    int64_t n_leave, n_enter, nchunk;
+    int64_t dsel, base, dv;
+    int64_t tpos;           // trace records written this run
+    int64_t eval_bytes, commit_bytes;
+    int32_t done, begun, selected, fault;
+    int32_t tenure, kick, cand, budget;
+    int32_t ul_valid, ul_ovf;
+    int32_t dtar, ncand, take, nq, use_list;
+    int32_t bi, bp, bv, oldv, nties, stamp, ncw;
+    int32_t evdone, cmdone;
+};
+
+struct Cell {
+    int q, n, R, M, q1, F, E;
+    int64_t P;              // (q-1)^R patterns per position subset
+    int64_t N, nzw, nblk, V, ncap, ulcap, tcap;
+    int64_t pw[MAXN];
+    uint32_t magic;         // ceil(2^32 / (q-1)): t / (q-1) = umulhi(t, magic)
+    uint32_t* cnt32;        // cnt as uint32 pairs
+    uint32_t* zb;
+    uint32_t* ob;
+    uint32_t* blk;
+    uint8_t* code;
+    uint64_t* cpk;
+    int64_t* cidx;
+    uint8_t* best;
+    int32_t* tabu;
+    uint32_t* fsub;         // C(n,R) R-subsets of all positions, 4 bits each
+    uint32_t* psub;         // n x C(n-1,R) R-subsets avoiding p
+    uint32_t* bsub;         // subsets of size 0..R, grouped by size
+    int64_t bsub_off[MAXR + 2];   // first subset of each size in bsub
+    int64_t bpat_off[MAXR + 2];   // first ball word of each size
+    int64_t* ul_idx;
+    uint64_t* ul_pk;
+    uint32_t* inl;          // list membership plane
+    uint32_t* shuf;         // shuffled candidate order (original indices)
+    int32_t* acc;           // slot accumulators, ACCW per slot
+    int32_t* slot_cw;       // slot -> codeword
+    uint8_t* slot_pos;      // slot -> differing positions (MAXN each)
+    uint64_t* rbuf;
+    int64_t* trace;         // tcap records
+    int64_t* scratch64;     // small reduction outputs
+    uint8_t* dist;          // distance transform output (N bytes), lazily
+    Ctl* ctl;
+};
+
+// ------------------------------------------------------------- helpers
+__device__ __forceinline__ uint16_t cnt_get(const Cell& C, int64_t w)
+{
+    return reinterpret_cast<const uint16_t*>(C.cnt32)[w];
+}
+__device__ __forceinline__ void cnt_put(const Cell& C, int64_t w, uint16_t v)
+{
+    reinterpret_cast<uint16_t*>(C.cnt32)[w] = v;
+}
+__device__ __forceinline__ uint32_t bit_of(const uint32_t* plane, int64_t w)
+{
+    return (plane[w >> 5] >> (w & 31)) & 1u;
+}
+__device__ __forceinline__ void bit_set(uint32_t* plane, int64_t w)
+{
+    atomicOr(&plane[w >> 5], 1u << (w & 31));
+}
+__device__ __forceinline__ void bit_clr(uint32_t* plane, int64_t w)
+{
+    atomicAnd(&plane[w >> 5], ~(1u << (w & 31)));
+}
+
+// nibble-wise "differs" mask: bit 4j set iff digit j of a and b differ
+__device__ __forceinline__ uint64_t nib_ne(uint64_t a, uint64_t b)
+{
+    uint64_t x = a ^ b;
+    return (x | (x >> 1) | (x >> 2) | (x >> 3)) & NIB1;
+}
+__device__ __forceinline__ int nib_dist(uint64_t a, uint64_t b)
+{
+    return __popcll(nib_ne(a, b));
+}
+__device__ __forceinline__ int nib_get(uint64_t pk, int j)
+{
+    return (int)((pk >> (4 * j)) & 15u);
+}
+__device__ __forceinline__ uint64_t nib_put(uint64_t pk, int j, int v)
+{
+    return (pk & ~(15ULL << (4 * j))) | ((uint64_t)v << (4 * j));
+}
+
+// digits of a word index, packed
+__device__ inline uint64_t pack_index(const Cell& C, int64_t w)
+{
+    uint64_t pk = 0;
+    for (int j = C.n - 1; j >= 0; j--) {
+        pk |= (uint64_t)(w % C.q) << (4 * j);
+        w /= C.q;
+    }
+    return pk;
+}
+
+__device__ __forceinline__ uint32_t div_q1(const Cell& C, uint32_t t)
+{
+    return __umulhi(t, C.magic);
+}
+
+// Word of pattern t (0 <= t < (q-1)^r) on the position subset `sub` (r
+// positions, 4 bits each, increasing) around the word with digits pk at base:
+// the last position varies fastest, and the k-th value of a position skips
+// the centre digit (v = k + (k >= c)).  Also returns the packed digits.
+__device__ __forceinline__ int64_t pattern_word(const Cell& C, int64_t base, uint64_t pk,
+                                                uint32_t sub, int r, uint32_t t,
+                                                uint64_t* wpk)
+{
+    int64_t a = base;
+    uint64_t p2 = pk;
+    for (int i = r - 1; i >= 0; i--) {
+        const uint32_t tq = div_q1(C, t);
+        const int k = (int)(t - tq * (uint32_t)C.q1);
+        t = tq;
+        const int j = (sub >> (4 * i)) & 15;
+        const int c = nib_get(pk, j);
+        const int v = k + (k >= c);
+        a += (int64_t)(v - c) * C.pw[j];
+        p2 = nib_put(p2, j, v);
+    }
+    if (wpk) *wpk = p2;
+    return a;
+}
+
+// ----------------------------------------------------------------- rng
+__device__ __forceinline__ uint64_t rotl64(uint64_t x, int k) { return (x << k) | (x >> (64 - k)); }
+
+// xoshiro256++ step (Blackman & Vigna)
+__device__ __forceinline__ uint64_t xo_next(uint64_t s[4])
+{
+    const uint64_t out = rotl64(s[0] + s[3], 23) + s[0];
+    const uint64_t t = s[1] << 17;
+    s[2] ^= s[0];
+    s[3] ^= s[1];
+    s[1] ^= s[2];
+    s[0] ^= s[3];
+    s[2] ^= t;
+    s[3] = rotl64(s[3], 45);
+    return out;
+}
+
+// Generate draws until rgen >= upto (never overwriting a draw still to be
+// read).  One thread only.  Returns the number generated.
+__device__ inline int64_t ring_fill(const Cell& C, Ctl* ctl, int64_t upto, int64_t budget)
+{
+    uint64_t s[4] = {ctl->rst[0], ctl->rst[1], ctl->rst[2], ctl->rst[3]};
+    int64_t g = ctl->rgen;
+    int64_t lim = ctl->rlow + RB;
+    if (upto < lim) lim = upto;
+    if (g + budget < lim) lim = g + budget;
+    const int64_t g0 = g;
+    while (g < lim) {
+        C.rbuf[g & (RB - 1)] = xo_next(s);
+        g++;
+    }
+    ctl->rst[0] = s[0]; ctl->rst[1] = s[1]; ctl->rst[2] = s[2]; ctl->rst[3] = s[3];
+    ctl->rgen = g;
+    return g - g0;
+}
+
+__device__ __forceinline__ uint64_t ring_at(const Cell& C, int64_t pos)
+{
+    return C.rbuf[pos & (RB - 1)];
+}
+
+// ------------------------------------------------- workgroup reductions
+struct WgRed {
+    int64_t v[WG / 64];
+    int64_t out;
+};
+
+__device__ __forceinline__ int64_t wave_sum64(int64_t x)
+{
+    for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o, 64);
+    return x;
+}
+__device__ __forceinline__ int64_t wave_min64(int64_t x)
+{
+    for (int o = 32; o > 0; o >>= 1) {
+        int64_t y = __shfl_xor(x, o, 64);
+        x = y < x ? y : x;
+    }
+    return x;
+}
+__device__ __forceinline__ int64_t wave_max64(int64_t x)
+{
+    for (int o = 32; o > 0; o >>= 1) {
+        int64_t y = __shfl_xor(x, o, 64);
+        x = y > x ? y : x;
+    }
+    return x;
+}
+
+// all threads of the workgroup must call these
+__device__ inline int64_t wg_sum(int64_t x, WgRed& r)
+{
+    x = wave_sum64(x);
+    const int w = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) r.v[w] = x;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int64_t s = 0;
+        for (int i = 0; i < WG / 64; i++) s += r.v[i];
+        r.out = s;
+    }
+    __syncthreads();
+    const int64_t res = r.out;
+    __syncthreads();
+    return res;
+}
+__device__ inline int64_t wg_min(int64_t x, WgRed& r)
+{
+    x = wave_min64(x);
+    const int w = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) r.v[w] = x;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int64_t s = r.v[0];
+        for (int i = 1; i < WG / 64; i++) s = r.v[i] < s ? r.v[i] : s;
+        r.out = s;
+    }
+    __syncthreads();
+    const int64_t res = r.out;
+    __syncthreads();
+    return res;
+}
+__device__ inline int64_t wg_max(int64_t x, WgRed& r)
+{
+    x = wave_max64(x);
+    const int w = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) r.v[w] = x;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int64_t s = r.v[0];
+        for (int i = 1; i < WG / 64; i++) s = r.v[i] > s ? r.v[i] : s;
+        r.out = s;
+    }
+    __syncthreads();
+    const int64_t res = r.out;
+    __syncthreads();
+    return res;
+}
+
+// inclusive prefix sum over the workgroup (thread order); total in *tot
+struct WgScan {
+    int64_t w[WG / 64];
+};
+__device__ inline int64_t wg_incl_sum(int64_t x, WgScan& s, int64_t* tot)
+{
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    for (int o = 1; o < 64; o <<= 1) {
+        int64_t y = __shfl_up(x, o, 64);
+        if (lane >= o) x += y;
+    }
+    if (lane == 63) s.w[wv] = x;
+    __syncthreads();
+    int64_t pre = 0, all = 0;
+    for (int i = 0; i < WG / 64; i++) {
+        if (i < wv) pre += s.w[i];
+        all += s.w[i];
+    }
+    __syncthreads();
+    *tot = all;
+    return x + pre;
+}
+__device__ inline int64_t wg_incl_min(int64_t x, WgScan& s)
+{
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    for (int o = 1; o < 64; o <<= 1) {
+        int64_t y = __shfl_up(x, o, 64);
+        if (lane >= o && y < x) x = y;
+    }
+    if (lane == 63) s.w[wv] = x;
+    __syncthreads();
+    for (int i = 0; i < wv; i++)
+        if (s.w[i] < x) x = s.w[i];
+    __syncthreads();
+    return x;
+}
+__device__ inline int64_t wg_incl_max(int64_t x, WgScan& s)
+{
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    for (int o = 1; o < 64; o <<= 1) {
+        int64_t y = __shfl_up(x, o, 64);
+        if (lane >= o && y > x) x = y;
+    }
+    if (lane == 63) s.w[wv] = x;
+    __syncthreads();
+    for (int i = 0; i < wv; i++)
+        if (s.w[i] > x) x = s.w[i];
+    __syncthreads();
+    return x;
+}
+
+}  // namespace cov

@@ -1,0 +1,1024 @@
+// cov_step.h -- the device-resident tabu step (kernel a) and its narrow phases.
+//
+// One search step of cov/engine/covsearch2e.c:1304-1580 is two launches:
+//
+//   k_eval    (wide)  units of work over the candidate codewords:
+//                       leaving side: per (codeword, R-subset of all n
+//                         positions) the words of the distance-R sphere with
+//                         cnt == 1, read from the ob plane; each count adds to
+//                         T and to H_j for the j in the subset (:696-716)
+//                       entering side: per (codeword, differing position p,
+//                         R-subset avoiding p) the words of c' = c(p:=u_p)'s
+//                         sphere with cnt == 0 (zb plane), or per (codeword,
+//                         chunk of the uncovered list) the x with d(x,c)=R+1
+//                         and x_p = u_p (:717-753)
+//                     wave 0 of workgroup 0 runs the Fisher-Yates shuffle of
+//                     the candidates meanwhile (the set of codewords to
+//                     evaluate does not depend on the order), wave 1 refills
+//                     the rng ring; the last workgroup to finish selects the
+//                     move (tabu, aspiration, reservoir ties) with a parallel
+//                     prefix formulation of the sequential scan (:1503-1518)
+//   k_commit  (wide)  --cnt[a], ++cnt[b=a+dv] over the C(n-1,R)(q-1)^R
+//                     patterns (every word once, no counter atomics, :562-590);
+//                     plane bits, block counts and the uncovered list follow
+//                     the transitions; the last workgroup applies the move,
+//                     stamps tabu, tracks the best code, kicks on stagnation,
+//                     writes the trace record and begins the next step: pick
+//                     (:973-1016), distances, candidates (:1339-1376).
+//
+// Every decision that consumes randomness reads the ring of xoshiro256++
+// draws at a cursor in the reference's order, so the trajectory is the
+// reference's bit for bit.
+#pragma once
+#include "cov_common.h"
+#include "cov_cover.h"
+
+namespace cov {
+
+constexpr int SHUF_LDS = 4096;     // candidates shuffled in LDS (else in HBM)
+
+struct NarrowSh {
+    WgRed red;
+    WgScan scan;
+    int64_t buf[WG];
+    int64_t v64[8];
+    int32_t v32[8];
+};
+
+// ---------------------------------------------------------- scanning
+// first word of the plane-word range [x0, x1) with a set zb bit (masked first
+// and last words), or INT64_MAX; all threads
+__device__ inline int64_t wg_first_bits(const Cell& C, int64_t x0, int64_t x1, uint32_t m0,
+                                        uint32_t m1, NarrowSh& sh)
+{
+    int64_t best = INT64_MAX;
+    for (int64_t xb = x0; xb < x1; xb += WG) {
+        const int64_t x = xb + threadIdx.x;
+        int64_t cand = INT64_MAX;
+        if (x < x1) {
+            uint32_t z = C.zb[x];
+            if (x == x0) z &= m0;
+            if (x == x1 - 1) z &= m1;
+            if (z) cand = x * 32 + __ffs(z) - 1;
+        }
+        const int64_t mn = wg_min(cand, sh.red);
+        if (mn != INT64_MAX) { best = mn; break; }
+    }
+    return best;
+}
+
+// first uncovered word at or after `start`, cyclically (the scan of
+// cov/engine/covsearch2e.c:1003-1015); -1 if none.  All threads.
+__device__ inline int64_t wg_first_from(const Cell& C, int64_t start, NarrowSh& sh)
+{
+    const int64_t sw = start >> 5;
+    const int64_t sb = start >> BLK_SHIFT;
+    const int64_t bw0 = sb << (BLK_SHIFT - 5);
+    const int64_t bw1 = (bw0 + (BLK_WORDS / 32) < C.nzw) ? bw0 + (BLK_WORDS / 32) : C.nzw;
+    const uint32_t hi_mask = ~0u << (start & 31);
+    // 1. rest of the start block
+    int64_t r = wg_first_bits(C, sw, bw1, hi_mask, ~0u, sh);
+    if (r != INT64_MAX) return r;
+    // 2. blocks after, 3. blocks before
+    for (int pass = 0; pass < 2; pass++) {
+        const int64_t b_lo = pass == 0 ? sb + 1 : 0;
+        const int64_t b_hi = pass == 0 ? C.nblk : sb;
+        for (int64_t bb = b_lo; bb < b_hi; bb += WG) {
+            const int64_t b = bb + threadIdx.x;
+            const int64_t cand = (b < b_hi && C.blk[b]) ? b : INT64_MAX;
+            const int64_t fb = wg_min(cand, sh.red);
+            if (fb != INT64_MAX) {
+                const int64_t x0 = fb << (BLK_SHIFT - 5);
+                const int64_t x1 = (x0 + (BLK_WORDS / 32) < C.nzw) ? x0 + (BLK_WORDS / 32) : C.nzw;
+                return wg_first_bits(C, x0, x1, ~0u, ~0u, sh);
+            }
+        }
+    }
+    // 4. start of the start block, up to (not including) start
+    const uint32_t lo_mask = (start & 31) ? ((1u << (start & 31)) - 1u) : 0u;
+    if (sw > bw0 || lo_mask) {
+        r = wg_first_bits(C, bw0, sw + 1, ~0u, lo_mask, sh);
+        if (r != INT64_MAX && r < start) return r;
+    }
+    return -1;
+}
+
+// make sure `need` draws past the cursor exist (thread 0 generates); all threads
+__device__ inline void wg_ensure(const Cell& C, Ctl* ctl, int64_t need)
+{
+    if (threadIdx.x == 0) {
+        if (ctl->rlow < ctl->ruse) ctl->rlow = ctl->ruse;
+        if (ctl->rgen < ctl->ruse + need) ring_fill(C, ctl, ctl->ruse + need, INT64_MAX);
+    }
+    __syncthreads();
+}
+
+// pick_uncovered: 64 probes, then the cyclic scan (:973-1016).  All threads.
+__device__ inline int64_t wg_pick(const Cell& C, Ctl* ctl, NarrowSh& sh)
+{
+    wg_ensure(C, ctl, 65);
+    const int64_t pos = ctl->ruse;
+    if (threadIdx.x < 64) {
+        const uint64_t rv = ring_at(C, pos + threadIdx.x);
+        const int64_t w = (int64_t)(rv % (uint64_t)C.N);
+        const bool hit = bit_of(C.zb, w) != 0;
+        const uint64_t mask = __ballot(hit);
+        const int fh = mask ? (__ffsll((unsigned long long)mask) - 1) : -1;
+        if (threadIdx.x == 0) sh.v32[0] = fh;
+        if (fh >= 0 && (int)threadIdx.x == fh) sh.v64[0] = w;
+        if (threadIdx.x == 0 && fh < 0) sh.v64[1] = (int64_t)(ring_at(C, pos + 64) % (uint64_t)C.N);
+    }
+    __syncthreads();
+    const int fh = sh.v32[0];
+    const int64_t w = sh.v64[0], start = sh.v64[1];
+    __syncthreads();
+    if (fh >= 0) {
+        if (threadIdx.x == 0) ctl->ruse = pos + fh + 1;
+        __syncthreads();
+        return w;
+    }
+    if (threadIdx.x == 0) ctl->ruse = pos + 65;
+    __syncthreads();
+    return wg_first_from(C, start, sh);
+}
+
+// ------------------------------------------------------ uncovered list
+__device__ inline void wg_list_rebuild(const Cell& C, Ctl* ctl, NarrowSh& sh)
+{
+    if (threadIdx.x == 0) { ctl->ul_n = 0; ctl->ul_ovf = 0; }
+    __syncthreads();
+    for (int64_t x = threadIdx.x; x < C.nzw; x += WG) {
+        const uint32_t z = C.zb[x];
+        C.inl[x] = z;
+        uint32_t m = z;
+        while (m) {
+            const int b = __ffs(m) - 1;
+            m &= m - 1;
+            const int64_t w = x * 32 + b;
+            const unsigned long long pos = atomicAdd(reinterpret_cast<unsigned long long*>(&ctl->ul_n), 1ULL);
+            if ((int64_t)pos < C.ulcap) { C.ul_idx[pos] = w; C.ul_pk[pos] = pack_index(C, w); }
+            else atomicOr(&ctl->ul_ovf, 1);
+        }
+    }
+    __threadfence();
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        ctl->ul_valid = ctl->ul_ovf ? 0 : 1;
+        if (ctl->ul_n > C.ulcap) ctl->ul_n = C.ulcap;
+        ctl->ul_ovf = 0;
+    }
+    __syncthreads();
+}
+
+// drop entries that are covered again (their membership bit is cleared)
+__device__ inline void wg_list_compact(const Cell& C, Ctl* ctl, NarrowSh& sh)
+{
+    const int64_t n0 = ctl->ul_n;
+    int64_t out = 0;
+    for (int64_t e0 = 0; e0 < n0; e0 += WG) {
+        const int64_t e = e0 + threadIdx.x;
+        int64_t idx = 0;
+        uint64_t pk = 0;
+        int keep = 0;
+        if (e < n0) {
+            idx = C.ul_idx[e];
+            pk = C.ul_pk[e];
+            keep = bit_of(C.zb, idx) != 0;
+            if (!keep) bit_clr(C.inl, idx);
+        }
+        int64_t tot;
+        const int64_t incl = wg_incl_sum(keep, sh.scan, &tot);
+        __syncthreads();
+        if (keep) { C.ul_idx[out + incl - 1] = idx; C.ul_pk[out + incl - 1] = pk; }
+        out += tot;
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) ctl->ul_n = out;
+    __syncthreads();
+}
+
+// ---------------------------------------------------------------- begin
+// Start step it+1 if the run allows it: pick u, collect the candidate
+// codewords (those at distance dtar from u, in index order), reserve the
+// shuffle draws and lay out the evaluator's units.  All threads of one WG.
+__device__ inline void wg_begin(const Cell& C, Ctl* ctl, NarrowSh& sh)
+{
+    __syncthreads();
+    const int done = ctl->done, begun = ctl->begun;
+    const int64_t unc = ctl->unc, it = ctl->it, it_stop = ctl->it_stop;
+    __syncthreads();
+    if (done || begun) return;
+    if (unc <= 0) {
+        if (threadIdx.x == 0) ctl->done = 1;
+        __syncthreads();
+        return;
+    }
+    if (it >= it_stop) return;
+    if (threadIdx.x == 0) ctl->it = it + 1;
+
+    // uncovered-list upkeep (speed only: the list never changes a value); an
+    // overflow since the last step (inserts beyond capacity) invalidates it
+    if (threadIdx.x == 0 && ctl->ul_ovf) {
+        ctl->ul_valid = 0;
+        ctl->ul_ovf = 0;
+        ctl->ul_n = 0;
+    }
+    __syncthreads();
+    {
+        const int valid = ctl->ul_valid;
+        const int64_t ul_n = ctl->ul_n;
+        __syncthreads();
+        if (!valid && unc <= C.ulcap / 4) wg_list_rebuild(C, ctl, sh);
+        else if (valid && ul_n > 2 * unc + 64) wg_list_compact(C, ctl, sh);
+    }
+
+    const int64_t u = wg_pick(C, ctl, sh);
+    if (u < 0) {
+        if (threadIdx.x == 0) ctl->done = 1;
+        __syncthreads();
+        return;
+    }
+    if (threadIdx.x == 0) sh.v64[2] = (int64_t)pack_index(C, u);
+    __syncthreads();
+    const uint64_t upk = (uint64_t)sh.v64[2];
+
+    // distances and the target distance (:1342-1351)
+    int64_t dmin = INT64_MAX;
+    for (int i = threadIdx.x; i < C.M; i += WG) {
+        const int64_t d = nib_dist(C.cpk[i], upk);
+        dmin = d < dmin ? d : dmin;
+    }
+    dmin = wg_min(dmin, sh.red);
+    const int dtar = (dmin <= C.R + 1) ? C.R + 1 : (int)dmin;
+
+    // qualifying codewords in index order -> slots
+    int64_t nq = 0;
+    for (int i0 = 0; i0 < C.M; i0 += WG) {
+        const int i = i0 + threadIdx.x;
+        uint64_t ne = 0;
+        int ok = 0;
+        if (i < C.M) {
+            ne = nib_ne(C.cpk[i], upk);
+            ok = __popcll(ne) == dtar;
+        }
+        int64_t tot;
+        const int64_t incl = wg_incl_sum(ok, sh.scan, &tot);
+        if (ok) {
+            const int64_t slot = nq + incl - 1;
+            C.slot_cw[slot] = i;
+            int t = 0;
+            for (int p = 0; p < C.n; p++)
+                if ((ne >> (4 * p)) & 1) C.slot_pos[slot * MAXN + t++] = (uint8_t)p;
+        }
+        nq += tot;
+    }
+    const int64_t ncand = nq * dtar;
+    const int64_t take = ncand < ctl->cand ? ncand : ctl->cand;
+
+    // list decision and unit counts
+    const int use_list = ctl->ul_valid &&
+                         (ctl->ul_n * C.n < (int64_t)(C.R + 1) * C.E * C.P);
+    const int64_t nchunk = (ctl->ul_n + LIST_CHUNK - 1) / LIST_CHUNK;
+    for (int64_t x = threadIdx.x; x < nq * ACCW; x += WG) C.acc[x] = 0;
+    __threadfence();
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        if (ncand == 0) { ctl->fault = 1; ctl->done = 1; }
+        // the shuffle's draws must fit in the ring with room for the select's
+        if (take > RB - 4096) { ctl->fault = 2; ctl->done = 1; }
+        ctl->u = u;
+        ctl->upk = upk;
+        ctl->dtar = dtar;
+        ctl->nq = (int32_t)nq;
+        ctl->ncand = (int32_t)ncand;
+        ctl->take = (int32_t)take;
+        ctl->use_list = use_list;
+        ctl->nchunk = use_list ? nchunk : 0;
+        ctl->n_leave = nq * C.F;
+        ctl->n_enter = use_list ? nq * nchunk : nq * dtar * C.E;
+        // the shuffle's draws (:1384-1389): reserve them; the evaluator's helper
+        // wave computes the permutation while the units run
+        ctl->sh_pos = ctl->ruse;
+        ctl->rlow = ctl->ruse;
+        ctl->ruse += take;
+        if (ctl->rgen < ctl->ruse) ring_fill(C, ctl, ctl->ruse, INT64_MAX);
+        ctl->eval_bytes += (int64_t)C.M * C.n + 2 * nq * C.F * C.P +
+                           (use_list ? nq * ctl->ul_n * C.n : 2 * nq * dtar * C.E * C.P);
+        ctl->selected = 0;
+        ctl->begun = (ncand > 0 && take <= RB - 4096) ? 1 : 0;
+    }
+    __syncthreads();
+}
+
+// ------------------------------------------------------- evaluator units
+// leaving side: words of the distance-R sphere of codeword slot on subset f
+__device__ __forceinline__ void unit_leave(const Cell& C, int slot, int f)
+{
+    const int lane = threadIdx.x & 63;
+    const int i = C.slot_cw[slot];
+    const int64_t base = C.cidx[i];
+    const uint64_t pk = C.cpk[i];
+    const uint32_t sub = C.fsub[f];
+    int64_t c1 = 0;
+    for (int64_t t0 = 0; t0 < C.P; t0 += 64) {
+        const int64_t t = t0 + lane;
+        uint32_t b = 0;
+        if (t < C.P) b = bit_of(C.ob, pattern_word(C, base, pk, sub, C.R, (uint32_t)t, nullptr));
+        c1 += __popcll(__ballot(b));
+    }
+    if (lane == 0 && c1) {
+        int32_t* a = C.acc + (int64_t)slot * ACCW;
+        atomicAdd(&a[0], (int)c1);
+        for (int k = 0; k < C.R; k++) atomicAdd(&a[1 + ((sub >> (4 * k)) & 15)], (int)c1);
+    }
+}
+
+// entering side by walks: words of the sphere of c(p:=u_p) on subset e
+// avoiding p, with cnt == 0
+__device__ __forceinline__ void unit_walk(const Cell& C, int slot, int p, int up, int e)
+{
+    const int lane = threadIdx.x & 63;
+    const int i = C.slot_cw[slot];
+    const uint64_t pk0 = C.cpk[i];
+    const int cp = nib_get(pk0, p);
+    const int64_t base = C.cidx[i] + (int64_t)(up - cp) * C.pw[p];
+    const uint64_t pk = nib_put(pk0, p, up);
+    const uint32_t sub = C.psub[(int64_t)p * C.E + e];
+    int64_t c0 = 0;
+    for (int64_t t0 = 0; t0 < C.P; t0 += 64) {
+        const int64_t t = t0 + lane;
+        uint32_t b = 0;
+        if (t < C.P) b = bit_of(C.zb, pattern_word(C, base, pk, sub, C.R, (uint32_t)t, nullptr));
+        c0 += __popcll(__ballot(b));
+    }
+    if (lane == 0 && c0) atomicAdd(&C.acc[(int64_t)slot * ACCW + 1 + MAXN + p], (int)c0);
+}
+
+// entering side from the uncovered list: entries e0..e1 against codeword slot
+__device__ __forceinline__ void unit_list(const Cell& C, int slot, int64_t e0, int64_t e1,
+                                          uint64_t upk, int64_t ul_n)
+{
+    const int lane = threadIdx.x & 63;
+    const int i = C.slot_cw[slot];
+    const uint64_t cpk = C.cpk[i];
+    const uint64_t diff_cu = nib_ne(cpk, upk);      // positions p with c_p != u_p
+    int32_t cnt[MAXN];
+#pragma unroll
+    for (int p = 0; p < MAXN; p++) cnt[p] = 0;
+    for (int64_t x0 = e0; x0 < e1; x0 += 64) {
+        const int64_t x = x0 + lane;
+        uint64_t hitmask = 0;
+        if (x < e1 && x < ul_n) {
+            const int64_t xi = C.ul_idx[x];
+            const uint64_t xpk = C.ul_pk[x];
+            if (bit_of(C.zb, xi) && nib_dist(xpk, cpk) == C.R + 1)
+                hitmask = ~nib_ne(xpk, upk) & diff_cu & NIB1;   // x_p == u_p != c_p
+        }
+#pragma unroll
+        for (int p = 0; p < MAXN; p++) {
+            if (p < C.n && ((diff_cu >> (4 * p)) & 1))
+                cnt[p] += __popcll(__ballot((hitmask >> (4 * p)) & 1));
+        }
+    }
+    if (lane == 0) {
+        int32_t* a = C.acc + (int64_t)slot * ACCW + 1 + MAXN;
+#pragma unroll
+        for (int p = 0; p < MAXN; p++)
+            if (cnt[p]) atomicAdd(&a[p], cnt[p]);
+    }
+}
+
+// run this wave's share of the step's evaluator units
+__device__ inline void eval_units(const Cell& C, const Ctl* ctl, int64_t wid, int64_t nw)
+{
+    const int64_t nl = ctl->n_leave, ne = ctl->n_enter;
+    const int use_list = ctl->use_list, dtar = ctl->dtar;
+    const int64_t nchunk = ctl->nchunk, ul_n = ctl->ul_n;
+    const uint64_t upk = ctl->upk;
+    const int64_t per_slot_walk = (int64_t)dtar * C.E;
+    for (int64_t g = wid; g < nl + ne; g += nw) {
+        if (g < nl) {
+            const int slot = (int)(g / C.F);
+            unit_leave(C, slot, (int)(g - (int64_t)slot * C.F));
+        } else if (use_list) {
+            const int64_t h = g - nl;
+            const int slot = (int)(h / nchunk);
+            const int64_t ch = h - (int64_t)slot * nchunk;
+            const int64_t e0 = ch * LIST_CHUNK;
+            const int64_t e1 = (e0 + LIST_CHUNK < ul_n) ? e0 + LIST_CHUNK : ul_n;
+            unit_list(C, slot, e0, e1, upk, ul_n);
+        } else {
+            const int64_t h = g - nl;
+            const int slot = (int)(h / per_slot_walk);
+            const int64_t r = h - (int64_t)slot * per_slot_walk;
+            const int t = (int)(r / C.E);
+            const int e = (int)(r - (int64_t)t * C.E);
+            const int p = C.slot_pos[(int64_t)slot * MAXN + t];
+            unit_walk(C, slot, p, nib_get(upk, p), e);
+        }
+    }
+}
+
+// ------------------------------------------------------------- shuffle
+// partial Fisher-Yates over the ncand candidates (original index k =
+// slot*dtar + t), `take` draws from ring position sh_pos (:1383-1389).  One
+// wave; writes C.shuf[0..take).
+__device__ inline void wave_shuffle(const Cell& C, const Ctl* ctl, uint32_t* lds_a)
+{
+    const int lane = threadIdx.x & 63;
+    const int64_t ncand = ctl->ncand, take = ctl->take, pos0 = ctl->sh_pos;
+    const bool in_lds = ncand <= SHUF_LDS;
+    uint32_t* a = in_lds ? lds_a : C.shuf;
+    __shared__ uint32_t jb[64];
+    for (int64_t k = lane; k < ncand; k += 64) a[k] = (uint32_t)k;
+    __threadfence_block();
+    for (int64_t s0 = 0; s0 < take; s0 += 64) {
+        const int64_t s = s0 + lane;
+        if (s < take) jb[lane] = (uint32_t)(s + (int64_t)(ring_at(C, pos0 + s) % (uint64_t)(ncand - s)));
+        __threadfence_block();
+        if (lane == 0) {
+            const int64_t lim = (take - s0 < 64) ? take - s0 : 64;
+            for (int64_t t = 0; t < lim; t++) {
+                const int64_t ss = s0 + t;
+                const uint32_t j = jb[t];
+                const uint32_t x = a[ss], y = a[j];
+                a[j] = x;
+                a[ss] = y;
+            }
+        }
+        __threadfence_block();
+    }
+    if (in_lds)
+        for (int64_t k = lane; k < take; k += 64) C.shuf[k] = a[k];
+}
+
+// ------------------------------------------------------------- select
+struct SelSh {
+    WgRed red;
+    WgScan scan;
+    int64_t inc[WG];
+};
+
+__device__ __forceinline__ int64_t cand_delta(const Cell& C, const Ctl* ctl, uint32_t k,
+                                              int* pi, int* pp, int* pv)
+{
+    const int dtar = ctl->dtar;
+    const int slot = (int)(k / (uint32_t)dtar);
+    const int t = (int)(k - (uint32_t)slot * (uint32_t)dtar);
+    const int p = C.slot_pos[(int64_t)slot * MAXN + t];
+    const int32_t* a = C.acc + (int64_t)slot * ACCW;
+    *pi = C.slot_cw[slot];
+    *pp = p;
+    *pv = nib_get(ctl->upk, p);
+    return (int64_t)(a[0] - a[1 + p]) - (int64_t)a[1 + MAXN + p];
+}
+
+// Selection of :1503-1518 in parallel.  In shuffled order, a valid candidate
+// (not tabu, or aspirating) with d below the running minimum restarts the tie
+// count; one equal to it draws rmod(nties).  The running minimum before s is
+// an exclusive prefix minimum; T(s) counts ties up to s and every tie's draw
+// is the T(s)-th of the step; nties(s) = 1 + T(s) - T(last restart <= s).
+// The winner is the last restart or successful tie.  All threads of one WG.
+__device__ inline void wg_select(const Cell& C, Ctl* ctl, SelSh& sh)
+{
+    const int tid = threadIdx.x;
+    if (tid == 0) {
+        ctl->rlow = ctl->ruse;
+        if (ctl->rgen < ctl->ruse + ctl->take + 2) ring_fill(C, ctl, ctl->ruse + ctl->take + 2, INT64_MAX);
+    }
+    __syncthreads();
+    const int64_t take = ctl->take, it = ctl->it, unc = ctl->unc, best = ctl->best_unc;
+    const int64_t r0 = ctl->ruse;
+    const int n = C.n, q = C.q;
+    int64_t carry_pm = INT64_MAX, carry_T = 0, carry_Tb = 0, winner = -1;
+    for (int64_t s0 = 0; s0 < take; s0 += WG) {
+        const int64_t s = s0 + tid;
+        bool valid = false;
+        int64_t d = INT64_MAX;
+        if (s < take) {
+            int i, p, v;
+            d = cand_delta(C, ctl, C.shuf[s], &i, &p, &v);
+            const bool is_tabu = C.tabu[((int64_t)i * n + p) * q + v] > it;
+            valid = !(is_tabu && unc + d >= best);
+        }
+        const int64_t x = valid ? d : INT64_MAX;
+        const int64_t incl = wg_incl_min(x, sh.scan);
+        sh.inc[tid] = incl;
+        __syncthreads();
+        int64_t excl = tid ? sh.inc[tid - 1] : INT64_MAX;
+        const int64_t chunk_min = sh.inc[WG - 1];
+        __syncthreads();
+        const int64_t pm = excl < carry_pm ? excl : carry_pm;
+        const bool nb = valid && d < pm;
+        const bool tie = valid && d == pm;
+        int64_t tot;
+        const int64_t T = carry_T + wg_incl_sum(tie ? 1 : 0, sh.scan, &tot);
+        int64_t Tb = wg_incl_max(nb ? T : -1, sh.scan);
+        if (Tb < carry_Tb) Tb = carry_Tb;
+        bool ev = nb;
+        if (tie) {
+            const uint64_t rv = ring_at(C, r0 + T - 1);
+            ev = (rv % (uint64_t)(1 + T - Tb)) == 0;
+        }
+        const int64_t wch = wg_max(ev ? s : -1, sh.red);
+        if (wch > winner) winner = wch;
+        // carries: last thread's values
+        if (tid == WG - 1) { sh.inc[0] = T; sh.inc[1] = Tb; }
+        __syncthreads();
+        carry_T = sh.inc[0];
+        carry_Tb = sh.inc[1];
+        if (chunk_min < carry_pm) carry_pm = chunk_min;
+        __syncthreads();
+    }
+    if (tid == 0) {
+        int64_t draws, dsel;
+        int nties;
+        int64_t s;
+        if (winner < 0) {
+            s = (int64_t)(ring_at(C, r0) % (uint64_t)take);
+            draws = 1;
+            dsel = DSEL_RANDOM;
+            nties = 0;
+        } else {
+            s = winner;
+            draws = carry_T;
+            nties = (int)(1 + carry_T - carry_Tb);
+            int i_, p_, v_;
+            dsel = cand_delta(C, ctl, C.shuf[s], &i_, &p_, &v_);
+        }
+        const uint64_t rs = ring_at(C, r0 + draws);
+        draws += 1;
+        ctl->ruse = r0 + draws;
+        ctl->rlow = ctl->ruse;
+        int bi, bp, bv;
+        cand_delta(C, ctl, C.shuf[s], &bi, &bp, &bv);
+        const int oldv = C.code[(int64_t)bi * n + bp];
+        ctl->bi = bi; ctl->bp = bp; ctl->bv = bv; ctl->oldv = oldv;
+        ctl->dsel = dsel;
+        ctl->nties = nties;
+        ctl->stamp = (int32_t)(it + ctl->tenure + (int64_t)(rs % 4));
+        ctl->base = C.cidx[bi];
+        ctl->dv = (int64_t)(bv - oldv) * C.pw[bp];
+        ctl->ncw = ctl->nq;
+        ctl->selected = 1;
+    }
+    __syncthreads();
+    // distinct codewords among the first `take` (differs from nq only when
+    // --cand cuts the sample)
+    if (take < ctl->ncand) {
+        int64_t cntd = 0;
+        for (int64_t s0 = 0; s0 < take; s0 += WG) {
+            const int64_t s = s0 + tid;
+            int first = 0;
+            if (s < take) {
+                const uint32_t slot = C.shuf[s] / (uint32_t)ctl->dtar;
+                first = 1;
+                for (int64_t s2 = 0; s2 < s; s2++)
+                    if (C.shuf[s2] / (uint32_t)ctl->dtar == slot) { first = 0; break; }
+            }
+            cntd += first;
+        }
+        cntd = wg_sum(cntd, sh.red);
+        if (tid == 0) ctl->ncw = (int32_t)cntd;
+        __syncthreads();
+    }
+}
+
+// --------------------------------------------------------------- kernels
+__global__ void __launch_bounds__(WG) k_eval(const Cell* __restrict__ pc)
+{
+    const Cell& C = *pc;
+    Ctl* ctl = C.ctl;
+    __shared__ uint32_t shuf_lds[SHUF_LDS];
+    __shared__ SelSh sel;
+    __shared__ int last;
+    if (!ctl->begun || ctl->done || ctl->selected) return;
+    const int wv = threadIdx.x >> 6;
+    const bool helper = blockIdx.x == 0 && wv < 2;
+    if (helper) {
+        if (wv == 0) wave_shuffle(C, ctl, shuf_lds);
+        else if ((threadIdx.x & 63) == 0) ring_fill(C, ctl, ctl->rlow + RB, ctl->budget);
+    } else {
+        const int64_t wid = (int64_t)blockIdx.x * WAVES + wv - 2;
+        const int64_t nw = (int64_t)gridDim.x * WAVES - 2;
+        eval_units(C, ctl, wid, nw);
+    }
+    __threadfence();
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const int prev = atomicAdd(&ctl->evdone, 1);
+        last = (prev == (int)gridDim.x - 1);
+    }
+    __syncthreads();
+    if (!last) return;
+    __threadfence();
+    wg_select(C, ctl, sel);
+    if (threadIdx.x == 0) ctl->evdone = 0;
+}
+
+// kick: teleport `kick` codewords near uncovered words (:1553-1578).  All threads.
+__device__ inline void wg_ball(const Cell& C, Ctl* ctl, int i, int delta, NarrowSh& sh)
+{
+    const int64_t base = C.cidx[i];
+    const uint64_t pk = C.cpk[i];
+    const bool list_on = ctl->ul_valid != 0;
+    int64_t d = 0;
+    for (int64_t g = threadIdx.x; g < C.V; g += WG) {
+        uint64_t wpk;
+        const int64_t w = ball_word_at(C, base, pk, g, &wpk);
+        d += ball_word(C, ctl, w, wpk, delta, list_on);
+    }
+    __threadfence();
+    d = wg_sum(d, sh.red);
+    if (threadIdx.x == 0) ctl->unc += d;
+    __syncthreads();
+}
+
+__device__ inline void wg_kick(const Cell& C, Ctl* ctl, NarrowSh& sh)
+{
+    const int n = C.n, q = C.q;
+    for (int t = 0; t < ctl->kick; t++) {
+        wg_ensure(C, ctl, 1);
+        if (threadIdx.x == 0) {
+            sh.v32[1] = (int)(ring_at(C, ctl->ruse) % (uint64_t)C.M);
+            ctl->ruse += 1;
+        }
+        __syncthreads();
+        const int i = sh.v32[1];
+        __syncthreads();
+        wg_ball(C, ctl, i, -1, sh);
+        const int64_t w = wg_pick(C, ctl, sh);
+        wg_ensure(C, ctl, 2 * n + 2);
+        if (threadIdx.x == 0) {
+            uint64_t pk;
+            int64_t pos = ctl->ruse;
+            if (w >= 0) {
+                pk = pack_index(C, w);
+                for (int t2 = 0; t2 < C.R; t2++) {
+                    // gcc evaluates the assigned value's draw before the index's
+                    const int val = (int)(ring_at(C, pos) % (uint64_t)q);
+                    const int at = (int)(ring_at(C, pos + 1) % (uint64_t)n);
+                    pos += 2;
+                    pk = nib_put(pk, at, val);
+                }
+            } else {
+                pk = 0;
+                for (int j = 0; j < n; j++) pk = nib_put(pk, j, (int)(ring_at(C, pos++) % (uint64_t)q));
+            }
+            ctl->ruse = pos;
+            int64_t idx = 0;
+            for (int j = 0; j < n; j++) {
+                const int v = nib_get(pk, j);
+                C.code[(int64_t)i * n + j] = (uint8_t)v;
+                idx += (int64_t)v * C.pw[j];
+            }
+            C.cpk[i] = pk;
+            C.cidx[i] = idx;
+        }
+        __threadfence();
+        __syncthreads();
+        wg_ball(C, ctl, i, +1, sh);
+    }
+    for (int64_t x = threadIdx.x; x < (int64_t)C.M * n * q; x += WG) C.tabu[x] = 0;
+    __threadfence();
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        ctl->nkicks += 1;
+        ctl->since = 0;
+    }
+    __syncthreads();
+}
+
+// apply the committed move and close the step.  All threads of one WG.
+__device__ inline void wg_post(const Cell& C, Ctl* ctl, NarrowSh& sh)
+{
+    const int n = C.n, q = C.q;
+    if (threadIdx.x == 0) {
+        const int64_t dunc = ctl->dunc;
+        ctl->dunc = 0;
+        ctl->unc += dunc;
+        const int bi = ctl->bi, bp = ctl->bp, bv = ctl->bv, oldv = ctl->oldv;
+        C.code[(int64_t)bi * n + bp] = (uint8_t)bv;
+        C.cpk[bi] = nib_put(C.cpk[bi], bp, bv);
+        C.cidx[bi] += ctl->dv;
+        C.tabu[((int64_t)bi * n + bp) * q + oldv] = ctl->stamp;
+        if (ctl->ul_ovf) { ctl->ul_valid = 0; ctl->ul_ovf = 0; }
+        ctl->commit_bytes += 8 * (int64_t)C.E * C.P;
+        sh.v64[3] = dunc;
+        sh.v32[2] = 0;
+        if (ctl->unc < ctl->best_unc) {
+            ctl->best_unc = ctl->unc;
+            ctl->since = 0;
+            sh.v32[2] = 1;
+            if (ctl->best_unc == 0) ctl->done = 1;
+        } else if (++ctl->since > ctl->stall) {
+            sh.v32[2] = 2;
+        }
+    }
+    __threadfence();
+    __syncthreads();
+    const int what = sh.v32[2];
+    const int64_t dunc = sh.v64[3];
+    __syncthreads();
+    if (what == 1)
+        for (int64_t x = threadIdx.x; x < (int64_t)C.M * n; x += WG) C.best[x] = C.code[x];
+    else if (what == 2)
+        wg_kick(C, ctl, sh);
+    if (threadIdx.x == 0) {
+        if (C.trace && ctl->tpos < C.tcap) {
+            int64_t* tr = C.trace + ctl->tpos * TRW;
+            tr[0] = ctl->it;
+            tr[1] = ctl->u;
+            tr[2] = ctl->ncand;
+            tr[3] = ctl->ncw;
+            tr[4] = ctl->bi;
+            tr[5] = ctl->bp;
+            tr[6] = ctl->bv;
+            tr[7] = ctl->oldv;
+            tr[8] = ctl->dsel;
+            tr[9] = dunc;
+            tr[10] = ctl->unc;
+            tr[11] = ctl->best_unc;
+            tr[12] = what == 2;
+            tr[13] = ctl->ruse;
+            tr[14] = ctl->since;
+            tr[15] = ctl->nties;
+        }
+        ctl->tpos += 1;
+        ctl->begun = 0;
+        ctl->selected = 0;
+    }
+    __threadfence();
+    __syncthreads();
+}
+
+// the commit units: every pattern of the subsets avoiding bp, flattened
+__device__ inline int64_t commit_units(const Cell& C, const Ctl* ctl, int64_t gt, int64_t nt)
+{
+    const int bi = ctl->bi, bp = ctl->bp, bv = ctl->bv;
+    const int64_t base = ctl->base, dv = ctl->dv;
+    const uint64_t pk = C.cpk[bi];
+    const bool list_on = ctl->ul_valid != 0;
+    const int64_t S = (int64_t)C.E * C.P;
+    int64_t d = 0;
+    for (int64_t g = gt; g < S; g += nt) {
+        const int64_t e = g / C.P;
+        const uint32_t t = (uint32_t)(g - e * C.P);
+        uint64_t apk;
+        const int64_t a = pattern_word(C, base, pk, C.psub[(int64_t)bp * C.E + e], C.R, t, &apk);
+        const int64_t b = a + dv;
+        const uint16_t ca = cnt_get(C, a), cb = cnt_get(C, b);
+        cnt_put(C, a, (uint16_t)(ca - 1));
+        cnt_put(C, b, (uint16_t)(cb + 1));
+        if (ca == 1) {
+            bit_set(C.zb, a); bit_clr(C.ob, a);
+            atomicAdd(&C.blk[a >> BLK_SHIFT], 1u);
+            if (list_on) list_insert(C, C.ctl, a, apk);
+            d++;
+        } else if (ca == 2) {
+            bit_set(C.ob, a);
+        }
+        if (cb == 0) {
+            bit_clr(C.zb, b); bit_set(C.ob, b);
+            atomicSub(&C.blk[b >> BLK_SHIFT], 1u);
+            d--;
+        } else if (cb == 1) {
+            bit_clr(C.ob, b);
+        }
+    }
+    return d;
+}
+
+__global__ void __launch_bounds__(WG) k_commit(const Cell* __restrict__ pc, int api)
+{
+    const Cell& C = *pc;
+    Ctl* ctl = C.ctl;
+    __shared__ NarrowSh sh;
+    __shared__ int last;
+    if (!ctl->begun || ctl->done || !ctl->selected) return;
+    const int64_t gt = (int64_t)blockIdx.x * WG + threadIdx.x;
+    const int64_t d = commit_units(C, ctl, gt, (int64_t)gridDim.x * WG);
+    const int64_t ds = wg_sum(d, sh.red);
+    if (threadIdx.x == 0 && ds) atomicAdd(reinterpret_cast<unsigned long long*>(&ctl->dunc), (unsigned long long)ds);
+    __threadfence();
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const int prev = atomicAdd(&ctl->cmdone, 1);
+        last = (prev == (int)gridDim.x - 1);
+    }
+    __syncthreads();
+    if (!last) return;
+    __threadfence();
+    if (threadIdx.x == 0) ctl->cmdone = 0;
+    if (api) {
+        // a single committed move requested through the C ABI: apply it only
+        if (threadIdx.x == 0) {
+            const int64_t dunc = ctl->dunc;
+            ctl->dunc = 0;
+            ctl->unc += dunc;
+            ctl->dsel = dunc;
+            C.code[(int64_t)ctl->bi * C.n + ctl->bp] = (uint8_t)ctl->bv;
+            C.cpk[ctl->bi] = nib_put(C.cpk[ctl->bi], ctl->bp, ctl->bv);
+            C.cidx[ctl->bi] += ctl->dv;
+            if (ctl->ul_ovf) { ctl->ul_valid = 0; ctl->ul_ovf = 0; }
+            ctl->begun = 0;
+            ctl->selected = 0;
+        }
+        return;
+    }
+    wg_post(C, ctl, sh);
+    wg_begin(C, ctl, sh);
+}
+
+// set the run's stop and begin the first step (one workgroup)
+__global__ void __launch_bounds__(WG) k_begin_run(const Cell* __restrict__ pc, int64_t nsteps)
+{
+    const Cell& C = *pc;
+    Ctl* ctl = C.ctl;
+    __shared__ NarrowSh sh;
+    if (threadIdx.x == 0) {
+        ctl->it_stop = ctl->it + nsteps - (ctl->begun ? 1 : 0);
+        ctl->tpos = 0;
+    }
+    __syncthreads();
+    wg_begin(C, ctl, sh);
+}
+
+// --------------------------------------------------------- init kernels
+// uniform random digits for codewords have..M-1, row-major (:1235-1238)
+__global__ void __launch_bounds__(WG) k_init_random(const Cell* __restrict__ pc, int have)
+{
+    const Cell& C = *pc;
+    Ctl* ctl = C.ctl;
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    const int64_t need = (int64_t)(C.M - have) * C.n;
+    int64_t done = 0;
+    while (done < need) {
+        if (ctl->rlow < ctl->ruse) ctl->rlow = ctl->ruse;
+        if (ctl->rgen <= ctl->ruse) ring_fill(C, ctl, ctl->ruse + RB / 2, INT64_MAX);
+        const int64_t avail = ctl->rgen - ctl->ruse;
+        const int64_t k = (need - done < avail) ? need - done : avail;
+        for (int64_t x = 0; x < k; x++)
+            C.code[(int64_t)have * C.n + done + x] = (uint8_t)(ring_at(C, ctl->ruse + x) % (uint64_t)C.q);
+        ctl->ruse += k;
+        done += k;
+    }
+}
+
+// digits -> packed digits and word index for every codeword
+__global__ void __launch_bounds__(WG) k_pack_code(const Cell* __restrict__ pc)
+{
+    const Cell& C = *pc;
+    for (int i = blockIdx.x * WG + threadIdx.x; i < C.M; i += gridDim.x * WG) {
+        uint64_t pk = 0;
+        int64_t idx = 0;
+        for (int j = 0; j < C.n; j++) {
+            const int v = C.code[(int64_t)i * C.n + j];
+            pk |= (uint64_t)v << (4 * j);
+            idx += (int64_t)v * C.pw[j];
+        }
+        C.cpk[i] = pk;
+        C.cidx[i] = idx;
+    }
+}
+
+// greedy placement of codeword i on the picked uncovered word (:1219-1233)
+__global__ void __launch_bounds__(WG) k_greedy_pick(const Cell* __restrict__ pc, int i)
+{
+    const Cell& C = *pc;
+    Ctl* ctl = C.ctl;
+    __shared__ NarrowSh sh;
+    int64_t w = wg_pick(C, ctl, sh);
+    if (threadIdx.x == 0) {
+        if (w < 0) {
+            if (ctl->rgen <= ctl->ruse) ring_fill(C, ctl, ctl->ruse + 1, INT64_MAX);
+            w = (int64_t)(ring_at(C, ctl->ruse) % (uint64_t)C.N);
+            ctl->ruse += 1;
+        }
+        const uint64_t pk = pack_index(C, w);
+        for (int j = 0; j < C.n; j++) C.code[(int64_t)i * C.n + j] = (uint8_t)nib_get(pk, j);
+        C.cpk[i] = pk;
+        C.cidx[i] = w;
+    }
+}
+
+// +-1 ball of codeword i (incremental, whole grid)
+__global__ void __launch_bounds__(WG) k_ball_cw(const Cell* __restrict__ pc, int i, int delta)
+{
+    const Cell& C = *pc;
+    Ctl* ctl = C.ctl;
+    __shared__ WgRed red;
+    const bool list_on = ctl->ul_valid != 0;
+    const int64_t base = C.cidx[i];
+    const uint64_t pk = C.cpk[i];
+    int64_t d = 0;
+    for (int64_t g = (int64_t)blockIdx.x * WG + threadIdx.x; g < C.V; g += (int64_t)gridDim.x * WG) {
+        uint64_t wpk;
+        const int64_t w = ball_word_at(C, base, pk, g, &wpk);
+        d += ball_word(C, ctl, w, wpk, delta, list_on);
+    }
+    const int64_t s = wg_sum(d, red);
+    if (threadIdx.x == 0 && s) atomicAdd(reinterpret_cast<unsigned long long*>(&ctl->unc), (unsigned long long)s);
+}
+
+// words of each given centre's ball with cnt == 1 (remove-and-repair, :927-946)
+__global__ void __launch_bounds__(WG) k_unique_cover(const Cell* __restrict__ pc, const int64_t* centers,
+                                                     int k, int64_t* out)
+{
+    const Cell& C = *pc;
+    __shared__ WgRed red;
+    for (int c = blockIdx.x; c < k; c += gridDim.x) {
+        const int64_t base = centers[c];
+        const uint64_t pk = pack_index(C, base);
+        int64_t u = 0;
+        for (int64_t g = threadIdx.x; g < C.V; g += WG)
+            u += bit_of(C.ob, ball_word_at(C, base, pk, g, nullptr));
+        u = wg_sum(u, red);
+        if (threadIdx.x == 0) out[c] = u;
+    }
+}
+
+// whole-grid list rebuild from the zb plane
+__global__ void __launch_bounds__(WG) k_list_rebuild(const Cell* __restrict__ pc)
+{
+    const Cell& C = *pc;
+    Ctl* ctl = C.ctl;
+    for (int64_t x = (int64_t)blockIdx.x * WG + threadIdx.x; x < C.nzw; x += (int64_t)gridDim.x * WG) {
+        const uint32_t z = C.zb[x];
+        C.inl[x] = z;
+        uint32_t m = z;
+        while (m) {
+            const int b = __ffs(m) - 1;
+            m &= m - 1;
+            const int64_t w = x * 32 + b;
+            const unsigned long long pos = atomicAdd(reinterpret_cast<unsigned long long*>(&ctl->ul_n), 1ULL);
+            if ((int64_t)pos < C.ulcap) { C.ul_idx[pos] = w; C.ul_pk[pos] = pack_index(C, w); }
+            else atomicOr(&ctl->ul_ovf, 1);
+        }
+    }
+}
+
+// first uncovered word at or after start (one workgroup)
+__global__ void __launch_bounds__(WG) k_first_from(const Cell* __restrict__ pc, int64_t start, int64_t* out)
+{
+    const Cell& C = *pc;
+    __shared__ NarrowSh sh;
+    const int64_t w = wg_first_from(C, start, sh);
+    if (threadIdx.x == 0) *out = w;
+}
+
+// --------------------------------------------- evaluator through the ABI
+// codewords slot_cw[0..k) against word u (ctl->upk), all differing positions
+__global__ void __launch_bounds__(WG) k_eval_probe(const Cell* __restrict__ pc, int k)
+{
+    const Cell& C = *pc;
+    const Ctl* ctl = C.ctl;
+    const uint64_t upk = ctl->upk;
+    const int64_t wid = (int64_t)blockIdx.x * WAVES + (threadIdx.x >> 6);
+    const int64_t nw = (int64_t)gridDim.x * WAVES;
+    const int64_t nl = (int64_t)k * C.F, ne = (int64_t)k * C.n * C.E;
+    for (int64_t g = wid; g < nl + ne; g += nw) {
+        if (g < nl) {
+            const int slot = (int)(g / C.F);
+            unit_leave(C, slot, (int)(g - (int64_t)slot * C.F));
+        } else {
+            const int64_t h = g - nl;
+            const int slot = (int)(h / ((int64_t)C.n * C.E));
+            const int64_t r = h - (int64_t)slot * C.n * C.E;
+            const int p = (int)(r / C.E);
+            const int e = (int)(r - (int64_t)p * C.E);
+            const int up = nib_get(upk, p);
+            if (nib_get(C.cpk[C.slot_cw[slot]], p) == up) continue;   // uniform per wave
+            unit_walk(C, slot, p, up, e);
+        }
+    }
+}
+
+__global__ void __launch_bounds__(WG) k_eval_finish(const Cell* __restrict__ pc, int k, int64_t* dvec)
+{
+    const Cell& C = *pc;
+    const uint64_t upk = C.ctl->upk;
+    for (int x = blockIdx.x * WG + threadIdx.x; x < k * C.n; x += gridDim.x * WG) {
+        const int slot = x / C.n, p = x - slot * C.n;
+        const int32_t* a = C.acc + (int64_t)slot * ACCW;
+        if (nib_get(C.cpk[C.slot_cw[slot]], p) == nib_get(upk, p)) dvec[x] = DSEL_RANDOM;
+        else dvec[x] = (int64_t)(a[0] - a[1 + p]) - (int64_t)a[1 + MAXN + p];
+    }
+}
+
+// set up a single move for k_commit(api = 1)
+__global__ void k_set_move(const Cell* __restrict__ pc, int i, int p, int v)
+{
+    const Cell& C = *pc;
+    Ctl* ctl = C.ctl;
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    const int oldv = C.code[(int64_t)i * C.n + p];
+    ctl->bi = i; ctl->bp = p; ctl->bv = v; ctl->oldv = oldv;
+    ctl->base = C.cidx[i];
+    ctl->dv = (int64_t)(v - oldv) * C.pw[p];
+    ctl->begun = 1;
+    ctl->selected = 1;
+    ctl->done = 0;
+}
+
+}  // namespace cov

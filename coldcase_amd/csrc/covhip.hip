@@ -1,0 +1,962 @@
+// covhip.hip -- host side of libcovhip.so: cell handles, tables, launches and
+// the C ABI declared in include/covhip.h.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstddef>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "covhip.h"
+#include "cov_common.h"
+#include "cov_cover.h"
+#include "cov_dist.h"
+#include "cov_step.h"
+
+using namespace cov;
+
+struct cov_cell {
+    Cell h{};                 // host copy of the device descriptor
+    Cell* d = nullptr;        // device descriptor
+    RecountPlan plan{};
+    hipStream_t stream = nullptr;
+    std::string err;
+    int eval_groups = 0, commit_groups = 0;
+    int64_t mem_bytes = 0;
+    std::vector<void*> allocs;
+    int64_t* d_small = nullptr;           // 64 int64 of device scratch
+    // graph of GS steps
+    hipGraphExec_t gexec = nullptr;
+    int g_steps = 0, g_eg = 0, g_cg = 0;
+    // timing
+    bool timing = false;
+    double ms[4] = {0, 0, 0, 0};
+    int64_t launches[4] = {0, 0, 0, 0};
+    std::vector<hipEvent_t> evs;
+};
+
+namespace {
+
+constexpr int GRAPH_STEPS = 32;
+
+#define HCK(h, x)                                                              \
+    do {                                                                       \
+        hipError_t e_ = (x);                                                   \
+        if (e_ != hipSuccess) {                                                \
+            if (h) (h)->err = std::string(#x) + ": " + hipGetErrorString(e_);  \
+            return -(100 + (int)e_);                                           \
+        }                                                                      \
+    } while (0)
+
+int64_t binom(int a, int b)
+{
+    if (b < 0 || b > a) return 0;
+    if (b > a - b) b = a - b;
+    int64_t r = 1;
+    for (int i = 0; i < b; i++) r = r * (a - i) / (i + 1);
+    return r;
+}
+
+// all r-subsets of `pos`, lexicographic, packed 4 bits per position
+std::vector<uint32_t> subsets(const std::vector<int>& pos, int r)
+{
+    std::vector<uint32_t> out;
+    const int np = (int)pos.size();
+    if (r > np) return out;
+    std::vector<int> c(r);
+    for (int i = 0; i < r; i++) c[i] = i;
+    for (;;) {
+        uint32_t s = 0;
+        for (int i = 0; i < r; i++) s |= (uint32_t)pos[c[i]] << (4 * i);
+        out.push_back(s);
+        int i = r - 1;
+        while (i >= 0 && c[i] == np - r + i) i--;
+        if (i < 0) break;
+        c[i]++;
+        for (int j = i + 1; j < r; j++) c[j] = c[j - 1] + 1;
+    }
+    return out;
+}
+
+struct Shape {
+    int q, n, R, M, q1, F, E;
+    int64_t P, N, V, nzw, nblk, ulcap;
+};
+
+int shape_of(int q, int n, int R, int M, Shape* s)
+{
+    if (q < 2 || q > MAXQ || n < 1 || n > MAXN || R < 0 || R > MAXR || R >= n || M < 1 || M > 65535)
+        return COV_EPARAM;
+    int64_t N = 1;
+    for (int i = 0; i < n; i++) {
+        if (N > (int64_t)4e10 / q) return COV_ESPACE;
+        N *= q;
+    }
+    s->q = q; s->n = n; s->R = R; s->M = M; s->q1 = q - 1;
+    s->F = (int)binom(n, R);
+    s->E = (int)binom(n - 1, R);
+    s->P = 1;
+    for (int i = 0; i < R; i++) s->P *= (q - 1);
+    if (s->P >= (1 << 24)) return COV_ELIMIT;
+    s->N = N;
+    s->V = 0;
+    for (int r = 0; r <= R; r++) {
+        int64_t t = binom(n, r);
+        for (int i = 0; i < r; i++) t *= (q - 1);
+        s->V += t;
+    }
+    s->nzw = (N + 31) / 32;
+    s->nblk = (N + BLK_WORDS - 1) / BLK_WORDS;
+    s->ulcap = std::min<int64_t>(N, (int64_t)1 << 20);
+    return COV_OK;
+}
+
+// device bytes of each array, in allocation order
+struct Sizes {
+    int64_t cnt, plane, blk, code, cpk, cidx, tabu, fsub, psub, bsub, ssub, ul_idx, ul_pk,
+        shuf, acc, slot_cw, slot_pos, rbuf, ctl, cell, small;
+};
+
+Sizes sizes_of(const Shape& s)
+{
+    Sizes z{};
+    z.cnt = ((s.N + 1) / 2 + 16) * 4;
+    z.plane = (s.nzw + 16) * 4;
+    z.blk = (s.nblk + 16) * 4;
+    z.code = (int64_t)s.M * s.n + 64;
+    z.cpk = (int64_t)s.M * 8;
+    z.cidx = (int64_t)s.M * 8;
+    z.tabu = (int64_t)s.M * s.n * s.q * 4;
+    z.fsub = (int64_t)s.F * 4 + 4;
+    z.psub = (int64_t)s.n * s.E * 4 + 4;
+    z.bsub = ((int64_t)1 << s.n) * 4 + 4;
+    z.ssub = ((int64_t)1 << s.n) * 4 + 4;
+    z.ul_idx = s.ulcap * 8;
+    z.ul_pk = s.ulcap * 8;
+    z.shuf = (int64_t)s.M * s.n * 4 + 4;
+    z.acc = (int64_t)s.M * ACCW * 4;
+    z.slot_cw = (int64_t)s.M * 4;
+    z.slot_pos = (int64_t)s.M * MAXN;
+    z.rbuf = (int64_t)RB * 8;
+    z.ctl = sizeof(Ctl);
+    z.cell = sizeof(Cell);
+    z.small = 64 * 8;
+    return z;
+}
+
+int64_t total_bytes(const Shape& s)
+{
+    const Sizes z = sizes_of(s);
+    return z.cnt + 3 * z.plane + z.blk + 2 * z.code + z.cpk + z.cidx + z.tabu + z.fsub + z.psub +
+           z.bsub + z.ssub + z.ul_idx + z.ul_pk + z.shuf + z.acc + z.slot_cw + z.slot_pos + z.rbuf +
+           z.ctl + z.cell + z.small;
+}
+
+template <typename T>
+int dalloc(cov_cell* h, T** p, int64_t bytes)
+{
+    void* v = nullptr;
+    hipError_t e = hipMalloc(&v, (size_t)bytes);
+    if (e != hipSuccess) {
+        h->err = std::string("hipMalloc: ") + hipGetErrorString(e);
+        return COV_ENOMEM;
+    }
+    e = hipMemset(v, 0, (size_t)bytes);
+    if (e != hipSuccess) {
+        (void)hipFree(v);
+        h->err = std::string("hipMemset: ") + hipGetErrorString(e);
+        return -(100 + (int)e);
+    }
+    h->allocs.push_back(v);
+    h->mem_bytes += bytes;
+    *p = reinterpret_cast<T*>(v);
+    return COV_OK;
+}
+
+template <typename T>
+int upload(cov_cell* h, T* dst, const void* src, int64_t bytes)
+{
+    HCK(h, hipMemcpyAsync(dst, src, (size_t)bytes, hipMemcpyHostToDevice, h->stream));
+    return COV_OK;
+}
+
+#define CTL_PTR(h, field) (reinterpret_cast<char*>((h)->h.ctl) + offsetof(Ctl, field))
+
+template <typename T>
+int set_ctl(cov_cell* h, size_t off, T v)
+{
+    HCK(h, hipMemcpyAsync(reinterpret_cast<char*>(h->h.ctl) + off, &v, sizeof(T), hipMemcpyHostToDevice,
+                          h->stream));
+    HCK(h, hipStreamSynchronize(h->stream));
+    return COV_OK;
+}
+
+int read_ctl(cov_cell* h, Ctl* out)
+{
+    HCK(h, hipMemcpyAsync(out, h->h.ctl, sizeof(Ctl), hipMemcpyDeviceToHost, h->stream));
+    HCK(h, hipStreamSynchronize(h->stream));
+    return COV_OK;
+}
+
+int push_cell(cov_cell* h)
+{
+    HCK(h, hipMemcpyAsync(h->d, &h->h, sizeof(Cell), hipMemcpyHostToDevice, h->stream));
+    HCK(h, hipStreamSynchronize(h->stream));
+    return COV_OK;
+}
+
+int grid_for(int64_t items, int per_wg, int lo, int hi)
+{
+    int64_t g = (items + per_wg - 1) / per_wg;
+    if (g < lo) g = lo;
+    if (g > hi) g = hi;
+    return (int)g;
+}
+
+void splitmix_seed(uint64_t seed, uint64_t s[4])
+{
+    for (int i = 0; i < 4; i++) {
+        seed += 0x9E3779B97F4A7C15ULL;
+        uint64_t z = seed;
+        z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+        z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+        s[i] = z ^ (z >> 31);
+    }
+}
+
+#define RC(x)                          \
+    do {                               \
+        int rc_ = (x);                 \
+        if (rc_ != COV_OK) return rc_; \
+    } while (0)
+
+int timed_begin(cov_cell* h, int kind, hipEvent_t* a)
+{
+    (void)kind;
+    if (!h->timing) return COV_OK;
+    HCK(h, hipEventCreate(a));
+    HCK(h, hipEventRecord(*a, h->stream));
+    return COV_OK;
+}
+
+int timed_end(cov_cell* h, int kind, hipEvent_t a)
+{
+    if (!h->timing) return COV_OK;
+    hipEvent_t b;
+    HCK(h, hipEventCreate(&b));
+    HCK(h, hipEventRecord(b, h->stream));
+    HCK(h, hipEventSynchronize(b));
+    float ms = 0;
+    HCK(h, hipEventElapsedTime(&ms, a, b));
+    h->ms[kind] += ms;
+    h->launches[kind] += 1;
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(b);
+    return COV_OK;
+}
+
+int do_recount(cov_cell* h)
+{
+    hipEvent_t a = nullptr;
+    RC(timed_begin(h, 3, &a));
+    const int g = (int)std::min<int64_t>(h->plan.ntile, 2048);
+    hipLaunchKernelGGL(k_recount, dim3(g), dim3(WG), (size_t)h->plan.T * 4, h->stream, h->d, h->plan);
+    HCK(h, hipGetLastError());
+    RC(timed_end(h, 3, a));
+    int64_t zero = 0;
+    HCK(h, hipMemcpyAsync(CTL_PTR(h, unc), &zero, 8, hipMemcpyHostToDevice, h->stream));
+    HCK(h, hipMemcpyAsync(CTL_PTR(h, ul_n), &zero, 8, hipMemcpyHostToDevice, h->stream));
+    int32_t zi = 0;
+    HCK(h, hipMemcpyAsync(CTL_PTR(h, ul_valid), &zi, 4, hipMemcpyHostToDevice, h->stream));
+    hipLaunchKernelGGL(k_derive, dim3(grid_for(h->h.nzw, WG, 1, 4096)), dim3(WG), 0, h->stream, h->d);
+    HCK(h, hipGetLastError());
+    return COV_OK;
+}
+
+int do_derive(cov_cell* h)
+{
+    int64_t zero = 0;
+    HCK(h, hipMemcpyAsync(CTL_PTR(h, unc), &zero, 8, hipMemcpyHostToDevice, h->stream));
+    hipLaunchKernelGGL(k_derive, dim3(grid_for(h->h.nzw, WG, 1, 4096)), dim3(WG), 0, h->stream, h->d);
+    HCK(h, hipGetLastError());
+    return COV_OK;
+}
+
+int list_rebuild(cov_cell* h)
+{
+    int64_t zero = 0;
+    int32_t zi = 0;
+    HCK(h, hipMemcpyAsync(CTL_PTR(h, ul_n), &zero, 8, hipMemcpyHostToDevice, h->stream));
+    HCK(h, hipMemcpyAsync(CTL_PTR(h, ul_ovf), &zi, 4, hipMemcpyHostToDevice, h->stream));
+    hipLaunchKernelGGL(k_list_rebuild, dim3(grid_for(h->h.nzw, WG, 1, 4096)), dim3(WG), 0, h->stream, h->d);
+    HCK(h, hipGetLastError());
+    Ctl c;
+    RC(read_ctl(h, &c));
+    const int32_t valid = c.ul_ovf ? 0 : 1;
+    RC(set_ctl<int32_t>(h, offsetof(Ctl, ul_valid), valid));
+    RC(set_ctl<int32_t>(h, offsetof(Ctl, ul_ovf), 0));
+    if (!valid) RC(set_ctl<int64_t>(h, offsetof(Ctl, ul_n), 0));
+    return COV_OK;
+}
+
+int launch_step(cov_cell* h)
+{
+    hipEvent_t a = nullptr;
+    RC(timed_begin(h, 0, &a));
+    hipLaunchKernelGGL(k_eval, dim3(h->eval_groups), dim3(WG), 0, h->stream, h->d);
+    HCK(h, hipGetLastError());
+    RC(timed_end(h, 0, a));
+    RC(timed_begin(h, 1, &a));
+    hipLaunchKernelGGL(k_commit, dim3(h->commit_groups), dim3(WG), 0, h->stream, h->d, 0);
+    HCK(h, hipGetLastError());
+    RC(timed_end(h, 1, a));
+    return COV_OK;
+}
+
+int ensure_graph(cov_cell* h)
+{
+    if (h->gexec && h->g_eg == h->eval_groups && h->g_cg == h->commit_groups) return COV_OK;
+    if (h->gexec) { (void)hipGraphExecDestroy(h->gexec); h->gexec = nullptr; }
+    hipGraph_t g;
+    HCK(h, hipStreamBeginCapture(h->stream, hipStreamCaptureModeRelaxed));
+    for (int s = 0; s < GRAPH_STEPS; s++) {
+        hipLaunchKernelGGL(k_eval, dim3(h->eval_groups), dim3(WG), 0, h->stream, h->d);
+        hipLaunchKernelGGL(k_commit, dim3(h->commit_groups), dim3(WG), 0, h->stream, h->d, 0);
+    }
+    HCK(h, hipStreamEndCapture(h->stream, &g));
+    hipError_t e = hipGraphInstantiate(&h->gexec, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    HCK(h, e);
+    h->g_steps = GRAPH_STEPS;
+    h->g_eg = h->eval_groups;
+    h->g_cg = h->commit_groups;
+    return COV_OK;
+}
+
+}  // namespace
+
+// =============================================================== the ABI
+extern "C" {
+
+int cov_version(void) { return 1; }
+
+const char* cov_strerror(int code)
+{
+    switch (code) {
+    case COV_OK: return "ok";
+    case COV_EPARAM: return "bad parameters";
+    case COV_ESPACE: return "q^n too large";
+    case COV_ENOMEM: return "device allocation failed";
+    case COV_ESTATE: return "call not valid in this state";
+    case COV_ELIMIT: return "input beyond a stated capacity";
+    default: return code <= -100 ? hipGetErrorString((hipError_t)(-code - 100)) : "unknown error";
+    }
+}
+
+int64_t cov_mem_bytes(int q, int n, int R, int M)
+{
+    Shape s;
+    const int rc = shape_of(q, n, R, M, &s);
+    if (rc != COV_OK) return rc;
+    return total_bytes(s);
+}
+
+const char* cov_last_error(const cov_cell_t* h) { return h ? h->err.c_str() : "null handle"; }
+
+int cov_create(int q, int n, int R, int M, cov_cell_t** out)
+{
+    if (!out) return COV_EPARAM;
+    *out = nullptr;
+    Shape s;
+    RC(shape_of(q, n, R, M, &s));
+    cov_cell* h = new cov_cell();
+    auto fail = [&](int rc) {
+        for (void* p : h->allocs) (void)hipFree(p);
+        if (h->stream) (void)hipStreamDestroy(h->stream);
+        delete h;
+        return rc;
+    };
+    {
+        const hipError_t e = hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking);
+        if (e != hipSuccess) { h->stream = nullptr; return fail(-(100 + (int)e)); }
+    }
+    Cell& C = h->h;
+    C.q = q; C.n = n; C.R = R; C.M = M; C.q1 = q - 1; C.F = s.F; C.E = s.E; C.P = s.P;
+    C.N = s.N; C.nzw = s.nzw; C.nblk = s.nblk; C.V = s.V; C.ncap = (int64_t)M * n; C.ulcap = s.ulcap;
+    for (int j = 0; j < n; j++) {
+        C.pw[j] = 1;
+        for (int e = 0; e < n - 1 - j; e++) C.pw[j] *= q;
+    }
+    C.magic = (q - 1 == 1) ? 0u : (uint32_t)((((uint64_t)1 << 32) + (uint64_t)(q - 2)) / (uint64_t)(q - 1));
+    const Sizes z = sizes_of(s);
+    int rc = COV_OK;
+#define AL(ptr, bytes) if ((rc = dalloc(h, &(ptr), (bytes))) != COV_OK) return fail(rc)
+    AL(C.cnt32, z.cnt);
+    AL(C.zb, z.plane);
+    AL(C.ob, z.plane);
+    AL(C.inl, z.plane);
+    AL(C.blk, z.blk);
+    AL(C.code, z.code);
+    AL(C.best, z.code);
+    AL(C.cpk, z.cpk);
+    AL(C.cidx, z.cidx);
+    AL(C.tabu, z.tabu);
+    AL(C.fsub, z.fsub);
+    AL(C.psub, z.psub);
+    AL(C.bsub, z.bsub);
+    AL(h->plan.ssub, z.ssub);
+    AL(C.ul_idx, z.ul_idx);
+    AL(C.ul_pk, z.ul_pk);
+    AL(C.shuf, z.shuf);
+    AL(C.acc, z.acc);
+    AL(C.slot_cw, z.slot_cw);
+    AL(C.slot_pos, z.slot_pos);
+    AL(C.rbuf, z.rbuf);
+    AL(C.ctl, z.ctl);
+    AL(h->d, z.cell);
+    AL(h->d_small, z.small);
+#undef AL
+    C.scratch64 = h->d_small;
+    C.trace = nullptr;
+    C.tcap = 0;
+
+    // position-subset tables
+    std::vector<int> all(n);
+    for (int j = 0; j < n; j++) all[j] = j;
+    std::vector<uint32_t> fs = subsets(all, R), ps;
+    for (int p = 0; p < n; p++) {
+        std::vector<int> rest;
+        for (int j = 0; j < n; j++) if (j != p) rest.push_back(j);
+        std::vector<uint32_t> t = subsets(rest, R);
+        ps.insert(ps.end(), t.begin(), t.end());
+    }
+    std::vector<uint32_t> bs;
+    for (int r = 0; r <= MAXR + 1; r++) { C.bsub_off[r] = 0; C.bpat_off[r] = 0; }
+    int64_t patacc = 0;
+    for (int r = 0; r <= R; r++) {
+        C.bsub_off[r] = (int64_t)bs.size();
+        C.bpat_off[r] = patacc;
+        std::vector<uint32_t> t = subsets(all, r);
+        bs.insert(bs.end(), t.begin(), t.end());
+        int64_t pr = 1;
+        for (int i = 0; i < r; i++) pr *= (q - 1);
+        patacc += (int64_t)t.size() * pr;
+    }
+    for (int r = R + 1; r <= MAXR + 1; r++) { C.bsub_off[r] = (int64_t)bs.size(); C.bpat_off[r] = patacc; }
+    // recount plan: tiles of q^(n-t) words with a common t-digit prefix
+    RecountPlan& pl = h->plan;
+    pl.t = 0;
+    pl.T = s.N;
+    while (pl.T > TILE_MAX) { pl.t++; pl.T /= q; }
+    pl.ntile = s.N / pl.T;
+    std::vector<int> suf;
+    for (int j = pl.t; j < n; j++) suf.push_back(j);
+    std::vector<uint32_t> ss;
+    int64_t sacc = 0;
+    for (int r = 0; r <= MAXR + 1; r++) {
+        pl.ssub_off[r] = (int64_t)ss.size();
+        pl.spat[r] = sacc;
+        int64_t pr = 1;
+        for (int i = 0; i < r; i++) pr *= (q - 1);
+        pl.spr[r] = pr;
+        pl.sinv[r] = 1.0f / (float)pr;
+        if (r <= R && r <= (int)suf.size()) {
+            std::vector<uint32_t> t = subsets(suf, r);
+            ss.insert(ss.end(), t.begin(), t.end());
+            sacc += (int64_t)t.size() * pr;
+        }
+    }
+    if ((rc = upload(h, C.fsub, fs.data(), (int64_t)fs.size() * 4)) != COV_OK) return fail(rc);
+    if (!ps.empty() && (rc = upload(h, C.psub, ps.data(), (int64_t)ps.size() * 4)) != COV_OK) return fail(rc);
+    if ((rc = upload(h, C.bsub, bs.data(), (int64_t)bs.size() * 4)) != COV_OK) return fail(rc);
+    if ((rc = upload(h, pl.ssub, ss.data(), (int64_t)ss.size() * 4)) != COV_OK) return fail(rc);
+
+    Ctl c{};
+    c.stall = 5000;
+    c.kick = 2;
+    c.cand = 100000;
+    c.budget = 1024;
+    c.tenure = 0;
+    c.it_stop = 0;
+    if ((rc = upload(h, C.ctl, &c, sizeof(Ctl))) != COV_OK) return fail(rc);
+    if ((rc = push_cell(h)) != COV_OK) return fail(rc);
+    if (hipFuncSetAttribute((const void*)k_recount, hipFuncAttributeMaxDynamicSharedMemorySize,
+                            (int)(pl.T * 4)) != hipSuccess) {
+        h->err = "recount tile does not fit in LDS";
+        return fail(COV_ELIMIT);
+    }
+    // default launch shape: about 32 candidate codewords per step
+    const int64_t eval_waves = 32 * ((int64_t)s.F + (int64_t)(R + 1) * s.E) *
+                               std::max<int64_t>(1, s.P / 64);
+    h->eval_groups = grid_for(eval_waves / 4, 8, 16, 1024);
+    h->commit_groups = grid_for((int64_t)s.E * s.P, 4 * WG, 1, 512);
+    *out = h;
+    return COV_OK;
+}
+
+int cov_destroy(cov_cell_t* h)
+{
+    if (!h) return COV_EPARAM;
+    (void)hipStreamSynchronize(h->stream);
+    if (h->gexec) (void)hipGraphExecDestroy(h->gexec);
+    for (void* p : h->allocs) (void)hipFree(p);
+    if (h->h.trace) (void)hipFree(h->h.trace);
+    if (h->h.dist) (void)hipFree(h->h.dist);
+    (void)hipStreamDestroy(h->stream);
+    delete h;
+    return COV_OK;
+}
+
+int cov_sync(cov_cell_t* h)
+{
+    if (!h) return COV_EPARAM;
+    HCK(h, hipStreamSynchronize(h->stream));
+    return COV_OK;
+}
+
+int cov_set_code(cov_cell_t* h, const uint8_t* digits)
+{
+    if (!h || !digits) return COV_EPARAM;
+    const Cell& C = h->h;
+    for (int64_t k = 0; k < (int64_t)C.M * C.n; k++)
+        if (digits[k] >= C.q) { h->err = "digit out of range"; return COV_EPARAM; }
+    RC(upload(h, C.code, digits, (int64_t)C.M * C.n));
+    hipLaunchKernelGGL(k_pack_code, dim3(grid_for(C.M, WG, 1, 256)), dim3(WG), 0, h->stream, h->d);
+    HCK(h, hipGetLastError());
+    RC(do_recount(h));
+    HCK(h, hipMemcpyAsync(C.best, C.code, (size_t)C.M * C.n, hipMemcpyDeviceToDevice, h->stream));
+    Ctl c;
+    RC(read_ctl(h, &c));
+    RC(set_ctl<int64_t>(h, offsetof(Ctl, best_unc), c.unc));
+    return COV_OK;
+}
+
+int cov_get_code(cov_cell_t* h, uint8_t* digits)
+{
+    if (!h || !digits) return COV_EPARAM;
+    HCK(h, hipMemcpyAsync(digits, h->h.code, (size_t)h->h.M * h->h.n, hipMemcpyDeviceToHost, h->stream));
+    HCK(h, hipStreamSynchronize(h->stream));
+    return COV_OK;
+}
+
+int cov_get_best(cov_cell_t* h, uint8_t* digits, int64_t* best_uncovered)
+{
+    if (!h) return COV_EPARAM;
+    if (digits) HCK(h, hipMemcpyAsync(digits, h->h.best, (size_t)h->h.M * h->h.n, hipMemcpyDeviceToHost, h->stream));
+    Ctl c;
+    RC(read_ctl(h, &c));
+    if (best_uncovered) *best_uncovered = c.best_unc;
+    return COV_OK;
+}
+
+int cov_recount(cov_cell_t* h)
+{
+    if (!h) return COV_EPARAM;
+    RC(do_recount(h));
+    HCK(h, hipStreamSynchronize(h->stream));
+    return COV_OK;
+}
+
+int cov_ball_update(cov_cell_t* h, const int64_t* centers, int k, int delta)
+{
+    if (!h || (k > 0 && !centers) || (delta != 1 && delta != -1) || k < 0) return COV_EPARAM;
+    if (k == 0) return COV_OK;
+    const Cell& C = h->h;
+    for (int i = 0; i < k; i++)
+        if (centers[i] < 0 || centers[i] >= C.N) { h->err = "centre out of range"; return COV_EPARAM; }
+    if (k == 1) {
+        int64_t w = centers[0];
+        uint64_t pk = 0;
+        for (int j = C.n - 1; j >= 0; j--) { pk |= (uint64_t)(w % C.q) << (4 * j); w /= C.q; }
+        hipLaunchKernelGGL(k_ball_one, dim3(grid_for(C.V, WG, 1, 1024)), dim3(WG), 0, h->stream, h->d,
+                           centers[0], pk, delta);
+        HCK(h, hipGetLastError());
+    } else {
+        int64_t* dc = nullptr;
+        HCK(h, hipMallocAsync((void**)&dc, (size_t)k * 8, h->stream));
+        HCK(h, hipMemcpyAsync(dc, centers, (size_t)k * 8, hipMemcpyHostToDevice, h->stream));
+        hipLaunchKernelGGL(k_ball_multi, dim3(grid_for((int64_t)k * C.V, WG, 1, 4096)), dim3(WG), 0,
+                           h->stream, h->d, dc, k, delta);
+        HCK(h, hipGetLastError());
+        HCK(h, hipFreeAsync(dc, h->stream));
+        RC(do_derive(h));
+        RC(set_ctl<int32_t>(h, offsetof(Ctl, ul_valid), 0));
+    }
+    HCK(h, hipStreamSynchronize(h->stream));
+    return COV_OK;
+}
+
+int64_t cov_count_uncovered(cov_cell_t* h)
+{
+    if (!h) return COV_EPARAM;
+    HCK(h, hipMemsetAsync(h->d_small, 0, 8, h->stream));
+    hipLaunchKernelGGL(k_count_zero, dim3(grid_for(h->h.N / 8, WG, 1, 2048)), dim3(WG), 0, h->stream, h->d,
+                       h->d_small);
+    HCK(h, hipGetLastError());
+    int64_t r = 0;
+    HCK(h, hipMemcpyAsync(&r, h->d_small, 8, hipMemcpyDeviceToHost, h->stream));
+    HCK(h, hipStreamSynchronize(h->stream));
+    return r;
+}
+
+int64_t cov_gather_uncovered(cov_cell_t* h, int64_t* out, int64_t cap)
+{
+    if (!h || cap < 0 || (cap > 0 && !out)) return COV_EPARAM;
+    const Cell& C = h->h;
+    int64_t *dz = nullptr, *dout = nullptr;
+    HCK(h, hipMallocAsync((void**)&dz, (size_t)C.nblk * 8 + 8, h->stream));
+    hipLaunchKernelGGL(k_block_zeros, dim3(grid_for(C.nblk, 1, 1, 8192)), dim3(WG), 0, h->stream, h->d, dz);
+    HCK(h, hipGetLastError());
+    std::vector<int64_t> bz((size_t)C.nblk);
+    HCK(h, hipMemcpyAsync(bz.data(), dz, (size_t)C.nblk * 8, hipMemcpyDeviceToHost, h->stream));
+    HCK(h, hipStreamSynchronize(h->stream));
+    int64_t total = 0;
+    for (auto& v : bz) { const int64_t t = v; v = total; total += t; }
+    const int64_t m = std::min(cap, total);
+    if (m > 0) {
+        HCK(h, hipMemcpyAsync(dz, bz.data(), (size_t)C.nblk * 8, hipMemcpyHostToDevice, h->stream));
+        HCK(h, hipMallocAsync((void**)&dout, (size_t)m * 8, h->stream));
+        hipLaunchKernelGGL(k_block_emit, dim3(grid_for(C.nblk, 1, 1, 8192)), dim3(WG), 0, h->stream, h->d, dz,
+                           dout, m);
+        HCK(h, hipGetLastError());
+        HCK(h, hipMemcpyAsync(out, dout, (size_t)m * 8, hipMemcpyDeviceToHost, h->stream));
+        HCK(h, hipFreeAsync(dout, h->stream));
+    }
+    HCK(h, hipFreeAsync(dz, h->stream));
+    HCK(h, hipStreamSynchronize(h->stream));
+    return total;
+}
+
+int64_t cov_first_uncovered_from(cov_cell_t* h, int64_t start)
+{
+    if (!h || start < 0 || start >= h->h.N) return COV_EPARAM;
+    hipLaunchKernelGGL(k_first_from, dim3(1), dim3(WG), 0, h->stream, h->d, start, h->d_small);
+    HCK(h, hipGetLastError());
+    int64_t r = 0;
+    HCK(h, hipMemcpyAsync(&r, h->d_small, 8, hipMemcpyDeviceToHost, h->stream));
+    HCK(h, hipStreamSynchronize(h->stream));
+    return r;
+}
+
+int cov_read_cnt(cov_cell_t* h, int64_t off, int64_t len, uint16_t* out)
+{
+    if (!h || off < 0 || len < 0 || off + len > h->h.N || (len && !out)) return COV_EPARAM;
+    const uint16_t* src = reinterpret_cast<const uint16_t*>(h->h.cnt32) + off;
+    HCK(h, hipMemcpyAsync(out, src, (size_t)len * 2, hipMemcpyDeviceToHost, h->stream));
+    HCK(h, hipStreamSynchronize(h->stream));
+    return COV_OK;
+}
+
+int cov_eval_moves(cov_cell_t* h, const uint8_t* u, int k, const int32_t* cw, int64_t* dvec)
+{
+    if (!h || !u || k < 0 || (k > 0 && (!cw || !dvec))) return COV_EPARAM;
+    const Cell& C = h->h;
+    if (k > C.M) return COV_ELIMIT;
+    if (k == 0) return COV_OK;
+    uint64_t upk = 0;
+    for (int j = 0; j < C.n; j++) {
+        if (u[j] >= C.q) return COV_EPARAM;
+        upk |= (uint64_t)u[j] << (4 * j);
+    }
+    for (int t = 0; t < k; t++)
+        if (cw[t] < 0 || cw[t] >= C.M) return COV_EPARAM;
+    RC(upload(h, C.slot_cw, cw, (int64_t)k * 4));
+    HCK(h, hipMemcpyAsync(CTL_PTR(h, upk), &upk, 8, hipMemcpyHostToDevice, h->stream));
+    HCK(h, hipMemsetAsync(C.acc, 0, (size_t)k * ACCW * 4, h->stream));
+    const int64_t units = (int64_t)k * (C.F + (int64_t)C.n * C.E);
+    hipLaunchKernelGGL(k_eval_probe, dim3(grid_for(units, 4 * 8, 1, 2048)), dim3(WG), 0, h->stream, h->d, k);
+    HCK(h, hipGetLastError());
+    int64_t* dd = nullptr;
+    HCK(h, hipMallocAsync((void**)&dd, (size_t)k * C.n * 8, h->stream));
+    hipLaunchKernelGGL(k_eval_finish, dim3(grid_for((int64_t)k * C.n, WG, 1, 256)), dim3(WG), 0, h->stream, h->d,
+                       k, dd);
+    HCK(h, hipGetLastError());
+    HCK(h, hipMemcpyAsync(dvec, dd, (size_t)k * C.n * 8, hipMemcpyDeviceToHost, h->stream));
+    HCK(h, hipFreeAsync(dd, h->stream));
+    HCK(h, hipStreamSynchronize(h->stream));
+    return COV_OK;
+}
+
+int cov_commit(cov_cell_t* h, int i, int p, int v, int64_t* dunc)
+{
+    if (!h) return COV_EPARAM;
+    const Cell& C = h->h;
+    if (i < 0 || i >= C.M || p < 0 || p >= C.n || v < 0 || v >= C.q) return COV_EPARAM;
+    hipLaunchKernelGGL(k_set_move, dim3(1), dim3(64), 0, h->stream, h->d, i, p, v);
+    HCK(h, hipGetLastError());
+    hipLaunchKernelGGL(k_commit, dim3(h->commit_groups), dim3(WG), 0, h->stream, h->d, 1);
+    HCK(h, hipGetLastError());
+    Ctl c;
+    RC(read_ctl(h, &c));
+    if (dunc) *dunc = c.dsel;
+    return COV_OK;
+}
+
+int cov_distance_transform(cov_cell_t* h, uint8_t* dist, int64_t* hist, int* radius)
+{
+    if (!h) return COV_EPARAM;
+    Cell& C = h->h;
+    if (!C.dist) {
+        HCK(h, hipMalloc((void**)&C.dist, (size_t)C.N + 64));
+        RC(push_cell(h));
+    }
+    const int gfill = grid_for(C.N / 16, WG, 1, 4096);
+    hipLaunchKernelGGL(k_dist_fill, dim3(gfill), dim3(WG), 0, h->stream, C.dist, C.N);
+    hipLaunchKernelGGL(k_dist_seed, dim3(grid_for(C.M, WG, 1, 64)), dim3(WG), 0, h->stream, h->d, C.dist);
+    HCK(h, hipGetLastError());
+    // inner axes in LDS tiles of q^a words, then the outer axes one by one
+    int a = 0;
+    int64_t T = 1;
+    while (a < C.n && T * C.q <= DT_TILE) { T *= C.q; a++; }
+    hipEvent_t ev = nullptr;
+    if (a > 0) {
+        RC(timed_begin(h, 2, &ev));
+        const int64_t per = DT_TILE / T;
+        hipLaunchKernelGGL(k_axis_inner, dim3(grid_for(C.N, (int)(per * T), 1, 8192)), dim3(WG), 0, h->stream,
+                           C.dist, C.N, C.q, a, T);
+        HCK(h, hipGetLastError());
+        RC(timed_end(h, 2, ev));
+    }
+    for (int j = C.n - 1 - a; j >= 0; j--) {
+        const int64_t s = C.pw[j];
+        RC(timed_begin(h, 2, &ev));
+        hipLaunchKernelGGL(k_axis_outer, dim3(grid_for(C.N / C.q / 16 + 1, WG, 1, 8192)), dim3(WG), 0, h->stream,
+                           C.dist, C.N, C.q, s);
+        HCK(h, hipGetLastError());
+        RC(timed_end(h, 2, ev));
+    }
+    HCK(h, hipMemsetAsync(h->d_small, 0, (MAXN + 2) * 8, h->stream));
+    hipLaunchKernelGGL(k_dist_hist, dim3(grid_for(C.N / 16, WG, 1, 2048)), dim3(WG), 0, h->stream, C.dist, C.N,
+                       C.n, reinterpret_cast<unsigned long long*>(h->d_small));
+    HCK(h, hipGetLastError());
+    int64_t hh[MAXN + 2];
+    HCK(h, hipMemcpyAsync(hh, h->d_small, (MAXN + 2) * 8, hipMemcpyDeviceToHost, h->stream));
+    if (dist) HCK(h, hipMemcpyAsync(dist, C.dist, (size_t)C.N, hipMemcpyDeviceToHost, h->stream));
+    HCK(h, hipStreamSynchronize(h->stream));
+    int rad = 0;
+    for (int d = 0; d <= C.n + 1; d++)
+        if (hh[d]) rad = d;
+    if (hist)
+        for (int d = 0; d <= C.n; d++) hist[d] = hh[d];
+    if (radius) *radius = rad;
+    return COV_OK;
+}
+
+int cov_search_params(cov_cell_t* h, int tenure, int64_t stall, int kick, int cand)
+{
+    if (!h || tenure < 0 || stall < 0 || kick < 0 || cand < 1) return COV_EPARAM;
+    RC(set_ctl<int32_t>(h, offsetof(Ctl, tenure), tenure));
+    RC(set_ctl<int64_t>(h, offsetof(Ctl, stall), stall));
+    RC(set_ctl<int32_t>(h, offsetof(Ctl, kick), kick));
+    RC(set_ctl<int32_t>(h, offsetof(Ctl, cand), cand));
+    return COV_OK;
+}
+
+int cov_search_init(cov_cell_t* h, uint64_t seed, const uint8_t* seed_code, int mseed, int greedy)
+{
+    if (!h || mseed < 0 || (mseed > 0 && !seed_code)) return COV_EPARAM;
+    Cell& C = h->h;
+    const int n = C.n, M = C.M;
+    for (int64_t k = 0; k < (int64_t)mseed * n; k++)
+        if (seed_code[k] >= C.q) { h->err = "digit out of range"; return COV_EPARAM; }
+    // fresh control block, keeping the search knobs
+    Ctl old;
+    RC(read_ctl(h, &old));
+    Ctl c{};
+    c.tenure = old.tenure; c.stall = old.stall; c.kick = old.kick; c.cand = old.cand; c.budget = old.budget;
+    splitmix_seed(seed, c.rst);
+    RC(upload(h, C.ctl, &c, sizeof(Ctl)));
+    HCK(h, hipMemsetAsync(C.tabu, 0, (size_t)M * n * C.q * 4, h->stream));
+    auto index_of = [&](const uint8_t* w) {
+        int64_t x = 0;
+        for (int j = 0; j < n; j++) x += (int64_t)w[j] * C.pw[j];
+        return x;
+    };
+    auto pack_of = [&](const uint8_t* w) {
+        uint64_t pk = 0;
+        for (int j = 0; j < n; j++) pk |= (uint64_t)w[j] << (4 * j);
+        return pk;
+    };
+    if (mseed > M) {
+        // remove-and-repair descent (:1156-1196): counts of all seed words, then
+        // repeatedly drop the first word with the fewest uniquely covered words
+        std::vector<uint8_t> sc(seed_code, seed_code + (size_t)mseed * n);
+        std::vector<int64_t> si((size_t)mseed);
+        for (int i = 0; i < mseed; i++) si[i] = index_of(&sc[(size_t)i * n]);
+        int64_t *dc = nullptr, *du = nullptr;
+        HCK(h, hipMalloc((void**)&dc, (size_t)mseed * 8));
+        HCK(h, hipMalloc((void**)&du, (size_t)mseed * 8));
+        HCK(h, hipMemsetAsync(C.cnt32, 0, (size_t)((C.N + 1) / 2) * 4, h->stream));
+        HCK(h, hipMemcpyAsync(dc, si.data(), (size_t)mseed * 8, hipMemcpyHostToDevice, h->stream));
+        hipLaunchKernelGGL(k_ball_multi, dim3(grid_for((int64_t)mseed * C.V, WG, 1, 4096)), dim3(WG), 0, h->stream,
+                           h->d, dc, mseed, 1);
+        HCK(h, hipGetLastError());
+        RC(do_derive(h));
+        std::vector<int64_t> uc((size_t)mseed);
+        while (mseed > M) {
+            HCK(h, hipMemcpyAsync(dc, si.data(), (size_t)mseed * 8, hipMemcpyHostToDevice, h->stream));
+            hipLaunchKernelGGL(k_unique_cover, dim3(std::min(mseed, 4096)), dim3(WG), 0, h->stream, h->d, dc, mseed,
+                               du);
+            HCK(h, hipGetLastError());
+            HCK(h, hipMemcpyAsync(uc.data(), du, (size_t)mseed * 8, hipMemcpyDeviceToHost, h->stream));
+            HCK(h, hipStreamSynchronize(h->stream));
+            int worst = 0;
+            for (int i = 1; i < mseed; i++)
+                if (uc[i] < uc[worst]) worst = i;
+            hipLaunchKernelGGL(k_ball_one, dim3(grid_for(C.V, WG, 1, 1024)), dim3(WG), 0, h->stream, h->d,
+                               si[worst], pack_of(&sc[(size_t)worst * n]), -1);
+            HCK(h, hipGetLastError());
+            sc.erase(sc.begin() + (size_t)worst * n, sc.begin() + (size_t)(worst + 1) * n);
+            si.erase(si.begin() + worst);
+            mseed--;
+        }
+        HCK(h, hipStreamSynchronize(h->stream));
+        (void)hipFree(dc);
+        (void)hipFree(du);
+        RC(upload(h, C.code, sc.data(), (int64_t)M * n));
+        hipLaunchKernelGGL(k_pack_code, dim3(grid_for(M, WG, 1, 256)), dim3(WG), 0, h->stream, h->d);
+        HCK(h, hipGetLastError());
+        RC(do_recount(h));
+    } else {
+        const int have = mseed;
+        if (have > 0) RC(upload(h, C.code, seed_code, (int64_t)have * n));
+        if (have < M && greedy) {
+            // greedy placement on uncovered words (:1197-1234)
+            HCK(h, hipMemsetAsync(C.cnt32, 0, (size_t)((C.N + 1) / 2) * 4, h->stream));
+            hipLaunchKernelGGL(k_pack_code, dim3(grid_for(M, WG, 1, 256)), dim3(WG), 0, h->stream, h->d);
+            HCK(h, hipGetLastError());
+            RC(do_derive(h));
+            for (int i = 0; i < have; i++) {
+                hipLaunchKernelGGL(k_ball_cw, dim3(grid_for(C.V, WG, 1, 1024)), dim3(WG), 0, h->stream, h->d, i, 1);
+                HCK(h, hipGetLastError());
+            }
+            for (int i = have; i < M; i++) {
+                hipLaunchKernelGGL(k_greedy_pick, dim3(1), dim3(WG), 0, h->stream, h->d, i);
+                hipLaunchKernelGGL(k_ball_cw, dim3(grid_for(C.V, WG, 1, 1024)), dim3(WG), 0, h->stream, h->d, i, 1);
+                HCK(h, hipGetLastError());
+            }
+        } else {
+            hipLaunchKernelGGL(k_init_random, dim3(1), dim3(64), 0, h->stream, h->d, have);
+            hipLaunchKernelGGL(k_pack_code, dim3(grid_for(M, WG, 1, 256)), dim3(WG), 0, h->stream, h->d);
+            HCK(h, hipGetLastError());
+            RC(do_recount(h));
+        }
+    }
+    HCK(h, hipMemcpyAsync(C.best, C.code, (size_t)M * n, hipMemcpyDeviceToDevice, h->stream));
+    Ctl now;
+    RC(read_ctl(h, &now));
+    RC(set_ctl<int64_t>(h, offsetof(Ctl, best_unc), now.unc));
+    if (now.unc <= C.ulcap / 2) RC(list_rebuild(h));
+    else RC(set_ctl<int32_t>(h, offsetof(Ctl, ul_valid), 0));
+    return COV_OK;
+}
+
+int cov_search_launch(cov_cell_t* h, int64_t nsteps, int mode)
+{
+    if (!h || nsteps < 0 || (mode != COV_MODE_PLAIN && mode != COV_MODE_GRAPH)) return COV_EPARAM;
+    if (nsteps == 0) return COV_OK;
+    hipLaunchKernelGGL(k_begin_run, dim3(1), dim3(WG), 0, h->stream, h->d, nsteps);
+    HCK(h, hipGetLastError());
+    int64_t left = nsteps;
+    if (mode == COV_MODE_GRAPH && !h->timing) {
+        RC(ensure_graph(h));
+        while (left >= h->g_steps) {
+            HCK(h, hipGraphLaunch(h->gexec, h->stream));
+            left -= h->g_steps;
+        }
+    }
+    for (; left > 0; left--) RC(launch_step(h));
+    return COV_OK;
+}
+
+int64_t cov_search_run(cov_cell_t* h, int64_t nsteps, int mode, int64_t* trace)
+{
+    if (!h || nsteps < 0) return COV_EPARAM;
+    Cell& C = h->h;
+    if (trace) {
+        if (C.tcap < nsteps) {
+            if (C.trace) (void)hipFree(C.trace);
+            C.trace = nullptr;
+            HCK(h, hipMalloc((void**)&C.trace, (size_t)nsteps * TRW * 8));
+            C.tcap = nsteps;
+            RC(push_cell(h));
+        }
+    } else if (C.tcap) {
+        // keep the buffer but stop writing records
+        Cell tmp = C;
+        tmp.tcap = 0;
+        HCK(h, hipMemcpyAsync(h->d, &tmp, sizeof(Cell), hipMemcpyHostToDevice, h->stream));
+    }
+    RC(cov_search_launch(h, nsteps, mode));
+    HCK(h, hipStreamSynchronize(h->stream));
+    Ctl c;
+    RC(read_ctl(h, &c));
+    if (c.fault == 1) { h->err = "inconsistent counters: no candidate move"; return COV_ESTATE; }
+    if (c.fault == 2) { h->err = "candidate sample larger than the rng ring allows"; return COV_ELIMIT; }
+    const int64_t ran = c.tpos;
+    if (trace && ran > 0) {
+        HCK(h, hipMemcpyAsync(trace, C.trace, (size_t)std::min(ran, nsteps) * TRW * 8, hipMemcpyDeviceToHost,
+                              h->stream));
+        HCK(h, hipStreamSynchronize(h->stream));
+    }
+    if (!trace && C.tcap) RC(push_cell(h));
+    return ran;
+}
+
+int cov_search_state(cov_cell_t* h, cov_state_t* st)
+{
+    if (!h || !st) return COV_EPARAM;
+    Ctl c;
+    RC(read_ctl(h, &c));
+    st->it = c.it;
+    st->uncovered = c.unc;
+    st->best_uncovered = c.best_unc;
+    st->kicks = c.nkicks;
+    st->since_improve = c.since;
+    st->draws = c.ruse;
+    st->list_n = c.ul_n;
+    st->done = c.done;
+    st->list_valid = c.ul_valid;
+    return COV_OK;
+}
+
+int cov_set_launch(cov_cell_t* h, int eval_groups, int commit_groups)
+{
+    if (!h || eval_groups < 1 || commit_groups < 1 || eval_groups > 65535 || commit_groups > 65535)
+        return COV_EPARAM;
+    h->eval_groups = eval_groups;
+    h->commit_groups = commit_groups;
+    return COV_OK;
+}
+
+int cov_timing(cov_cell_t* h, int on)
+{
+    if (!h) return COV_EPARAM;
+    h->timing = on != 0;
+    for (int k = 0; k < 4; k++) { h->ms[k] = 0; h->launches[k] = 0; }
+    return COV_OK;
+}
+
+int cov_timing_read(cov_cell_t* h, double* ms, int64_t* launches, int nkinds)
+{
+    if (!h || nkinds < 0 || nkinds > 4) return COV_EPARAM;
+    for (int k = 0; k < nkinds; k++) {
+        if (ms) ms[k] = h->ms[k];
+        if (launches) launches[k] = h->launches[k];
+    }
+    return COV_OK;
+}
+
+int cov_search_bytes(cov_cell_t* h, int64_t* eval_bytes, int64_t* commit_bytes)
+{
+    if (!h) return COV_EPARAM;
+    Ctl c;
+    RC(read_ctl(h, &c));
+    if (eval_bytes) *eval_bytes = c.eval_bytes;
+    if (commit_bytes) *commit_bytes = c.commit_bytes;
+    return COV_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,266 @@
+"""ctypes binding of libcovhip.so and the `Cell` handle.
+
+Follows the reference's ctypes convention (cov/gemm2/dctlib.py:19-39,
+cov/gemm2/chain.py:11-57): plain pointers and sizes, int status codes, the
+library owning device memory.  Unlike dct.cu's process-global state
+(cov/gemm2/dct.cu:35-49) every cell is its own handle with its own HIP stream,
+so several cells can be resident and running at once.
+
+There is no CPU fallback: if the library is missing or cannot load, every
+entry point raises.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB = os.path.join(HERE, 'lib', 'libcovhip.so')
+
+COV_OK = 0
+COV_TR_W = 16
+COV_DSEL_RANDOM = 1 << 60
+MODE_PLAIN, MODE_GRAPH = 0, 1
+TR_FIELDS = ['it', 'u', 'ncand', 'ncw', 'bi', 'bp', 'bv', 'oldv', 'dsel', 'dunc',
+             'unc', 'best', 'kick', 'draws', 'since', 'nties']
+
+# the exported symbols, as declared in include/covhip.h
+EXPORTS = ['cov_version', 'cov_strerror', 'cov_mem_bytes', 'cov_create', 'cov_destroy',
+           'cov_last_error', 'cov_sync', 'cov_set_code', 'cov_get_code', 'cov_get_best',
+           'cov_recount', 'cov_ball_update', 'cov_count_uncovered', 'cov_gather_uncovered',
+           'cov_first_uncovered_from', 'cov_read_cnt', 'cov_eval_moves', 'cov_commit',
+           'cov_distance_transform', 'cov_search_init', 'cov_search_params',
+           'cov_search_run', 'cov_search_launch', 'cov_search_state', 'cov_set_launch',
+           'cov_timing', 'cov_timing_read', 'cov_search_bytes']
+
+
+class CovError(RuntimeError):
+    pass
+
+
+class State(ctypes.Structure):
+    _fields_ = [('it', ctypes.c_int64), ('uncovered', ctypes.c_int64),
+                ('best_uncovered', ctypes.c_int64), ('kicks', ctypes.c_int64),
+                ('since_improve', ctypes.c_int64), ('draws', ctypes.c_int64),
+                ('list_n', ctypes.c_int64), ('done', ctypes.c_int32),
+                ('list_valid', ctypes.c_int32)]
+
+
+_lib = None
+
+
+def lib_path():
+    return LIB
+
+
+def lib():
+    """Load libcovhip.so once; raise if it is not built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB):
+        raise CovError('libcovhip.so is not built (run __graft_entry__.build())')
+    L = ctypes.CDLL(LIB)
+    vp, i32, i64, u64 = ctypes.c_void_p, ctypes.c_int, ctypes.c_int64, ctypes.c_uint64
+    sig = {
+        'cov_version': (i32, []),
+        'cov_strerror': (ctypes.c_char_p, [i32]),
+        'cov_mem_bytes': (i64, [i32, i32, i32, i32]),
+        'cov_create': (i32, [i32, i32, i32, i32, ctypes.POINTER(vp)]),
+        'cov_destroy': (i32, [vp]),
+        'cov_last_error': (ctypes.c_char_p, [vp]),
+        'cov_sync': (i32, [vp]),
+        'cov_set_code': (i32, [vp, vp]),
+        'cov_get_code': (i32, [vp, vp]),
+        'cov_get_best': (i32, [vp, vp, vp]),
+        'cov_recount': (i32, [vp]),
+        'cov_ball_update': (i32, [vp, vp, i32, i32]),
+        'cov_count_uncovered': (i64, [vp]),
+        'cov_gather_uncovered': (i64, [vp, vp, i64]),
+        'cov_first_uncovered_from': (i64, [vp, i64]),
+        'cov_read_cnt': (i32, [vp, i64, i64, vp]),
+        'cov_eval_moves': (i32, [vp, vp, i32, vp, vp]),
+        'cov_commit': (i32, [vp, i32, i32, i32, vp]),
+        'cov_distance_transform': (i32, [vp, vp, vp, vp]),
+        'cov_search_init': (i32, [vp, u64, vp, i32, i32]),
+        'cov_search_params': (i32, [vp, i32, i64, i32, i32]),
+        'cov_search_run': (i64, [vp, i64, i32, vp]),
+        'cov_search_launch': (i32, [vp, i64, i32]),
+        'cov_search_state': (i32, [vp, ctypes.POINTER(State)]),
+        'cov_set_launch': (i32, [vp, i32, i32]),
+        'cov_timing': (i32, [vp, i32]),
+        'cov_timing_read': (i32, [vp, vp, vp, i32]),
+        'cov_search_bytes': (i32, [vp, vp, vp]),
+    }
+    for name, (res, args) in sig.items():
+        f = getattr(L, name)
+        f.restype = res
+        f.argtypes = args
+    _lib = L
+    return L
+
+
+def _p(a):
+    return a.ctypes.data_as(ctypes.c_void_p) if a is not None else None
+
+
+def mem_bytes(q, n, R, M):
+    """Device bytes a cell of this shape needs (the serving loop's admission)."""
+    r = lib().cov_mem_bytes(q, n, R, M)
+    if r < 0:
+        raise CovError('cov_mem_bytes: %s' % lib().cov_strerror(int(r)).decode())
+    return r
+
+
+class Cell:
+    """One (q, n, R, M) cell resident on the GPU: coverage counts, the code,
+    and the device-resident tabu search."""
+
+    def __init__(self, q, n, R, M):
+        self.q, self.n, self.R, self.M = q, n, R, M
+        self.N = q ** n
+        h = ctypes.c_void_p()
+        rc = lib().cov_create(q, n, R, M, ctypes.byref(h))
+        if rc != COV_OK:
+            raise CovError('cov_create(q=%d n=%d R=%d M=%d): %s'
+                           % (q, n, R, M, lib().cov_strerror(rc).decode()))
+        self.h = h
+        self.pw = np.array([q ** (n - 1 - p) for p in range(n)], dtype=np.int64)
+
+    # ------------------------------------------------------------ plumbing
+    def _check(self, rc, what):
+        if rc < 0:
+            msg = lib().cov_last_error(self.h)
+            raise CovError('%s: %s (%s)' % (what, lib().cov_strerror(int(rc)).decode(),
+                                             msg.decode() if msg else ''))
+        return rc
+
+    def close(self):
+        if getattr(self, 'h', None):
+            lib().cov_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def sync(self):
+        self._check(lib().cov_sync(self.h), 'cov_sync')
+
+    # ------------------------------------------------------- code, coverage
+    def word_index(self, words):
+        return np.asarray(words, dtype=np.int64).reshape(-1, self.n) @ self.pw
+
+    def set_code(self, code):
+        c = np.ascontiguousarray(code, dtype=np.uint8).reshape(self.M, self.n)
+        self._check(lib().cov_set_code(self.h, _p(c)), 'cov_set_code')
+
+    def code(self):
+        c = np.zeros((self.M, self.n), np.uint8)
+        self._check(lib().cov_get_code(self.h, _p(c)), 'cov_get_code')
+        return c
+
+    def best(self):
+        c = np.zeros((self.M, self.n), np.uint8)
+        b = np.zeros(1, np.int64)
+        self._check(lib().cov_get_best(self.h, _p(c), _p(b)), 'cov_get_best')
+        return c, int(b[0])
+
+    def recount(self):
+        self._check(lib().cov_recount(self.h), 'cov_recount')
+
+    def ball_update(self, centers, delta):
+        c = np.ascontiguousarray(centers, dtype=np.int64).ravel()
+        self._check(lib().cov_ball_update(self.h, _p(c), len(c), int(delta)), 'cov_ball_update')
+
+    def uncovered(self):
+        return self._check(lib().cov_count_uncovered(self.h), 'cov_count_uncovered')
+
+    def gather_uncovered(self, cap):
+        out = np.zeros(max(int(cap), 1), np.int64)
+        tot = self._check(lib().cov_gather_uncovered(self.h, _p(out), int(cap)), 'cov_gather_uncovered')
+        return out[:min(int(cap), tot)], tot
+
+    def first_uncovered_from(self, start):
+        return lib().cov_first_uncovered_from(self.h, int(start))
+
+    def cnt(self, off=0, length=None):
+        length = self.N - off if length is None else length
+        out = np.zeros(length, np.uint16)
+        self._check(lib().cov_read_cnt(self.h, int(off), int(length), _p(out)), 'cov_read_cnt')
+        return out
+
+    # ------------------------------------------------------------- evaluator
+    def eval_moves(self, u, codewords):
+        u = np.ascontiguousarray(u, dtype=np.uint8)
+        cw = np.ascontiguousarray(codewords, dtype=np.int32).ravel()
+        d = np.zeros((len(cw), self.n), np.int64)
+        self._check(lib().cov_eval_moves(self.h, _p(u), len(cw), _p(cw), _p(d)), 'cov_eval_moves')
+        return d
+
+    def commit(self, i, p, v):
+        d = np.zeros(1, np.int64)
+        self._check(lib().cov_commit(self.h, int(i), int(p), int(v), _p(d)), 'cov_commit')
+        return int(d[0])
+
+    # -------------------------------------------------- distance transform
+    def distance_transform(self, want_array=False):
+        dist = np.zeros(self.N, np.uint8) if want_array else None
+        hist = np.zeros(self.n + 1, np.int64)
+        rad = ctypes.c_int(0)
+        self._check(lib().cov_distance_transform(self.h, _p(dist), _p(hist), ctypes.byref(rad)),
+                    'cov_distance_transform')
+        return dist, hist, rad.value
+
+    # ---------------------------------------------------------------- search
+    def search_init(self, seed, seed_code=None, greedy=False):
+        if seed_code is None:
+            sc, m = None, 0
+        else:
+            sc = np.ascontiguousarray(seed_code, dtype=np.uint8).reshape(-1, self.n)
+            m = sc.shape[0]
+        self._check(lib().cov_search_init(self.h, ctypes.c_uint64(int(seed)), _p(sc), m, int(bool(greedy))),
+                    'cov_search_init')
+
+    def search_params(self, tenure=0, stall=5000, kick=2, cand=100000):
+        self._check(lib().cov_search_params(self.h, int(tenure), int(stall), int(kick), int(cand)),
+                    'cov_search_params')
+
+    def run(self, nsteps, mode=MODE_PLAIN, trace=False):
+        tr = np.zeros((max(int(nsteps), 1), COV_TR_W), np.int64) if trace else None
+        ran = self._check(lib().cov_search_run(self.h, int(nsteps), int(mode), _p(tr)), 'cov_search_run')
+        return tr[:ran] if trace else ran
+
+    def launch(self, nsteps, mode=MODE_GRAPH):
+        self._check(lib().cov_search_launch(self.h, int(nsteps), int(mode)), 'cov_search_launch')
+
+    def state(self):
+        s = State()
+        self._check(lib().cov_search_state(self.h, ctypes.byref(s)), 'cov_search_state')
+        return {k: getattr(s, k) for k, _ in State._fields_}
+
+    def set_launch(self, eval_groups, commit_groups):
+        self._check(lib().cov_set_launch(self.h, int(eval_groups), int(commit_groups)), 'cov_set_launch')
+
+    def timing(self, on):
+        self._check(lib().cov_timing(self.h, int(bool(on))), 'cov_timing')
+
+    def timing_read(self):
+        ms = np.zeros(4, np.float64)
+        n = np.zeros(4, np.int64)
+        self._check(lib().cov_timing_read(self.h, _p(ms), _p(n), 4), 'cov_timing_read')
+        return ms, n
+
+    def search_bytes(self):
+        e = np.zeros(1, np.int64)
+        c = np.zeros(1, np.int64)
+        self._check(lib().cov_search_bytes(self.h, _p(e), _p(c)), 'cov_search_bytes')
+        return int(e[0]), int(c[0])

@@ -1,0 +1,148 @@
+/*
+ * covhip.h -- C ABI of libcovhip.so, the MI355X (gfx950) engine for the
+ * covering-code search hot path: coverage counts over Z_q^n, the shared-sphere
+ * tabu move evaluator, the uint8 axis-DP distance transform and the
+ * uncovered-set reductions, plus the device-resident search loop built on them.
+ *
+ * Conventions (those of the reference's ctypes engines, cov/gemm2/dct.cu:299-504
+ * and cov/gemm2/chainsolve.cu:370-433):
+ *   - every call returns int (or int64 for counts); 0 / a count = success,
+ *     negative = error: COV_E* below, or -(100 + hipError_t) from the runtime;
+ *   - host buffers are owned by the caller and passed as plain pointers;
+ *     device buffers are owned by the cell handle;
+ *   - a cell owns one HIP stream; calls on one cell are not thread-safe, calls
+ *     on distinct cells may run concurrently (the serving loop relies on it);
+ *   - words are indexed MSB-first in base q, w = sum_j d_j q^(n-1-j), as in
+ *     cov/engine/covsearch2e.c:916-921 and cov/verify_cov.py:114;
+ *   - no exception crosses the ABI; cov_last_error() gives the text.
+ * Limits of this build: 2 <= q <= 16, 1 <= n <= 16, 0 <= R <= 8, R < n,
+ * 1 <= M <= 65535 (uint16 counters, as cov/engine/covsearch2e.c:1117),
+ * q^n <= 4e10 (as :1111-1114).
+ */
+#ifndef COVHIP_H
+#define COVHIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define COV_OK       0
+#define COV_EPARAM  -1   /* bad parameters (as dct_init's -1)               */
+#define COV_ESPACE  -2   /* q^n too large (as dct_init's -2)                */
+#define COV_ENOMEM  -3   /* device allocation failed                        */
+#define COV_ESTATE  -4   /* call not valid in the cell's current state      */
+#define COV_ELIMIT  -5   /* input beyond a stated capacity                  */
+
+/* per-step trace record: COV_TR_W int64 per step (same layout as the oracle) */
+#define COV_TR_W 16
+#define COV_DSEL_RANDOM (1LL << 60)
+
+/* search-loop modes of cov_search_run */
+#define COV_MODE_PLAIN 0   /* one launch sequence per step                 */
+#define COV_MODE_GRAPH 1   /* steps replayed from a captured hipGraph      */
+
+typedef struct cov_cell cov_cell_t;
+
+typedef struct {
+    int64_t it;               /* iterations done                              */
+    int64_t uncovered;        /* current uncovered count                      */
+    int64_t best_uncovered;   /* best seen (the RESULT line's uncovered=)     */
+    int64_t kicks;
+    int64_t since_improve;
+    int64_t draws;            /* rng draws consumed so far                    */
+    int64_t list_n;           /* uncovered-list length (incl. stale entries)  */
+    int32_t done;             /* 1 once the search stopped (covered)          */
+    int32_t list_valid;
+} cov_state_t;
+
+/* ------------------------------------------------------------- lifecycle */
+int         cov_version(void);
+const char *cov_strerror(int code);
+/* Device bytes a cell of this shape needs; the serving loop admits by it. */
+int64_t     cov_mem_bytes(int q, int n, int R, int M);
+/* Replaces dct_init (cov/gemm2/dct.cu:302) with a handle instead of globals. */
+int         cov_create(int q, int n, int R, int M, cov_cell_t **out);
+/* Replaces dct_free (cov/gemm2/dct.cu:338). */
+int         cov_destroy(cov_cell_t *h);
+const char *cov_last_error(const cov_cell_t *h);
+int         cov_sync(cov_cell_t *h);
+
+/* ------------------------------------------------------ code and coverage */
+/* Upload M*n digits (row-major) and rebuild cnt by the recount kernel.
+ * Replaces dct_set_code + dct_transform(0,2) (cov/gemm2/dct.cu:351-411) and
+ * rebuild() (cov/engine/covsearch2e.c:958-967). */
+int     cov_set_code(cov_cell_t *h, const uint8_t *digits);
+int     cov_get_code(cov_cell_t *h, uint8_t *digits);
+int     cov_get_best(cov_cell_t *h, uint8_t *digits, int64_t *best_uncovered);
+/* (b) recount: cnt from the current code, LDS-aggregated per word tile. */
+int     cov_recount(cov_cell_t *h);
+/* (b) +-1 over the radius-R balls of k words given by index.
+ * Replaces dct_ball_update (cov/gemm2/dct.cu:413) and ball_apply
+ * (cov/engine/covsearch2e.c:370-380). */
+int     cov_ball_update(cov_cell_t *h, const int64_t *centers, int k, int delta);
+/* (d) #{w : cnt[w] == 0}.  Replaces dct_count_eq(0) (cov/gemm2/dct.cu:440). */
+int64_t cov_count_uncovered(cov_cell_t *h);
+/* (d) the first min(cap, total) uncovered word indices in ascending order;
+ * returns the total.  Replaces gather_uncovered_kernel (cov/search/covsearch.cu:213)
+ * with a deterministic order, and verify_pure's witnesses (cov/verify_cov.py:142-151). */
+int64_t cov_gather_uncovered(cov_cell_t *h, int64_t *out, int64_t cap);
+/* (d) first uncovered word at or after start, cyclically; -1 if none.
+ * The scan of pick_uncovered (cov/engine/covsearch2e.c:1003-1015). */
+int64_t cov_first_uncovered_from(cov_cell_t *h, int64_t start);
+/* Replaces dct_read_cnt (cov/gemm2/dct.cu:498). */
+int     cov_read_cnt(cov_cell_t *h, int64_t off, int64_t len, uint16_t *out);
+
+/* ----------------------------------------------------------- evaluator */
+/* (a) Shared-sphere evaluation: for each of the k codewords cw[t], the delta in
+ * the uncovered count of moving c_p to u_p, for every p with c_p != u_p, into
+ * dvec[t*n + p]; entries with c_p == u_p are set to COV_DSEL_RANDOM.
+ * Replaces eval_codeword (cov/engine/covsearch2e.c:696-760). */
+int     cov_eval_moves(cov_cell_t *h, const uint8_t *u, int k, const int32_t *cw,
+                       int64_t *dvec);
+/* Commit the move c_i[p] := v; *dunc receives the change in uncovered.
+ * Replaces move_commit (cov/engine/covsearch2e.c:562-590). */
+int     cov_commit(cov_cell_t *h, int i, int p, int v, int64_t *dunc);
+
+/* --------------------------------------------------- distance transform */
+/* (c) dist[w] = min Hamming distance from w to the code (uint8), by n axis
+ * passes of D <- min(D, 1 + min over the fiber).  dist may be NULL (then it
+ * stays on the device); hist[d] for d = 0..n (n+1 entries) and the covering
+ * radius are returned.  Replaces run_mindist / run_dilate
+ * (cov/verify_independent.py:128-185) and covering_radius (cov/verify_cov.py:237). */
+int     cov_distance_transform(cov_cell_t *h, uint8_t *dist, int64_t *hist, int *radius);
+
+/* --------------------------------------------------------------- search */
+/* Seed the rng (splitmix64 -> xoshiro256++, cov/engine/covsearch2e.c:211-219)
+ * and build the initial code: seed_code (mseed words, may be NULL/0), then
+ * remove-and-repair if mseed > M, else greedy (greedy=1) or uniform random
+ * digits for the rest (:1156-1239). */
+int     cov_search_init(cov_cell_t *h, uint64_t seed, const uint8_t *seed_code,
+                        int mseed, int greedy);
+/* Search knobs: --tabu, --stall, --kick, --cand (:1030-1066). */
+int     cov_search_params(cov_cell_t *h, int tenure, int64_t stall, int kick, int cand);
+/* Run up to nsteps more tabu steps (:1304-1580) on the device.  trace, if not
+ * NULL, receives COV_TR_W int64 per step actually run.  Returns the number of
+ * steps run (fewer when the code covers). */
+int64_t cov_search_run(cov_cell_t *h, int64_t nsteps, int mode, int64_t *trace);
+/* Non-blocking form: enqueue the steps and return; cov_sync() waits. */
+int     cov_search_launch(cov_cell_t *h, int64_t nsteps, int mode);
+int     cov_search_state(cov_cell_t *h, cov_state_t *st);
+
+/* -------------------------------------------------- launch shape, timing */
+/* Workgroups of the evaluator and commit kernels (the autotuner sets them). */
+int     cov_set_launch(cov_cell_t *h, int eval_groups, int commit_groups);
+/* Accumulate per-kernel device time with HIP events on the cell's stream
+ * (on = 1), read totals in ms and launch counts for kernel kinds
+ * 0 = evaluator, 1 = commit, 2 = distance transform pass, 3 = recount. */
+int     cov_timing(cov_cell_t *h, int on);
+int     cov_timing_read(cov_cell_t *h, double *ms, int64_t *launches, int nkinds);
+/* Algorithmic bytes moved by the evaluator and commit kernels since the last
+ * cov_search_init (2 B per counter read or update, as the uint16 cnt). */
+int     cov_search_bytes(cov_cell_t *h, int64_t *eval_bytes, int64_t *commit_bytes);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -47,6 +47,7 @@ struct cso {
     int64_t best_unc;
     int32_t *tabu;
     uint64_t rs[4];
+    int64_t ndraw;       /* rng draws consumed */
     int64_t it, since, nkicks;
     int kick_order;
     int threads;
@@ -99,7 +100,8 @@ static void rng_seed(uint64_t *s, uint64_t seed)
     }
 }
 
-static inline uint64_t rng_mod(cso_t *s, uint64_t m) { return rng_next(s->rs) % m; }
+static inline uint64_t draw(cso_t *s) { s->ndraw++; return rng_next(s->rs); }
+static inline uint64_t rng_mod(cso_t *s, uint64_t m) { return draw(s) % m; }
 
 /* ------------------------------------------------------------ helpers */
 static int64_t binom64(int a, int b)
@@ -369,10 +371,10 @@ int64_t cso_pick(cso_t *s)
 {
     const uint64_t N = (uint64_t)s->N;
     for (int t = 0; t < 64; t++) {
-        int64_t w = (int64_t)(rng_next(s->rs) % N);
+        int64_t w = (int64_t)(draw(s) % N);
         if (s->cnt[w] == 0) return w;
     }
-    const int64_t start = (int64_t)(rng_next(s->rs) % N);
+    const int64_t start = (int64_t)(draw(s) % N);
     /* first word with cnt == 0 in cyclic order from start; a short direct scan,
      * then the exact set (same answer, less work when uncovered words are rare) */
     int64_t lim = s->N < 4096 ? s->N : 4096;
@@ -727,7 +729,7 @@ int64_t cso_run(cso_t *s, int64_t iters, int tabu_len, int64_t stall,
             tr[CSO_TR_UNC] = s->uncovered;
             tr[CSO_TR_BEST] = s->best_unc;
             tr[CSO_TR_KICK] = kicked;
-            tr[CSO_TR_RNG0] = (int64_t)s->rs[0];
+            tr[CSO_TR_DRAWS] = s->ndraw;
             tr[CSO_TR_SINCE] = s->since;
             tr[CSO_TR_NTIES] = nties;
         }
@@ -749,6 +751,7 @@ const uint8_t *cso_best(const cso_t *s) { return s->best; }
 const uint16_t *cso_cnt(const cso_t *s) { return s->cnt; }
 void cso_rng_state(const cso_t *s, uint64_t out[4]) { memcpy(out, s->rs, sizeof s->rs); }
 void cso_set_rng_state(cso_t *s, const uint64_t in[4]) { memcpy(s->rs, in, sizeof s->rs); }
+int64_t cso_draws(const cso_t *s) { return s->ndraw; }
 
 typedef struct { int64_t x; int pos; } xp_t;
 

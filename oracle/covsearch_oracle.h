@@ -35,7 +35,7 @@ enum {
     CSO_TR_UNC,       /* uncovered after the step (after a kick if any)       */
     CSO_TR_BEST,      /* best uncovered after the step                        */
     CSO_TR_KICK,      /* 1 if the step ended with a kick                      */
-    CSO_TR_RNG0,      /* rng word s[0] after the step                         */
+    CSO_TR_DRAWS,     /* rng draws consumed so far, after the step            */
     CSO_TR_SINCE,     /* steps since the last improvement, after the step     */
     CSO_TR_NTIES,     /* candidates tied at the chosen delta                  */
     CSO_TR_W          /* record width                                         */
@@ -72,6 +72,7 @@ const uint8_t  *cso_best(const cso_t *s);
 const uint16_t *cso_cnt(const cso_t *s);
 void    cso_rng_state(const cso_t *s, uint64_t out[4]);
 void    cso_set_rng_state(cso_t *s, const uint64_t in[4]);
+int64_t cso_draws(const cso_t *s);
 
 /* Overwrite the current code and rebuild the counters (no rng draws). */
 int     cso_set_code(cso_t *s, const uint8_t *code);

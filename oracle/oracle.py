@@ -15,7 +15,7 @@ CLI_PATH = os.path.join(HERE, 'cso')
 REF_PATH = os.path.join(HERE, '_ref', 'covsearch2e')
 
 TR_FIELDS = ['it', 'u', 'ncand', 'ncw', 'bi', 'bp', 'bv', 'oldv', 'dsel', 'dunc',
-             'unc', 'best', 'kick', 'rng0', 'since', 'nties']
+             'unc', 'best', 'kick', 'draws', 'since', 'nties']
 TR_W = len(TR_FIELDS)
 DSEL_RANDOM = 1 << 60
 
@@ -46,6 +46,8 @@ def lib():
         L.cso_rng_state.argtypes = [vp, vp]
         L.cso_set_rng_state.argtypes = [vp, vp]
         L.cso_set_code.argtypes = [vp, vp]
+        L.cso_draws.restype = i64
+        L.cso_draws.argtypes = [vp]
         L.cso_move_delta.restype = i64
         L.cso_move_delta.argtypes = [vp, i32, i32, i32]
         L.cso_eval_codeword.argtypes = [vp, i32, vp, vp, i32]
@@ -117,6 +119,10 @@ class Oracle:
     @property
     def iters(self):
         return lib().cso_iters(self.h)
+
+    @property
+    def draws(self):
+        return lib().cso_draws(self.h)
 
     @property
     def kicks(self):

@@ -1,0 +1,75 @@
+"""GPU search vs the oracle, step by step, through the C ABI.
+
+Every step's record (picked word, candidate count, chosen move, evaluated and
+committed deltas, uncovered/best counts, kicks, rng draws consumed) must be
+identical, and so must the final counters and best code.
+"""
+import numpy as np
+import pytest
+
+from oracle.oracle import TR_FIELDS, Oracle
+
+pytestmark = pytest.mark.gpu
+
+
+def _cell(q, n, R, M):
+    from coldcase_amd import Cell
+    return Cell(q, n, R, M)
+
+
+def compare(q, n, R, M, seed, steps, tabu=10, stall=5000, cand=100000, greedy=False,
+            seed_code=None, mode=0, check_cnt=True, threads=1):
+    print('case K%d(%d,%d) M=%d seed=%d steps=%d stall=%d mode=%d' % (q, n, R, M, seed, steps, stall, mode),
+          flush=True)
+    cell = _cell(q, n, R, M)
+    cell.search_params(tenure=tabu, stall=stall, kick=2, cand=cand)
+    cell.search_init(seed, seed_code=seed_code, greedy=greedy)
+    o = Oracle(q, n, R, M, seed, threads=threads).init(seed_code, greedy=greedy)
+    st0 = cell.state()
+    assert st0['uncovered'] == o.uncovered, (st0, o.uncovered)
+    assert st0['draws'] == o.draws
+    assert (cell.code() == o.code()).all()
+    gpu = cell.run(steps, mode=mode, trace=True)
+    ref = o.run(steps, tabu=tabu, stall=stall, cand=cand, trace=True)
+    assert gpu.shape == ref.shape
+    diff = np.argwhere(gpu != ref)
+    if diff.size:
+        s, f = diff[0]
+        raise AssertionError('step %d field %s: gpu %d oracle %d' % (s, TR_FIELDS[f], gpu[s, f], ref[s, f]))
+    st = cell.state()
+    assert (st['it'], st['best_uncovered'], st['kicks'], st['uncovered']) == \
+        (o.iters, o.best_uncovered, o.kicks, o.uncovered)
+    best, bu = cell.best()
+    assert (best == o.best()).all()
+    assert (cell.code() == o.code()).all()
+    if check_cnt:
+        assert (cell.cnt() == o.cnt()).all()
+        assert cell.uncovered() == o.uncovered
+    cell.close()
+    return gpu
+
+
+def test_k5_7_2_random_tabu():
+    compare(5, 7, 2, 400, 1, 2000)
+
+
+def test_k6_6_3_kicks():
+    tr = compare(6, 6, 3, 41, 2, 600, tabu=3, stall=20)
+    assert tr[:, TR_FIELDS.index('kick')].sum() > 5
+
+
+def test_small_cells_greedy():
+    for (q, n, R, M, it) in [(3, 4, 1, 9, 200), (2, 7, 2, 8, 300), (7, 3, 1, 20, 300), (5, 4, 3, 3, 50)]:
+        compare(q, n, R, M, 7, it, tabu=0, greedy=True)
+
+
+def test_cand_sample():
+    compare(6, 6, 3, 41, 5, 300, cand=3)
+
+
+def test_graph_mode_identical():
+    compare(5, 7, 2, 400, 3, 1000, mode=1)
+
+
+def test_k6_9_4_random_tabu():
+    compare(6, 9, 4, 660, 1, 200, threads=8)
