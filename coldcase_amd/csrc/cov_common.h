@@ -40,7 +40,7 @@ constexpr uint64_t NIB1 = 0x1111111111111111ULL;
 
 // Device-resident search state.  Scalars written by the narrow phases (one
 // workgroup) and read by the wide kernels of the same step.
-struct Ctl {
+struct alignas(16) Ctl {
     uint64_t rst[4];        // generator state after rgen draws
     int64_t rgen;           // draws generated into the ring
     int64_t ruse;           // draws consumed
@@ -62,12 +62,47 @@ struct Ctl {
     int32_t dtar, ncand, take, nq, use_list;
     int32_t bi, bp, bv, oldv, nties, stamp, ncw;
     int32_t evdone, cmdone;
+    // wall_clock64 ticks spent per phase (diagnostics): 0 select, 1 post,
+    // 2 begin, 3 pick, 4 list upkeep, 5 shuffle helper, 6 rng helper, 7 kick,
+    // 8 evaluator span before select, 9 commit span before post, 10 gap commit
+    // end -> next evaluator start, 11 gap evaluator end -> commit start
+    int64_t tph[12];
+    int64_t t_eve, t_cme;
+    int64_t* trace;         // per-run trace records (tcap of them), or null
+    int64_t tcap;
 };
+static_assert(sizeof(Ctl) % 16 == 0, "Ctl is copied in 8-byte words");
+
+// The single-workgroup phases work on a copy of the control block in LDS:
+// one parallel load at entry and one store at exit instead of a dependent
+// global round trip per field.  All threads call these.
+__device__ inline void ctl_to_lds(Ctl* lds, const Ctl* g)
+{
+    const int64_t* s = reinterpret_cast<const int64_t*>(g);
+    int64_t* d = reinterpret_cast<int64_t*>(lds);
+    for (int x = threadIdx.x; x < (int)(sizeof(Ctl) / 8); x += WG) d[x] = s[x];
+    __syncthreads();
+}
+__device__ inline void ctl_from_lds(Ctl* g, const Ctl* lds)
+{
+    __syncthreads();
+    const int64_t* s = reinterpret_cast<const int64_t*>(lds);
+    int64_t* d = reinterpret_cast<int64_t*>(g);
+    for (int x = threadIdx.x; x < (int)(sizeof(Ctl) / 8); x += WG) d[x] = s[x];
+    __syncthreads();
+}
+
+__device__ __forceinline__ uint64_t uniform64(uint64_t v)
+{
+    const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)v);
+    const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(v >> 32));
+    return ((uint64_t)hi << 32) | lo;
+}
 
 struct Cell {
     int q, n, R, M, q1, F, E;
     int64_t P;              // (q-1)^R patterns per position subset
-    int64_t N, nzw, nblk, V, ncap, ulcap, tcap;
+    int64_t N, nzw, nblk, V, ncap, ulcap;
     int64_t pw[MAXN];
     uint32_t magic;         // ceil(2^32 / (q-1)): t / (q-1) = umulhi(t, magic)
     uint32_t* cnt32;        // cnt as uint32 pairs
@@ -92,7 +127,6 @@ struct Cell {
     int32_t* slot_cw;       // slot -> codeword
     uint8_t* slot_pos;      // slot -> differing positions (MAXN each)
     uint64_t* rbuf;
-    int64_t* trace;         // tcap records
     int64_t* scratch64;     // small reduction outputs
     uint8_t* dist;          // distance transform output (N bytes), lazily
     Ctl* ctl;
@@ -139,10 +173,20 @@ __device__ __forceinline__ uint64_t nib_put(uint64_t pk, int j, int v)
     return (pk & ~(15ULL << (4 * j))) | ((uint64_t)v << (4 * j));
 }
 
-// digits of a word index, packed
+// digits of a word index, packed (32-bit arithmetic when q^n fits)
 __device__ inline uint64_t pack_index(const Cell& C, int64_t w)
 {
     uint64_t pk = 0;
+    if (C.N <= 0xFFFFFFFFLL) {
+        uint32_t x = (uint32_t)w;
+        const uint32_t q = (uint32_t)C.q;
+        for (int j = C.n - 1; j >= 0; j--) {
+            const uint32_t y = x / q;
+            pk |= (uint64_t)(x - y * q) << (4 * j);
+            x = y;
+        }
+        return pk;
+    }
     for (int j = C.n - 1; j >= 0; j--) {
         pk |= (uint64_t)(w % C.q) << (4 * j);
         w /= C.q;
@@ -150,9 +194,16 @@ __device__ inline uint64_t pack_index(const Cell& C, int64_t w)
     return pk;
 }
 
+// relaxed agent-scope load: sees device atomics of this kernel without
+// relying on the vector L1 (used by the single-workgroup phases)
+__device__ __forceinline__ uint32_t ld_acq32(const uint32_t* p)
+{
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
 __device__ __forceinline__ uint32_t div_q1(const Cell& C, uint32_t t)
 {
-    return __umulhi(t, C.magic);
+    return C.q1 == 1 ? t : __umulhi(t, C.magic);
 }
 
 // Word of pattern t (0 <= t < (q-1)^r) on the position subset `sub` (r
@@ -196,24 +247,41 @@ __device__ __forceinline__ uint64_t xo_next(uint64_t s[4])
     return out;
 }
 
-// Generate draws until rgen >= upto (never overwriting a draw still to be
-// read).  One thread only.  Returns the number generated.
-__device__ inline int64_t ring_fill(const Cell& C, Ctl* ctl, int64_t upto, int64_t budget)
+// Generate draws until rgen >= upto, at most `budget`, never overwriting a draw
+// still to be read (positions below rlow + RB).  Called by all 64 lanes of one
+// wave: the xoshiro state is wave-uniform (scalar unit), lane k keeps the k-th
+// draw of each group of 64 and the group is stored at once.
+__device__ inline int64_t wave_ring_fill(const Cell& C, Ctl* ctl, int64_t upto, int64_t budget)
 {
-    uint64_t s[4] = {ctl->rst[0], ctl->rst[1], ctl->rst[2], ctl->rst[3]};
-    int64_t g = ctl->rgen;
-    int64_t lim = ctl->rlow + RB;
+    const int lane = threadIdx.x & 63;
+    // wave-uniform state in scalar registers
+    uint64_t s[4] = {uniform64(ctl->rst[0]), uniform64(ctl->rst[1]), uniform64(ctl->rst[2]),
+                     uniform64(ctl->rst[3])};
+    const int64_t g0 = (int64_t)uniform64((uint64_t)ctl->rgen);
+    int64_t lim = (int64_t)uniform64((uint64_t)(ctl->rlow + RB));
+    upto = (int64_t)uniform64((uint64_t)upto);
+    budget = (int64_t)uniform64((uint64_t)budget);
     if (upto < lim) lim = upto;
-    if (g + budget < lim) lim = g + budget;
-    const int64_t g0 = g;
+    if (g0 + budget < lim) lim = g0 + budget;
+    int64_t g = g0;
     while (g < lim) {
-        C.rbuf[g & (RB - 1)] = xo_next(s);
-        g++;
+        const int nb = (lim - g < 64) ? (int)(lim - g) : 64;
+        uint64_t mine = 0;
+        for (int k = 0; k < nb; k++) {
+            const uint64_t r = xo_next(s);
+            if (lane == k) mine = r;
+        }
+        if (lane < nb) C.rbuf[(g + lane) & (RB - 1)] = mine;
+        g += nb;
     }
-    ctl->rst[0] = s[0]; ctl->rst[1] = s[1]; ctl->rst[2] = s[2]; ctl->rst[3] = s[3];
-    ctl->rgen = g;
+    if (lane == 0 && g != g0) {
+        ctl->rst[0] = s[0]; ctl->rst[1] = s[1]; ctl->rst[2] = s[2]; ctl->rst[3] = s[3];
+        ctl->rgen = g;
+    }
     return g - g0;
 }
+
+__device__ __forceinline__ int64_t clk() { return (int64_t)wall_clock64(); }
 
 __device__ __forceinline__ uint64_t ring_at(const Cell& C, int64_t pos)
 {

@@ -45,9 +45,8 @@ __device__ __forceinline__ uint32_t udiv_small(uint32_t x, uint32_t d, float inv
     return (uint32_t)qf;
 }
 
-__global__ void __launch_bounds__(WG) k_recount(const Cell* __restrict__ pc, RecountPlan plan)
+__global__ void __launch_bounds__(WG) k_recount(const Cell C, RecountPlan plan)
 {
-    const Cell& C = *pc;
     extern __shared__ uint32_t tile[];
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
     const int t = plan.t;
@@ -93,9 +92,8 @@ __global__ void __launch_bounds__(WG) k_recount(const Cell* __restrict__ pc, Rec
 }
 
 // cnt -> planes, block counts and the uncovered count; invalidates the list
-__global__ void __launch_bounds__(WG) k_derive(const Cell* __restrict__ pc)
+__global__ void __launch_bounds__(WG) k_derive(const Cell C)
 {
-    const Cell& C = *pc;
     __shared__ WgRed red;
     __shared__ uint32_t bc[2];
     const int64_t groups_per_wg = WG;              // 256 groups of 32 words = 2 blocks
@@ -213,10 +211,9 @@ __device__ __forceinline__ int64_t ball_word_at(const Cell& C, int64_t base, uin
 }
 
 // incremental single-ball update over the whole grid (k == 1 centre)
-__global__ void __launch_bounds__(WG) k_ball_one(const Cell* __restrict__ pc, int64_t base,
+__global__ void __launch_bounds__(WG) k_ball_one(const Cell C, int64_t base,
                                                  uint64_t pk, int delta)
 {
-    const Cell& C = *pc;
     Ctl* ctl = C.ctl;
     __shared__ WgRed red;
     const bool list_on = ctl->ul_valid != 0;
@@ -232,11 +229,10 @@ __global__ void __launch_bounds__(WG) k_ball_one(const Cell* __restrict__ pc, in
 }
 
 // counts only, several centres (planes re-derived afterwards)
-__global__ void __launch_bounds__(WG) k_ball_multi(const Cell* __restrict__ pc,
+__global__ void __launch_bounds__(WG) k_ball_multi(const Cell C,
                                                    const int64_t* __restrict__ centers, int k,
                                                    int delta)
 {
-    const Cell& C = *pc;
     const int64_t total = (int64_t)k * C.V;
     for (int64_t g = (int64_t)blockIdx.x * WG + threadIdx.x; g < total; g += (int64_t)gridDim.x * WG) {
         const int64_t ci = g / C.V;
@@ -249,9 +245,8 @@ __global__ void __launch_bounds__(WG) k_ball_multi(const Cell* __restrict__ pc,
 }
 
 // (d) #{w : cnt[w] == 0}, streaming 16 counters per thread per step
-__global__ void __launch_bounds__(WG) k_count_zero(const Cell* __restrict__ pc, int64_t* out)
+__global__ void __launch_bounds__(WG) k_count_zero(const Cell C, int64_t* out)
 {
-    const Cell& C = *pc;
     __shared__ WgRed red;
     const uint16_t* cnt = reinterpret_cast<const uint16_t*>(C.cnt32);
     const int64_t nvec = C.N / 8;
@@ -270,9 +265,8 @@ __global__ void __launch_bounds__(WG) k_count_zero(const Cell* __restrict__ pc, 
 }
 
 // (d) ordered gather, pass 1: zeros per 4096-word block
-__global__ void __launch_bounds__(WG) k_block_zeros(const Cell* __restrict__ pc, int64_t* bz)
+__global__ void __launch_bounds__(WG) k_block_zeros(const Cell C, int64_t* bz)
 {
-    const Cell& C = *pc;
     __shared__ WgRed red;
     const uint16_t* cnt = reinterpret_cast<const uint16_t*>(C.cnt32);
     for (int64_t b = blockIdx.x; b < C.nblk; b += gridDim.x) {
@@ -288,10 +282,9 @@ __global__ void __launch_bounds__(WG) k_block_zeros(const Cell* __restrict__ pc,
 }
 
 // (d) ordered gather, pass 2: block b writes its zeros at out[off[b]...]
-__global__ void __launch_bounds__(WG) k_block_emit(const Cell* __restrict__ pc, const int64_t* off,
+__global__ void __launch_bounds__(WG) k_block_emit(const Cell C, const int64_t* off,
                                                    int64_t* out, int64_t cap)
 {
-    const Cell& C = *pc;
     __shared__ WgScan sc;
     const uint16_t* cnt = reinterpret_cast<const uint16_t*>(C.cnt32);
     constexpr int PER = BLK_WORDS / WG;   // 16 consecutive words per thread

@@ -31,9 +31,8 @@ __global__ void __launch_bounds__(WG) k_dist_fill(uint8_t* D, int64_t N)
         D[w] = 0xFF;
 }
 
-__global__ void __launch_bounds__(WG) k_dist_seed(const Cell* __restrict__ pc, uint8_t* D)
+__global__ void __launch_bounds__(WG) k_dist_seed(const Cell C, uint8_t* D)
 {
-    const Cell& C = *pc;
     for (int i = blockIdx.x * WG + threadIdx.x; i < C.M; i += gridDim.x * WG) D[C.cidx[i]] = 0;
 }
 

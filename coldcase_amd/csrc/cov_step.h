@@ -12,10 +12,10 @@
 //                         sphere with cnt == 0 (zb plane), or per (codeword,
 //                         chunk of the uncovered list) the x with d(x,c)=R+1
 //                         and x_p = u_p (:717-753)
-//                     wave 0 of workgroup 0 runs the Fisher-Yates shuffle of
-//                     the candidates meanwhile (the set of codewords to
-//                     evaluate does not depend on the order), wave 1 refills
-//                     the rng ring; the last workgroup to finish selects the
+//                     workgroup 0 computes the Fisher-Yates shuffle of the
+//                     candidates meanwhile (the set of codewords to evaluate
+//                     does not depend on the order), one wave of workgroup 1
+//                     refills the rng ring; the last workgroup to finish selects the
 //                     move (tabu, aspiration, reservoir ties) with a parallel
 //                     prefix formulation of the sequential scan (:1503-1518)
 //   k_commit  (wide)  --cnt[a], ++cnt[b=a+dv] over the C(n-1,R)(q-1)^R
@@ -29,13 +29,20 @@
 // Every decision that consumes randomness reads the ring of xoshiro256++
 // draws at a cursor in the reference's order, so the trajectory is the
 // reference's bit for bit.
+//
+// Synchronisation: a wide kernel's workgroups publish their results with one
+// agent-scope release atomic each on a counter; only the workgroup that sees
+// the count complete acquires (one L2 invalidate per launch), so workgroups
+// still gathering keep their XCD's L2 warm.  The single-workgroup phases read
+// data that device atomics change (planes, block counts) with agent-scope
+// loads.
 #pragma once
 #include "cov_common.h"
 #include "cov_cover.h"
 
 namespace cov {
 
-constexpr int SHUF_LDS = 4096;     // candidates shuffled in LDS (else in HBM)
+constexpr int SHUF_PAR = 1024;     // draws handled by the parallel shuffle
 
 struct NarrowSh {
     WgRed red;
@@ -44,6 +51,11 @@ struct NarrowSh {
     int64_t v64[8];
     int32_t v32[8];
 };
+
+__device__ __forceinline__ uint32_t zbit_acq(const Cell& C, int64_t w)
+{
+    return (ld_acq32(&C.zb[w >> 5]) >> (w & 31)) & 1u;
+}
 
 // ---------------------------------------------------------- scanning
 // first word of the plane-word range [x0, x1) with a set zb bit (masked first
@@ -56,7 +68,7 @@ __device__ inline int64_t wg_first_bits(const Cell& C, int64_t x0, int64_t x1, u
         const int64_t x = xb + threadIdx.x;
         int64_t cand = INT64_MAX;
         if (x < x1) {
-            uint32_t z = C.zb[x];
+            uint32_t z = ld_acq32(&C.zb[x]);
             if (x == x0) z &= m0;
             if (x == x1 - 1) z &= m1;
             if (z) cand = x * 32 + __ffs(z) - 1;
@@ -85,7 +97,7 @@ __device__ inline int64_t wg_first_from(const Cell& C, int64_t start, NarrowSh& 
         const int64_t b_hi = pass == 0 ? C.nblk : sb;
         for (int64_t bb = b_lo; bb < b_hi; bb += WG) {
             const int64_t b = bb + threadIdx.x;
-            const int64_t cand = (b < b_hi && C.blk[b]) ? b : INT64_MAX;
+            const int64_t cand = (b < b_hi && ld_acq32(&C.blk[b])) ? b : INT64_MAX;
             const int64_t fb = wg_min(cand, sh.red);
             if (fb != INT64_MAX) {
                 const int64_t x0 = fb << (BLK_SHIFT - 5);
@@ -103,14 +115,20 @@ __device__ inline int64_t wg_first_from(const Cell& C, int64_t start, NarrowSh& 
     return -1;
 }
 
-// make sure `need` draws past the cursor exist (thread 0 generates); all threads
+// make sure `need` draws past the cursor exist (wave 0 generates, with 1024
+// to spare); all threads
 __device__ inline void wg_ensure(const Cell& C, Ctl* ctl, int64_t need)
 {
-    if (threadIdx.x == 0) {
-        if (ctl->rlow < ctl->ruse) ctl->rlow = ctl->ruse;
-        if (ctl->rgen < ctl->ruse + need) ring_fill(C, ctl, ctl->ruse + need, INT64_MAX);
-    }
     __syncthreads();
+    const int64_t ruse = ctl->ruse;
+    const bool short_ = ctl->rgen < ruse + need;
+    __syncthreads();
+    if (short_) {
+        if (threadIdx.x == 0 && ctl->rlow < ruse) ctl->rlow = ruse;
+        __syncthreads();
+        if (threadIdx.x < 64) wave_ring_fill(C, ctl, ruse + need + 1024, INT64_MAX);
+        __syncthreads();
+    }
 }
 
 // pick_uncovered: 64 probes, then the cyclic scan (:973-1016).  All threads.
@@ -121,7 +139,7 @@ __device__ inline int64_t wg_pick(const Cell& C, Ctl* ctl, NarrowSh& sh)
     if (threadIdx.x < 64) {
         const uint64_t rv = ring_at(C, pos + threadIdx.x);
         const int64_t w = (int64_t)(rv % (uint64_t)C.N);
-        const bool hit = bit_of(C.zb, w) != 0;
+        const bool hit = zbit_acq(C, w) != 0;
         const uint64_t mask = __ballot(hit);
         const int fh = mask ? (__ffsll((unsigned long long)mask) - 1) : -1;
         if (threadIdx.x == 0) sh.v32[0] = fh;
@@ -148,7 +166,7 @@ __device__ inline void wg_list_rebuild(const Cell& C, Ctl* ctl, NarrowSh& sh)
     if (threadIdx.x == 0) { ctl->ul_n = 0; ctl->ul_ovf = 0; }
     __syncthreads();
     for (int64_t x = threadIdx.x; x < C.nzw; x += WG) {
-        const uint32_t z = C.zb[x];
+        const uint32_t z = ld_acq32(&C.zb[x]);
         C.inl[x] = z;
         uint32_t m = z;
         while (m) {
@@ -160,7 +178,6 @@ __device__ inline void wg_list_rebuild(const Cell& C, Ctl* ctl, NarrowSh& sh)
             else atomicOr(&ctl->ul_ovf, 1);
         }
     }
-    __threadfence();
     __syncthreads();
     if (threadIdx.x == 0) {
         ctl->ul_valid = ctl->ul_ovf ? 0 : 1;
@@ -183,7 +200,7 @@ __device__ inline void wg_list_compact(const Cell& C, Ctl* ctl, NarrowSh& sh)
         if (e < n0) {
             idx = C.ul_idx[e];
             pk = C.ul_pk[e];
-            keep = bit_of(C.zb, idx) != 0;
+            keep = zbit_acq(C, idx) != 0;
             if (!keep) bit_clr(C.inl, idx);
         }
         int64_t tot;
@@ -203,6 +220,7 @@ __device__ inline void wg_list_compact(const Cell& C, Ctl* ctl, NarrowSh& sh)
 // shuffle draws and lay out the evaluator's units.  All threads of one WG.
 __device__ inline void wg_begin(const Cell& C, Ctl* ctl, NarrowSh& sh)
 {
+    const int64_t tb0 = clk();
     __syncthreads();
     const int done = ctl->done, begun = ctl->begun;
     const int64_t unc = ctl->unc, it = ctl->it, it_stop = ctl->it_stop;
@@ -218,6 +236,7 @@ __device__ inline void wg_begin(const Cell& C, Ctl* ctl, NarrowSh& sh)
 
     // uncovered-list upkeep (speed only: the list never changes a value); an
     // overflow since the last step (inserts beyond capacity) invalidates it
+    const int64_t tl0 = clk();
     if (threadIdx.x == 0 && ctl->ul_ovf) {
         ctl->ul_valid = 0;
         ctl->ul_ovf = 0;
@@ -232,33 +251,55 @@ __device__ inline void wg_begin(const Cell& C, Ctl* ctl, NarrowSh& sh)
         else if (valid && ul_n > 2 * unc + 64) wg_list_compact(C, ctl, sh);
     }
 
+    const int64_t tp0 = clk();
     const int64_t u = wg_pick(C, ctl, sh);
+    if (threadIdx.x == 0) { ctl->tph[4] += tp0 - tl0; ctl->tph[3] += clk() - tp0; }
     if (u < 0) {
         if (threadIdx.x == 0) ctl->done = 1;
         __syncthreads();
         return;
     }
-    if (threadIdx.x == 0) sh.v64[2] = (int64_t)pack_index(C, u);
-    __syncthreads();
-    const uint64_t upk = (uint64_t)sh.v64[2];
+    // digits of u, one position per lane
+    uint64_t part = 0;
+    if (threadIdx.x < C.n) {
+        const int j = threadIdx.x;
+        int64_t d;
+        if (C.N <= 0xFFFFFFFFLL) d = (int64_t)(((uint32_t)u / (uint32_t)C.pw[j]) % (uint32_t)C.q);
+        else d = (u / C.pw[j]) % C.q;
+        part = (uint64_t)d << (4 * j);
+    }
+    const uint64_t upk = (uint64_t)wg_sum((int64_t)part, sh.red);
 
-    // distances and the target distance (:1342-1351)
+    // distances and the target distance (:1342-1351), then the codewords at
+    // distance dtar in index order -> slots; each thread keeps its words'
+    // packed digits in registers across both passes
+    constexpr int KEEP = 4;
+    uint64_t cpk_r[KEEP];
     int64_t dmin = INT64_MAX;
-    for (int i = threadIdx.x; i < C.M; i += WG) {
+#pragma unroll
+    for (int k = 0; k < KEEP; k++) {
+        const int i = k * WG + threadIdx.x;
+        cpk_r[k] = i < C.M ? C.cpk[i] : 0;
+    }
+#pragma unroll
+    for (int k = 0; k < KEEP; k++) {
+        const int i = k * WG + threadIdx.x;
+        if (i < C.M) { const int64_t d = nib_dist(cpk_r[k], upk); dmin = d < dmin ? d : dmin; }
+    }
+    for (int i = KEEP * WG + threadIdx.x; i < C.M; i += WG) {
         const int64_t d = nib_dist(C.cpk[i], upk);
         dmin = d < dmin ? d : dmin;
     }
     dmin = wg_min(dmin, sh.red);
     const int dtar = (dmin <= C.R + 1) ? C.R + 1 : (int)dmin;
 
-    // qualifying codewords in index order -> slots
     int64_t nq = 0;
-    for (int i0 = 0; i0 < C.M; i0 += WG) {
+    auto qualify = [&](int i0, uint64_t mine) {
         const int i = i0 + threadIdx.x;
         uint64_t ne = 0;
         int ok = 0;
         if (i < C.M) {
-            ne = nib_ne(C.cpk[i], upk);
+            ne = nib_ne(mine, upk);
             ok = __popcll(ne) == dtar;
         }
         int64_t tot;
@@ -271,6 +312,13 @@ __device__ inline void wg_begin(const Cell& C, Ctl* ctl, NarrowSh& sh)
                 if ((ne >> (4 * p)) & 1) C.slot_pos[slot * MAXN + t++] = (uint8_t)p;
         }
         nq += tot;
+    };
+#pragma unroll
+    for (int k = 0; k < KEEP; k++)
+        if (k * WG < C.M) qualify(k * WG, cpk_r[k]);
+    for (int i0 = KEEP * WG; i0 < C.M; i0 += WG) {
+        const int i = i0 + threadIdx.x;
+        qualify(i0, i < C.M ? C.cpk[i] : 0);
     }
     const int64_t ncand = nq * dtar;
     const int64_t take = ncand < ctl->cand ? ncand : ctl->cand;
@@ -279,12 +327,11 @@ __device__ inline void wg_begin(const Cell& C, Ctl* ctl, NarrowSh& sh)
     const int use_list = ctl->ul_valid &&
                          (ctl->ul_n * C.n < (int64_t)(C.R + 1) * C.E * C.P);
     const int64_t nchunk = (ctl->ul_n + LIST_CHUNK - 1) / LIST_CHUNK;
-    for (int64_t x = threadIdx.x; x < nq * ACCW; x += WG) C.acc[x] = 0;
-    __threadfence();
-    __syncthreads();
+    // the shuffle's draws (:1384-1389) are reserved now; the evaluator's
+    // helper wave computes the permutation while the units run
+    if (take <= RB - 4096) wg_ensure(C, ctl, take);
     if (threadIdx.x == 0) {
         if (ncand == 0) { ctl->fault = 1; ctl->done = 1; }
-        // the shuffle's draws must fit in the ring with room for the select's
         if (take > RB - 4096) { ctl->fault = 2; ctl->done = 1; }
         ctl->u = u;
         ctl->upk = upk;
@@ -296,22 +343,21 @@ __device__ inline void wg_begin(const Cell& C, Ctl* ctl, NarrowSh& sh)
         ctl->nchunk = use_list ? nchunk : 0;
         ctl->n_leave = nq * C.F;
         ctl->n_enter = use_list ? nq * nchunk : nq * dtar * C.E;
-        // the shuffle's draws (:1384-1389): reserve them; the evaluator's helper
-        // wave computes the permutation while the units run
         ctl->sh_pos = ctl->ruse;
         ctl->rlow = ctl->ruse;
         ctl->ruse += take;
-        if (ctl->rgen < ctl->ruse) ring_fill(C, ctl, ctl->ruse, INT64_MAX);
         ctl->eval_bytes += (int64_t)C.M * C.n + 2 * nq * C.F * C.P +
                            (use_list ? nq * ctl->ul_n * C.n : 2 * nq * dtar * C.E * C.P);
         ctl->selected = 0;
         ctl->begun = (ncand > 0 && take <= RB - 4096) ? 1 : 0;
+        ctl->tph[2] += clk() - tb0;
     }
     __syncthreads();
 }
 
 // ------------------------------------------------------- evaluator units
-// leaving side: words of the distance-R sphere of codeword slot on subset f
+// leaving side: words of the distance-R sphere of codeword slot on subset f;
+// four independent gathers in flight per lane
 __device__ __forceinline__ void unit_leave(const Cell& C, int slot, int f)
 {
     const int lane = threadIdx.x & 63;
@@ -320,11 +366,16 @@ __device__ __forceinline__ void unit_leave(const Cell& C, int slot, int f)
     const uint64_t pk = C.cpk[i];
     const uint32_t sub = C.fsub[f];
     int64_t c1 = 0;
-    for (int64_t t0 = 0; t0 < C.P; t0 += 64) {
-        const int64_t t = t0 + lane;
-        uint32_t b = 0;
-        if (t < C.P) b = bit_of(C.ob, pattern_word(C, base, pk, sub, C.R, (uint32_t)t, nullptr));
-        c1 += __popcll(__ballot(b));
+    for (int64_t t0 = 0; t0 < C.P; t0 += 256) {
+        uint32_t b[4];
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            const int64_t t = t0 + k * 64 + lane;
+            b[k] = 0;
+            if (t < C.P) b[k] = bit_of(C.ob, pattern_word(C, base, pk, sub, C.R, (uint32_t)t, nullptr));
+        }
+#pragma unroll
+        for (int k = 0; k < 4; k++) c1 += __popcll(__ballot(b[k]));
     }
     if (lane == 0 && c1) {
         int32_t* a = C.acc + (int64_t)slot * ACCW;
@@ -345,11 +396,16 @@ __device__ __forceinline__ void unit_walk(const Cell& C, int slot, int p, int up
     const uint64_t pk = nib_put(pk0, p, up);
     const uint32_t sub = C.psub[(int64_t)p * C.E + e];
     int64_t c0 = 0;
-    for (int64_t t0 = 0; t0 < C.P; t0 += 64) {
-        const int64_t t = t0 + lane;
-        uint32_t b = 0;
-        if (t < C.P) b = bit_of(C.zb, pattern_word(C, base, pk, sub, C.R, (uint32_t)t, nullptr));
-        c0 += __popcll(__ballot(b));
+    for (int64_t t0 = 0; t0 < C.P; t0 += 256) {
+        uint32_t b[4];
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            const int64_t t = t0 + k * 64 + lane;
+            b[k] = 0;
+            if (t < C.P) b[k] = bit_of(C.zb, pattern_word(C, base, pk, sub, C.R, (uint32_t)t, nullptr));
+        }
+#pragma unroll
+        for (int k = 0; k < 4; k++) c0 += __popcll(__ballot(b[k]));
     }
     if (lane == 0 && c0) atomicAdd(&C.acc[(int64_t)slot * ACCW + 1 + MAXN + p], (int)c0);
 }
@@ -420,36 +476,95 @@ __device__ inline void eval_units(const Cell& C, const Ctl* ctl, int64_t wid, in
 }
 
 // ------------------------------------------------------------- shuffle
-// partial Fisher-Yates over the ncand candidates (original index k =
-// slot*dtar + t), `take` draws from ring position sh_pos (:1383-1389).  One
-// wave; writes C.shuf[0..take).
-__device__ inline void wave_shuffle(const Cell& C, const Ctl* ctl, uint32_t* lds_a)
+struct ShufSh {
+    uint32_t j[SHUF_PAR + 4] __attribute__((aligned(16)));   // swap partner of each draw
+    int32_t g[SHUF_PAR];
+    int32_t f[SHUF_PAR];
+};
+
+// Partial Fisher-Yates over the ncand candidates (original index k =
+// slot*dtar + t): `take` draws from ring position sh_pos, the s-th swapping
+// slot s with j_s = s + r_s mod (ncand - s) (:1383-1389).  Writes C.shuf.
+//
+// Parallel form (take <= SHUF_PAR).  Slot s is final after swap s and holds
+// what slot j_s held just before it.  A slot p >= s is written before time s
+// only by swaps t < s with j_t = p (its own swap comes at time p >= s), and
+// such a swap puts there what slot t held at time t.  So with
+//   g(s) = last t < s with j_t = j_s,   f(t) = last t' < t with j_t' = t,
+// out[s] = j_s if g(s) < 0, else root(g(s)), where root(t) = t if f(t) < 0,
+// else root(f(t)).  All threads of one workgroup: the (draw, chunk of 32
+// earlier draws) pairs are scanned in parallel with LDS maxima, then every
+// thread follows its own chains.
+__device__ inline void wg_shuffle(const Cell& C, const Ctl* ctl, ShufSh& sh)
 {
-    const int lane = threadIdx.x & 63;
+    const int tid = threadIdx.x;
     const int64_t ncand = ctl->ncand, take = ctl->take, pos0 = ctl->sh_pos;
-    const bool in_lds = ncand <= SHUF_LDS;
-    uint32_t* a = in_lds ? lds_a : C.shuf;
-    __shared__ uint32_t jb[64];
-    for (int64_t k = lane; k < ncand; k += 64) a[k] = (uint32_t)k;
-    __threadfence_block();
-    for (int64_t s0 = 0; s0 < take; s0 += 64) {
-        const int64_t s = s0 + lane;
-        if (s < take) jb[lane] = (uint32_t)(s + (int64_t)(ring_at(C, pos0 + s) % (uint64_t)(ncand - s)));
-        __threadfence_block();
-        if (lane == 0) {
-            const int64_t lim = (take - s0 < 64) ? take - s0 : 64;
+    if (take <= SHUF_PAR) {
+        for (int64_t s = tid; s < take; s += WG) {
+            sh.j[s] = (uint32_t)(s + (int64_t)(ring_at(C, pos0 + s) % (uint64_t)(ncand - s)));
+            sh.g[s] = -1;
+            sh.f[s] = -1;
+        }
+        if (tid < 4 && take + tid < SHUF_PAR + 4) sh.j[take + tid] = 0xFFFFFFFFu;
+        __syncthreads();
+        // every (draw s, chunk of 32 earlier draws) pair in parallel
+        const int64_t nchunk = (take + 31) / 32;
+        for (int64_t x = tid; x < take * nchunk; x += WG) {
+            const int64_t s = x % take, c = x / take;
+            const int64_t t0 = c * 32;
+            if (t0 >= s) continue;
+            const int64_t t1 = (t0 + 32 < s) ? t0 + 32 : s;
+            const uint32_t js = sh.j[s];
+            int32_t g = -1, f = -1;
+            for (int64_t t4 = t0; t4 < t1; t4 += 4) {
+                const uint4 q4 = *reinterpret_cast<const uint4*>(&sh.j[t4]);
+                const uint32_t v[4] = {q4.x, q4.y, q4.z, q4.w};
+#pragma unroll
+                for (int k = 0; k < 4; k++) {
+                    const int32_t t = (int32_t)(t4 + k);
+                    if (t < t1) {
+                        if (v[k] == js) g = t;
+                        if (v[k] == (uint32_t)s) f = t;
+                    }
+                }
+            }
+            if (g >= 0) atomicMax(&sh.g[s], g);
+            if (f >= 0) atomicMax(&sh.f[s], f);
+        }
+        __syncthreads();
+        for (int64_t s = tid; s < take; s += WG) {
+            int32_t t = sh.g[s];
+            uint32_t out;
+            if (t < 0) {
+                out = sh.j[s];
+            } else {
+                while (sh.f[t] >= 0) t = sh.f[t];
+                out = (uint32_t)t;
+            }
+            C.shuf[s] = out;
+        }
+        return;
+    }
+    // serial path for large samples (thread 0, the candidate array in HBM)
+    uint32_t* a = C.shuf;
+    for (int64_t k = tid; k < ncand; k += WG) a[k] = (uint32_t)k;
+    __syncthreads();
+    for (int64_t s0 = 0; s0 < take; s0 += WG) {
+        const int64_t s = s0 + tid;
+        if (s < take) sh.j[tid] = (uint32_t)(s + (int64_t)(ring_at(C, pos0 + s) % (uint64_t)(ncand - s)));
+        __syncthreads();
+        if (tid == 0) {
+            const int64_t lim = (take - s0 < WG) ? take - s0 : WG;
             for (int64_t t = 0; t < lim; t++) {
                 const int64_t ss = s0 + t;
-                const uint32_t j = jb[t];
+                const uint32_t j = sh.j[t];
                 const uint32_t x = a[ss], y = a[j];
                 a[j] = x;
                 a[ss] = y;
             }
         }
-        __threadfence_block();
+        __syncthreads();
     }
-    if (in_lds)
-        for (int64_t k = lane; k < take; k += 64) C.shuf[k] = a[k];
 }
 
 // ------------------------------------------------------------- select
@@ -482,10 +597,11 @@ __device__ __forceinline__ int64_t cand_delta(const Cell& C, const Ctl* ctl, uin
 __device__ inline void wg_select(const Cell& C, Ctl* ctl, SelSh& sh)
 {
     const int tid = threadIdx.x;
-    if (tid == 0) {
-        ctl->rlow = ctl->ruse;
-        if (ctl->rgen < ctl->ruse + ctl->take + 2) ring_fill(C, ctl, ctl->ruse + ctl->take + 2, INT64_MAX);
-    }
+    if (tid == 0) ctl->rlow = ctl->ruse;
+    __syncthreads();
+    const bool short_ = ctl->rgen < ctl->ruse + ctl->take + 2;
+    __syncthreads();
+    if (short_ && tid < 64) wave_ring_fill(C, ctl, ctl->ruse + ctl->take + 2 + 1024, INT64_MAX);
     __syncthreads();
     const int64_t take = ctl->take, it = ctl->it, unc = ctl->unc, best = ctl->best_unc;
     const int64_t r0 = ctl->ruse;
@@ -563,6 +679,8 @@ __device__ inline void wg_select(const Cell& C, Ctl* ctl, SelSh& sh)
         ctl->selected = 1;
     }
     __syncthreads();
+    // the accumulators start the next step at zero
+    for (int64_t x = tid; x < (int64_t)ctl->nq * ACCW; x += WG) C.acc[x] = 0;
     // distinct codewords among the first `take` (differs from nq only when
     // --cand cuts the sample)
     if (take < ctl->ncand) {
@@ -584,36 +702,67 @@ __device__ inline void wg_select(const Cell& C, Ctl* ctl, SelSh& sh)
     }
 }
 
-// --------------------------------------------------------------- kernels
-__global__ void __launch_bounds__(WG) k_eval(const Cell* __restrict__ pc)
+// One release per workgroup on `counter`; returns true in the workgroup that
+// completes the count, which then acquires.  All threads.
+__device__ inline bool wg_arrive_last(int32_t* counter, int* flag)
 {
-    const Cell& C = *pc;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const int prev = __hip_atomic_fetch_add(counter, 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+        *flag = (prev == (int)gridDim.x - 1);
+    }
+    __syncthreads();
+    const bool last = *flag != 0;
+    if (last) {
+        __threadfence();
+        __syncthreads();
+    }
+    return last;
+}
+
+// --------------------------------------------------------------- kernels
+__global__ void __launch_bounds__(WG, 4) k_eval(const Cell C)
+{
     Ctl* ctl = C.ctl;
-    __shared__ uint32_t shuf_lds[SHUF_LDS];
+    __shared__ ShufSh shs;
     __shared__ SelSh sel;
     __shared__ int last;
     if (!ctl->begun || ctl->done || ctl->selected) return;
     const int wv = threadIdx.x >> 6;
-    const bool helper = blockIdx.x == 0 && wv < 2;
-    if (helper) {
-        if (wv == 0) wave_shuffle(C, ctl, shuf_lds);
-        else if ((threadIdx.x & 63) == 0) ring_fill(C, ctl, ctl->rlow + RB, ctl->budget);
-    } else {
-        const int64_t wid = (int64_t)blockIdx.x * WAVES + wv - 2;
-        const int64_t nw = (int64_t)gridDim.x * WAVES - 2;
-        eval_units(C, ctl, wid, nw);
+    const int64_t th0 = clk();
+    if (blockIdx.x == 0 && threadIdx.x == 0 && ctl->t_cme) ctl->tph[10] += th0 - ctl->t_cme;
+    // workgroup 0: the shuffle; wave 0 of workgroup 1 (or of 0 on a one-group
+    // grid, after the shuffle): the rng refill, keeping ~4096 draws ahead
+    const int rng_wg = gridDim.x > 1 ? 1 : 0;
+    if (blockIdx.x == 0) {
+        wg_shuffle(C, ctl, shs);
+        if (threadIdx.x == 0) ctl->tph[5] += clk() - th0;
     }
-    __threadfence();
-    __syncthreads();
+    if (blockIdx.x == rng_wg && wv == 0) {
+        if (ctl->rgen < ctl->ruse + 4096) wave_ring_fill(C, ctl, ctl->ruse + 4096, ctl->budget);
+        if (threadIdx.x == 0) ctl->tph[6] += clk() - th0;
+    }
+    if (blockIdx.x != 0) {
+        // units over all waves of workgroups >= 1, except the rng wave
+        const int64_t wid = (int64_t)(blockIdx.x - 1) * WAVES + wv - (rng_wg == 1 ? 1 : 0);
+        const int64_t nw = (int64_t)(gridDim.x - 1) * WAVES - 1;
+        if (!(blockIdx.x == 1 && wv == 0)) eval_units(C, ctl, wid, nw);
+    } else if (gridDim.x == 1) {
+        eval_units(C, ctl, wv, WAVES);
+    }
+    if (!wg_arrive_last(&ctl->evdone, &last)) return;
+    const int64_t ts0 = clk();
+    __shared__ Ctl lc;
+    ctl_to_lds(&lc, ctl);
+    if (threadIdx.x == 0 && lc.t_cme) lc.tph[8] += ts0 - lc.t_cme;
+    wg_select(C, &lc, sel);
     if (threadIdx.x == 0) {
-        const int prev = atomicAdd(&ctl->evdone, 1);
-        last = (prev == (int)gridDim.x - 1);
+        lc.evdone = 0;
+        const int64_t te = clk();
+        lc.tph[0] += te - ts0;
+        lc.t_eve = te;
     }
-    __syncthreads();
-    if (!last) return;
-    __threadfence();
-    wg_select(C, ctl, sel);
-    if (threadIdx.x == 0) ctl->evdone = 0;
+    ctl_from_lds(ctl, &lc);
 }
 
 // kick: teleport `kick` codewords near uncovered words (:1553-1578).  All threads.
@@ -628,7 +777,6 @@ __device__ inline void wg_ball(const Cell& C, Ctl* ctl, int i, int delta, Narrow
         const int64_t w = ball_word_at(C, base, pk, g, &wpk);
         d += ball_word(C, ctl, w, wpk, delta, list_on);
     }
-    __threadfence();
     d = wg_sum(d, sh.red);
     if (threadIdx.x == 0) ctl->unc += d;
     __syncthreads();
@@ -675,12 +823,10 @@ __device__ inline void wg_kick(const Cell& C, Ctl* ctl, NarrowSh& sh)
             C.cpk[i] = pk;
             C.cidx[i] = idx;
         }
-        __threadfence();
         __syncthreads();
         wg_ball(C, ctl, i, +1, sh);
     }
     for (int64_t x = threadIdx.x; x < (int64_t)C.M * n * q; x += WG) C.tabu[x] = 0;
-    __threadfence();
     __syncthreads();
     if (threadIdx.x == 0) {
         ctl->nkicks += 1;
@@ -692,6 +838,7 @@ __device__ inline void wg_kick(const Cell& C, Ctl* ctl, NarrowSh& sh)
 // apply the committed move and close the step.  All threads of one WG.
 __device__ inline void wg_post(const Cell& C, Ctl* ctl, NarrowSh& sh)
 {
+    const int64_t tq0 = clk();
     const int n = C.n, q = C.q;
     if (threadIdx.x == 0) {
         const int64_t dunc = ctl->dunc;
@@ -702,7 +849,7 @@ __device__ inline void wg_post(const Cell& C, Ctl* ctl, NarrowSh& sh)
         C.cpk[bi] = nib_put(C.cpk[bi], bp, bv);
         C.cidx[bi] += ctl->dv;
         C.tabu[((int64_t)bi * n + bp) * q + oldv] = ctl->stamp;
-        if (ctl->ul_ovf) { ctl->ul_valid = 0; ctl->ul_ovf = 0; }
+        if (ctl->ul_ovf) { ctl->ul_valid = 0; ctl->ul_ovf = 0; ctl->ul_n = 0; }
         ctl->commit_bytes += 8 * (int64_t)C.E * C.P;
         sh.v64[3] = dunc;
         sh.v32[2] = 0;
@@ -715,18 +862,24 @@ __device__ inline void wg_post(const Cell& C, Ctl* ctl, NarrowSh& sh)
             sh.v32[2] = 2;
         }
     }
-    __threadfence();
     __syncthreads();
     const int what = sh.v32[2];
     const int64_t dunc = sh.v64[3];
     __syncthreads();
-    if (what == 1)
-        for (int64_t x = threadIdx.x; x < (int64_t)C.M * n; x += WG) C.best[x] = C.code[x];
-    else if (what == 2)
+    if (what == 1) {
+        const int64_t nb = (int64_t)C.M * n, n16 = nb / 16;
+        const uint4* src = reinterpret_cast<const uint4*>(C.code);
+        uint4* dst = reinterpret_cast<uint4*>(C.best);
+        for (int64_t x = threadIdx.x; x < n16; x += WG) dst[x] = src[x];
+        for (int64_t x = n16 * 16 + threadIdx.x; x < nb; x += WG) C.best[x] = C.code[x];
+    } else if (what == 2) {
+        const int64_t tk0 = clk();
         wg_kick(C, ctl, sh);
+        if (threadIdx.x == 0) ctl->tph[7] += clk() - tk0;
+    }
     if (threadIdx.x == 0) {
-        if (C.trace && ctl->tpos < C.tcap) {
-            int64_t* tr = C.trace + ctl->tpos * TRW;
+        if (ctl->trace && ctl->tpos < ctl->tcap) {
+            int64_t* tr = ctl->trace + ctl->tpos * TRW;
             tr[0] = ctl->it;
             tr[1] = ctl->u;
             tr[2] = ctl->ncand;
@@ -747,15 +900,15 @@ __device__ inline void wg_post(const Cell& C, Ctl* ctl, NarrowSh& sh)
         ctl->tpos += 1;
         ctl->begun = 0;
         ctl->selected = 0;
+        ctl->tph[1] += clk() - tq0;
     }
-    __threadfence();
     __syncthreads();
 }
 
 // the commit units: every pattern of the subsets avoiding bp, flattened
 __device__ inline int64_t commit_units(const Cell& C, const Ctl* ctl, int64_t gt, int64_t nt)
 {
-    const int bi = ctl->bi, bp = ctl->bp, bv = ctl->bv;
+    const int bi = ctl->bi, bp = ctl->bp;
     const int64_t base = ctl->base, dv = ctl->dv;
     const uint64_t pk = C.cpk[bi];
     const bool list_on = ctl->ul_valid != 0;
@@ -789,86 +942,95 @@ __device__ inline int64_t commit_units(const Cell& C, const Ctl* ctl, int64_t gt
     return d;
 }
 
-__global__ void __launch_bounds__(WG) k_commit(const Cell* __restrict__ pc, int api)
+__global__ void __launch_bounds__(WG) k_commit(const Cell C, int api)
 {
-    const Cell& C = *pc;
     Ctl* ctl = C.ctl;
     __shared__ NarrowSh sh;
     __shared__ int last;
     if (!ctl->begun || ctl->done || !ctl->selected) return;
+    if (blockIdx.x == 0 && threadIdx.x == 0 && ctl->t_eve) ctl->tph[11] += clk() - ctl->t_eve;
+    const int64_t tc0 = clk();
     const int64_t gt = (int64_t)blockIdx.x * WG + threadIdx.x;
     const int64_t d = commit_units(C, ctl, gt, (int64_t)gridDim.x * WG);
     const int64_t ds = wg_sum(d, sh.red);
     if (threadIdx.x == 0 && ds) atomicAdd(reinterpret_cast<unsigned long long*>(&ctl->dunc), (unsigned long long)ds);
-    __threadfence();
-    __syncthreads();
+    if (!wg_arrive_last(&ctl->cmdone, &last)) return;
+    __shared__ Ctl lc;
+    ctl_to_lds(&lc, ctl);
+    Ctl* L = &lc;
     if (threadIdx.x == 0) {
-        const int prev = atomicAdd(&ctl->cmdone, 1);
-        last = (prev == (int)gridDim.x - 1);
+        L->cmdone = 0;
+        L->tph[9] += clk() - tc0;
     }
-    __syncthreads();
-    if (!last) return;
-    __threadfence();
-    if (threadIdx.x == 0) ctl->cmdone = 0;
     if (api) {
         // a single committed move requested through the C ABI: apply it only
         if (threadIdx.x == 0) {
-            const int64_t dunc = ctl->dunc;
-            ctl->dunc = 0;
-            ctl->unc += dunc;
-            ctl->dsel = dunc;
-            C.code[(int64_t)ctl->bi * C.n + ctl->bp] = (uint8_t)ctl->bv;
-            C.cpk[ctl->bi] = nib_put(C.cpk[ctl->bi], ctl->bp, ctl->bv);
-            C.cidx[ctl->bi] += ctl->dv;
-            if (ctl->ul_ovf) { ctl->ul_valid = 0; ctl->ul_ovf = 0; }
-            ctl->begun = 0;
-            ctl->selected = 0;
+            const int64_t dunc = L->dunc;
+            L->dunc = 0;
+            L->unc += dunc;
+            L->dsel = dunc;
+            C.code[(int64_t)L->bi * C.n + L->bp] = (uint8_t)L->bv;
+            C.cpk[L->bi] = nib_put(C.cpk[L->bi], L->bp, L->bv);
+            C.cidx[L->bi] += L->dv;
+            if (L->ul_ovf) { L->ul_valid = 0; L->ul_ovf = 0; L->ul_n = 0; }
+            L->begun = 0;
+            L->selected = 0;
         }
+        ctl_from_lds(C.ctl, &lc);
         return;
     }
-    wg_post(C, ctl, sh);
-    wg_begin(C, ctl, sh);
+    wg_post(C, L, sh);
+    wg_begin(C, L, sh);
+    if (threadIdx.x == 0) L->t_cme = clk();
+    ctl_from_lds(C.ctl, &lc);
 }
 
 // set the run's stop and begin the first step (one workgroup)
-__global__ void __launch_bounds__(WG) k_begin_run(const Cell* __restrict__ pc, int64_t nsteps)
+__global__ void __launch_bounds__(WG) k_begin_run(const Cell C, int64_t nsteps)
 {
-    const Cell& C = *pc;
-    Ctl* ctl = C.ctl;
     __shared__ NarrowSh sh;
+    __shared__ Ctl lc;
+    ctl_to_lds(&lc, C.ctl);
+    Ctl* ctl = &lc;
     if (threadIdx.x == 0) {
         ctl->it_stop = ctl->it + nsteps - (ctl->begun ? 1 : 0);
         ctl->tpos = 0;
     }
     __syncthreads();
     wg_begin(C, ctl, sh);
+    if (threadIdx.x == 0) ctl->t_cme = clk();
+    ctl_from_lds(C.ctl, &lc);
 }
 
 // --------------------------------------------------------- init kernels
-// uniform random digits for codewords have..M-1, row-major (:1235-1238)
-__global__ void __launch_bounds__(WG) k_init_random(const Cell* __restrict__ pc, int have)
+// uniform random digits for codewords have..M-1, row-major (:1235-1238).
+// One wave.
+__global__ void __launch_bounds__(64) k_init_random(const Cell C, int have)
 {
-    const Cell& C = *pc;
     Ctl* ctl = C.ctl;
-    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    const int lane = threadIdx.x & 63;
     const int64_t need = (int64_t)(C.M - have) * C.n;
     int64_t done = 0;
     while (done < need) {
-        if (ctl->rlow < ctl->ruse) ctl->rlow = ctl->ruse;
-        if (ctl->rgen <= ctl->ruse) ring_fill(C, ctl, ctl->ruse + RB / 2, INT64_MAX);
-        const int64_t avail = ctl->rgen - ctl->ruse;
+        const int64_t r0 = ctl->ruse;
+        if (lane == 0 && ctl->rlow < r0) ctl->rlow = r0;
+        __threadfence_block();
+        if (ctl->rgen <= r0) wave_ring_fill(C, ctl, r0 + RB / 2, INT64_MAX);
+        __threadfence_block();
+        const int64_t avail = ctl->rgen - r0;
         const int64_t k = (need - done < avail) ? need - done : avail;
-        for (int64_t x = 0; x < k; x++)
-            C.code[(int64_t)have * C.n + done + x] = (uint8_t)(ring_at(C, ctl->ruse + x) % (uint64_t)C.q);
-        ctl->ruse += k;
+        for (int64_t x = lane; x < k; x += 64)
+            C.code[(int64_t)have * C.n + done + x] = (uint8_t)(ring_at(C, r0 + x) % (uint64_t)C.q);
+        __threadfence_block();
+        if (lane == 0) ctl->ruse = r0 + k;
+        __threadfence_block();
         done += k;
     }
 }
 
 // digits -> packed digits and word index for every codeword
-__global__ void __launch_bounds__(WG) k_pack_code(const Cell* __restrict__ pc)
+__global__ void __launch_bounds__(WG) k_pack_code(const Cell C)
 {
-    const Cell& C = *pc;
     for (int i = blockIdx.x * WG + threadIdx.x; i < C.M; i += gridDim.x * WG) {
         uint64_t pk = 0;
         int64_t idx = 0;
@@ -883,15 +1045,16 @@ __global__ void __launch_bounds__(WG) k_pack_code(const Cell* __restrict__ pc)
 }
 
 // greedy placement of codeword i on the picked uncovered word (:1219-1233)
-__global__ void __launch_bounds__(WG) k_greedy_pick(const Cell* __restrict__ pc, int i)
+__global__ void __launch_bounds__(WG) k_greedy_pick(const Cell C, int i)
 {
-    const Cell& C = *pc;
-    Ctl* ctl = C.ctl;
     __shared__ NarrowSh sh;
+    __shared__ Ctl lc;
+    ctl_to_lds(&lc, C.ctl);
+    Ctl* ctl = &lc;
     int64_t w = wg_pick(C, ctl, sh);
+    if (w < 0) wg_ensure(C, ctl, 1);
     if (threadIdx.x == 0) {
         if (w < 0) {
-            if (ctl->rgen <= ctl->ruse) ring_fill(C, ctl, ctl->ruse + 1, INT64_MAX);
             w = (int64_t)(ring_at(C, ctl->ruse) % (uint64_t)C.N);
             ctl->ruse += 1;
         }
@@ -900,12 +1063,12 @@ __global__ void __launch_bounds__(WG) k_greedy_pick(const Cell* __restrict__ pc,
         C.cpk[i] = pk;
         C.cidx[i] = w;
     }
+    ctl_from_lds(C.ctl, &lc);
 }
 
 // +-1 ball of codeword i (incremental, whole grid)
-__global__ void __launch_bounds__(WG) k_ball_cw(const Cell* __restrict__ pc, int i, int delta)
+__global__ void __launch_bounds__(WG) k_ball_cw(const Cell C, int i, int delta)
 {
-    const Cell& C = *pc;
     Ctl* ctl = C.ctl;
     __shared__ WgRed red;
     const bool list_on = ctl->ul_valid != 0;
@@ -922,10 +1085,9 @@ __global__ void __launch_bounds__(WG) k_ball_cw(const Cell* __restrict__ pc, int
 }
 
 // words of each given centre's ball with cnt == 1 (remove-and-repair, :927-946)
-__global__ void __launch_bounds__(WG) k_unique_cover(const Cell* __restrict__ pc, const int64_t* centers,
+__global__ void __launch_bounds__(WG) k_unique_cover(const Cell C, const int64_t* centers,
                                                      int k, int64_t* out)
 {
-    const Cell& C = *pc;
     __shared__ WgRed red;
     for (int c = blockIdx.x; c < k; c += gridDim.x) {
         const int64_t base = centers[c];
@@ -939,9 +1101,8 @@ __global__ void __launch_bounds__(WG) k_unique_cover(const Cell* __restrict__ pc
 }
 
 // whole-grid list rebuild from the zb plane
-__global__ void __launch_bounds__(WG) k_list_rebuild(const Cell* __restrict__ pc)
+__global__ void __launch_bounds__(WG) k_list_rebuild(const Cell C)
 {
-    const Cell& C = *pc;
     Ctl* ctl = C.ctl;
     for (int64_t x = (int64_t)blockIdx.x * WG + threadIdx.x; x < C.nzw; x += (int64_t)gridDim.x * WG) {
         const uint32_t z = C.zb[x];
@@ -959,9 +1120,8 @@ __global__ void __launch_bounds__(WG) k_list_rebuild(const Cell* __restrict__ pc
 }
 
 // first uncovered word at or after start (one workgroup)
-__global__ void __launch_bounds__(WG) k_first_from(const Cell* __restrict__ pc, int64_t start, int64_t* out)
+__global__ void __launch_bounds__(WG) k_first_from(const Cell C, int64_t start, int64_t* out)
 {
-    const Cell& C = *pc;
     __shared__ NarrowSh sh;
     const int64_t w = wg_first_from(C, start, sh);
     if (threadIdx.x == 0) *out = w;
@@ -969,9 +1129,8 @@ __global__ void __launch_bounds__(WG) k_first_from(const Cell* __restrict__ pc, 
 
 // --------------------------------------------- evaluator through the ABI
 // codewords slot_cw[0..k) against word u (ctl->upk), all differing positions
-__global__ void __launch_bounds__(WG) k_eval_probe(const Cell* __restrict__ pc, int k)
+__global__ void __launch_bounds__(WG) k_eval_probe(const Cell C, int k)
 {
-    const Cell& C = *pc;
     const Ctl* ctl = C.ctl;
     const uint64_t upk = ctl->upk;
     const int64_t wid = (int64_t)blockIdx.x * WAVES + (threadIdx.x >> 6);
@@ -994,9 +1153,8 @@ __global__ void __launch_bounds__(WG) k_eval_probe(const Cell* __restrict__ pc, 
     }
 }
 
-__global__ void __launch_bounds__(WG) k_eval_finish(const Cell* __restrict__ pc, int k, int64_t* dvec)
+__global__ void __launch_bounds__(WG) k_eval_finish(const Cell C, int k, int64_t* dvec)
 {
-    const Cell& C = *pc;
     const uint64_t upk = C.ctl->upk;
     for (int x = blockIdx.x * WG + threadIdx.x; x < k * C.n; x += gridDim.x * WG) {
         const int slot = x / C.n, p = x - slot * C.n;
@@ -1007,9 +1165,8 @@ __global__ void __launch_bounds__(WG) k_eval_finish(const Cell* __restrict__ pc,
 }
 
 // set up a single move for k_commit(api = 1)
-__global__ void k_set_move(const Cell* __restrict__ pc, int i, int p, int v)
+__global__ void k_set_move(const Cell C, int i, int p, int v)
 {
-    const Cell& C = *pc;
     Ctl* ctl = C.ctl;
     if (threadIdx.x != 0 || blockIdx.x != 0) return;
     const int oldv = C.code[(int64_t)i * C.n + p];

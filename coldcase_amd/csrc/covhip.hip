@@ -35,6 +35,8 @@ struct cov_cell {
     double ms[4] = {0, 0, 0, 0};
     int64_t launches[4] = {0, 0, 0, 0};
     std::vector<hipEvent_t> evs;
+    int64_t* trace = nullptr;             // device trace buffer
+    int64_t tcap = 0;
 };
 
 namespace {
@@ -262,7 +264,7 @@ int do_recount(cov_cell* h)
     hipEvent_t a = nullptr;
     RC(timed_begin(h, 3, &a));
     const int g = (int)std::min<int64_t>(h->plan.ntile, 2048);
-    hipLaunchKernelGGL(k_recount, dim3(g), dim3(WG), (size_t)h->plan.T * 4, h->stream, h->d, h->plan);
+    hipLaunchKernelGGL(k_recount, dim3(g), dim3(WG), (size_t)h->plan.T * 4, h->stream, h->h, h->plan);
     HCK(h, hipGetLastError());
     RC(timed_end(h, 3, a));
     int64_t zero = 0;
@@ -270,7 +272,7 @@ int do_recount(cov_cell* h)
     HCK(h, hipMemcpyAsync(CTL_PTR(h, ul_n), &zero, 8, hipMemcpyHostToDevice, h->stream));
     int32_t zi = 0;
     HCK(h, hipMemcpyAsync(CTL_PTR(h, ul_valid), &zi, 4, hipMemcpyHostToDevice, h->stream));
-    hipLaunchKernelGGL(k_derive, dim3(grid_for(h->h.nzw, WG, 1, 4096)), dim3(WG), 0, h->stream, h->d);
+    hipLaunchKernelGGL(k_derive, dim3(grid_for(h->h.nzw, WG, 1, 4096)), dim3(WG), 0, h->stream, h->h);
     HCK(h, hipGetLastError());
     return COV_OK;
 }
@@ -279,7 +281,7 @@ int do_derive(cov_cell* h)
 {
     int64_t zero = 0;
     HCK(h, hipMemcpyAsync(CTL_PTR(h, unc), &zero, 8, hipMemcpyHostToDevice, h->stream));
-    hipLaunchKernelGGL(k_derive, dim3(grid_for(h->h.nzw, WG, 1, 4096)), dim3(WG), 0, h->stream, h->d);
+    hipLaunchKernelGGL(k_derive, dim3(grid_for(h->h.nzw, WG, 1, 4096)), dim3(WG), 0, h->stream, h->h);
     HCK(h, hipGetLastError());
     return COV_OK;
 }
@@ -290,7 +292,7 @@ int list_rebuild(cov_cell* h)
     int32_t zi = 0;
     HCK(h, hipMemcpyAsync(CTL_PTR(h, ul_n), &zero, 8, hipMemcpyHostToDevice, h->stream));
     HCK(h, hipMemcpyAsync(CTL_PTR(h, ul_ovf), &zi, 4, hipMemcpyHostToDevice, h->stream));
-    hipLaunchKernelGGL(k_list_rebuild, dim3(grid_for(h->h.nzw, WG, 1, 4096)), dim3(WG), 0, h->stream, h->d);
+    hipLaunchKernelGGL(k_list_rebuild, dim3(grid_for(h->h.nzw, WG, 1, 4096)), dim3(WG), 0, h->stream, h->h);
     HCK(h, hipGetLastError());
     Ctl c;
     RC(read_ctl(h, &c));
@@ -305,11 +307,11 @@ int launch_step(cov_cell* h)
 {
     hipEvent_t a = nullptr;
     RC(timed_begin(h, 0, &a));
-    hipLaunchKernelGGL(k_eval, dim3(h->eval_groups), dim3(WG), 0, h->stream, h->d);
+    hipLaunchKernelGGL(k_eval, dim3(h->eval_groups), dim3(WG), 0, h->stream, h->h);
     HCK(h, hipGetLastError());
     RC(timed_end(h, 0, a));
     RC(timed_begin(h, 1, &a));
-    hipLaunchKernelGGL(k_commit, dim3(h->commit_groups), dim3(WG), 0, h->stream, h->d, 0);
+    hipLaunchKernelGGL(k_commit, dim3(h->commit_groups), dim3(WG), 0, h->stream, h->h, 0);
     HCK(h, hipGetLastError());
     RC(timed_end(h, 1, a));
     return COV_OK;
@@ -322,8 +324,8 @@ int ensure_graph(cov_cell* h)
     hipGraph_t g;
     HCK(h, hipStreamBeginCapture(h->stream, hipStreamCaptureModeRelaxed));
     for (int s = 0; s < GRAPH_STEPS; s++) {
-        hipLaunchKernelGGL(k_eval, dim3(h->eval_groups), dim3(WG), 0, h->stream, h->d);
-        hipLaunchKernelGGL(k_commit, dim3(h->commit_groups), dim3(WG), 0, h->stream, h->d, 0);
+        hipLaunchKernelGGL(k_eval, dim3(h->eval_groups), dim3(WG), 0, h->stream, h->h);
+        hipLaunchKernelGGL(k_commit, dim3(h->commit_groups), dim3(WG), 0, h->stream, h->h, 0);
     }
     HCK(h, hipStreamEndCapture(h->stream, &g));
     hipError_t e = hipGraphInstantiate(&h->gexec, g, nullptr, nullptr, 0);
@@ -419,8 +421,6 @@ int cov_create(int q, int n, int R, int M, cov_cell_t** out)
     AL(h->d_small, z.small);
 #undef AL
     C.scratch64 = h->d_small;
-    C.trace = nullptr;
-    C.tcap = 0;
 
     // position-subset tables
     std::vector<int> all(n);
@@ -491,7 +491,7 @@ int cov_create(int q, int n, int R, int M, cov_cell_t** out)
     const int64_t eval_waves = 32 * ((int64_t)s.F + (int64_t)(R + 1) * s.E) *
                                std::max<int64_t>(1, s.P / 64);
     h->eval_groups = grid_for(eval_waves / 4, 8, 16, 1024);
-    h->commit_groups = grid_for((int64_t)s.E * s.P, 4 * WG, 1, 512);
+    h->commit_groups = grid_for((int64_t)s.E * s.P, WG, 1, 2048);
     *out = h;
     return COV_OK;
 }
@@ -502,7 +502,7 @@ int cov_destroy(cov_cell_t* h)
     (void)hipStreamSynchronize(h->stream);
     if (h->gexec) (void)hipGraphExecDestroy(h->gexec);
     for (void* p : h->allocs) (void)hipFree(p);
-    if (h->h.trace) (void)hipFree(h->h.trace);
+    if (h->trace) (void)hipFree(h->trace);
     if (h->h.dist) (void)hipFree(h->h.dist);
     (void)hipStreamDestroy(h->stream);
     delete h;
@@ -523,7 +523,7 @@ int cov_set_code(cov_cell_t* h, const uint8_t* digits)
     for (int64_t k = 0; k < (int64_t)C.M * C.n; k++)
         if (digits[k] >= C.q) { h->err = "digit out of range"; return COV_EPARAM; }
     RC(upload(h, C.code, digits, (int64_t)C.M * C.n));
-    hipLaunchKernelGGL(k_pack_code, dim3(grid_for(C.M, WG, 1, 256)), dim3(WG), 0, h->stream, h->d);
+    hipLaunchKernelGGL(k_pack_code, dim3(grid_for(C.M, WG, 1, 256)), dim3(WG), 0, h->stream, h->h);
     HCK(h, hipGetLastError());
     RC(do_recount(h));
     HCK(h, hipMemcpyAsync(C.best, C.code, (size_t)C.M * C.n, hipMemcpyDeviceToDevice, h->stream));
@@ -570,7 +570,7 @@ int cov_ball_update(cov_cell_t* h, const int64_t* centers, int k, int delta)
         int64_t w = centers[0];
         uint64_t pk = 0;
         for (int j = C.n - 1; j >= 0; j--) { pk |= (uint64_t)(w % C.q) << (4 * j); w /= C.q; }
-        hipLaunchKernelGGL(k_ball_one, dim3(grid_for(C.V, WG, 1, 1024)), dim3(WG), 0, h->stream, h->d,
+        hipLaunchKernelGGL(k_ball_one, dim3(grid_for(C.V, WG, 1, 1024)), dim3(WG), 0, h->stream, h->h,
                            centers[0], pk, delta);
         HCK(h, hipGetLastError());
     } else {
@@ -578,7 +578,7 @@ int cov_ball_update(cov_cell_t* h, const int64_t* centers, int k, int delta)
         HCK(h, hipMallocAsync((void**)&dc, (size_t)k * 8, h->stream));
         HCK(h, hipMemcpyAsync(dc, centers, (size_t)k * 8, hipMemcpyHostToDevice, h->stream));
         hipLaunchKernelGGL(k_ball_multi, dim3(grid_for((int64_t)k * C.V, WG, 1, 4096)), dim3(WG), 0,
-                           h->stream, h->d, dc, k, delta);
+                           h->stream, h->h, dc, k, delta);
         HCK(h, hipGetLastError());
         HCK(h, hipFreeAsync(dc, h->stream));
         RC(do_derive(h));
@@ -592,7 +592,7 @@ int64_t cov_count_uncovered(cov_cell_t* h)
 {
     if (!h) return COV_EPARAM;
     HCK(h, hipMemsetAsync(h->d_small, 0, 8, h->stream));
-    hipLaunchKernelGGL(k_count_zero, dim3(grid_for(h->h.N / 8, WG, 1, 2048)), dim3(WG), 0, h->stream, h->d,
+    hipLaunchKernelGGL(k_count_zero, dim3(grid_for(h->h.N / 8, WG, 1, 2048)), dim3(WG), 0, h->stream, h->h,
                        h->d_small);
     HCK(h, hipGetLastError());
     int64_t r = 0;
@@ -607,7 +607,7 @@ int64_t cov_gather_uncovered(cov_cell_t* h, int64_t* out, int64_t cap)
     const Cell& C = h->h;
     int64_t *dz = nullptr, *dout = nullptr;
     HCK(h, hipMallocAsync((void**)&dz, (size_t)C.nblk * 8 + 8, h->stream));
-    hipLaunchKernelGGL(k_block_zeros, dim3(grid_for(C.nblk, 1, 1, 8192)), dim3(WG), 0, h->stream, h->d, dz);
+    hipLaunchKernelGGL(k_block_zeros, dim3(grid_for(C.nblk, 1, 1, 8192)), dim3(WG), 0, h->stream, h->h, dz);
     HCK(h, hipGetLastError());
     std::vector<int64_t> bz((size_t)C.nblk);
     HCK(h, hipMemcpyAsync(bz.data(), dz, (size_t)C.nblk * 8, hipMemcpyDeviceToHost, h->stream));
@@ -618,7 +618,7 @@ int64_t cov_gather_uncovered(cov_cell_t* h, int64_t* out, int64_t cap)
     if (m > 0) {
         HCK(h, hipMemcpyAsync(dz, bz.data(), (size_t)C.nblk * 8, hipMemcpyHostToDevice, h->stream));
         HCK(h, hipMallocAsync((void**)&dout, (size_t)m * 8, h->stream));
-        hipLaunchKernelGGL(k_block_emit, dim3(grid_for(C.nblk, 1, 1, 8192)), dim3(WG), 0, h->stream, h->d, dz,
+        hipLaunchKernelGGL(k_block_emit, dim3(grid_for(C.nblk, 1, 1, 8192)), dim3(WG), 0, h->stream, h->h, dz,
                            dout, m);
         HCK(h, hipGetLastError());
         HCK(h, hipMemcpyAsync(out, dout, (size_t)m * 8, hipMemcpyDeviceToHost, h->stream));
@@ -632,7 +632,7 @@ int64_t cov_gather_uncovered(cov_cell_t* h, int64_t* out, int64_t cap)
 int64_t cov_first_uncovered_from(cov_cell_t* h, int64_t start)
 {
     if (!h || start < 0 || start >= h->h.N) return COV_EPARAM;
-    hipLaunchKernelGGL(k_first_from, dim3(1), dim3(WG), 0, h->stream, h->d, start, h->d_small);
+    hipLaunchKernelGGL(k_first_from, dim3(1), dim3(WG), 0, h->stream, h->h, start, h->d_small);
     HCK(h, hipGetLastError());
     int64_t r = 0;
     HCK(h, hipMemcpyAsync(&r, h->d_small, 8, hipMemcpyDeviceToHost, h->stream));
@@ -666,11 +666,11 @@ int cov_eval_moves(cov_cell_t* h, const uint8_t* u, int k, const int32_t* cw, in
     HCK(h, hipMemcpyAsync(CTL_PTR(h, upk), &upk, 8, hipMemcpyHostToDevice, h->stream));
     HCK(h, hipMemsetAsync(C.acc, 0, (size_t)k * ACCW * 4, h->stream));
     const int64_t units = (int64_t)k * (C.F + (int64_t)C.n * C.E);
-    hipLaunchKernelGGL(k_eval_probe, dim3(grid_for(units, 4 * 8, 1, 2048)), dim3(WG), 0, h->stream, h->d, k);
+    hipLaunchKernelGGL(k_eval_probe, dim3(grid_for(units, 4 * 8, 1, 2048)), dim3(WG), 0, h->stream, h->h, k);
     HCK(h, hipGetLastError());
     int64_t* dd = nullptr;
     HCK(h, hipMallocAsync((void**)&dd, (size_t)k * C.n * 8, h->stream));
-    hipLaunchKernelGGL(k_eval_finish, dim3(grid_for((int64_t)k * C.n, WG, 1, 256)), dim3(WG), 0, h->stream, h->d,
+    hipLaunchKernelGGL(k_eval_finish, dim3(grid_for((int64_t)k * C.n, WG, 1, 256)), dim3(WG), 0, h->stream, h->h,
                        k, dd);
     HCK(h, hipGetLastError());
     HCK(h, hipMemcpyAsync(dvec, dd, (size_t)k * C.n * 8, hipMemcpyDeviceToHost, h->stream));
@@ -684,9 +684,9 @@ int cov_commit(cov_cell_t* h, int i, int p, int v, int64_t* dunc)
     if (!h) return COV_EPARAM;
     const Cell& C = h->h;
     if (i < 0 || i >= C.M || p < 0 || p >= C.n || v < 0 || v >= C.q) return COV_EPARAM;
-    hipLaunchKernelGGL(k_set_move, dim3(1), dim3(64), 0, h->stream, h->d, i, p, v);
+    hipLaunchKernelGGL(k_set_move, dim3(1), dim3(64), 0, h->stream, h->h, i, p, v);
     HCK(h, hipGetLastError());
-    hipLaunchKernelGGL(k_commit, dim3(h->commit_groups), dim3(WG), 0, h->stream, h->d, 1);
+    hipLaunchKernelGGL(k_commit, dim3(h->commit_groups), dim3(WG), 0, h->stream, h->h, 1);
     HCK(h, hipGetLastError());
     Ctl c;
     RC(read_ctl(h, &c));
@@ -704,7 +704,7 @@ int cov_distance_transform(cov_cell_t* h, uint8_t* dist, int64_t* hist, int* rad
     }
     const int gfill = grid_for(C.N / 16, WG, 1, 4096);
     hipLaunchKernelGGL(k_dist_fill, dim3(gfill), dim3(WG), 0, h->stream, C.dist, C.N);
-    hipLaunchKernelGGL(k_dist_seed, dim3(grid_for(C.M, WG, 1, 64)), dim3(WG), 0, h->stream, h->d, C.dist);
+    hipLaunchKernelGGL(k_dist_seed, dim3(grid_for(C.M, WG, 1, 64)), dim3(WG), 0, h->stream, h->h, C.dist);
     HCK(h, hipGetLastError());
     // inner axes in LDS tiles of q^a words, then the outer axes one by one
     int a = 0;
@@ -791,13 +791,13 @@ int cov_search_init(cov_cell_t* h, uint64_t seed, const uint8_t* seed_code, int 
         HCK(h, hipMemsetAsync(C.cnt32, 0, (size_t)((C.N + 1) / 2) * 4, h->stream));
         HCK(h, hipMemcpyAsync(dc, si.data(), (size_t)mseed * 8, hipMemcpyHostToDevice, h->stream));
         hipLaunchKernelGGL(k_ball_multi, dim3(grid_for((int64_t)mseed * C.V, WG, 1, 4096)), dim3(WG), 0, h->stream,
-                           h->d, dc, mseed, 1);
+                           h->h, dc, mseed, 1);
         HCK(h, hipGetLastError());
         RC(do_derive(h));
         std::vector<int64_t> uc((size_t)mseed);
         while (mseed > M) {
             HCK(h, hipMemcpyAsync(dc, si.data(), (size_t)mseed * 8, hipMemcpyHostToDevice, h->stream));
-            hipLaunchKernelGGL(k_unique_cover, dim3(std::min(mseed, 4096)), dim3(WG), 0, h->stream, h->d, dc, mseed,
+            hipLaunchKernelGGL(k_unique_cover, dim3(std::min(mseed, 4096)), dim3(WG), 0, h->stream, h->h, dc, mseed,
                                du);
             HCK(h, hipGetLastError());
             HCK(h, hipMemcpyAsync(uc.data(), du, (size_t)mseed * 8, hipMemcpyDeviceToHost, h->stream));
@@ -805,7 +805,7 @@ int cov_search_init(cov_cell_t* h, uint64_t seed, const uint8_t* seed_code, int 
             int worst = 0;
             for (int i = 1; i < mseed; i++)
                 if (uc[i] < uc[worst]) worst = i;
-            hipLaunchKernelGGL(k_ball_one, dim3(grid_for(C.V, WG, 1, 1024)), dim3(WG), 0, h->stream, h->d,
+            hipLaunchKernelGGL(k_ball_one, dim3(grid_for(C.V, WG, 1, 1024)), dim3(WG), 0, h->stream, h->h,
                                si[worst], pack_of(&sc[(size_t)worst * n]), -1);
             HCK(h, hipGetLastError());
             sc.erase(sc.begin() + (size_t)worst * n, sc.begin() + (size_t)(worst + 1) * n);
@@ -816,7 +816,7 @@ int cov_search_init(cov_cell_t* h, uint64_t seed, const uint8_t* seed_code, int 
         (void)hipFree(dc);
         (void)hipFree(du);
         RC(upload(h, C.code, sc.data(), (int64_t)M * n));
-        hipLaunchKernelGGL(k_pack_code, dim3(grid_for(M, WG, 1, 256)), dim3(WG), 0, h->stream, h->d);
+        hipLaunchKernelGGL(k_pack_code, dim3(grid_for(M, WG, 1, 256)), dim3(WG), 0, h->stream, h->h);
         HCK(h, hipGetLastError());
         RC(do_recount(h));
     } else {
@@ -825,21 +825,21 @@ int cov_search_init(cov_cell_t* h, uint64_t seed, const uint8_t* seed_code, int 
         if (have < M && greedy) {
             // greedy placement on uncovered words (:1197-1234)
             HCK(h, hipMemsetAsync(C.cnt32, 0, (size_t)((C.N + 1) / 2) * 4, h->stream));
-            hipLaunchKernelGGL(k_pack_code, dim3(grid_for(M, WG, 1, 256)), dim3(WG), 0, h->stream, h->d);
+            hipLaunchKernelGGL(k_pack_code, dim3(grid_for(M, WG, 1, 256)), dim3(WG), 0, h->stream, h->h);
             HCK(h, hipGetLastError());
             RC(do_derive(h));
             for (int i = 0; i < have; i++) {
-                hipLaunchKernelGGL(k_ball_cw, dim3(grid_for(C.V, WG, 1, 1024)), dim3(WG), 0, h->stream, h->d, i, 1);
+                hipLaunchKernelGGL(k_ball_cw, dim3(grid_for(C.V, WG, 1, 1024)), dim3(WG), 0, h->stream, h->h, i, 1);
                 HCK(h, hipGetLastError());
             }
             for (int i = have; i < M; i++) {
-                hipLaunchKernelGGL(k_greedy_pick, dim3(1), dim3(WG), 0, h->stream, h->d, i);
-                hipLaunchKernelGGL(k_ball_cw, dim3(grid_for(C.V, WG, 1, 1024)), dim3(WG), 0, h->stream, h->d, i, 1);
+                hipLaunchKernelGGL(k_greedy_pick, dim3(1), dim3(WG), 0, h->stream, h->h, i);
+                hipLaunchKernelGGL(k_ball_cw, dim3(grid_for(C.V, WG, 1, 1024)), dim3(WG), 0, h->stream, h->h, i, 1);
                 HCK(h, hipGetLastError());
             }
         } else {
-            hipLaunchKernelGGL(k_init_random, dim3(1), dim3(64), 0, h->stream, h->d, have);
-            hipLaunchKernelGGL(k_pack_code, dim3(grid_for(M, WG, 1, 256)), dim3(WG), 0, h->stream, h->d);
+            hipLaunchKernelGGL(k_init_random, dim3(1), dim3(64), 0, h->stream, h->h, have);
+            hipLaunchKernelGGL(k_pack_code, dim3(grid_for(M, WG, 1, 256)), dim3(WG), 0, h->stream, h->h);
             HCK(h, hipGetLastError());
             RC(do_recount(h));
         }
@@ -857,7 +857,7 @@ int cov_search_launch(cov_cell_t* h, int64_t nsteps, int mode)
 {
     if (!h || nsteps < 0 || (mode != COV_MODE_PLAIN && mode != COV_MODE_GRAPH)) return COV_EPARAM;
     if (nsteps == 0) return COV_OK;
-    hipLaunchKernelGGL(k_begin_run, dim3(1), dim3(WG), 0, h->stream, h->d, nsteps);
+    hipLaunchKernelGGL(k_begin_run, dim3(1), dim3(WG), 0, h->stream, h->h, nsteps);
     HCK(h, hipGetLastError());
     int64_t left = nsteps;
     if (mode == COV_MODE_GRAPH && !h->timing) {
@@ -874,21 +874,19 @@ int cov_search_launch(cov_cell_t* h, int64_t nsteps, int mode)
 int64_t cov_search_run(cov_cell_t* h, int64_t nsteps, int mode, int64_t* trace)
 {
     if (!h || nsteps < 0) return COV_EPARAM;
-    Cell& C = h->h;
-    if (trace) {
-        if (C.tcap < nsteps) {
-            if (C.trace) (void)hipFree(C.trace);
-            C.trace = nullptr;
-            HCK(h, hipMalloc((void**)&C.trace, (size_t)nsteps * TRW * 8));
-            C.tcap = nsteps;
-            RC(push_cell(h));
-        }
-    } else if (C.tcap) {
-        // keep the buffer but stop writing records
-        Cell tmp = C;
-        tmp.tcap = 0;
-        HCK(h, hipMemcpyAsync(h->d, &tmp, sizeof(Cell), hipMemcpyHostToDevice, h->stream));
+    // the trace buffer and its capacity live in the control block, so captured
+    // graphs stay valid whether or not a run records a trace
+    if (trace && h->tcap < nsteps) {
+        if (h->trace) (void)hipFree(h->trace);
+        h->trace = nullptr;
+        h->tcap = 0;
+        HCK(h, hipMalloc((void**)&h->trace, (size_t)nsteps * TRW * 8));
+        h->tcap = nsteps;
     }
+    int64_t* tp = trace ? h->trace : nullptr;
+    const int64_t tc = trace ? h->tcap : 0;
+    HCK(h, hipMemcpyAsync(CTL_PTR(h, trace), &tp, 8, hipMemcpyHostToDevice, h->stream));
+    HCK(h, hipMemcpyAsync(CTL_PTR(h, tcap), &tc, 8, hipMemcpyHostToDevice, h->stream));
     RC(cov_search_launch(h, nsteps, mode));
     HCK(h, hipStreamSynchronize(h->stream));
     Ctl c;
@@ -897,11 +895,10 @@ int64_t cov_search_run(cov_cell_t* h, int64_t nsteps, int mode, int64_t* trace)
     if (c.fault == 2) { h->err = "candidate sample larger than the rng ring allows"; return COV_ELIMIT; }
     const int64_t ran = c.tpos;
     if (trace && ran > 0) {
-        HCK(h, hipMemcpyAsync(trace, C.trace, (size_t)std::min(ran, nsteps) * TRW * 8, hipMemcpyDeviceToHost,
+        HCK(h, hipMemcpyAsync(trace, h->trace, (size_t)std::min(ran, nsteps) * TRW * 8, hipMemcpyDeviceToHost,
                               h->stream));
         HCK(h, hipStreamSynchronize(h->stream));
     }
-    if (!trace && C.tcap) RC(push_cell(h));
     return ran;
 }
 
@@ -946,6 +943,15 @@ int cov_timing_read(cov_cell_t* h, double* ms, int64_t* launches, int nkinds)
         if (ms) ms[k] = h->ms[k];
         if (launches) launches[k] = h->launches[k];
     }
+    return COV_OK;
+}
+
+int cov_search_phases(cov_cell_t* h, double* us, int n)
+{
+    if (!h || n < 0 || n > 12 || (n && !us)) return COV_EPARAM;
+    Ctl c;
+    RC(read_ctl(h, &c));
+    for (int k = 0; k < n; k++) us[k] = (double)c.tph[k] * 0.01;   // 100 MHz wall clock
     return COV_OK;
 }
 

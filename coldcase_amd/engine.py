@@ -31,7 +31,7 @@ EXPORTS = ['cov_version', 'cov_strerror', 'cov_mem_bytes', 'cov_create', 'cov_de
            'cov_first_uncovered_from', 'cov_read_cnt', 'cov_eval_moves', 'cov_commit',
            'cov_distance_transform', 'cov_search_init', 'cov_search_params',
            'cov_search_run', 'cov_search_launch', 'cov_search_state', 'cov_set_launch',
-           'cov_timing', 'cov_timing_read', 'cov_search_bytes']
+           'cov_timing', 'cov_timing_read', 'cov_search_phases', 'cov_search_bytes']
 
 
 class CovError(RuntimeError):
@@ -90,6 +90,7 @@ def lib():
         'cov_set_launch': (i32, [vp, i32, i32]),
         'cov_timing': (i32, [vp, i32]),
         'cov_timing_read': (i32, [vp, vp, vp, i32]),
+        'cov_search_phases': (i32, [vp, vp, i32]),
         'cov_search_bytes': (i32, [vp, vp, vp]),
     }
     for name, (res, args) in sig.items():
@@ -258,6 +259,13 @@ class Cell:
         n = np.zeros(4, np.int64)
         self._check(lib().cov_timing_read(self.h, _p(ms), _p(n), 4), 'cov_timing_read')
         return ms, n
+
+    def phases(self):
+        """Microseconds in the narrow phases since search_init."""
+        us = np.zeros(12, np.float64)
+        self._check(lib().cov_search_phases(self.h, _p(us), 12), 'cov_search_phases')
+        return dict(zip(['select', 'post', 'begin', 'pick', 'list', 'shuffle', 'rng', 'kick',
+                         'eval_span', 'commit_span', 'gap_c2e', 'gap_e2c'], us.tolist()))
 
     def search_bytes(self):
         e = np.zeros(1, np.int64)

@@ -138,6 +138,12 @@ int     cov_set_launch(cov_cell_t *h, int eval_groups, int commit_groups);
  * 0 = evaluator, 1 = commit, 2 = distance transform pass, 3 = recount. */
 int     cov_timing(cov_cell_t *h, int on);
 int     cov_timing_read(cov_cell_t *h, double *ms, int64_t *launches, int nkinds);
+/* Diagnostics: microseconds spent since cov_search_init, from the device wall
+ * clock, in: 0 select, 1 post, 2 begin, 3 pick, 4 list upkeep, 5 shuffle,
+ * 6 rng refill, 7 kicks, 8 evaluator span before select, 9 commit span
+ * before post, 10 gap commit -> next evaluator, 11 gap evaluator -> commit
+ * (n <= 12 entries). */
+int     cov_search_phases(cov_cell_t *h, double *us, int n);
 /* Algorithmic bytes moved by the evaluator and commit kernels since the last
  * cov_search_init (2 B per counter read or update, as the uint16 cnt). */
 int     cov_search_bytes(cov_cell_t *h, int64_t *eval_bytes, int64_t *commit_bytes);
