@@ -5,7 +5,9 @@ Workload (a "step" is one tabu move of the device-resident search):
   q=6, n=9, R=4 (10,077,696 words, ball 90,196), M=660 codewords, uniform
   random initial code from seed 1, tabu tenure 10, stall 5000, kick 2, all
   candidates evaluated; steps replayed from a captured hipGraph.
-  W warmup steps, then K timed steps; inputs and state resident in HBM.
+  W warmup steps, then K timed steps (10^6 by default, the cell's stated
+  run length); inputs and state resident in HBM.  The same cell then runs
+  a shorter window with one launch sequence per step (plain mode).
 
 Printed: one JSON line with the metric, the evaluator kernel's roofline
 (algorithmic bytes per launch / average launch time from HIP events on the
@@ -14,6 +16,12 @@ solver compiled from source, else the oracle, on the host's cores over a
 bounded prefix of the same search).
 
   python bench.py [--gpus 1] [--steps K] [--warmup W] [--mode graph|plain]
+
+Secondary workloads (one JSON line each, not the headline):
+  --workload transform   K7(9,4) M=1480 random code: the uint8 axis-DP
+                         distance transform (2*n*q^n algorithmic bytes) and
+                         the uint16 coverage recount (2*q^n + 4*M*V bytes,
+                         M*V atomics), K timed repetitions of each
 """
 import argparse
 import json
@@ -58,33 +66,88 @@ def cpu_baseline(cores, sample_iters):
                       '%.1f s with init), --preset p5b --threads %d' % (it, tsec, wall, cores)}
 
 
-def traffic_from_profile():
-    """HBM bytes per evaluator launch measured by rocprofv3 PMC counters,
-    committed under profiles/ (None when absent)."""
-    p = os.path.join(ROOT, 'profiles', 'eval_traffic_K6_9_4.json')
-    if not os.path.exists(p):
-        return None
-    try:
-        return json.load(open(p)).get('bytes_per_launch')
-    except Exception:
-        return None
+def bench_transform(a):
+    """K7(9,4), M=1480 uniform random codewords (numpy seed 3)."""
+    import numpy as np
+    import torch
+    from coldcase_amd import Cell
+    torch.cuda.set_device(0)
+    q, n, R, M, V = 7, 9, 4, 1480, 182791
+    N = q ** n
+    code = np.random.default_rng(3).integers(0, q, size=(M, n)).astype(np.uint8)
+    cell = Cell(q, n, R, M)
+    if a.lds:
+        cell.set_transform_tile(a.lds)
+    cell.set_code(code)
+    for _ in range(max(a.warmup, 1)):
+        cell.distance_transform()
+        cell.recount()
+    cell.sync()
+    steps = a.steps
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        _, hist, rad = cell.distance_transform()
+    cell.sync()
+    el_t = (time.perf_counter() - t0) / steps
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        cell.recount()
+    cell.sync()
+    el_r = (time.perf_counter() - t0) / steps
+    cell.timing(True)
+    for _ in range(steps):
+        cell.distance_transform()
+        cell.recount()
+    ms, launches = cell.timing_read()
+    cell.timing(False)
+    dt_us = ms[2] * 1e3 / steps            # all passes of one transform
+    rc_us = ms[3] * 1e3 / max(int(launches[3]), 1)
+    alg_t = 2 * n * N
+    alg_r = 2 * N + 4 * M * V
+    ach = alg_t / (dt_us * 1e-6) / 1e9
+    out = {
+        'metric': 'axis-DP distance transform on K7(9,4)', 'value': round(ach, 1), 'unit': 'GB/s (algorithmic)',
+        'n_gpus': 1, 'steps': steps, 'warmup': a.warmup, 'ms_per_step': round(el_t * 1e3, 4),
+        'higher_is_better': True, 'scaling': 'weak', 'vs_baseline': None, 'dtype': 'u8', 'data': 'synthetic',
+        'config': {'workload': 'K7(9,4) M=1480 random code (numpy seed 3): distance transform + recount',
+                   'words': N, 'passes_per_transform': int(launches[2]) // steps,
+                   'hist': [int(x) for x in hist], 'radius': rad},
+        'roofline': {'bound': 'hbm', 'achieved': round(ach, 1), 'peak': HBM_PEAK_GBS, 'unit': 'GB/s',
+                     'frac': round(ach / HBM_PEAK_GBS, 4), 'traffic': None, 'kernel': 'k_dt_inner+k_dt_outer',
+                     'avg_transform_us': round(dt_us, 2), 'alg_bytes_per_transform': alg_t},
+        'recount': {'avg_us': round(rc_us, 2), 'wall_ms': round(el_r * 1e3, 4), 'alg_bytes': alg_r,
+                    'achieved_GBs': round(alg_r / (rc_us * 1e-6) / 1e9, 1),
+                    'atomics_per_s': round(M * V / (rc_us * 1e-6), 1)},
+    }
+    print(json.dumps(out))
+    cell.close()
+    return 0
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument('--gpus', type=int, default=1)
-    ap.add_argument('--steps', type=int, default=20000)
+    ap.add_argument('--steps', type=int, default=1000000)
     ap.add_argument('--warmup', type=int, default=1000)
     ap.add_argument('--mode', choices=['graph', 'plain'], default='graph')
     ap.add_argument('--timing-steps', type=int, default=300)
-    ap.add_argument('--cpu-iters', type=int, default=400)
+    ap.add_argument('--plain-steps', type=int, default=20000)
+    ap.add_argument('--cpu-iters', type=int, default=4000)
     ap.add_argument('--no-cpu', action='store_true')
+    ap.add_argument('--workload', choices=['search', 'transform'], default='search')
+    ap.add_argument('--lds', type=int, default=0, help='transform tile LDS bytes (0 = default)')
     a = ap.parse_args()
     if a.gpus != 1:
         print('bench.py measures one GPU: every box here has one GPU, and the search '
               'path shards by independent cells, not within one (--gpus %d refused)' % a.gpus,
               file=sys.stderr)
         return 2
+    if a.workload == 'transform':
+        if a.steps == ap.get_default('steps'):
+            a.steps = 20
+        if a.warmup == ap.get_default('warmup'):
+            a.warmup = 2
+        return bench_transform(a)
 
     import torch
     from coldcase_amd import Cell
@@ -109,6 +172,17 @@ def main():
     el = t1 - t0
     value = moves / el
 
+    # the same cell, one launch sequence per step
+    plain = None
+    if a.plain_steps > 0 and mode == MODE_GRAPH and not st1['done']:
+        p0 = cell.state()['it']
+        t0 = time.perf_counter()
+        cell.launch(a.plain_steps, mode=MODE_PLAIN)
+        cell.sync()
+        el_p = time.perf_counter() - t0
+        pm = cell.state()['it'] - p0
+        plain = {'value': round(pm / el_p, 1), 'steps': pm, 'ms_per_step': round(el_p * 1e3 / max(pm, 1), 5)}
+
     # roofline of the evaluator: per-launch HIP events on the cell's stream
     e0, _ = cell.search_bytes()
     cell.timing(True)
@@ -121,7 +195,7 @@ def main():
     bytes_per_launch = (e1 - e0) / max(n_eval, 1)
     achieved = bytes_per_launch / avg_s / 1e9 if avg_s > 0 else 0.0
     roof = {'bound': 'hbm', 'achieved': round(achieved, 1), 'peak': HBM_PEAK_GBS, 'unit': 'GB/s',
-            'frac': round(achieved / HBM_PEAK_GBS, 4), 'traffic': traffic_from_profile(),
+            'frac': round(achieved / HBM_PEAK_GBS, 4), 'traffic': None,
             'kernel': 'k_eval', 'avg_launch_us': round(avg_s * 1e6, 2),
             'alg_bytes_per_launch': int(bytes_per_launch),
             'commit_avg_us': round(ms[1] / 1e3 / max(int(launches[1]), 1) * 1e6, 2)}
@@ -132,7 +206,7 @@ def main():
         cpu = cpu_baseline(cores, a.cpu_iters)
 
     out = {
-        'metric': 'tabu moves/s on K6(9,4)',
+        'metric': 'tabu moves/s on K6(9,4); batched vs plain solves/s; p50/p99 solve latency',
         'value': round(value, 1),
         'unit': 'moves/s',
         'n_gpus': 1,
@@ -149,6 +223,7 @@ def main():
                    'words': Q ** N_, 'ball': 90196, 'M': M_, 'mode': a.mode,
                    'uncovered_at_window': [st0['uncovered'], st1['uncovered']],
                    'best_uncovered': st1['best_uncovered']},
+        'plain_mode': plain,
         'roofline': roof,
         'cpu_baseline': cpu,
     }

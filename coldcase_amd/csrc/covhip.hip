@@ -34,9 +34,14 @@ struct cov_cell {
     bool timing = false;
     double ms[4] = {0, 0, 0, 0};
     int64_t launches[4] = {0, 0, 0, 0};
-    std::vector<hipEvent_t> evs;
+    std::vector<hipEvent_t> evs;          // event pool
+    struct Pend { int kind; hipEvent_t a, b; };
+    std::vector<Pend> pend;               // recorded, not yet read
     int64_t* trace = nullptr;             // device trace buffer
     int64_t tcap = 0;
+    // distance transform
+    uint32_t* cwb = nullptr;              // codeword bitmap, all zero between calls
+    int dt_lds = 0;                       // LDS bytes per transform tile
 };
 
 namespace {
@@ -151,9 +156,10 @@ Sizes sizes_of(const Shape& s)
 int64_t total_bytes(const Shape& s)
 {
     const Sizes z = sizes_of(s);
+    // plus the distance array and codeword bitmap the transform allocates on first use
     return z.cnt + 3 * z.plane + z.blk + 2 * z.code + z.cpk + z.cidx + z.tabu + z.fsub + z.psub +
            z.bsub + z.ssub + z.ul_idx + z.ul_pk + z.shuf + z.acc + z.slot_cw + z.slot_pos + z.rbuf +
-           z.ctl + z.cell + z.small;
+           z.ctl + z.cell + z.small + (s.N + 64) + ((s.N >> 5) + 2) * 4;
 }
 
 template <typename T>
@@ -234,11 +240,41 @@ void splitmix_seed(uint64_t seed, uint64_t s[4])
         if (rc_ != COV_OK) return rc_; \
     } while (0)
 
+// Per-launch device time: an event before and after the launch on the cell's
+// stream, read back in batches (no synchronisation between launches).
+int timing_collect(cov_cell* h)
+{
+    if (h->pend.empty()) return COV_OK;
+    HCK(h, hipEventSynchronize(h->pend.back().b));
+    for (const auto& p : h->pend) {
+        float ms = 0;
+        HCK(h, hipEventElapsedTime(&ms, p.a, p.b));
+        h->ms[p.kind] += ms;
+        h->launches[p.kind] += 1;
+        h->evs.push_back(p.a);
+        h->evs.push_back(p.b);
+    }
+    h->pend.clear();
+    return COV_OK;
+}
+
+int pool_event(cov_cell* h, hipEvent_t* e)
+{
+    if (h->evs.empty()) {
+        HCK(h, hipEventCreate(e));
+    } else {
+        *e = h->evs.back();
+        h->evs.pop_back();
+    }
+    return COV_OK;
+}
+
 int timed_begin(cov_cell* h, int kind, hipEvent_t* a)
 {
     (void)kind;
     if (!h->timing) return COV_OK;
-    HCK(h, hipEventCreate(a));
+    if (h->pend.size() >= 4096) RC(timing_collect(h));
+    RC(pool_event(h, a));
     HCK(h, hipEventRecord(*a, h->stream));
     return COV_OK;
 }
@@ -247,15 +283,9 @@ int timed_end(cov_cell* h, int kind, hipEvent_t a)
 {
     if (!h->timing) return COV_OK;
     hipEvent_t b;
-    HCK(h, hipEventCreate(&b));
+    RC(pool_event(h, &b));
     HCK(h, hipEventRecord(b, h->stream));
-    HCK(h, hipEventSynchronize(b));
-    float ms = 0;
-    HCK(h, hipEventElapsedTime(&ms, a, b));
-    h->ms[kind] += ms;
-    h->launches[kind] += 1;
-    (void)hipEventDestroy(a);
-    (void)hipEventDestroy(b);
+    h->pend.push_back({kind, a, b});
     return COV_OK;
 }
 
@@ -504,6 +534,9 @@ int cov_destroy(cov_cell_t* h)
     for (void* p : h->allocs) (void)hipFree(p);
     if (h->trace) (void)hipFree(h->trace);
     if (h->h.dist) (void)hipFree(h->h.dist);
+    if (h->cwb) (void)hipFree(h->cwb);
+    for (const auto& p : h->pend) { (void)hipEventDestroy(p.a); (void)hipEventDestroy(p.b); }
+    for (hipEvent_t e : h->evs) (void)hipEventDestroy(e);
     (void)hipStreamDestroy(h->stream);
     delete h;
     return COV_OK;
@@ -702,37 +735,40 @@ int cov_distance_transform(cov_cell_t* h, uint8_t* dist, int64_t* hist, int* rad
         HCK(h, hipMalloc((void**)&C.dist, (size_t)C.N + 64));
         RC(push_cell(h));
     }
-    const int gfill = grid_for(C.N / 16, WG, 1, 4096);
-    hipLaunchKernelGGL(k_dist_fill, dim3(gfill), dim3(WG), 0, h->stream, C.dist, C.N);
-    hipLaunchKernelGGL(k_dist_seed, dim3(grid_for(C.M, WG, 1, 64)), dim3(WG), 0, h->stream, h->h, C.dist);
+    if (!h->cwb) {
+        const size_t wb = ((size_t)(C.N >> 5) + 2) * 4;
+        HCK(h, hipMalloc((void**)&h->cwb, wb));
+        HCK(h, hipMemsetAsync(h->cwb, 0, wb, h->stream));
+    }
+    DtPass pass[MAXN + 1];
+    const int np = dt_plan(C.q, C.n, C.N, h->dt_lds ? h->dt_lds : DT_LDS_DEFAULT, pass);
+    if (np <= 0) {
+        h->err = "distance transform: no tile plan for this shape and LDS budget";
+        return COV_ELIMIT;
+    }
+    unsigned long long* dh = reinterpret_cast<unsigned long long*>(h->d_small);
+    HCK(h, hipMemsetAsync(dh, 0, DT_HB * 8, h->stream));
+    const int gm = grid_for(C.M, WG, 1, 64);
+    hipLaunchKernelGGL(k_cwb_mark, dim3(gm), dim3(WG), 0, h->stream, h->h, h->cwb, 1);
     HCK(h, hipGetLastError());
-    // inner axes in LDS tiles of q^a words, then the outer axes one by one
-    int a = 0;
-    int64_t T = 1;
-    while (a < C.n && T * C.q <= DT_TILE) { T *= C.q; a++; }
-    hipEvent_t ev = nullptr;
-    if (a > 0) {
+    for (int i = 0; i < np; i++) {
+        const DtPass& p = pass[i];
+        const int g = (int)std::min<int64_t>(p.ntiles, 8192);
+        hipEvent_t ev = nullptr;
         RC(timed_begin(h, 2, &ev));
-        const int64_t per = DT_TILE / T;
-        hipLaunchKernelGGL(k_axis_inner, dim3(grid_for(C.N, (int)(per * T), 1, 8192)), dim3(WG), 0, h->stream,
-                           C.dist, C.N, C.q, a, T);
+        if (i == 0)
+            hipLaunchKernelGGL(k_dt_inner, dim3(g), dim3(WG), (size_t)p.P, h->stream, p, C.dist, h->cwb, dh);
+        else
+            hipLaunchKernelGGL(k_dt_outer, dim3(g), dim3(WG), (size_t)(p.rows * p.P), h->stream, p, C.dist, dh);
         HCK(h, hipGetLastError());
         RC(timed_end(h, 2, ev));
+        if (i == 0) {
+            hipLaunchKernelGGL(k_cwb_mark, dim3(gm), dim3(WG), 0, h->stream, h->h, h->cwb, 0);
+            HCK(h, hipGetLastError());
+        }
     }
-    for (int j = C.n - 1 - a; j >= 0; j--) {
-        const int64_t s = C.pw[j];
-        RC(timed_begin(h, 2, &ev));
-        hipLaunchKernelGGL(k_axis_outer, dim3(grid_for(C.N / C.q / 16 + 1, WG, 1, 8192)), dim3(WG), 0, h->stream,
-                           C.dist, C.N, C.q, s);
-        HCK(h, hipGetLastError());
-        RC(timed_end(h, 2, ev));
-    }
-    HCK(h, hipMemsetAsync(h->d_small, 0, (MAXN + 2) * 8, h->stream));
-    hipLaunchKernelGGL(k_dist_hist, dim3(grid_for(C.N / 16, WG, 1, 2048)), dim3(WG), 0, h->stream, C.dist, C.N,
-                       C.n, reinterpret_cast<unsigned long long*>(h->d_small));
-    HCK(h, hipGetLastError());
-    int64_t hh[MAXN + 2];
-    HCK(h, hipMemcpyAsync(hh, h->d_small, (MAXN + 2) * 8, hipMemcpyDeviceToHost, h->stream));
+    int64_t hh[DT_HB];
+    HCK(h, hipMemcpyAsync(hh, dh, DT_HB * 8, hipMemcpyDeviceToHost, h->stream));
     if (dist) HCK(h, hipMemcpyAsync(dist, C.dist, (size_t)C.N, hipMemcpyDeviceToHost, h->stream));
     HCK(h, hipStreamSynchronize(h->stream));
     int rad = 0;
@@ -928,9 +964,20 @@ int cov_set_launch(cov_cell_t* h, int eval_groups, int commit_groups)
     return COV_OK;
 }
 
+int cov_set_transform_tile(cov_cell_t* h, int lds_bytes)
+{
+    if (!h || lds_bytes < 0) return COV_EPARAM;
+    DtPass pass[MAXN + 1];
+    const Cell& C = h->h;
+    if (lds_bytes && dt_plan(C.q, C.n, C.N, lds_bytes, pass) <= 0) return COV_EPARAM;
+    h->dt_lds = lds_bytes;
+    return COV_OK;
+}
+
 int cov_timing(cov_cell_t* h, int on)
 {
     if (!h) return COV_EPARAM;
+    RC(timing_collect(h));
     h->timing = on != 0;
     for (int k = 0; k < 4; k++) { h->ms[k] = 0; h->launches[k] = 0; }
     return COV_OK;
@@ -939,6 +986,7 @@ int cov_timing(cov_cell_t* h, int on)
 int cov_timing_read(cov_cell_t* h, double* ms, int64_t* launches, int nkinds)
 {
     if (!h || nkinds < 0 || nkinds > 4) return COV_EPARAM;
+    RC(timing_collect(h));
     for (int k = 0; k < nkinds; k++) {
         if (ms) ms[k] = h->ms[k];
         if (launches) launches[k] = h->launches[k];

@@ -31,7 +31,8 @@ EXPORTS = ['cov_version', 'cov_strerror', 'cov_mem_bytes', 'cov_create', 'cov_de
            'cov_first_uncovered_from', 'cov_read_cnt', 'cov_eval_moves', 'cov_commit',
            'cov_distance_transform', 'cov_search_init', 'cov_search_params',
            'cov_search_run', 'cov_search_launch', 'cov_search_state', 'cov_set_launch',
-           'cov_timing', 'cov_timing_read', 'cov_search_phases', 'cov_search_bytes']
+           'cov_set_transform_tile', 'cov_timing', 'cov_timing_read', 'cov_search_phases',
+           'cov_search_bytes']
 
 
 class CovError(RuntimeError):
@@ -88,6 +89,7 @@ def lib():
         'cov_search_launch': (i32, [vp, i64, i32]),
         'cov_search_state': (i32, [vp, ctypes.POINTER(State)]),
         'cov_set_launch': (i32, [vp, i32, i32]),
+        'cov_set_transform_tile': (i32, [vp, i32]),
         'cov_timing': (i32, [vp, i32]),
         'cov_timing_read': (i32, [vp, vp, vp, i32]),
         'cov_search_phases': (i32, [vp, vp, i32]),
@@ -250,6 +252,9 @@ class Cell:
 
     def set_launch(self, eval_groups, commit_groups):
         self._check(lib().cov_set_launch(self.h, int(eval_groups), int(commit_groups)), 'cov_set_launch')
+
+    def set_transform_tile(self, lds_bytes):
+        self._check(lib().cov_set_transform_tile(self.h, int(lds_bytes)), 'cov_set_transform_tile')
 
     def timing(self, on):
         self._check(lib().cov_timing(self.h, int(bool(on))), 'cov_timing')

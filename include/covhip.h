@@ -133,8 +133,12 @@ int     cov_search_state(cov_cell_t *h, cov_state_t *st);
 /* -------------------------------------------------- launch shape, timing */
 /* Workgroups of the evaluator and commit kernels (the autotuner sets them). */
 int     cov_set_launch(cov_cell_t *h, int eval_groups, int commit_groups);
-/* Accumulate per-kernel device time with HIP events on the cell's stream
- * (on = 1), read totals in ms and launch counts for kernel kinds
+/* LDS bytes per distance-transform tile (0 = the default, 32768); the tile
+ * plan groups the n axes into as few HBM passes as tiles of this size allow. */
+int     cov_set_transform_tile(cov_cell_t *h, int lds_bytes);
+/* Accumulate per-kernel device time with HIP events recorded around each
+ * launch on the cell's stream (on = 1; read back in batches, no host sync
+ * between launches), read totals in ms and launch counts for kernel kinds
  * 0 = evaluator, 1 = commit, 2 = distance transform pass, 3 = recount. */
 int     cov_timing(cov_cell_t *h, int on);
 int     cov_timing_read(cov_cell_t *h, double *ms, int64_t *launches, int nkinds);

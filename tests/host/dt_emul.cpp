@@ -1,0 +1,134 @@
+// Host emulation of the distance-transform kernels: runs the pass plan and the
+// per-thread phases of cov_dist.h thread by thread (phases separated where the
+// kernels place a barrier) over exact-size heap buffers, and compares the
+// result with a brute-force minimum distance.  Built with AddressSanitizer by
+// tests/test_dt_emul.py, so any access outside a buffer aborts.
+//
+//   dt_emul q n M seed budget  ->  prints "ok passes=P hist=..." or "FAIL ..."
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <algorithm>
+#include <vector>
+#include "cov_dist.h"
+
+using namespace cov;
+
+static uint64_t sm64(uint64_t& x)
+{
+    uint64_t z = (x += 0x9E3779B97F4A7C15ull);
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+
+int main(int argc, char** argv)
+{
+    if (argc < 6) return 2;
+    const int q = atoi(argv[1]), n = atoi(argv[2]), M = atoi(argv[3]);
+    uint64_t seed = strtoull(argv[4], 0, 10);
+    const int budget = atoi(argv[5]);
+    int64_t N = 1;
+    for (int j = 0; j < n; j++) N *= q;
+    std::vector<int64_t> cw(M);
+    std::vector<std::vector<int>> dig(M, std::vector<int>(n));
+    for (int i = 0; i < M; i++) {
+        int64_t w = 0;
+        for (int j = 0; j < n; j++) {
+            dig[i][j] = (int)(sm64(seed) % (uint64_t)q);
+            w = w * q + dig[i][j];
+        }
+        cw[i] = w;
+    }
+    DtPass pass[MAXN + 1];
+    const int np = dt_plan(q, n, N, budget, pass);
+    if (np <= 0) { printf("FAIL plan\n"); return 1; }
+    int axes = 0;
+    for (int i = 0; i < np; i++) axes += pass[i].nax;
+    if (axes != n) { printf("FAIL axes %d\n", axes); return 1; }
+    uint8_t* D = new uint8_t[N + 64];
+    memset(D, 0xAB, N + 64);
+    const int64_t nw = (N >> 5) + 2;
+    uint32_t* cwb = new uint32_t[nw];
+    memset(cwb, 0, nw * 4);
+    for (int i = 0; i < M; i++) cwb[cw[i] >> 5] |= 1u << (cw[i] & 31);
+    uint64_t hist[DT_HB] = {0};
+    const int nth = WG;
+    for (int pi = 0; pi < np; pi++) {
+        const DtPass& p = pass[pi];
+        const bool inner = pi == 0;
+        const int64_t lds = inner ? p.P : p.rows * p.P;
+        if (lds + (inner ? 0 : 1024) + 4 * DT_HB > 65536) { printf("FAIL lds %lld\n", (long long)lds); return 1; }
+        uint8_t* sm = new uint8_t[lds];
+        uint8_t* shr = new uint8_t[p.rows];
+        for (int64_t t = 0; t < p.ntiles; t++) {
+            std::vector<DtHist> h(nth);
+            for (auto& x : h) memset(&x, 0, sizeof x);
+            if (inner) {
+                for (int tid = 0; tid < nth; tid++) dt_inner_load(p, t, tid, nth, sm, D, cwb);
+                for (int ax = 0; ax < p.nax; ax++)
+                    for (int tid = 0; tid < nth; tid++) dt_inner_axis(p, t, ax, tid, nth, sm);
+                for (int tid = 0; tid < nth; tid++) dt_inner_store(p, t, tid, nth, sm, D, h[tid]);
+            } else {
+                const DtTile T = dt_outer_tile(p, t);
+                for (int tid = 0; tid < nth; tid++) dt_outer_load(p, T, tid, nth, sm, shr, D);
+                for (int ax = 0; ax < p.nax; ax++)
+                    for (int tid = 0; tid < nth; tid++) dt_outer_axis(p, T, ax, tid, nth, sm, shr);
+                for (int tid = 0; tid < nth; tid++) dt_outer_store(p, T, tid, nth, sm, D, h[tid]);
+            }
+            if (p.last)
+                for (int tid = 0; tid < nth; tid++)
+                    for (int b = 0; b <= n + 1; b++) hist[b] += (h[tid].w[b >> 2] >> (16 * (b & 3))) & 0xFFFF;
+        }
+        delete[] sm;
+        delete[] shr;
+    }
+    // brute force on small spaces; on large ones the textbook axis DP, one
+    // full-array pass per axis
+    std::vector<uint8_t> ref;
+    if (N * (int64_t)M > 400000000LL) {
+        ref.assign(N, 255);
+        for (int i = 0; i < M; i++) ref[cw[i]] = 0;
+        int64_t s = 1;
+        for (int j = n - 1; j >= 0; j--, s *= q)
+            for (int64_t hi = 0; hi < N; hi += s * q)
+                for (int64_t lo = 0; lo < s; lo++) {
+                    int m = 255;
+                    for (int v = 0; v < q; v++) m = std::min<int>(m, ref[hi + lo + v * s]);
+                    const int m1 = m < 255 ? m + 1 : 255;
+                    for (int v = 0; v < q; v++) ref[hi + lo + v * s] = (uint8_t)std::min<int>(ref[hi + lo + v * s], m1);
+                }
+    }
+    std::vector<int> d(n);
+    uint64_t bh[DT_HB] = {0};
+    int64_t bad = 0;
+    for (int64_t w = 0; w < N; w++) {
+        int best = 255;
+        if (!ref.empty()) {
+            best = ref[w];
+        } else {
+            int64_t x = w;
+            for (int j = n - 1; j >= 0; j--) { d[j] = (int)(x % q); x /= q; }
+            for (int i = 0; i < M; i++) {
+                int dd = 0;
+                for (int j = 0; j < n; j++) dd += d[j] != dig[i][j];
+                if (dd < best) best = dd;
+            }
+        }
+        if (D[w] != best) { if (bad < 5) printf("mismatch w=%lld got %d want %d\n", (long long)w, D[w], best); bad++; }
+        bh[best <= n ? best : n + 1]++;
+    }
+    for (int b = 0; b <= n + 1; b++)
+        if (bh[b] != hist[b]) { printf("FAIL hist bin %d %llu vs %llu\n", b, (unsigned long long)hist[b], (unsigned long long)bh[b]); return 1; }
+    for (int64_t k = N; k < N + 64; k++)
+        if (D[k] != 0xAB) { printf("FAIL wrote past N at %lld\n", (long long)k); return 1; }
+    if (bad) { printf("FAIL %lld mismatches\n", (long long)bad); return 1; }
+    printf("ok passes=%d tiles=", np);
+    for (int i = 0; i < np; i++) printf("%lld%s", (long long)pass[i].ntiles, i + 1 < np ? "," : "");
+    printf(" hist=");
+    for (int b = 0; b <= n; b++) printf("%llu ", (unsigned long long)hist[b]);
+    printf("\n");
+    delete[] D;
+    delete[] cwb;
+    return 0;
+}

@@ -35,8 +35,8 @@ def test_version_and_errors():
 def test_mem_bytes_and_limits():
     from coldcase_amd.engine import CovError, mem_bytes
     b = mem_bytes(6, 9, 4, 660)
-    # cnt (2 B/word) + three bit planes + list: a little over 2.4 B per word
-    assert 2.0 * 6 ** 9 < b < 4.5 * 6 ** 9
+    # cnt (2 B/word) + three bit planes + list + distance array and bitmap
+    assert 3.5 * 6 ** 9 < b < 6.0 * 6 ** 9
     assert mem_bytes(6, 10, 4, 2784) > 121e6
     for bad in [(1, 5, 1, 3), (17, 3, 1, 3), (3, 5, 5, 3), (3, 5, 1, 70000), (3, 17, 1, 3)]:
         with pytest.raises(CovError):

@@ -113,6 +113,25 @@ def test_direct_sum_and_single_word():
         c.set_code(one)
         assert c.uncovered() == 2 ** 5
 
+@pytest.mark.parametrize('q,n,R,M,lds', [(3, 8, 3, 30, 4096), (5, 7, 2, 400, 4096), (2, 12, 4, 60, 8192),
+                                         (6, 6, 3, 41, 63488), (7, 5, 2, 9, 4096), (16, 3, 1, 50, 8192),
+                                         (3, 11, 3, 25, 4096), (2, 16, 4, 30, 4096), (4, 9, 3, 20, 4096)])
+def test_distance_transform_tile_plans(q, n, R, M, lds):
+    """Small LDS budgets force many-pass plans with ragged row chunks; the
+    codeword bitmap must be clean again for the next code."""
+    print('case transform plans K%d(%d,%d) M=%d lds=%d' % (q, n, R, M, lds), flush=True)
+    rng = np.random.default_rng(lds + q)
+    with cell(q, n, R, M) as c:
+        c.set_transform_tile(lds)
+        for rep in range(3):
+            code = rng.integers(0, q, size=(M, n)).astype(np.uint8)
+            want = mindist(code, n, q)
+            c.set_code(code)
+            dist, hist, rad = c.distance_transform(want_array=True)
+            assert (dist == want).all(), rep
+            assert rad == int(want.max())
+            assert (hist == np.bincount(want, minlength=n + 1)[:n + 1]).all()
+
 
 @pytest.mark.parametrize('fname,q,n,R', [('K6_9_4_M660.txt', 6, 9, 4), ('K7_9_4_M1480.txt', 7, 9, 4),
                                          ('K6_10_4_M2784.txt', 6, 10, 4), ('K6_8_4_M169.txt', 6, 8, 4)])
