@@ -99,37 +99,46 @@ __device__ __forceinline__ uint64_t uniform64(uint64_t v)
     return ((uint64_t)hi << 32) | lo;
 }
 
+// Device pointers of a cell.  In device code they carry the global address
+// space, so every access through them is a global_* instruction whether the
+// descriptor comes in a kernel argument or from a table in device memory.
+#ifdef __HIP_DEVICE_COMPILE__
+#define GPTR(T) __attribute__((address_space(1))) T*
+#else
+#define GPTR(T) T*
+#endif
+
 struct Cell {
     int q, n, R, M, q1, F, E;
     int64_t P;              // (q-1)^R patterns per position subset
     int64_t N, nzw, nblk, V, ncap, ulcap;
     int64_t pw[MAXN];
     uint32_t magic;         // ceil(2^32 / (q-1)): t / (q-1) = umulhi(t, magic)
-    uint32_t* cnt32;        // cnt as uint32 pairs
-    uint32_t* zb;
-    uint32_t* ob;
-    uint32_t* blk;
-    uint8_t* code;
-    uint64_t* cpk;
-    int64_t* cidx;
-    uint8_t* best;
-    int32_t* tabu;
-    uint32_t* fsub;         // C(n,R) R-subsets of all positions, 4 bits each
-    uint32_t* psub;         // n x C(n-1,R) R-subsets avoiding p
-    uint32_t* bsub;         // subsets of size 0..R, grouped by size
+    GPTR(uint32_t) cnt32;        // cnt as uint32 pairs
+    GPTR(uint32_t) zb;
+    GPTR(uint32_t) ob;
+    GPTR(uint32_t) blk;
+    GPTR(uint8_t) code;
+    GPTR(uint64_t) cpk;
+    GPTR(int64_t) cidx;
+    GPTR(uint8_t) best;
+    GPTR(int32_t) tabu;
+    GPTR(uint32_t) fsub;         // C(n,R) R-subsets of all positions, 4 bits each
+    GPTR(uint32_t) psub;         // n x C(n-1,R) R-subsets avoiding p
+    GPTR(uint32_t) bsub;         // subsets of size 0..R, grouped by size
     int64_t bsub_off[MAXR + 2];   // first subset of each size in bsub
     int64_t bpat_off[MAXR + 2];   // first ball word of each size
-    int64_t* ul_idx;
-    uint64_t* ul_pk;
-    uint32_t* inl;          // list membership plane
-    uint32_t* shuf;         // shuffled candidate order (original indices)
-    int32_t* acc;           // slot accumulators, ACCW per slot
-    int32_t* slot_cw;       // slot -> codeword
-    uint8_t* slot_pos;      // slot -> differing positions (MAXN each)
-    uint64_t* rbuf;
-    int64_t* scratch64;     // small reduction outputs
-    uint8_t* dist;          // distance transform output (N bytes), lazily
-    Ctl* ctl;
+    GPTR(int64_t) ul_idx;
+    GPTR(uint64_t) ul_pk;
+    GPTR(uint32_t) inl;          // list membership plane
+    GPTR(uint32_t) shuf;         // shuffled candidate order (original indices)
+    GPTR(int32_t) acc;           // slot accumulators, ACCW per slot
+    GPTR(int32_t) slot_cw;       // slot -> codeword
+    GPTR(uint8_t) slot_pos;      // slot -> differing positions (MAXN each)
+    GPTR(uint64_t) rbuf;
+    GPTR(int64_t) scratch64;     // small reduction outputs
+    GPTR(uint8_t) dist;          // distance transform output (N bytes), lazily
+    GPTR(Ctl) ctl;
 };
 
 // ------------------------------------------------------------- helpers

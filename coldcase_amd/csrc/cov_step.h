@@ -60,7 +60,7 @@ __device__ __forceinline__ uint32_t zbit_acq(const Cell& C, int64_t w)
 // ---------------------------------------------------------- scanning
 // first word of the plane-word range [x0, x1) with a set zb bit (masked first
 // and last words), or INT64_MAX; all threads
-__device__ inline int64_t wg_first_bits(const Cell& C, int64_t x0, int64_t x1, uint32_t m0,
+__device__ __forceinline__ int64_t wg_first_bits(const Cell& C, int64_t x0, int64_t x1, uint32_t m0,
                                         uint32_t m1, NarrowSh& sh)
 {
     int64_t best = INT64_MAX;
@@ -81,7 +81,7 @@ __device__ inline int64_t wg_first_bits(const Cell& C, int64_t x0, int64_t x1, u
 
 // first uncovered word at or after `start`, cyclically (the scan of
 // cov/engine/covsearch2e.c:1003-1015); -1 if none.  All threads.
-__device__ inline int64_t wg_first_from(const Cell& C, int64_t start, NarrowSh& sh)
+__device__ __forceinline__ int64_t wg_first_from(const Cell& C, int64_t start, NarrowSh& sh)
 {
     const int64_t sw = start >> 5;
     const int64_t sb = start >> BLK_SHIFT;
@@ -117,7 +117,7 @@ __device__ inline int64_t wg_first_from(const Cell& C, int64_t start, NarrowSh& 
 
 // make sure `need` draws past the cursor exist (wave 0 generates, with 1024
 // to spare); all threads
-__device__ inline void wg_ensure(const Cell& C, Ctl* ctl, int64_t need)
+__device__ __forceinline__ void wg_ensure(const Cell& C, Ctl* ctl, int64_t need)
 {
     __syncthreads();
     const int64_t ruse = ctl->ruse;
@@ -132,7 +132,7 @@ __device__ inline void wg_ensure(const Cell& C, Ctl* ctl, int64_t need)
 }
 
 // pick_uncovered: 64 probes, then the cyclic scan (:973-1016).  All threads.
-__device__ inline int64_t wg_pick(const Cell& C, Ctl* ctl, NarrowSh& sh)
+__device__ __forceinline__ int64_t wg_pick(const Cell& C, Ctl* ctl, NarrowSh& sh)
 {
     wg_ensure(C, ctl, 65);
     const int64_t pos = ctl->ruse;
@@ -161,7 +161,7 @@ __device__ inline int64_t wg_pick(const Cell& C, Ctl* ctl, NarrowSh& sh)
 }
 
 // ------------------------------------------------------ uncovered list
-__device__ inline void wg_list_rebuild(const Cell& C, Ctl* ctl, NarrowSh& sh)
+__device__ __forceinline__ void wg_list_rebuild(const Cell& C, Ctl* ctl, NarrowSh& sh)
 {
     if (threadIdx.x == 0) { ctl->ul_n = 0; ctl->ul_ovf = 0; }
     __syncthreads();
@@ -188,7 +188,7 @@ __device__ inline void wg_list_rebuild(const Cell& C, Ctl* ctl, NarrowSh& sh)
 }
 
 // drop entries that are covered again (their membership bit is cleared)
-__device__ inline void wg_list_compact(const Cell& C, Ctl* ctl, NarrowSh& sh)
+__device__ __forceinline__ void wg_list_compact(const Cell& C, Ctl* ctl, NarrowSh& sh)
 {
     const int64_t n0 = ctl->ul_n;
     int64_t out = 0;
@@ -218,7 +218,7 @@ __device__ inline void wg_list_compact(const Cell& C, Ctl* ctl, NarrowSh& sh)
 // Start step it+1 if the run allows it: pick u, collect the candidate
 // codewords (those at distance dtar from u, in index order), reserve the
 // shuffle draws and lay out the evaluator's units.  All threads of one WG.
-__device__ inline void wg_begin(const Cell& C, Ctl* ctl, NarrowSh& sh)
+__device__ __forceinline__ void wg_begin(const Cell& C, Ctl* ctl, NarrowSh& sh)
 {
     const int64_t tb0 = clk();
     __syncthreads();
@@ -445,7 +445,7 @@ __device__ __forceinline__ void unit_list(const Cell& C, int slot, int64_t e0, i
 }
 
 // run this wave's share of the step's evaluator units
-__device__ inline void eval_units(const Cell& C, const Ctl* ctl, int64_t wid, int64_t nw)
+__device__ __forceinline__ void eval_units(const Cell& C, const Ctl* ctl, int64_t wid, int64_t nw)
 {
     const int64_t nl = ctl->n_leave, ne = ctl->n_enter;
     const int use_list = ctl->use_list, dtar = ctl->dtar;
@@ -495,7 +495,7 @@ struct ShufSh {
 // else root(f(t)).  All threads of one workgroup: the (draw, chunk of 32
 // earlier draws) pairs are scanned in parallel with LDS maxima, then every
 // thread follows its own chains.
-__device__ inline void wg_shuffle(const Cell& C, const Ctl* ctl, ShufSh& sh)
+__device__ __forceinline__ void wg_shuffle(const Cell& C, const Ctl* ctl, ShufSh& sh)
 {
     const int tid = threadIdx.x;
     const int64_t ncand = ctl->ncand, take = ctl->take, pos0 = ctl->sh_pos;
@@ -594,7 +594,7 @@ __device__ __forceinline__ int64_t cand_delta(const Cell& C, const Ctl* ctl, uin
 // an exclusive prefix minimum; T(s) counts ties up to s and every tie's draw
 // is the T(s)-th of the step; nties(s) = 1 + T(s) - T(last restart <= s).
 // The winner is the last restart or successful tie.  All threads of one WG.
-__device__ inline void wg_select(const Cell& C, Ctl* ctl, SelSh& sh)
+__device__ __forceinline__ void wg_select(const Cell& C, Ctl* ctl, SelSh& sh)
 {
     const int tid = threadIdx.x;
     if (tid == 0) ctl->rlow = ctl->ruse;
@@ -702,14 +702,15 @@ __device__ inline void wg_select(const Cell& C, Ctl* ctl, SelSh& sh)
     }
 }
 
-// One release per workgroup on `counter`; returns true in the workgroup that
-// completes the count, which then acquires.  All threads.
-__device__ inline bool wg_arrive_last(int32_t* counter, int* flag)
+// One release per workgroup on `counter` (nblk workgroups take part); returns
+// true in the workgroup that completes the count, which then acquires.  All
+// threads.
+__device__ __forceinline__ bool wg_arrive_last(int32_t* counter, int* flag, int nblk)
 {
     __syncthreads();
     if (threadIdx.x == 0) {
         const int prev = __hip_atomic_fetch_add(counter, 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-        *flag = (prev == (int)gridDim.x - 1);
+        *flag = (prev == nblk - 1);
     }
     __syncthreads();
     const bool last = *flag != 0;
@@ -721,7 +722,8 @@ __device__ inline bool wg_arrive_last(int32_t* counter, int* flag)
 }
 
 // --------------------------------------------------------------- kernels
-__global__ void __launch_bounds__(WG, 4) k_eval(const Cell C)
+// The evaluator of one cell on nblk workgroups; this one is number bid.
+__device__ __forceinline__ void eval_body(const Cell& C, int bid, int nblk)
 {
     Ctl* ctl = C.ctl;
     __shared__ ShufSh shs;
@@ -730,27 +732,27 @@ __global__ void __launch_bounds__(WG, 4) k_eval(const Cell C)
     if (!ctl->begun || ctl->done || ctl->selected) return;
     const int wv = threadIdx.x >> 6;
     const int64_t th0 = clk();
-    if (blockIdx.x == 0 && threadIdx.x == 0 && ctl->t_cme) ctl->tph[10] += th0 - ctl->t_cme;
+    if (bid == 0 && threadIdx.x == 0 && ctl->t_cme) ctl->tph[10] += th0 - ctl->t_cme;
     // workgroup 0: the shuffle; wave 0 of workgroup 1 (or of 0 on a one-group
     // grid, after the shuffle): the rng refill, keeping ~4096 draws ahead
-    const int rng_wg = gridDim.x > 1 ? 1 : 0;
-    if (blockIdx.x == 0) {
+    const int rng_wg = nblk > 1 ? 1 : 0;
+    if (bid == 0) {
         wg_shuffle(C, ctl, shs);
         if (threadIdx.x == 0) ctl->tph[5] += clk() - th0;
     }
-    if (blockIdx.x == rng_wg && wv == 0) {
+    if (bid == rng_wg && wv == 0) {
         if (ctl->rgen < ctl->ruse + 4096) wave_ring_fill(C, ctl, ctl->ruse + 4096, ctl->budget);
         if (threadIdx.x == 0) ctl->tph[6] += clk() - th0;
     }
-    if (blockIdx.x != 0) {
+    if (bid != 0) {
         // units over all waves of workgroups >= 1, except the rng wave
-        const int64_t wid = (int64_t)(blockIdx.x - 1) * WAVES + wv - (rng_wg == 1 ? 1 : 0);
-        const int64_t nw = (int64_t)(gridDim.x - 1) * WAVES - 1;
-        if (!(blockIdx.x == 1 && wv == 0)) eval_units(C, ctl, wid, nw);
-    } else if (gridDim.x == 1) {
+        const int64_t wid = (int64_t)(bid - 1) * WAVES + wv - (rng_wg == 1 ? 1 : 0);
+        const int64_t nw = (int64_t)(nblk - 1) * WAVES - 1;
+        if (!(bid == 1 && wv == 0)) eval_units(C, ctl, wid, nw);
+    } else if (nblk == 1) {
         eval_units(C, ctl, wv, WAVES);
     }
-    if (!wg_arrive_last(&ctl->evdone, &last)) return;
+    if (!wg_arrive_last(&ctl->evdone, &last, nblk)) return;
     const int64_t ts0 = clk();
     __shared__ Ctl lc;
     ctl_to_lds(&lc, ctl);
@@ -765,8 +767,13 @@ __global__ void __launch_bounds__(WG, 4) k_eval(const Cell C)
     ctl_from_lds(ctl, &lc);
 }
 
+__global__ void __launch_bounds__(WG, 4) k_eval(const Cell C)
+{
+    eval_body(C, blockIdx.x, gridDim.x);
+}
+
 // kick: teleport `kick` codewords near uncovered words (:1553-1578).  All threads.
-__device__ inline void wg_ball(const Cell& C, Ctl* ctl, int i, int delta, NarrowSh& sh)
+__device__ __forceinline__ void wg_ball(const Cell& C, Ctl* ctl, int i, int delta, NarrowSh& sh)
 {
     const int64_t base = C.cidx[i];
     const uint64_t pk = C.cpk[i];
@@ -782,7 +789,7 @@ __device__ inline void wg_ball(const Cell& C, Ctl* ctl, int i, int delta, Narrow
     __syncthreads();
 }
 
-__device__ inline void wg_kick(const Cell& C, Ctl* ctl, NarrowSh& sh)
+__device__ __forceinline__ void wg_kick(const Cell& C, Ctl* ctl, NarrowSh& sh)
 {
     const int n = C.n, q = C.q;
     for (int t = 0; t < ctl->kick; t++) {
@@ -836,7 +843,7 @@ __device__ inline void wg_kick(const Cell& C, Ctl* ctl, NarrowSh& sh)
 }
 
 // apply the committed move and close the step.  All threads of one WG.
-__device__ inline void wg_post(const Cell& C, Ctl* ctl, NarrowSh& sh)
+__device__ __forceinline__ void wg_post(const Cell& C, Ctl* ctl, NarrowSh& sh)
 {
     const int64_t tq0 = clk();
     const int n = C.n, q = C.q;
@@ -906,7 +913,7 @@ __device__ inline void wg_post(const Cell& C, Ctl* ctl, NarrowSh& sh)
 }
 
 // the commit units: every pattern of the subsets avoiding bp, flattened
-__device__ inline int64_t commit_units(const Cell& C, const Ctl* ctl, int64_t gt, int64_t nt)
+__device__ __forceinline__ int64_t commit_units(const Cell& C, const Ctl* ctl, int64_t gt, int64_t nt)
 {
     const int bi = ctl->bi, bp = ctl->bp;
     const int64_t base = ctl->base, dv = ctl->dv;
@@ -942,19 +949,20 @@ __device__ inline int64_t commit_units(const Cell& C, const Ctl* ctl, int64_t gt
     return d;
 }
 
-__global__ void __launch_bounds__(WG) k_commit(const Cell C, int api)
+// The commit of one cell on nblk workgroups; this one is number bid.
+__device__ __forceinline__ void commit_body(const Cell& C, int bid, int nblk, int api)
 {
     Ctl* ctl = C.ctl;
     __shared__ NarrowSh sh;
     __shared__ int last;
     if (!ctl->begun || ctl->done || !ctl->selected) return;
-    if (blockIdx.x == 0 && threadIdx.x == 0 && ctl->t_eve) ctl->tph[11] += clk() - ctl->t_eve;
+    if (bid == 0 && threadIdx.x == 0 && ctl->t_eve) ctl->tph[11] += clk() - ctl->t_eve;
     const int64_t tc0 = clk();
-    const int64_t gt = (int64_t)blockIdx.x * WG + threadIdx.x;
-    const int64_t d = commit_units(C, ctl, gt, (int64_t)gridDim.x * WG);
+    const int64_t gt = (int64_t)bid * WG + threadIdx.x;
+    const int64_t d = commit_units(C, ctl, gt, (int64_t)nblk * WG);
     const int64_t ds = wg_sum(d, sh.red);
     if (threadIdx.x == 0 && ds) atomicAdd(reinterpret_cast<unsigned long long*>(&ctl->dunc), (unsigned long long)ds);
-    if (!wg_arrive_last(&ctl->cmdone, &last)) return;
+    if (!wg_arrive_last(&ctl->cmdone, &last, nblk)) return;
     __shared__ Ctl lc;
     ctl_to_lds(&lc, ctl);
     Ctl* L = &lc;
@@ -985,8 +993,13 @@ __global__ void __launch_bounds__(WG) k_commit(const Cell C, int api)
     ctl_from_lds(C.ctl, &lc);
 }
 
+__global__ void __launch_bounds__(WG) k_commit(const Cell C, int api)
+{
+    commit_body(C, blockIdx.x, gridDim.x, api);
+}
+
 // set the run's stop and begin the first step (one workgroup)
-__global__ void __launch_bounds__(WG) k_begin_run(const Cell C, int64_t nsteps)
+__device__ __forceinline__ void begin_run_body(const Cell& C, int64_t nsteps)
 {
     __shared__ NarrowSh sh;
     __shared__ Ctl lc;
@@ -1000,6 +1013,11 @@ __global__ void __launch_bounds__(WG) k_begin_run(const Cell C, int64_t nsteps)
     wg_begin(C, ctl, sh);
     if (threadIdx.x == 0) ctl->t_cme = clk();
     ctl_from_lds(C.ctl, &lc);
+}
+
+__global__ void __launch_bounds__(WG) k_begin_run(const Cell C, int64_t nsteps)
+{
+    begin_run_body(C, nsteps);
 }
 
 // --------------------------------------------------------- init kernels

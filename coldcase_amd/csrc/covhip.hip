@@ -13,6 +13,7 @@
 #include "cov_common.h"
 #include "cov_cover.h"
 #include "cov_dist.h"
+#include "cov_batch.h"
 #include "cov_step.h"
 
 using namespace cov;
@@ -42,6 +43,18 @@ struct cov_cell {
     // distance transform
     uint32_t* cwb = nullptr;              // codeword bitmap, all zero between calls
     int dt_lds = 0;                       // LDS bytes per transform tile
+};
+
+// Many cells stepped by the same launches (cov_batch.h).
+struct cov_batch {
+    std::vector<cov_cell*> cells;
+    hipStream_t stream = nullptr;
+    Cell* d_cells = nullptr;              // the cells' descriptors, in device memory
+    int32_t* d_eoff = nullptr;            // evaluator workgroup ranges, ncell + 1
+    int32_t* d_coff = nullptr;            // commit workgroup ranges, ncell + 1
+    int eval_total = 0, commit_total = 0;
+    hipGraphExec_t gexec = nullptr;
+    std::string err;
 };
 
 namespace {
@@ -179,7 +192,7 @@ int dalloc(cov_cell* h, T** p, int64_t bytes)
     }
     h->allocs.push_back(v);
     h->mem_bytes += bytes;
-    *p = reinterpret_cast<T*>(v);
+    *p = (T*)v;
     return COV_OK;
 }
 
@@ -450,7 +463,7 @@ int cov_create(int q, int n, int R, int M, cov_cell_t** out)
     AL(h->d, z.cell);
     AL(h->d_small, z.small);
 #undef AL
-    C.scratch64 = h->d_small;
+    C.scratch64 = (decltype(C.scratch64))h->d_small;
 
     // position-subset tables
     std::vector<int> all(n);
@@ -732,7 +745,9 @@ int cov_distance_transform(cov_cell_t* h, uint8_t* dist, int64_t* hist, int* rad
     if (!h) return COV_EPARAM;
     Cell& C = h->h;
     if (!C.dist) {
-        HCK(h, hipMalloc((void**)&C.dist, (size_t)C.N + 64));
+        void* v = nullptr;
+        HCK(h, hipMalloc(&v, (size_t)C.N + 64));
+        C.dist = (decltype(C.dist))v;
         RC(push_cell(h));
     }
     if (!h->cwb) {
@@ -756,10 +771,7 @@ int cov_distance_transform(cov_cell_t* h, uint8_t* dist, int64_t* hist, int* rad
         const int g = (int)std::min<int64_t>(p.ntiles, 8192);
         hipEvent_t ev = nullptr;
         RC(timed_begin(h, 2, &ev));
-        if (i == 0)
-            hipLaunchKernelGGL(k_dt_inner, dim3(g), dim3(WG), (size_t)p.P, h->stream, p, C.dist, h->cwb, dh);
-        else
-            hipLaunchKernelGGL(k_dt_outer, dim3(g), dim3(WG), (size_t)(p.rows * p.P), h->stream, p, C.dist, dh);
+        hipLaunchKernelGGL(k_dt_pass, dim3(g), dim3(WG), (size_t)dt_lds_bytes(p), h->stream, p, C.dist, h->cwb, dh);
         HCK(h, hipGetLastError());
         RC(timed_end(h, 2, ev));
         if (i == 0) {
@@ -1011,6 +1023,142 @@ int cov_search_bytes(cov_cell_t* h, int64_t* eval_bytes, int64_t* commit_bytes)
     if (eval_bytes) *eval_bytes = c.eval_bytes;
     if (commit_bytes) *commit_bytes = c.commit_bytes;
     return COV_OK;
+}
+
+// ============================================================ batches
+int cov_batch_create(cov_cell_t* const* cells, int k, cov_batch_t** out)
+{
+    if (!cells || k < 1 || !out) return COV_EPARAM;
+    *out = nullptr;
+    for (int i = 0; i < k; i++) {
+        if (!cells[i]) return COV_EPARAM;
+        for (int j = 0; j < i; j++)
+            if (cells[j] == cells[i]) return COV_EPARAM;
+    }
+    cov_batch* b = new cov_batch();
+    auto fail = [&](int rc) {
+        cov_batch_destroy(b);
+        return rc;
+    };
+    b->cells.assign(cells, cells + k);
+    for (cov_cell* c : b->cells) {
+        const hipError_t e = hipStreamSynchronize(c->stream);
+        if (e != hipSuccess) return fail(-(100 + (int)e));
+    }
+    {
+        const hipError_t e = hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking);
+        if (e != hipSuccess) { b->stream = nullptr; return fail(-(100 + (int)e)); }
+    }
+    std::vector<Cell> tab(k);
+    std::vector<int32_t> eo(k + 1, 0), co(k + 1, 0);
+    for (int i = 0; i < k; i++) {
+        tab[i] = b->cells[i]->h;
+        eo[i + 1] = eo[i] + b->cells[i]->eval_groups;
+        co[i + 1] = co[i] + b->cells[i]->commit_groups;
+    }
+    if (eo[k] > (1 << 24) || co[k] > (1 << 24)) return fail(COV_ELIMIT);
+    b->eval_total = eo[k];
+    b->commit_total = co[k];
+    void* v = nullptr;
+    hipError_t e = hipMalloc(&v, sizeof(Cell) * k);
+    if (e != hipSuccess) return fail(COV_ENOMEM);
+    b->d_cells = (Cell*)v;
+    e = hipMalloc(&v, 4 * (k + 1));
+    if (e != hipSuccess) return fail(COV_ENOMEM);
+    b->d_eoff = (int32_t*)v;
+    e = hipMalloc(&v, 4 * (k + 1));
+    if (e != hipSuccess) return fail(COV_ENOMEM);
+    b->d_coff = (int32_t*)v;
+    if ((e = hipMemcpy(b->d_cells, tab.data(), sizeof(Cell) * k, hipMemcpyHostToDevice)) != hipSuccess ||
+        (e = hipMemcpy(b->d_eoff, eo.data(), 4 * (k + 1), hipMemcpyHostToDevice)) != hipSuccess ||
+        (e = hipMemcpy(b->d_coff, co.data(), 4 * (k + 1), hipMemcpyHostToDevice)) != hipSuccess)
+        return fail(-(100 + (int)e));
+    *out = b;
+    return COV_OK;
+}
+
+int cov_batch_destroy(cov_batch_t* b)
+{
+    if (!b) return COV_EPARAM;
+    if (b->stream) (void)hipStreamSynchronize(b->stream);
+    if (b->gexec) (void)hipGraphExecDestroy(b->gexec);
+    if (b->d_cells) (void)hipFree(b->d_cells);
+    if (b->d_eoff) (void)hipFree(b->d_eoff);
+    if (b->d_coff) (void)hipFree(b->d_coff);
+    if (b->stream) (void)hipStreamDestroy(b->stream);
+    delete b;
+    return COV_OK;
+}
+
+const char* cov_batch_last_error(const cov_batch_t* b)
+{
+    return b ? b->err.c_str() : "null batch";
+}
+
+int cov_batch_size(const cov_batch_t* b)
+{
+    return b ? (int)b->cells.size() : COV_EPARAM;
+}
+
+int cov_batch_launch(cov_batch_t* b, int64_t nsteps, int mode)
+{
+    if (!b || nsteps < 0 || (mode != COV_MODE_PLAIN && mode != COV_MODE_GRAPH)) return COV_EPARAM;
+    if (nsteps == 0) return COV_OK;
+    const int k = (int)b->cells.size();
+    // no trace records from batched runs
+    const int64_t zero = 0;
+    for (cov_cell* h : b->cells) {
+        HCK(b, hipMemcpyAsync(CTL_PTR(h, trace), &zero, 8, hipMemcpyHostToDevice, b->stream));
+        HCK(b, hipMemcpyAsync(CTL_PTR(h, tcap), &zero, 8, hipMemcpyHostToDevice, b->stream));
+    }
+    hipLaunchKernelGGL(k_begin_run_b, dim3(k), dim3(WG), 0, b->stream, b->d_cells, nsteps);
+    HCK(b, hipGetLastError());
+    int64_t left = nsteps;
+    if (mode == COV_MODE_GRAPH) {
+        if (!b->gexec) {
+            hipGraph_t g;
+            HCK(b, hipStreamBeginCapture(b->stream, hipStreamCaptureModeRelaxed));
+            for (int st = 0; st < GRAPH_STEPS; st++) {
+                hipLaunchKernelGGL(k_eval_b, dim3(b->eval_total), dim3(WG), 0, b->stream, b->d_cells, b->d_eoff, k);
+                hipLaunchKernelGGL(k_commit_b, dim3(b->commit_total), dim3(WG), 0, b->stream, b->d_cells, b->d_coff,
+                                   k);
+            }
+            HCK(b, hipStreamEndCapture(b->stream, &g));
+            const hipError_t e = hipGraphInstantiate(&b->gexec, g, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(g);
+            HCK(b, e);
+        }
+        while (left >= GRAPH_STEPS) {
+            HCK(b, hipGraphLaunch(b->gexec, b->stream));
+            left -= GRAPH_STEPS;
+        }
+    }
+    for (; left > 0; left--) {
+        hipLaunchKernelGGL(k_eval_b, dim3(b->eval_total), dim3(WG), 0, b->stream, b->d_cells, b->d_eoff, k);
+        hipLaunchKernelGGL(k_commit_b, dim3(b->commit_total), dim3(WG), 0, b->stream, b->d_cells, b->d_coff, k);
+        HCK(b, hipGetLastError());
+    }
+    return COV_OK;
+}
+
+int cov_batch_sync(cov_batch_t* b)
+{
+    if (!b) return COV_EPARAM;
+    HCK(b, hipStreamSynchronize(b->stream));
+    for (size_t i = 0; i < b->cells.size(); i++) {
+        Ctl c;
+        const int rc = read_ctl(b->cells[i], &c);
+        if (rc != COV_OK) { b->err = b->cells[i]->err; return rc; }
+        if (c.fault == 1) { b->err = "cell " + std::to_string(i) + ": inconsistent counters"; return COV_ESTATE; }
+        if (c.fault == 2) { b->err = "cell " + std::to_string(i) + ": candidate sample too large"; return COV_ELIMIT; }
+    }
+    return COV_OK;
+}
+
+int cov_batch_run(cov_batch_t* b, int64_t nsteps, int mode)
+{
+    RC(cov_batch_launch(b, nsteps, mode));
+    return cov_batch_sync(b);
 }
 
 }  // extern "C"

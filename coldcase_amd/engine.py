@@ -32,7 +32,8 @@ EXPORTS = ['cov_version', 'cov_strerror', 'cov_mem_bytes', 'cov_create', 'cov_de
            'cov_distance_transform', 'cov_search_init', 'cov_search_params',
            'cov_search_run', 'cov_search_launch', 'cov_search_state', 'cov_set_launch',
            'cov_set_transform_tile', 'cov_timing', 'cov_timing_read', 'cov_search_phases',
-           'cov_search_bytes']
+           'cov_search_bytes', 'cov_batch_create', 'cov_batch_destroy', 'cov_batch_last_error',
+           'cov_batch_size', 'cov_batch_launch', 'cov_batch_sync', 'cov_batch_run']
 
 
 class CovError(RuntimeError):
@@ -94,6 +95,13 @@ def lib():
         'cov_timing_read': (i32, [vp, vp, vp, i32]),
         'cov_search_phases': (i32, [vp, vp, i32]),
         'cov_search_bytes': (i32, [vp, vp, vp]),
+        'cov_batch_create': (i32, [ctypes.POINTER(vp), i32, ctypes.POINTER(vp)]),
+        'cov_batch_destroy': (i32, [vp]),
+        'cov_batch_last_error': (ctypes.c_char_p, [vp]),
+        'cov_batch_size': (i32, [vp]),
+        'cov_batch_launch': (i32, [vp, i64, i32]),
+        'cov_batch_sync': (i32, [vp]),
+        'cov_batch_run': (i32, [vp, i64, i32]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -277,3 +285,52 @@ class Cell:
         c = np.zeros(1, np.int64)
         self._check(lib().cov_search_bytes(self.h, _p(e), _p(c)), 'cov_search_bytes')
         return int(e[0]), int(c[0])
+
+
+class Batch:
+    """Cells of any shapes advanced by the same launches (one evaluator and
+    one commit launch per batched step).  Each cell keeps the trajectory it
+    has alone.  The cells stay owned by the caller and must outlive the batch;
+    while a batched run is in flight they are not to be used."""
+
+    def __init__(self, cells):
+        self.cells = list(cells)
+        arr = (ctypes.c_void_p * len(self.cells))(*[c.h.value for c in self.cells])
+        h = ctypes.c_void_p()
+        rc = lib().cov_batch_create(arr, len(self.cells), ctypes.byref(h))
+        if rc != COV_OK:
+            raise CovError('cov_batch_create(%d cells): %s' % (len(self.cells), lib().cov_strerror(rc).decode()))
+        self.h = h
+
+    def _check(self, rc, what):
+        if rc < 0:
+            msg = lib().cov_batch_last_error(self.h)
+            raise CovError('%s: %s (%s)' % (what, lib().cov_strerror(int(rc)).decode(),
+                                             msg.decode() if msg else ''))
+        return rc
+
+    def launch(self, nsteps, mode=MODE_GRAPH):
+        self._check(lib().cov_batch_launch(self.h, int(nsteps), int(mode)), 'cov_batch_launch')
+
+    def sync(self):
+        self._check(lib().cov_batch_sync(self.h), 'cov_batch_sync')
+
+    def run(self, nsteps, mode=MODE_GRAPH):
+        self._check(lib().cov_batch_run(self.h, int(nsteps), int(mode)), 'cov_batch_run')
+
+    def close(self):
+        if getattr(self, 'h', None):
+            lib().cov_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()

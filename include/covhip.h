@@ -44,6 +44,7 @@ extern "C" {
 #define COV_MODE_GRAPH 1   /* steps replayed from a captured hipGraph      */
 
 typedef struct cov_cell cov_cell_t;
+typedef struct cov_batch cov_batch_t;
 
 typedef struct {
     int64_t it;               /* iterations done                              */
@@ -151,6 +152,27 @@ int     cov_search_phases(cov_cell_t *h, double *us, int n);
 /* Algorithmic bytes moved by the evaluator and commit kernels since the last
  * cov_search_init (2 B per counter read or update, as the uint16 cnt). */
 int     cov_search_bytes(cov_cell_t *h, int64_t *eval_bytes, int64_t *commit_bytes);
+
+/* ------------------------------------------------------------ batches */
+/* Many cells, each created, configured and initialised on its own (any
+ * shapes), advanced by the same launches: every batched step is one
+ * evaluator and one commit launch whose workgroups are split among the cells
+ * (each cell's share is its cov_set_launch groups, fixed at creation).  A
+ * cell's trajectory in a batch is the one it has alone.  The reference runs
+ * one cell per process (cov/engine/covsearch2e.c:1304 is a single-cell loop);
+ * this has no counterpart there.  While a batch runs its cells belong to it:
+ * no call on a member cell until cov_batch_sync returns, and no member is
+ * destroyed before the batch. */
+int         cov_batch_create(cov_cell_t *const *cells, int k, cov_batch_t **out);
+int         cov_batch_destroy(cov_batch_t *b);
+const char *cov_batch_last_error(const cov_batch_t *b);
+int         cov_batch_size(const cov_batch_t *b);
+/* nsteps more steps of every cell (fewer for a cell that covers); graph mode
+ * replays captured batched steps.  launch returns at once, sync waits and
+ * reports the first cell that stopped on an error; run = launch + sync. */
+int         cov_batch_launch(cov_batch_t *b, int64_t nsteps, int mode);
+int         cov_batch_sync(cov_batch_t *b);
+int         cov_batch_run(cov_batch_t *b, int64_t nsteps, int mode);
 
 #ifdef __cplusplus
 }

@@ -43,11 +43,13 @@ int main(int argc, char** argv)
     DtPass pass[MAXN + 1];
     const int np = dt_plan(q, n, N, budget, pass);
     if (np <= 0) { printf("FAIL plan\n"); return 1; }
-    int axes = 0;
+    int axes = 1;                       // the innermost axis, inside the inner pass's rows
     for (int i = 0; i < np; i++) axes += pass[i].nax;
     if (axes != n) { printf("FAIL axes %d\n", axes); return 1; }
-    uint8_t* D = new uint8_t[N + 64];
-    memset(D, 0xAB, N + 64);
+    // the device allocation: N + 64 bytes, 4-aligned
+    uint32_t* Dw = new uint32_t[(N + 64) / 4];
+    uint8_t* D = reinterpret_cast<uint8_t*>(Dw);
+    memset(D, 0xAB, (N + 64) / 4 * 4);
     const int64_t nw = (N >> 5) + 2;
     uint32_t* cwb = new uint32_t[nw];
     memset(cwb, 0, nw * 4);
@@ -56,32 +58,25 @@ int main(int argc, char** argv)
     const int nth = WG;
     for (int pi = 0; pi < np; pi++) {
         const DtPass& p = pass[pi];
-        const bool inner = pi == 0;
-        const int64_t lds = inner ? p.P : p.rows * p.P;
-        if (lds + (inner ? 0 : 1024) + 4 * DT_HB > 65536) { printf("FAIL lds %lld\n", (long long)lds); return 1; }
-        uint8_t* sm = new uint8_t[lds];
-        uint8_t* shr = new uint8_t[p.rows];
+        const int64_t lds = dt_lds_bytes(p);
+        if (lds + 4 * DT_HB > 65536 || lds > budget) { printf("FAIL lds %lld\n", (long long)lds); return 1; }
+        if ((p.inner != 0) != (pi == 0) || (p.first != 0) != (pi == 0)) { printf("FAIL flags\n"); return 1; }
+        uint32_t* sm = new uint32_t[lds / 4];
         for (int64_t t = 0; t < p.ntiles; t++) {
+            const DtTile T = dt_tile(p, t);
             std::vector<DtHist> h(nth);
             for (auto& x : h) memset(&x, 0, sizeof x);
-            if (inner) {
-                for (int tid = 0; tid < nth; tid++) dt_inner_load(p, t, tid, nth, sm, D, cwb);
-                for (int ax = 0; ax < p.nax; ax++)
-                    for (int tid = 0; tid < nth; tid++) dt_inner_axis(p, t, ax, tid, nth, sm);
-                for (int tid = 0; tid < nth; tid++) dt_inner_store(p, t, tid, nth, sm, D, h[tid]);
-            } else {
-                const DtTile T = dt_outer_tile(p, t);
-                for (int tid = 0; tid < nth; tid++) dt_outer_load(p, T, tid, nth, sm, shr, D);
-                for (int ax = 0; ax < p.nax; ax++)
-                    for (int tid = 0; tid < nth; tid++) dt_outer_axis(p, T, ax, tid, nth, sm, shr);
-                for (int tid = 0; tid < nth; tid++) dt_outer_store(p, T, tid, nth, sm, D, h[tid]);
-            }
+            for (int tid = 0; tid < nth; tid++) dt_load(p, T, tid, nth, sm, reinterpret_cast<const uint32_t*>(D), cwb);
+            if (p.inner)
+                for (int tid = 0; tid < nth; tid++) dt_inrow(p, tid, nth, sm);
+            for (int ax = 0; ax < p.nax; ax++)
+                for (int tid = 0; tid < nth; tid++) dt_axis(p, T, ax, tid, nth, sm);
+            for (int tid = 0; tid < nth; tid++) dt_store(p, T, tid, nth, sm, D, h[tid]);
             if (p.last)
                 for (int tid = 0; tid < nth; tid++)
                     for (int b = 0; b <= n + 1; b++) hist[b] += (h[tid].w[b >> 2] >> (16 * (b & 3))) & 0xFFFF;
         }
         delete[] sm;
-        delete[] shr;
     }
     // brute force on small spaces; on large ones the textbook axis DP, one
     // full-array pass per axis
@@ -115,12 +110,13 @@ int main(int argc, char** argv)
                 if (dd < best) best = dd;
             }
         }
+        if (best == 255) best = 127;    // unreached (no codeword), never with M >= 1
         if (D[w] != best) { if (bad < 5) printf("mismatch w=%lld got %d want %d\n", (long long)w, D[w], best); bad++; }
         bh[best <= n ? best : n + 1]++;
     }
     for (int b = 0; b <= n + 1; b++)
         if (bh[b] != hist[b]) { printf("FAIL hist bin %d %llu vs %llu\n", b, (unsigned long long)hist[b], (unsigned long long)bh[b]); return 1; }
-    for (int64_t k = N; k < N + 64; k++)
+    for (int64_t k = N; k < (N + 64) / 4 * 4; k++)
         if (D[k] != 0xAB) { printf("FAIL wrote past N at %lld\n", (long long)k); return 1; }
     if (bad) { printf("FAIL %lld mismatches\n", (long long)bad); return 1; }
     printf("ok passes=%d tiles=", np);
@@ -128,7 +124,7 @@ int main(int argc, char** argv)
     printf(" hist=");
     for (int b = 0; b <= n; b++) printf("%llu ", (unsigned long long)hist[b]);
     printf("\n");
-    delete[] D;
+    delete[] Dw;
     delete[] cwb;
     return 0;
 }

@@ -1,0 +1,102 @@
+"""Batched cells (one evaluator and one commit launch per step for all cells)
+against the same cells run alone: every cell's counters, code, best code and
+coverage counts must be identical, whatever the mix of shapes, launch modes,
+run splits, kicks and early covers; one cell is also checked against the
+oracle step for step."""
+import numpy as np
+import pytest
+
+from oracle.oracle import Oracle
+
+pytestmark = pytest.mark.gpu
+
+# (q, n, R, M, seed, tenure, stall)
+MIX = [(5, 7, 2, 400, 1, 10, 5000), (6, 6, 3, 41, 2, 3, 20), (4, 8, 3, 60, 3, 10, 5000),
+       (3, 8, 2, 120, 4, 10, 5000), (7, 5, 2, 60, 5, 10, 5000), (6, 9, 4, 660, 6, 10, 5000),
+       (3, 4, 1, 9, 7, 10, 5000), (2, 10, 3, 70, 8, 10, 300)]
+
+
+def _cell(spec):
+    from coldcase_amd import Cell
+    q, n, R, M, seed, tenure, stall = spec
+    c = Cell(q, n, R, M)
+    c.search_params(tenure=tenure, stall=stall, kick=2)
+    c.search_init(seed)
+    return c
+
+
+def _snap(c):
+    st = c.state()
+    best, bu = c.best()
+    return (st['it'], st['uncovered'], st['best_uncovered'], st['kicks'], st['draws'], st['done'],
+            c.code().tobytes(), best.tobytes(), bu, c.cnt().tobytes())
+
+
+def _solo(spec, steps):
+    c = _cell(spec)
+    c.run(steps, mode=1)
+    s = _snap(c)
+    c.close()
+    return s
+
+
+@pytest.mark.parametrize('mode', [0, 1])
+def test_batch_equals_solo(mode):
+    from coldcase_amd import Batch
+    steps = [137, 250]
+    print('case batch mix mode=%d steps=%s' % (mode, steps), flush=True)
+    cells = [_cell(s) for s in MIX]
+    with Batch(cells) as b:
+        for k in steps:
+            b.run(k, mode=mode)
+    for spec, c in zip(MIX, cells):
+        want = _solo(spec, sum(steps))
+        got = _snap(c)
+        assert got[:6] == want[:6], (spec, got[:6], want[:6])
+        assert got[6:] == want[6:], spec
+        c.close()
+
+
+def test_batch_kicks_and_covers():
+    """A cell that covers early stops; the others go on; kicks happen inside
+    the batch."""
+    from coldcase_amd import Batch
+    specs = [(3, 4, 1, 9, 7, 10, 5000), (6, 6, 3, 41, 2, 3, 20), (5, 7, 2, 400, 3, 10, 5000)]
+    print('case batch kicks/covers', flush=True)
+    cells = [_cell(s) for s in specs]
+    with Batch(cells) as b:
+        b.run(400, mode=1)
+    st = [c.state() for c in cells]
+    assert st[1]['kicks'] > 5
+    for spec, c in zip(specs, cells):
+        assert _snap(c) == _solo(spec, 400), spec
+        c.close()
+
+
+def test_batch_cell_matches_oracle():
+    from coldcase_amd import Batch
+    spec = (5, 7, 2, 400, 11, 10, 5000)
+    print('case batch vs oracle', flush=True)
+    cells = [_cell(spec), _cell((6, 6, 3, 41, 12, 3, 20))]
+    with Batch(cells) as b:
+        b.run(700, mode=1)
+    o = Oracle(5, 7, 2, 400, 11).init()
+    o.run(700, tabu=10, stall=5000)
+    c = cells[0]
+    st = c.state()
+    assert (st['it'], st['uncovered'], st['best_uncovered'], st['kicks'], st['draws']) == \
+        (o.iters, o.uncovered, o.best_uncovered, o.kicks, o.draws)
+    assert (c.code() == o.code()).all() and (c.best()[0] == o.best()).all()
+    assert (c.cnt() == o.cnt()).all()
+    for c in cells:
+        c.close()
+
+
+def test_batch_rejects_bad_input():
+    from coldcase_amd import Batch, CovError
+    c = _cell(MIX[0])
+    with pytest.raises(CovError):
+        Batch([c, c])
+    with pytest.raises(CovError):
+        Batch([])
+    c.close()
