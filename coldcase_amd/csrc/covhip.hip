@@ -976,6 +976,14 @@ int cov_set_launch(cov_cell_t* h, int eval_groups, int commit_groups)
     return COV_OK;
 }
 
+int cov_get_launch(const cov_cell_t* h, int* eval_groups, int* commit_groups)
+{
+    if (!h) return COV_EPARAM;
+    if (eval_groups) *eval_groups = h->eval_groups;
+    if (commit_groups) *commit_groups = h->commit_groups;
+    return COV_OK;
+}
+
 int cov_set_transform_tile(cov_cell_t* h, int lds_bytes)
 {
     if (!h || lds_bytes < 0) return COV_EPARAM;

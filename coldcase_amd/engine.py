@@ -31,7 +31,7 @@ EXPORTS = ['cov_version', 'cov_strerror', 'cov_mem_bytes', 'cov_create', 'cov_de
            'cov_first_uncovered_from', 'cov_read_cnt', 'cov_eval_moves', 'cov_commit',
            'cov_distance_transform', 'cov_search_init', 'cov_search_params',
            'cov_search_run', 'cov_search_launch', 'cov_search_state', 'cov_set_launch',
-           'cov_set_transform_tile', 'cov_timing', 'cov_timing_read', 'cov_search_phases',
+           'cov_get_launch', 'cov_set_transform_tile', 'cov_timing', 'cov_timing_read', 'cov_search_phases',
            'cov_search_bytes', 'cov_batch_create', 'cov_batch_destroy', 'cov_batch_last_error',
            'cov_batch_size', 'cov_batch_launch', 'cov_batch_sync', 'cov_batch_run']
 
@@ -90,6 +90,7 @@ def lib():
         'cov_search_launch': (i32, [vp, i64, i32]),
         'cov_search_state': (i32, [vp, ctypes.POINTER(State)]),
         'cov_set_launch': (i32, [vp, i32, i32]),
+        'cov_get_launch': (i32, [vp, vp, vp]),
         'cov_set_transform_tile': (i32, [vp, i32]),
         'cov_timing': (i32, [vp, i32]),
         'cov_timing_read': (i32, [vp, vp, vp, i32]),
@@ -260,6 +261,12 @@ class Cell:
 
     def set_launch(self, eval_groups, commit_groups):
         self._check(lib().cov_set_launch(self.h, int(eval_groups), int(commit_groups)), 'cov_set_launch')
+
+    def get_launch(self):
+        e = np.zeros(1, np.int32)
+        c = np.zeros(1, np.int32)
+        self._check(lib().cov_get_launch(self.h, _p(e), _p(c)), 'cov_get_launch')
+        return int(e[0]), int(c[0])
 
     def set_transform_tile(self, lds_bytes):
         self._check(lib().cov_set_transform_tile(self.h, int(lds_bytes)), 'cov_set_transform_tile')
