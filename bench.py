@@ -22,6 +22,15 @@ Secondary workloads (one JSON line each, not the headline):
                          distance transform (2*n*q^n algorithmic bytes) and
                          the uint16 coverage recount (2*q^n + 4*M*V bytes,
                          M*V atomics), K timed repetitions of each
+  --workload batched     64 instances, (q,n,R) uniform over {5,6,7,8} x
+                         {7,8,9} x {2,3,4} (numpy seed 2026, M > 65535
+                         redrawn), M = ceil(1.3 q^n / |ball|), seeds
+                         1000003 + 7919 i: batched (one launch pair per step
+                         for all) vs plain (one cell at a time) over the same
+                         prefix of steps, then the batched run to 10^5 moves
+  --workload serve       Poisson arrivals (seed 7, 20 req/s) of verify and
+                         10^5-move search requests over cells up to K6(10,4),
+                         continuous batching with HBM-budget admission
 """
 import argparse
 import json
@@ -124,6 +133,137 @@ def bench_transform(a):
     return 0
 
 
+def c4_specs(seed=2026, count=64):
+    """The batched config's instances and how many draws were redrawn."""
+    import math
+    import numpy as np
+
+    def ball(q, n, R):
+        return sum(math.comb(n, i) * (q - 1) ** i for i in range(R + 1))
+
+    rng = np.random.default_rng(seed)
+    specs, redrawn = [], []
+    while len(specs) < count:
+        q = int(rng.choice([5, 6, 7, 8]))
+        n = int(rng.choice([7, 8, 9]))
+        R = int(rng.choice([2, 3, 4]))
+        M = math.ceil(1.3 * q ** n / ball(q, n, R))
+        if M > 65535:
+            redrawn.append((q, n, R, M))
+            continue
+        specs.append((q, n, R, M))
+    return specs, redrawn
+
+
+def bench_batched(a):
+    import torch
+    from coldcase_amd import Batch, Cell
+    from coldcase_amd.autotune import tune_batch
+    from coldcase_amd.engine import MODE_GRAPH
+    torch.cuda.set_device(0)
+    specs, redrawn = c4_specs()
+    moves, prefix = a.steps, a.prefix
+
+    def make(i, s):
+        q, n, R, M = s
+        c = Cell(q, n, R, M)
+        c.search_params(tenure=10, stall=5000, kick=2)
+        c.search_init(1 * 1000003 + i * 7919)
+        return c
+
+    # plain: one cell at a time, its own launches (hipGraph steps), default shape
+    plain_s = 0.0
+    for i, s in enumerate(specs):
+        with make(i, s) as c:
+            c.run(32, mode=MODE_GRAPH)
+            c.sync()
+            t0 = time.perf_counter()
+            c.run(prefix, mode=MODE_GRAPH)
+            plain_s += time.perf_counter() - t0
+        print('plain %d/%d' % (i + 1, len(specs)), file=sys.stderr, flush=True)
+    # batched: all cells in one launch pair per step, batch scale autotuned
+    cells = [make(i, s) for i, s in enumerate(specs)]
+    tuned = tune_batch(cells)
+    for c in cells:                      # restart every cell from its seed after tuning
+        c.search_init(cells.index(c) * 7919 + 1000003)
+    with Batch(cells) as b:
+        b.run(32)
+        t0 = time.perf_counter()
+        b.run(prefix)
+        batched_prefix_s = time.perf_counter() - t0
+        print('batched prefix done', file=sys.stderr, flush=True)
+        done = 32 + prefix
+        t0 = time.perf_counter()
+        while done < moves:
+            k = min(5000, moves - done)
+            b.run(k)
+            done += k
+            print('batched %d/%d' % (done, moves), file=sys.stderr, flush=True)
+        rest_s = time.perf_counter() - t0
+    total_batched = batched_prefix_s + rest_s
+    st = [c.state() for c in cells]
+    # the timed region holds moves - 32 steps of every cell (32 warmed up the graph)
+    solves_batched = len(cells) / (total_batched * moves / (moves - 32)) if moves > 32 else 0.0
+    plain_rate = len(cells) / (plain_s / prefix * moves)             # projected from the prefix
+    batched_prefix_rate = len(cells) / (batched_prefix_s / prefix * moves)
+    out = {
+        'metric': 'batched vs plain solves/s (64 instances, 10^5 moves each)', 'value': round(solves_batched, 4),
+        'unit': 'solves/s', 'n_gpus': 1, 'steps': moves, 'warmup': 32, 'ms_per_step': round(total_batched * 1e3 / max(moves - 32, 1), 4),
+        'higher_is_better': True, 'scaling': 'weak', 'vs_baseline': None, 'dtype': 'u16', 'data': 'synthetic',
+        'config': {'workload': '64 instances (q,n,R) from {5,6,7,8}x{7,8,9}x{2,3,4}, numpy seed 2026, '
+                               'M = ceil(1.3 q^n/|ball|), seeds 1000003 + 7919 i, tenure 10',
+                   'redrawn': redrawn, 'prefix_steps': prefix,
+                   'batch_scale': tuned['factor']},
+        'batched_full_solves_per_s': round(solves_batched, 4),
+        'prefix': {'plain_s': round(plain_s, 3), 'batched_s': round(batched_prefix_s, 3),
+                   'plain_solves_per_s_projected': round(plain_rate, 5),
+                   'batched_solves_per_s_projected': round(batched_prefix_rate, 5),
+                   'ratio': round(plain_s / batched_prefix_s, 3)},
+        'covered': sum(1 for x in st if x['done']),
+    }
+    print(json.dumps(out))
+    for c in cells:
+        c.close()
+    return 0
+
+
+def bench_serve(a):
+    import numpy as np
+    import torch
+    from coldcase_amd.serve import Server, poisson_workload, summarize
+    torch.cuda.set_device(0)
+    shapes = [(5, 7, 2, 400), (6, 6, 3, 41), (6, 8, 4, 169), (6, 9, 4, 660), (7, 9, 4, 1480), (6, 10, 4, 2784)]
+    codes = {}
+    data = os.path.join(ROOT, 'tests', 'golden', 'data')
+    for fn, key in [('K6_10_4_M2784.txt', (6, 10, 4, 2784)), ('K7_9_4_M1480.txt', (7, 9, 4, 1480)),
+                    ('K6_9_4_M660.txt', (6, 9, 4, 660)), ('K6_8_4_M169.txt', (6, 8, 4, 169)),
+                    ('K6_6_3_M41.txt', (6, 6, 3, 41))]:
+        p = os.path.join(data, fn)
+        if os.path.exists(p):
+            from coldcase_amd.verify_cov import parse_code
+            code, _ = parse_code(p, key[0], key[1])
+            codes[key] = np.array(code, dtype=np.uint8)
+    reqs = poisson_workload(7, a.requests, 20.0, shapes, verify_frac=0.3, moves=a.steps, codes=codes)
+    srv = Server(hbm_budget=int(a.budget_gb * (1 << 30)), max_cells=64, tick_steps=a.tick)
+    t0 = time.perf_counter()
+    res = srv.run(reqs, realtime=True)
+    el = time.perf_counter() - t0
+    s = summarize(res)
+    out = {'metric': 'p50/p99 solve latency (served)', 'value': round(s['p50_s'], 4), 'unit': 's',
+           'n_gpus': 1, 'steps': a.steps, 'warmup': 0, 'ms_per_step': None, 'higher_is_better': False,
+           'scaling': 'weak', 'vs_baseline': None, 'dtype': 'u16', 'data': 'synthetic',
+           'config': {'workload': 'Poisson arrivals 20 req/s (seed 7), %d requests, 30%% verify / 70%% search '
+                                  'bursts of %d moves over K5(7,2)..K6(10,4); record codes as verify inputs'
+                                  % (a.requests, a.steps),
+                      'hbm_budget_gb': a.budget_gb, 'tick_steps': a.tick},
+           'latency': s, 'makespan_s': round(el, 3), 'peak_cells': srv.peak_cells,
+           'peak_gb': round(srv.peak_bytes / (1 << 30), 3), 'ticks': srv.ticks,
+           'verify_record_uncovered': sorted({r.uncovered for q_, r in zip(reqs, res)
+                                              if q_.kind == 'verify' and (q_.q, q_.n, q_.R, q_.M) in codes})}
+    print(json.dumps(out))
+    return 0
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument('--gpus', type=int, default=1)
@@ -134,8 +274,12 @@ def main():
     ap.add_argument('--plain-steps', type=int, default=20000)
     ap.add_argument('--cpu-iters', type=int, default=4000)
     ap.add_argument('--no-cpu', action='store_true')
-    ap.add_argument('--workload', choices=['search', 'transform'], default='search')
+    ap.add_argument('--workload', choices=['search', 'transform', 'batched', 'serve'], default='search')
     ap.add_argument('--lds', type=int, default=0, help='transform tile LDS bytes (0 = default)')
+    ap.add_argument('--prefix', type=int, default=1000, help='batched: steps timed in both modes')
+    ap.add_argument('--requests', type=int, default=60, help='serve: requests in the trace')
+    ap.add_argument('--tick', type=int, default=256, help='serve: steps per batched tick at most')
+    ap.add_argument('--budget-gb', type=float, default=64.0, help='serve: HBM budget')
     a = ap.parse_args()
     if a.gpus != 1:
         print('bench.py measures one GPU: every box here has one GPU, and the search '
@@ -148,6 +292,14 @@ def main():
         if a.warmup == ap.get_default('warmup'):
             a.warmup = 2
         return bench_transform(a)
+    if a.workload == 'batched':
+        if a.steps == ap.get_default('steps'):
+            a.steps = 100000
+        return bench_batched(a)
+    if a.workload == 'serve':
+        if a.steps == ap.get_default('steps'):
+            a.steps = 100000
+        return bench_serve(a)
 
     import torch
     from coldcase_amd import Cell

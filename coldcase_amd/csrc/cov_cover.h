@@ -264,41 +264,53 @@ __global__ void __launch_bounds__(WG) k_count_zero(const Cell C, int64_t* out)
     if (threadIdx.x == 0 && s) atomicAdd(reinterpret_cast<unsigned long long*>(out), (unsigned long long)s);
 }
 
-// (d) ordered gather, pass 1: zeros per 4096-word block
-__global__ void __launch_bounds__(WG) k_block_zeros(const Cell C, int64_t* bz)
+// (d) ordered gather of the words a predicate selects: uncovered words
+// (cnt == 0) or words farther than r from the code (dist > r)
+struct SelZero {
+    const uint16_t* cnt;
+    __device__ __forceinline__ bool operator()(int64_t w) const { return cnt[w] == 0; }
+};
+struct SelFar {
+    const uint8_t* dist;
+    int r;
+    __device__ __forceinline__ bool operator()(int64_t w) const { return dist[w] > r; }
+};
+
+// pass 1: selected words per 4096-word block
+template <class Sel>
+__global__ void __launch_bounds__(WG) k_block_count(const Cell C, Sel sel, int64_t* bz)
 {
     __shared__ WgRed red;
-    const uint16_t* cnt = reinterpret_cast<const uint16_t*>(C.cnt32);
     for (int64_t b = blockIdx.x; b < C.nblk; b += gridDim.x) {
         int64_t z = 0;
         const int64_t w0 = b * BLK_WORDS;
         for (int k = threadIdx.x; k < BLK_WORDS; k += WG) {
             const int64_t w = w0 + k;
-            if (w < C.N) z += (cnt[w] == 0);
+            if (w < C.N) z += sel(w);
         }
         const int64_t s = wg_sum(z, red);
         if (threadIdx.x == 0) bz[b] = s;
     }
 }
 
-// (d) ordered gather, pass 2: block b writes its zeros at out[off[b]...]
-__global__ void __launch_bounds__(WG) k_block_emit(const Cell C, const int64_t* off,
+// pass 2: block b writes its selected words at out[off[b]...], ascending
+template <class Sel>
+__global__ void __launch_bounds__(WG) k_block_emit(const Cell C, Sel sel, const int64_t* off,
                                                    int64_t* out, int64_t cap)
 {
     __shared__ WgScan sc;
-    const uint16_t* cnt = reinterpret_cast<const uint16_t*>(C.cnt32);
     constexpr int PER = BLK_WORDS / WG;   // 16 consecutive words per thread
     for (int64_t b = blockIdx.x; b < C.nblk; b += gridDim.x) {
         const int64_t o = off[b];
         if (o >= cap) continue;            // uniform per workgroup
         const int64_t w0 = b * BLK_WORDS + (int64_t)threadIdx.x * PER;
         int64_t mine = 0;
-        for (int k = 0; k < PER; k++) mine += (w0 + k < C.N && cnt[w0 + k] == 0);
+        for (int k = 0; k < PER; k++) mine += (w0 + k < C.N && sel(w0 + k));
         int64_t tot;
         int64_t pos = o + wg_incl_sum(mine, sc, &tot) - mine;
         for (int k = 0; k < PER; k++) {
             const int64_t w = w0 + k;
-            if (w < C.N && cnt[w] == 0) {
+            if (w < C.N && sel(w)) {
                 if (pos < cap) out[pos] = w;
                 pos++;
             }

@@ -43,6 +43,8 @@ struct cov_cell {
     // distance transform
     uint32_t* cwb = nullptr;              // codeword bitmap, all zero between calls
     int dt_lds = 0;                       // LDS bytes per transform tile
+    bool dist_valid = false;              // dist holds the transform of the current code
+    bool cnt_valid = true;                // cnt, planes and counts match the code
 };
 
 // Many cells stepped by the same launches (cov_batch.h).
@@ -304,6 +306,7 @@ int timed_end(cov_cell* h, int kind, hipEvent_t a)
 
 int do_recount(cov_cell* h)
 {
+    h->cnt_valid = true;
     hipEvent_t a = nullptr;
     RC(timed_begin(h, 3, &a));
     const int g = (int)std::min<int64_t>(h->plan.ntile, 2048);
@@ -378,6 +381,36 @@ int ensure_graph(cov_cell* h)
     h->g_eg = h->eval_groups;
     h->g_cg = h->commit_groups;
     return COV_OK;
+}
+
+// first min(cap, total) selected words in ascending order; returns the total
+template <class Sel>
+int64_t ordered_gather(cov_cell* h, Sel sel, int64_t* out, int64_t cap)
+{
+    const Cell& C = h->h;
+    int64_t *dz = nullptr, *dout = nullptr;
+    HCK(h, hipMallocAsync((void**)&dz, (size_t)C.nblk * 8 + 8, h->stream));
+    hipLaunchKernelGGL(k_block_count<Sel>, dim3(grid_for(C.nblk, 1, 1, 8192)), dim3(WG), 0, h->stream, h->h, sel,
+                       dz);
+    HCK(h, hipGetLastError());
+    std::vector<int64_t> bz((size_t)C.nblk);
+    HCK(h, hipMemcpyAsync(bz.data(), dz, (size_t)C.nblk * 8, hipMemcpyDeviceToHost, h->stream));
+    HCK(h, hipStreamSynchronize(h->stream));
+    int64_t total = 0;
+    for (auto& v : bz) { const int64_t t = v; v = total; total += t; }
+    const int64_t m = std::min(cap, total);
+    if (m > 0) {
+        HCK(h, hipMemcpyAsync(dz, bz.data(), (size_t)C.nblk * 8, hipMemcpyHostToDevice, h->stream));
+        HCK(h, hipMallocAsync((void**)&dout, (size_t)m * 8, h->stream));
+        hipLaunchKernelGGL(k_block_emit<Sel>, dim3(grid_for(C.nblk, 1, 1, 8192)), dim3(WG), 0, h->stream, h->h, sel,
+                           dz, dout, m);
+        HCK(h, hipGetLastError());
+        HCK(h, hipMemcpyAsync(out, dout, (size_t)m * 8, hipMemcpyDeviceToHost, h->stream));
+        HCK(h, hipFreeAsync(dout, h->stream));
+    }
+    HCK(h, hipFreeAsync(dz, h->stream));
+    HCK(h, hipStreamSynchronize(h->stream));
+    return total;
 }
 
 }  // namespace
@@ -562,21 +595,32 @@ int cov_sync(cov_cell_t* h)
     return COV_OK;
 }
 
-int cov_set_code(cov_cell_t* h, const uint8_t* digits)
+int cov_set_code_flags(cov_cell_t* h, const uint8_t* digits, int flags)
 {
-    if (!h || !digits) return COV_EPARAM;
+    if (h) h->dist_valid = false;
+    if (!h || !digits || (flags & ~COV_NO_RECOUNT)) return COV_EPARAM;
     const Cell& C = h->h;
     for (int64_t k = 0; k < (int64_t)C.M * C.n; k++)
         if (digits[k] >= C.q) { h->err = "digit out of range"; return COV_EPARAM; }
     RC(upload(h, C.code, digits, (int64_t)C.M * C.n));
     hipLaunchKernelGGL(k_pack_code, dim3(grid_for(C.M, WG, 1, 256)), dim3(WG), 0, h->stream, h->h);
     HCK(h, hipGetLastError());
+    if (flags & COV_NO_RECOUNT) {
+        h->cnt_valid = false;
+        HCK(h, hipStreamSynchronize(h->stream));
+        return COV_OK;
+    }
     RC(do_recount(h));
     HCK(h, hipMemcpyAsync(C.best, C.code, (size_t)C.M * C.n, hipMemcpyDeviceToDevice, h->stream));
     Ctl c;
     RC(read_ctl(h, &c));
     RC(set_ctl<int64_t>(h, offsetof(Ctl, best_unc), c.unc));
     return COV_OK;
+}
+
+int cov_set_code(cov_cell_t* h, const uint8_t* digits)
+{
+    return cov_set_code_flags(h, digits, 0);
 }
 
 int cov_get_code(cov_cell_t* h, uint8_t* digits)
@@ -607,6 +651,7 @@ int cov_recount(cov_cell_t* h)
 
 int cov_ball_update(cov_cell_t* h, const int64_t* centers, int k, int delta)
 {
+    if (h && !h->cnt_valid) { h->err = "coverage counts not built (cov_recount)"; return COV_ESTATE; }
     if (!h || (k > 0 && !centers) || (delta != 1 && delta != -1) || k < 0) return COV_EPARAM;
     if (k == 0) return COV_OK;
     const Cell& C = h->h;
@@ -636,6 +681,7 @@ int cov_ball_update(cov_cell_t* h, const int64_t* centers, int k, int delta)
 
 int64_t cov_count_uncovered(cov_cell_t* h)
 {
+    if (h && !h->cnt_valid) { h->err = "coverage counts not built (cov_recount)"; return COV_ESTATE; }
     if (!h) return COV_EPARAM;
     HCK(h, hipMemsetAsync(h->d_small, 0, 8, h->stream));
     hipLaunchKernelGGL(k_count_zero, dim3(grid_for(h->h.N / 8, WG, 1, 2048)), dim3(WG), 0, h->stream, h->h,
@@ -649,34 +695,24 @@ int64_t cov_count_uncovered(cov_cell_t* h)
 
 int64_t cov_gather_uncovered(cov_cell_t* h, int64_t* out, int64_t cap)
 {
+    if (h && !h->cnt_valid) { h->err = "coverage counts not built (cov_recount)"; return COV_ESTATE; }
     if (!h || cap < 0 || (cap > 0 && !out)) return COV_EPARAM;
-    const Cell& C = h->h;
-    int64_t *dz = nullptr, *dout = nullptr;
-    HCK(h, hipMallocAsync((void**)&dz, (size_t)C.nblk * 8 + 8, h->stream));
-    hipLaunchKernelGGL(k_block_zeros, dim3(grid_for(C.nblk, 1, 1, 8192)), dim3(WG), 0, h->stream, h->h, dz);
-    HCK(h, hipGetLastError());
-    std::vector<int64_t> bz((size_t)C.nblk);
-    HCK(h, hipMemcpyAsync(bz.data(), dz, (size_t)C.nblk * 8, hipMemcpyDeviceToHost, h->stream));
-    HCK(h, hipStreamSynchronize(h->stream));
-    int64_t total = 0;
-    for (auto& v : bz) { const int64_t t = v; v = total; total += t; }
-    const int64_t m = std::min(cap, total);
-    if (m > 0) {
-        HCK(h, hipMemcpyAsync(dz, bz.data(), (size_t)C.nblk * 8, hipMemcpyHostToDevice, h->stream));
-        HCK(h, hipMallocAsync((void**)&dout, (size_t)m * 8, h->stream));
-        hipLaunchKernelGGL(k_block_emit, dim3(grid_for(C.nblk, 1, 1, 8192)), dim3(WG), 0, h->stream, h->h, dz,
-                           dout, m);
-        HCK(h, hipGetLastError());
-        HCK(h, hipMemcpyAsync(out, dout, (size_t)m * 8, hipMemcpyDeviceToHost, h->stream));
-        HCK(h, hipFreeAsync(dout, h->stream));
+    return ordered_gather(h, SelZero{reinterpret_cast<const uint16_t*>(h->h.cnt32)}, out, cap);
+}
+
+int64_t cov_gather_far(cov_cell_t* h, int r, int64_t* out, int64_t cap)
+{
+    if (!h || r < 0 || cap < 0 || (cap > 0 && !out)) return COV_EPARAM;
+    if (!h->h.dist || !h->dist_valid) {
+        h->err = "no distance transform of the current code";
+        return COV_ESTATE;
     }
-    HCK(h, hipFreeAsync(dz, h->stream));
-    HCK(h, hipStreamSynchronize(h->stream));
-    return total;
+    return ordered_gather(h, SelFar{h->h.dist, r}, out, cap);
 }
 
 int64_t cov_first_uncovered_from(cov_cell_t* h, int64_t start)
 {
+    if (h && !h->cnt_valid) { h->err = "coverage counts not built (cov_recount)"; return COV_ESTATE; }
     if (!h || start < 0 || start >= h->h.N) return COV_EPARAM;
     hipLaunchKernelGGL(k_first_from, dim3(1), dim3(WG), 0, h->stream, h->h, start, h->d_small);
     HCK(h, hipGetLastError());
@@ -688,6 +724,7 @@ int64_t cov_first_uncovered_from(cov_cell_t* h, int64_t start)
 
 int cov_read_cnt(cov_cell_t* h, int64_t off, int64_t len, uint16_t* out)
 {
+    if (h && !h->cnt_valid) { h->err = "coverage counts not built (cov_recount)"; return COV_ESTATE; }
     if (!h || off < 0 || len < 0 || off + len > h->h.N || (len && !out)) return COV_EPARAM;
     const uint16_t* src = reinterpret_cast<const uint16_t*>(h->h.cnt32) + off;
     HCK(h, hipMemcpyAsync(out, src, (size_t)len * 2, hipMemcpyDeviceToHost, h->stream));
@@ -697,6 +734,7 @@ int cov_read_cnt(cov_cell_t* h, int64_t off, int64_t len, uint16_t* out)
 
 int cov_eval_moves(cov_cell_t* h, const uint8_t* u, int k, const int32_t* cw, int64_t* dvec)
 {
+    if (h && !h->cnt_valid) { h->err = "coverage counts not built (cov_recount)"; return COV_ESTATE; }
     if (!h || !u || k < 0 || (k > 0 && (!cw || !dvec))) return COV_EPARAM;
     const Cell& C = h->h;
     if (k > C.M) return COV_ELIMIT;
@@ -727,6 +765,8 @@ int cov_eval_moves(cov_cell_t* h, const uint8_t* u, int k, const int32_t* cw, in
 
 int cov_commit(cov_cell_t* h, int i, int p, int v, int64_t* dunc)
 {
+    if (h && !h->cnt_valid) { h->err = "coverage counts not built (cov_recount)"; return COV_ESTATE; }
+    if (h) h->dist_valid = false;
     if (!h) return COV_EPARAM;
     const Cell& C = h->h;
     if (i < 0 || i >= C.M || p < 0 || p >= C.n || v < 0 || v >= C.q) return COV_EPARAM;
@@ -789,6 +829,7 @@ int cov_distance_transform(cov_cell_t* h, uint8_t* dist, int64_t* hist, int* rad
     if (hist)
         for (int d = 0; d <= C.n; d++) hist[d] = hh[d];
     if (radius) *radius = rad;
+    h->dist_valid = true;
     return COV_OK;
 }
 
@@ -804,6 +845,7 @@ int cov_search_params(cov_cell_t* h, int tenure, int64_t stall, int kick, int ca
 
 int cov_search_init(cov_cell_t* h, uint64_t seed, const uint8_t* seed_code, int mseed, int greedy)
 {
+    if (h) h->dist_valid = false;
     if (!h || mseed < 0 || (mseed > 0 && !seed_code)) return COV_EPARAM;
     Cell& C = h->h;
     const int n = C.n, M = C.M;
@@ -903,6 +945,8 @@ int cov_search_init(cov_cell_t* h, uint64_t seed, const uint8_t* seed_code, int 
 
 int cov_search_launch(cov_cell_t* h, int64_t nsteps, int mode)
 {
+    if (h && !h->cnt_valid) { h->err = "coverage counts not built (cov_recount)"; return COV_ESTATE; }
+    if (h) h->dist_valid = false;
     if (!h || nsteps < 0 || (mode != COV_MODE_PLAIN && mode != COV_MODE_GRAPH)) return COV_EPARAM;
     if (nsteps == 0) return COV_OK;
     hipLaunchKernelGGL(k_begin_run, dim3(1), dim3(WG), 0, h->stream, h->h, nsteps);
@@ -1039,7 +1083,7 @@ int cov_batch_create(cov_cell_t* const* cells, int k, cov_batch_t** out)
     if (!cells || k < 1 || !out) return COV_EPARAM;
     *out = nullptr;
     for (int i = 0; i < k; i++) {
-        if (!cells[i]) return COV_EPARAM;
+        if (!cells[i] || !cells[i]->cnt_valid) return COV_EPARAM;
         for (int j = 0; j < i; j++)
             if (cells[j] == cells[i]) return COV_EPARAM;
     }
@@ -1113,6 +1157,7 @@ int cov_batch_launch(cov_batch_t* b, int64_t nsteps, int mode)
     if (!b || nsteps < 0 || (mode != COV_MODE_PLAIN && mode != COV_MODE_GRAPH)) return COV_EPARAM;
     if (nsteps == 0) return COV_OK;
     const int k = (int)b->cells.size();
+    for (cov_cell* c : b->cells) c->dist_valid = false;
     // no trace records from batched runs
     const int64_t zero = 0;
     for (cov_cell* h : b->cells) {

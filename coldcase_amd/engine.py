@@ -26,10 +26,10 @@ TR_FIELDS = ['it', 'u', 'ncand', 'ncw', 'bi', 'bp', 'bv', 'oldv', 'dsel', 'dunc'
 
 # the exported symbols, as declared in include/covhip.h
 EXPORTS = ['cov_version', 'cov_strerror', 'cov_mem_bytes', 'cov_create', 'cov_destroy',
-           'cov_last_error', 'cov_sync', 'cov_set_code', 'cov_get_code', 'cov_get_best',
+           'cov_last_error', 'cov_sync', 'cov_set_code', 'cov_set_code_flags', 'cov_get_code', 'cov_get_best',
            'cov_recount', 'cov_ball_update', 'cov_count_uncovered', 'cov_gather_uncovered',
            'cov_first_uncovered_from', 'cov_read_cnt', 'cov_eval_moves', 'cov_commit',
-           'cov_distance_transform', 'cov_search_init', 'cov_search_params',
+           'cov_distance_transform', 'cov_gather_far', 'cov_search_init', 'cov_search_params',
            'cov_search_run', 'cov_search_launch', 'cov_search_state', 'cov_set_launch',
            'cov_get_launch', 'cov_set_transform_tile', 'cov_timing', 'cov_timing_read', 'cov_search_phases',
            'cov_search_bytes', 'cov_batch_create', 'cov_batch_destroy', 'cov_batch_last_error',
@@ -73,6 +73,7 @@ def lib():
         'cov_last_error': (ctypes.c_char_p, [vp]),
         'cov_sync': (i32, [vp]),
         'cov_set_code': (i32, [vp, vp]),
+        'cov_set_code_flags': (i32, [vp, vp, i32]),
         'cov_get_code': (i32, [vp, vp]),
         'cov_get_best': (i32, [vp, vp, vp]),
         'cov_recount': (i32, [vp]),
@@ -84,6 +85,7 @@ def lib():
         'cov_eval_moves': (i32, [vp, vp, i32, vp, vp]),
         'cov_commit': (i32, [vp, i32, i32, i32, vp]),
         'cov_distance_transform': (i32, [vp, vp, vp, vp]),
+        'cov_gather_far': (i64, [vp, i32, vp, i64]),
         'cov_search_init': (i32, [vp, u64, vp, i32, i32]),
         'cov_search_params': (i32, [vp, i32, i64, i32, i32]),
         'cov_search_run': (i64, [vp, i64, i32, vp]),
@@ -171,9 +173,11 @@ class Cell:
     def word_index(self, words):
         return np.asarray(words, dtype=np.int64).reshape(-1, self.n) @ self.pw
 
-    def set_code(self, code):
+    def set_code(self, code, recount=True):
+        """Upload M x n digits; rebuild the coverage counts unless recount=False
+        (enough for distance_transform / gather_far)."""
         c = np.ascontiguousarray(code, dtype=np.uint8).reshape(self.M, self.n)
-        self._check(lib().cov_set_code(self.h, _p(c)), 'cov_set_code')
+        self._check(lib().cov_set_code_flags(self.h, _p(c), 0 if recount else 1), 'cov_set_code')
 
     def code(self):
         c = np.zeros((self.M, self.n), np.uint8)
@@ -231,6 +235,12 @@ class Cell:
         self._check(lib().cov_distance_transform(self.h, _p(dist), _p(hist), ctypes.byref(rad)),
                     'cov_distance_transform')
         return dist, hist, rad.value
+
+    def gather_far(self, r, cap):
+        """After distance_transform(): (first `cap` words with dist > r, total)."""
+        out = np.zeros(max(int(cap), 1), np.int64)
+        tot = self._check(lib().cov_gather_far(self.h, int(r), _p(out), int(cap)), 'cov_gather_far')
+        return out[:min(int(cap), tot)], tot
 
     # ---------------------------------------------------------------- search
     def search_init(self, seed, seed_code=None, greedy=False):

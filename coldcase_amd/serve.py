@@ -1,0 +1,302 @@
+"""serve -- a continuous-batching request loop on one GPU.
+
+Two request kinds, over cells of any shape (up to the engine's limits):
+
+  verify   a code is checked exhaustively: the uint8 distance transform gives
+           the uncovered count, the covering radius and the distance
+           histogram; the first uncovered words in ascending order are the
+           witnesses (the answer of cov/verify_cov.py:109-230)
+  search   a burst of tabu moves from a seeded start (random, greedy or a
+           given seed code), the reference search of
+           cov/engine/covsearch2e.c:1304-1580; the answer is the best code
+           found, its uncovered count and the run's counters
+
+Requests are admitted as they arrive while the HBM budget (the sum of
+cov_mem_bytes of resident cells) and the cell slots allow; a request that
+does not fit waits, one larger than the whole budget is refused.  Admitted
+searches stay resident and advance together: every tick is one batched run
+(coldcase_amd.engine.Batch, one evaluator and one commit launch per step for
+all of them) of as many steps as the closest-to-done search still needs, up
+to `tick_steps`; searches that finish leave and newly admitted ones join at
+the next tick.  Verifies run on their own streams while a tick is on the
+device.  A request's result does not depend on what else is resident: each
+cell keeps its own counters, rng stream and tabu table.
+"""
+import collections
+import hashlib
+import math
+import time
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from .engine import MODE_GRAPH, Batch, Cell, CovError, mem_bytes
+
+
+@dataclass(eq=False)
+class Request:
+    rid: int
+    kind: str                      # 'verify' or 'search'
+    q: int
+    n: int
+    R: int
+    M: int
+    arrival: float = 0.0           # seconds after the start of the run
+    code: object = None            # verify: the code; search: an optional seed code
+    seed: int = 1
+    moves: int = 100000
+    greedy: bool = False
+    tenure: int = 10
+    stall: int = 5000
+    kick: int = 2
+
+
+@dataclass(eq=False)
+class Result:
+    rid: int
+    kind: str
+    arrival: float
+    admitted: float = 0.0
+    finished: float = 0.0
+    error: str = ''
+    # verify
+    uncovered: int = -1
+    witnesses: list = field(default_factory=list)
+    radius: int = -1
+    hist: list = field(default_factory=list)
+    # search
+    best_uncovered: int = -1
+    iters: int = 0
+    kicks: int = 0
+    best_sha256: str = ''
+
+    @property
+    def latency(self):
+        return self.finished - self.arrival
+
+    def answer(self):
+        """What the request computed, without timings."""
+        if self.kind == 'verify':
+            return (self.error, self.uncovered, tuple(map(tuple, self.witnesses)), self.radius, tuple(self.hist))
+        return (self.error, self.best_uncovered, self.uncovered, self.iters, self.kicks, self.best_sha256)
+
+
+def request_bytes(r):
+    return mem_bytes(r.q, r.n, r.R, r.M)
+
+
+def _digits(w, q, n):
+    out = []
+    for _ in range(n):
+        out.append(w % q)
+        w //= q
+    return tuple(reversed(out))
+
+
+def run_verify(r, witness_cap=5):
+    """One verify request on a fresh cell (code upload, transform, witnesses)."""
+    res = Result(r.rid, r.kind, r.arrival)
+    code = np.ascontiguousarray(r.code, dtype=np.uint8).reshape(-1, r.n)
+    with Cell(r.q, r.n, r.R, code.shape[0]) as c:
+        c.set_code(code, recount=False)
+        _, hist, rad = c.distance_transform()
+        res.hist = [int(x) for x in hist]
+        res.radius = int(rad)
+        res.uncovered = int(r.q ** r.n - hist[:r.R + 1].sum())
+        if res.uncovered and witness_cap:
+            far, _ = c.gather_far(r.R, witness_cap)
+            res.witnesses = [_digits(int(x), r.q, r.n) for x in far]
+    return res
+
+
+def start_search(r):
+    c = Cell(r.q, r.n, r.R, r.M)
+    c.search_params(tenure=r.tenure, stall=r.stall, kick=r.kick)
+    seed_code = None if r.code is None else np.ascontiguousarray(r.code, dtype=np.uint8).reshape(-1, r.n)
+    c.search_init(r.seed, seed_code=seed_code, greedy=r.greedy)
+    return c
+
+
+def finish_search(r, c):
+    res = Result(r.rid, r.kind, r.arrival)
+    st = c.state()
+    best, bu = c.best()
+    res.best_uncovered = int(bu)
+    res.uncovered = int(st['uncovered'])
+    res.iters = int(st['it'])
+    res.kicks = int(st['kicks'])
+    res.best_sha256 = hashlib.sha256(best.tobytes()).hexdigest()
+    return res
+
+
+class Server:
+    """Continuous batching with HBM-budget admission (see the module text).
+
+    hbm_budget  bytes of device memory the resident cells may use
+    max_cells   resident searches at once
+    tick_steps  steps per batched tick at most
+    launch      optional callable(Cell) that sets the cell's launch shape
+                (the autotuner's choice for batched runs)
+    """
+
+    def __init__(self, hbm_budget=64 << 30, max_cells=64, tick_steps=256, mode=MODE_GRAPH, launch=None):
+        self.budget = int(hbm_budget)
+        self.max_cells = int(max_cells)
+        self.tick_steps = int(tick_steps)
+        self.mode = mode
+        self.launch = launch
+        self.peak_bytes = 0
+        self.peak_cells = 0
+        self.ticks = 0
+
+    def run(self, requests, realtime=True):
+        """Serve `requests` (arrival times in seconds from now); returns the
+        results in request order."""
+        reqs = sorted(requests, key=lambda r: (r.arrival, r.rid))
+        results = {}
+        waiting = collections.deque()
+        active = []                     # [request, cell, steps left, admitted time]
+        batch, members = None, ()
+        used = 0
+        i = 0
+        t0 = time.perf_counter()
+
+        def now():
+            return time.perf_counter() - t0
+
+        try:
+            while i < len(reqs) or waiting or active:
+                t = now()
+                if not realtime and not waiting and not active and i < len(reqs):
+                    t = max(t, reqs[i].arrival)
+                while i < len(reqs) and reqs[i].arrival <= t:
+                    waiting.append(reqs[i])
+                    i += 1
+                # admission, first fit in arrival order
+                verifies = []
+                for r in list(waiting):
+                    try:
+                        need = request_bytes(r)
+                    except CovError as e:
+                        need, err = None, str(e)
+                    if need is None or need > self.budget:
+                        waiting.remove(r)
+                        res = Result(r.rid, r.kind, r.arrival, admitted=now(), finished=now(),
+                                     error=err if need is None else 'larger than the HBM budget')
+                        results[r.rid] = res
+                        continue
+                    if used + need > self.budget:
+                        continue
+                    if r.kind == 'search' and len(active) >= self.max_cells:
+                        continue
+                    waiting.remove(r)
+                    used += need
+                    if r.kind == 'verify':
+                        verifies.append((r, need, now()))
+                    else:
+                        try:
+                            c = start_search(r)
+                            if self.launch:
+                                self.launch(c)
+                            active.append([r, c, r.moves, now()])
+                        except CovError as e:
+                            used -= need
+                            results[r.rid] = Result(r.rid, r.kind, r.arrival, admitted=now(), finished=now(),
+                                                    error=str(e))
+                self.peak_bytes = max(self.peak_bytes, used)
+                self.peak_cells = max(self.peak_cells, len(active))
+                # one tick of every resident search
+                if active:
+                    cells = tuple(a[1] for a in active)
+                    if cells != members:
+                        if batch is not None:
+                            batch.close()
+                        batch, members = Batch(list(cells)), cells
+                    k = min(self.tick_steps, min(a[2] for a in active))
+                    batch.launch(k, mode=self.mode)
+                # verifies meanwhile, on their own streams
+                for r, need, ta in verifies:
+                    try:
+                        res = run_verify(r)
+                    except CovError as e:
+                        res = Result(r.rid, r.kind, r.arrival, error=str(e))
+                    res.admitted, res.finished = ta, now()
+                    results[r.rid] = res
+                    used -= need
+                if active:
+                    batch.sync()
+                    self.ticks += 1
+                    left = []
+                    for a in active:
+                        a[2] -= k
+                        st_done = a[2] <= 0
+                        if not st_done and a[1].state()['done']:
+                            st_done = True
+                        if st_done:
+                            res = finish_search(a[0], a[1])
+                            res.admitted, res.finished = a[3], now()
+                            results[a[0].rid] = res
+                            used -= request_bytes(a[0])
+                        else:
+                            left.append(a)
+                    if len(left) != len(active):
+                        batch.close()
+                        batch, members = None, ()
+                        keep = {id(a) for a in left}
+                        for a in active:
+                            if id(a) not in keep:
+                                a[1].close()
+                    active = left
+                elif not verifies and i < len(reqs) and not waiting:
+                    if realtime:
+                        time.sleep(max(0.0, min(reqs[i].arrival - now(), 0.05)))
+        finally:
+            if batch is not None:
+                batch.close()
+            for a in active:
+                a[1].close()
+        return [results[r.rid] for r in sorted(requests, key=lambda r: r.rid)]
+
+
+def summarize(results):
+    """p50 / p99 latency and throughput of a served run."""
+    ok = [r for r in results if not r.error]
+    lat = np.array([r.latency for r in ok]) if ok else np.zeros(1)
+    span = (max(r.finished for r in ok) - min(r.arrival for r in ok)) if ok else 0.0
+    out = {'requests': len(results), 'completed': len(ok), 'errors': len(results) - len(ok),
+           'p50_s': float(np.percentile(lat, 50)), 'p99_s': float(np.percentile(lat, 99)),
+           'max_s': float(lat.max()), 'throughput_rps': len(ok) / span if span > 0 else 0.0}
+    for kind in ('verify', 'search'):
+        lk = np.array([r.latency for r in ok if r.kind == kind])
+        if lk.size:
+            out[kind] = {'n': int(lk.size), 'p50_s': float(np.percentile(lk, 50)),
+                         'p99_s': float(np.percentile(lk, 99))}
+    return out
+
+
+def poisson_workload(seed, n_req, rate, shapes, verify_frac=0.3, moves=100000, codes=None):
+    """A seeded request trace: exponential inter-arrival times of mean
+    1/rate, each request a verify (probability verify_frac) or a search
+    burst of `moves` steps, on a shape drawn uniformly from `shapes`
+    ((q, n, R, M) tuples).  Verify codes come from `codes[(q, n, R, M)]` when
+    given (for instance the record codes), else uniform random digits;
+    search seeds follow seed * 1000003 + rid * 7919."""
+    rng = np.random.default_rng(seed)
+    t = 0.0
+    out = []
+    for rid in range(n_req):
+        t += float(rng.exponential(1.0 / rate))
+        q, n, R, M = shapes[int(rng.integers(len(shapes)))]
+        if rng.random() < verify_frac:
+            code = (codes or {}).get((q, n, R, M))
+            if code is None:
+                code = rng.integers(0, q, size=(M, n)).astype(np.uint8)
+            out.append(Request(rid, 'verify', q, n, R, M, arrival=t, code=code))
+        else:
+            out.append(Request(rid, 'search', q, n, R, M, arrival=t, seed=seed * 1000003 + rid * 7919,
+                               moves=moves))
+    return out
+
+
+def ball_volume(q, n, R):
+    return sum(math.comb(n, i) * (q - 1) ** i for i in range(R + 1))

@@ -140,13 +140,13 @@ def verify_numpy(code, n, q, R, want_witness=True, max_witness=5, n1=None):
         return 0, []
     arr = _as_array(code, n)
     with Cell(q, n, R, arr.shape[0]) as cell:
-        cell.set_code(arr)
-        dist, hist, _ = cell.distance_transform(want_array=bool(want_witness))
-    n_unc = int(q ** n - hist[:R + 1].sum())
-    wit = []
-    if n_unc and want_witness:
-        far = np.flatnonzero(dist > R)[:max_witness]
-        wit = [_digits_of(int(x), q, n) for x in far]
+        cell.set_code(arr, recount=False)
+        _, hist, _ = cell.distance_transform()
+        n_unc = int(q ** n - hist[:R + 1].sum())
+        wit = []
+        if n_unc and want_witness:
+            far, _ = cell.gather_far(R, max_witness)
+            wit = [_digits_of(int(x), q, n) for x in far]
     return n_unc, wit
 
 

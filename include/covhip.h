@@ -75,6 +75,11 @@ int         cov_sync(cov_cell_t *h);
  * Replaces dct_set_code + dct_transform(0,2) (cov/gemm2/dct.cu:351-411) and
  * rebuild() (cov/engine/covsearch2e.c:958-967). */
 int     cov_set_code(cov_cell_t *h, const uint8_t *digits);
+/* The same with flags: COV_NO_RECOUNT uploads the code only (for a distance
+ * transform alone); calls that need the counts then return COV_ESTATE until
+ * cov_recount. */
+#define COV_NO_RECOUNT 1
+int     cov_set_code_flags(cov_cell_t *h, const uint8_t *digits, int flags);
 int     cov_get_code(cov_cell_t *h, uint8_t *digits);
 int     cov_get_best(cov_cell_t *h, uint8_t *digits, int64_t *best_uncovered);
 /* (b) recount: cnt from the current code, LDS-aggregated per word tile. */
@@ -113,6 +118,12 @@ int     cov_commit(cov_cell_t *h, int i, int p, int v, int64_t *dunc);
  * radius are returned.  Replaces run_mindist / run_dilate
  * (cov/verify_independent.py:128-185) and covering_radius (cov/verify_cov.py:237). */
 int     cov_distance_transform(cov_cell_t *h, uint8_t *dist, int64_t *hist, int *radius);
+
+/* (d) after cov_distance_transform of the current code: #{w : dist[w] > r},
+ * and the first min(cap, total) such words in ascending order; COV_ESTATE if
+ * the code changed since the transform.  With r = R these are the uncovered
+ * words and verify_numpy's witnesses (cov/verify_cov.py:169-230). */
+int64_t cov_gather_far(cov_cell_t *h, int r, int64_t *out, int64_t cap);
 
 /* --------------------------------------------------------------- search */
 /* Seed the rng (splitmix64 -> xoshiro256++, cov/engine/covsearch2e.c:211-219)

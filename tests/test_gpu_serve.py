@@ -1,0 +1,71 @@
+"""The serving loop on the GPU: every served result equals the same request
+run alone (tolerance 0), verify answers match the oracle, admission keeps
+resident memory within the budget, and oversized requests are refused."""
+import numpy as np
+import pytest
+
+from oracle.verify_oracle import mindist
+
+pytestmark = pytest.mark.gpu
+
+SHAPES = [(5, 7, 2, 400), (6, 6, 3, 41), (4, 8, 3, 60), (3, 8, 2, 120), (6, 9, 4, 660)]
+
+
+def _trace(n_req=14, rate=60.0, moves=300, seed=5):
+    from coldcase_amd.serve import poisson_workload
+    return poisson_workload(seed, n_req, rate, SHAPES, verify_frac=0.35, moves=moves)
+
+
+def _alone(r):
+    from coldcase_amd.serve import Server
+    return Server(tick_steps=10 ** 9).run([r], realtime=False)[0]
+
+
+@pytest.mark.parametrize('tick', [64, 1000])
+def test_served_equals_alone(tick):
+    from coldcase_amd.serve import Server, summarize
+    print('case serve mix tick=%d' % tick, flush=True)
+    reqs = _trace()
+    assert {r.kind for r in reqs} == {'verify', 'search'}
+    srv = Server(tick_steps=tick)
+    res = srv.run(reqs, realtime=True)
+    s = summarize(res)
+    assert s['errors'] == 0 and s['completed'] == len(reqs)
+    assert srv.peak_cells >= 2                      # searches really overlapped
+    for r, got in zip(reqs, res):
+        assert got.rid == r.rid
+        assert got.answer() == _alone(r).answer(), r.rid
+        assert got.finished >= got.admitted >= got.arrival - 1e-6
+
+
+def test_verify_answers_match_oracle():
+    from coldcase_amd.serve import Request, Server
+    rng = np.random.default_rng(9)
+    reqs = []
+    for rid, (q, n, R, M) in enumerate([(5, 7, 2, 400), (4, 8, 3, 60), (3, 8, 2, 30)]):
+        code = rng.integers(0, q, size=(M, n)).astype(np.uint8)
+        reqs.append(Request(rid, 'verify', q, n, R, M, code=code))
+    res = Server().run(reqs, realtime=False)
+    for r, got in zip(reqs, res):
+        d = mindist(r.code, r.n, r.q)
+        far = np.flatnonzero(d > r.R)
+        assert got.uncovered == far.size
+        assert got.radius == int(d.max())
+        assert got.hist == np.bincount(d, minlength=r.n + 1)[:r.n + 1].tolist()
+        want = [tuple(int(x) for x in np.base_repr(int(w), r.q).zfill(r.n)) for w in far[:5]]
+        assert [tuple(w) for w in got.witnesses] == want
+
+
+def test_budget_admission_and_refusal():
+    from coldcase_amd.engine import mem_bytes
+    from coldcase_amd.serve import Request, Server
+    print('case serve budget', flush=True)
+    small = [Request(i, 'search', 5, 7, 2, 400, seed=100 + i, moves=200) for i in range(5)]
+    one = mem_bytes(5, 7, 2, 400)
+    big = Request(9, 'search', 6, 10, 4, 2784, moves=10)
+    srv = Server(hbm_budget=2 * one + one // 2)      # room for two at a time
+    res = srv.run(small + [big], realtime=False)
+    assert srv.peak_bytes <= 2 * one + one // 2 and srv.peak_cells == 2
+    for r, got in zip(small, res[:5]):
+        assert not got.error and got.answer() == _alone(r).answer()
+    assert res[5].error
