@@ -271,7 +271,7 @@ __device__ inline int64_t wave_ring_fill(const Cell& C, Ctl* ctl, int64_t upto, 
     upto = (int64_t)uniform64((uint64_t)upto);
     budget = (int64_t)uniform64((uint64_t)budget);
     if (upto < lim) lim = upto;
-    if (g0 + budget < lim) lim = g0 + budget;
+    if (budget < lim - g0) lim = g0 + budget;     // (budget may be INT64_MAX: no g0 + budget)
     int64_t g = g0;
     while (g < lim) {
         const int nb = (lim - g < 64) ? (int)(lim - g) : 64;

@@ -73,3 +73,12 @@ def test_graph_mode_identical():
 
 def test_k6_9_4_random_tabu():
     compare(6, 9, 4, 660, 1, 200, threads=8)
+
+
+def test_large_cells_random_init():
+    """Random inits that end with no rng draws to spare (K6(10,4): 27840 of
+    them) and shuffles longer than what is left after the pick: every later
+    draw must come from a freshly generated ring slot."""
+    compare(6, 10, 4, 2784, 1, 12, check_cnt=False, threads=8)
+    compare(7, 9, 4, 1480, 1, 12, check_cnt=False, threads=8)
+    compare(6, 9, 4, 1400, 1, 60, check_cnt=False, threads=8)
