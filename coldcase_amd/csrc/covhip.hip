@@ -1208,6 +1208,16 @@ int cov_batch_sync(cov_batch_t* b)
     return COV_OK;
 }
 
+int cov_batch_query(cov_batch_t* b)
+{
+    if (!b) return COV_EPARAM;
+    const hipError_t e = hipStreamQuery(b->stream);
+    if (e == hipSuccess) return 1;
+    if (e == hipErrorNotReady) return 0;
+    b->err = std::string("hipStreamQuery: ") + hipGetErrorString(e);
+    return -(100 + (int)e);
+}
+
 int cov_batch_run(cov_batch_t* b, int64_t nsteps, int mode)
 {
     RC(cov_batch_launch(b, nsteps, mode));

@@ -33,7 +33,7 @@ EXPORTS = ['cov_version', 'cov_strerror', 'cov_mem_bytes', 'cov_create', 'cov_de
            'cov_search_run', 'cov_search_launch', 'cov_search_state', 'cov_set_launch',
            'cov_get_launch', 'cov_set_transform_tile', 'cov_timing', 'cov_timing_read', 'cov_search_phases',
            'cov_search_bytes', 'cov_batch_create', 'cov_batch_destroy', 'cov_batch_last_error',
-           'cov_batch_size', 'cov_batch_launch', 'cov_batch_sync', 'cov_batch_run']
+           'cov_batch_size', 'cov_batch_launch', 'cov_batch_sync', 'cov_batch_query', 'cov_batch_run']
 
 
 class CovError(RuntimeError):
@@ -104,6 +104,7 @@ def lib():
         'cov_batch_size': (i32, [vp]),
         'cov_batch_launch': (i32, [vp, i64, i32]),
         'cov_batch_sync': (i32, [vp]),
+        'cov_batch_query': (i32, [vp]),
         'cov_batch_run': (i32, [vp, i64, i32]),
     }
     for name, (res, args) in sig.items():
@@ -331,6 +332,10 @@ class Batch:
 
     def sync(self):
         self._check(lib().cov_batch_sync(self.h), 'cov_batch_sync')
+
+    def query(self):
+        """True when the launched steps have finished (does not wait)."""
+        return self._check(lib().cov_batch_query(self.h), 'cov_batch_query') == 1
 
     def run(self, nsteps, mode=MODE_GRAPH):
         self._check(lib().cov_batch_run(self.h, int(nsteps), int(mode)), 'cov_batch_run')

@@ -153,109 +153,111 @@ class Server:
         """Serve `requests` (arrival times in seconds from now); returns the
         results in request order."""
         reqs = sorted(requests, key=lambda r: (r.arrival, r.rid))
-        results = {}
-        waiting = collections.deque()
+        self._results = {}
+        self._waiting = collections.deque()
+        self._joining = []              # admitted searches waiting for the next tick
+        self._used = 0
+        self._next = 0
         active = []                     # [request, cell, steps left, admitted time]
         batch, members = None, ()
-        used = 0
-        i = 0
-        t0 = time.perf_counter()
-
-        def now():
-            return time.perf_counter() - t0
-
+        self._t0 = time.perf_counter()
         try:
-            while i < len(reqs) or waiting or active:
-                t = now()
-                if not realtime and not waiting and not active and i < len(reqs):
-                    t = max(t, reqs[i].arrival)
-                while i < len(reqs) and reqs[i].arrival <= t:
-                    waiting.append(reqs[i])
-                    i += 1
-                # admission, first fit in arrival order
-                verifies = []
-                for r in list(waiting):
-                    try:
-                        need = request_bytes(r)
-                    except CovError as e:
-                        need, err = None, str(e)
-                    if need is None or need > self.budget:
-                        waiting.remove(r)
-                        res = Result(r.rid, r.kind, r.arrival, admitted=now(), finished=now(),
-                                     error=err if need is None else 'larger than the HBM budget')
-                        results[r.rid] = res
-                        continue
-                    if used + need > self.budget:
-                        continue
-                    if r.kind == 'search' and len(active) >= self.max_cells:
-                        continue
-                    waiting.remove(r)
-                    used += need
-                    if r.kind == 'verify':
-                        verifies.append((r, need, now()))
-                    else:
-                        try:
-                            c = start_search(r)
-                            if self.launch:
-                                self.launch(c)
-                            active.append([r, c, r.moves, now()])
-                        except CovError as e:
-                            used -= need
-                            results[r.rid] = Result(r.rid, r.kind, r.arrival, admitted=now(), finished=now(),
-                                                    error=str(e))
-                self.peak_bytes = max(self.peak_bytes, used)
+            while self._next < len(reqs) or self._waiting or self._joining or active:
+                self._admit(reqs, realtime, len(active))
+                active.extend(self._joining)
+                self._joining = []
                 self.peak_cells = max(self.peak_cells, len(active))
-                # one tick of every resident search
-                if active:
-                    cells = tuple(a[1] for a in active)
-                    if cells != members:
-                        if batch is not None:
-                            batch.close()
-                        batch, members = Batch(list(cells)), cells
-                    k = min(self.tick_steps, min(a[2] for a in active))
-                    batch.launch(k, mode=self.mode)
-                # verifies meanwhile, on their own streams
-                for r, need, ta in verifies:
-                    try:
-                        res = run_verify(r)
-                    except CovError as e:
-                        res = Result(r.rid, r.kind, r.arrival, error=str(e))
-                    res.admitted, res.finished = ta, now()
-                    results[r.rid] = res
-                    used -= need
-                if active:
-                    batch.sync()
-                    self.ticks += 1
-                    left = []
-                    for a in active:
-                        a[2] -= k
-                        st_done = a[2] <= 0
-                        if not st_done and a[1].state()['done']:
-                            st_done = True
-                        if st_done:
-                            res = finish_search(a[0], a[1])
-                            res.admitted, res.finished = a[3], now()
-                            results[a[0].rid] = res
-                            used -= request_bytes(a[0])
-                        else:
-                            left.append(a)
-                    if len(left) != len(active):
+                if not active:
+                    if self._next < len(reqs) and realtime:
+                        time.sleep(max(0.0, min(reqs[self._next].arrival - self._now(), 0.01)))
+                    continue
+                cells = tuple(a[1] for a in active)
+                if cells != members:
+                    if batch is not None:
                         batch.close()
-                        batch, members = None, ()
-                        keep = {id(a) for a in left}
-                        for a in active:
-                            if id(a) not in keep:
-                                a[1].close()
-                    active = left
-                elif not verifies and i < len(reqs) and not waiting:
-                    if realtime:
-                        time.sleep(max(0.0, min(reqs[i].arrival - now(), 0.05)))
+                    batch, members = Batch(list(cells)), cells
+                k = min(self.tick_steps, min(a[2] for a in active))
+                batch.launch(k, mode=self.mode)
+                # arrivals while the tick runs: verifies at once, searches join next tick
+                while not batch.query():
+                    if not self._admit(reqs, realtime, len(active) + len(self._joining)):
+                        time.sleep(0.0005)
+                batch.sync()
+                self.ticks += 1
+                left = []
+                for a in active:
+                    a[2] -= k
+                    if a[2] <= 0 or a[1].state()['done']:
+                        res = finish_search(a[0], a[1])
+                        res.admitted, res.finished = a[3], self._now()
+                        self._results[a[0].rid] = res
+                        self._used -= request_bytes(a[0])
+                        a[1].close()
+                    else:
+                        left.append(a)
+                active = left
         finally:
             if batch is not None:
                 batch.close()
-            for a in active:
+            for a in active + self._joining:
                 a[1].close()
-        return [results[r.rid] for r in sorted(requests, key=lambda r: r.rid)]
+        return [self._results[r.rid] for r in sorted(requests, key=lambda r: r.rid)]
+
+    def _now(self):
+        return time.perf_counter() - self._t0
+
+    def _admit(self, reqs, realtime, resident):
+        """Move arrivals into the queue, admit what fits (first fit in arrival
+        order), run admitted verifies, start admitted searches; returns
+        whether it did anything."""
+        did = False
+        t = self._now()
+        if not realtime and not self._waiting and resident == 0 and self._next < len(reqs):
+            t = max(t, reqs[self._next].arrival)
+        while self._next < len(reqs) and reqs[self._next].arrival <= t:
+            self._waiting.append(reqs[self._next])
+            self._next += 1
+            did = True
+        for r in list(self._waiting):
+            try:
+                need, err = request_bytes(r), ''
+            except CovError as e:
+                need, err = None, str(e)
+            if need is None or need > self.budget:
+                self._waiting.remove(r)
+                self._results[r.rid] = Result(r.rid, r.kind, r.arrival, admitted=self._now(),
+                                              finished=self._now(), error=err or 'larger than the HBM budget')
+                did = True
+                continue
+            if self._used + need > self.budget:
+                continue
+            if r.kind == 'search' and resident >= self.max_cells:
+                continue
+            self._waiting.remove(r)
+            self._used += need
+            self.peak_bytes = max(self.peak_bytes, self._used)
+            did = True
+            ta = self._now()
+            if r.kind == 'verify':
+                try:
+                    res = run_verify(r)
+                except CovError as e:
+                    res = Result(r.rid, r.kind, r.arrival, error=str(e))
+                res.admitted, res.finished = ta, self._now()
+                self._results[r.rid] = res
+                self._used -= need
+            else:
+                try:
+                    c = start_search(r)
+                    if self.launch:
+                        self.launch(c)
+                    self._joining.append([r, c, r.moves, ta])
+                    resident += 1
+                except CovError as e:
+                    self._used -= need
+                    self._results[r.rid] = Result(r.rid, r.kind, r.arrival, admitted=ta, finished=self._now(),
+                                                  error=str(e))
+        return did
 
 
 def summarize(results):

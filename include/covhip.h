@@ -184,6 +184,8 @@ int         cov_batch_size(const cov_batch_t *b);
  * reports the first cell that stopped on an error; run = launch + sync. */
 int         cov_batch_launch(cov_batch_t *b, int64_t nsteps, int mode);
 int         cov_batch_sync(cov_batch_t *b);
+/* 1 when the launched steps have finished, 0 while they run (no wait). */
+int         cov_batch_query(cov_batch_t *b);
 int         cov_batch_run(cov_batch_t *b, int64_t nsteps, int mode);
 
 #ifdef __cplusplus
