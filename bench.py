@@ -85,11 +85,12 @@ def bench_transform(a):
     N = q ** n
     code = np.random.default_rng(3).integers(0, q, size=(M, n)).astype(np.uint8)
     cell = Cell(q, n, R, M)
-    if a.lds:
-        cell.set_transform_tile(a.lds)
+    if a.lds or a.threads:
+        cell.set_transform_tile(a.lds, a.threads)
     cell.set_code(code)
     for _ in range(max(a.warmup, 1)):
         cell.distance_transform()
+        cell.distance_stats(R)
         cell.recount()
     cell.sync()
     steps = a.steps
@@ -103,13 +104,22 @@ def bench_transform(a):
         cell.recount()
     cell.sync()
     el_r = (time.perf_counter() - t0) / steps
+    # the transform as verify uses it (covered count and radius), then with the
+    # full histogram, then the recount; HIP events around every launch
+    cell.timing(True)
+    for _ in range(steps):
+        within, rad_s = cell.distance_stats(R)
+    ms, launches = cell.timing_read()
+    cell.timing(False)
+    dt_us = ms[2] * 1e3 / steps            # all passes of one transform
+    npass = int(launches[2]) // steps
     cell.timing(True)
     for _ in range(steps):
         cell.distance_transform()
         cell.recount()
     ms, launches = cell.timing_read()
     cell.timing(False)
-    dt_us = ms[2] * 1e3 / steps            # all passes of one transform
+    dt_hist_us = ms[2] * 1e3 / steps
     rc_us = ms[3] * 1e3 / max(int(launches[3]), 1)
     alg_t = 2 * n * N
     alg_r = 2 * N + 4 * M * V
@@ -119,11 +129,13 @@ def bench_transform(a):
         'n_gpus': 1, 'steps': steps, 'warmup': a.warmup, 'ms_per_step': round(el_t * 1e3, 4),
         'higher_is_better': True, 'scaling': 'weak', 'vs_baseline': None, 'dtype': 'u8', 'data': 'synthetic',
         'config': {'workload': 'K7(9,4) M=1480 random code (numpy seed 3): distance transform + recount',
-                   'words': N, 'passes_per_transform': int(launches[2]) // steps,
-                   'hist': [int(x) for x in hist], 'radius': rad},
+                   'words': N, 'passes_per_transform': npass,
+                   'hist': [int(x) for x in hist], 'radius': rad, 'covered': within,
+                   'uncovered': N - within},
         'roofline': {'bound': 'hbm', 'achieved': round(ach, 1), 'peak': HBM_PEAK_GBS, 'unit': 'GB/s',
                      'frac': round(ach / HBM_PEAK_GBS, 4), 'traffic': None, 'kernel': 'k_dt_inner+k_dt_outer',
-                     'avg_transform_us': round(dt_us, 2), 'alg_bytes_per_transform': alg_t},
+                     'avg_transform_us': round(dt_us, 2), 'alg_bytes_per_transform': alg_t,
+                     'with_histogram_us': round(dt_hist_us, 2)},
         'recount': {'avg_us': round(rc_us, 2), 'wall_ms': round(el_r * 1e3, 4), 'alg_bytes': alg_r,
                     'achieved_GBs': round(alg_r / (rc_us * 1e-6) / 1e9, 1),
                     'atomics_per_s': round(M * V / (rc_us * 1e-6), 1)},
@@ -276,6 +288,7 @@ def main():
     ap.add_argument('--no-cpu', action='store_true')
     ap.add_argument('--workload', choices=['search', 'transform', 'batched', 'serve'], default='search')
     ap.add_argument('--lds', type=int, default=0, help='transform tile LDS bytes (0 = default)')
+    ap.add_argument('--threads', type=int, default=0, help='transform workgroup threads (0 = default)')
     ap.add_argument('--prefix', type=int, default=1000, help='batched: steps timed in both modes')
     ap.add_argument('--requests', type=int, default=60, help='serve: requests in the trace')
     ap.add_argument('--tick', type=int, default=256, help='serve: steps per batched tick at most')

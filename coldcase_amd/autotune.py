@@ -80,18 +80,19 @@ def tune_transform(q, n, R, M, seed=1, reps=5):
     best, times = None, {}
     with Cell(q, n, R, M) as cell:
         cell.set_code(code, recount=False)
-        for lds in (8192, 16384, 24576, 32768, 49152, 63488):
-            try:
-                cell.set_transform_tile(lds)
-            except CovError:
-                continue
-            cell.distance_transform()
-            t = time.perf_counter()
-            for _ in range(reps):
+        for lds in (16384, 24576, 32768, 40960, 49152, 63488):
+            for nt in (256, 512, 1024):
+                try:
+                    cell.set_transform_tile(lds, nt)
+                except CovError:
+                    continue
                 cell.distance_transform()
-            times[lds] = (time.perf_counter() - t) / reps
+                t = time.perf_counter()
+                for _ in range(reps):
+                    cell.distance_transform()
+                times[(lds, nt)] = (time.perf_counter() - t) / reps
         best = min(times, key=times.get)
-    return {'lds': best, 'transform_us': times[best] * 1e6}
+    return {'lds': best[0], 'threads': best[1], 'transform_us': times[best] * 1e6}
 
 
 def tune_batch(cells, factors=(1.0, 0.5, 0.25, 0.125, 0.0625), steps=64):
@@ -147,7 +148,7 @@ class Table:
         if 'eval_groups' in row:
             cell.set_launch(max(2, int(row['eval_groups'] * scale)), max(1, int(row['commit_groups'] * scale)))
         if 'lds' in row:
-            cell.set_transform_tile(row['lds'])
+            cell.set_transform_tile(row['lds'], row.get('threads', 0))
         return True
 
 

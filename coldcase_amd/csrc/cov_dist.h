@@ -35,23 +35,33 @@ constexpr int DT_HB = MAXN + 2;      // histogram bins: 0..n, and unreached in b
 constexpr uint32_t DT_INF = 0x7Fu;   // "not reached yet"
 constexpr uint32_t DT_INF4 = 0x7F7F7F7Fu;
 
-// one pass of the transform, planned on the host
+// one pass of the transform, planned on the host.  LDS holds, per tile, the
+// rows realigned to 16 bytes ("aligned", at offset 0) and the 16-byte global
+// chunks they were read from ("raw", at raw_off).
 struct DtPass {
     int64_t N;
     int64_t nrow;        // rows per tile
-    int64_t S_row;       // words between consecutive rows of a tile
-    int64_t S_tile;      // words between consecutive row blocks (nrow * S_row)
+    int64_t S_row;       // words between consecutive rows (inner: q)
+    int64_t S_tile;      // words between consecutive row blocks = nrow * S_row
     int64_t Lc;          // words per row chunk (inner: q)
     int64_t nch;         // chunks per row (inner: 1)
     int64_t ntiles;
-    int32_t Pr;          // LDS bytes per row: inner 4 * 2^ds, outer Lc + 16
-    int32_t ds;          // log2 of the dword slots per row (inner: 1, 2, 4; outer: Lc / 4)
+    int64_t lds;         // LDS bytes of a tile
+    int32_t PA;          // aligned bytes per row: inner W = 4, 8 or 16; outer Lc + 16
+    int32_t PR;          // outer: raw bytes per row, Lc + 32
+    int32_t raw_off;     // LDS offset of the raw chunks
+    int32_t nkr;         // outer: raw chunk slots per row, Lc / 16 + 2
+    uint32_t mkr;        // magic of nkr
+    int32_t nvp;         // outer: 16-byte aligned chunks per row, Lc / 16
+    uint32_t mvp;        // magic of nvp
+    uint32_t nda;        // dwords per aligned row, PA / 4
+    uint32_t mda;        // magic of nda
     int32_t q, nax, first, last, hn, inner;
+    int32_t full;        // last pass: 1 = full histogram, 0 = #{values <= rcut} and the maximum
+    int32_t rcut;
     uint32_t u[MAXN];    // per row axis: stride in rows
-    uint32_t mg[MAXN];   // floor(2^32 / u) + 1, for u > 1
-    uint32_t mq;         // floor(2^32 / q) + 1
-    uint32_t nk;         // outer store: dword slots per row, Lc / 4 + 1
-    uint32_t mk;         // floor(2^32 / nk) + 1
+    uint32_t mg[MAXN];   // magic of u
+    uint32_t mq;         // magic of q
 };
 
 // exact floor(f / d) for f * d < 2^32, with m = floor(2^32 / d) + 1 (d > 1)
@@ -72,6 +82,21 @@ __host__ __device__ inline uint32_t dt_align(uint32_t hi, uint32_t lo, uint32_t 
 #else
     return (uint32_t)(((((uint64_t)hi) << 32) | lo) >> (8 * s));
 #endif
+}
+
+// 16 bytes starting at byte s (0..15) of the 32 bytes a:b
+__host__ __device__ inline uint4 dt_funnel(uint4 a, uint4 b, uint32_t s)
+{
+    const uint32_t w[8] = {a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w};
+    const uint32_t sb = s & 3, sd = s >> 2;
+    uint32_t y[7];
+#pragma unroll
+    for (int i = 0; i < 7; i++) y[i] = dt_align(w[i + 1], w[i], sb);
+    uint32_t o[4];
+#pragma unroll
+    for (int j = 0; j < 4; j++)
+        o[j] = sd == 0 ? y[j] : (sd == 1 ? y[j + 1] : (sd == 2 ? y[j + 2] : y[j + 3]));
+    return make_uint4(o[0], o[1], o[2], o[3]);
 }
 
 // per-byte min of bytes <= 127: byte i of (a | 0x80) - b is 128 + a_i - b_i,
@@ -96,22 +121,33 @@ __host__ __device__ inline uint4 dt_pinc4(uint4 a)
 {
     return make_uint4(dt_pinc(a.x), dt_pinc(a.y), dt_pinc(a.z), dt_pinc(a.w));
 }
-
-// packed 16-bit histogram counters: bin x in word x >> 2, field x & 3
-struct DtHist {
-    uint64_t w[(DT_HB + 3) / 4];
-};
-__host__ __device__ inline void dt_hist_add(DtHist& h, uint32_t x, int n)
+// bytes [k, 16) of x set to unreached
+__host__ __device__ inline uint4 dt_keep4(uint4 x, int32_t k)
 {
-    const int b = (int)x <= n ? (int)x : n + 1;
-    const uint64_t one = (uint64_t)1 << (16 * (b & 3));
+    uint32_t w[4] = {x.x, x.y, x.z, x.w};
 #pragma unroll
-    for (int k = 0; k < (DT_HB + 3) / 4; k++) h.w[k] += (k == (b >> 2)) ? one : 0;
+    for (int j = 0; j < 4; j++) {
+        const int32_t kk = k - 4 * j;
+        if (kk <= 0) w[j] = DT_INF4;
+        else if (kk < 4) {
+            const uint32_t m = (1u << (8 * kk)) - 1u;
+            w[j] = (w[j] & m) | (DT_INF4 & ~m);
+        }
+    }
+    return make_uint4(w[0], w[1], w[2], w[3]);
 }
-__host__ __device__ inline void dt_hist_add4(DtHist& h, uint32_t v, int n)
+
+__host__ __device__ inline uint4 dt_bits_to_bytes(uint32_t bits16)
 {
-#pragma unroll
-    for (int k = 0; k < 4; k++) dt_hist_add(h, (v >> (8 * k)) & 0xFFu, n);
+    // bit k set (a codeword) -> byte k = 0, else unreached
+    uint32_t w[4];
+    for (int k = 0; k < 4; k++) {
+        uint32_t x = 0;
+        for (int b = 0; b < 4; b++)
+            x |= (((bits16 >> (4 * k + b)) & 1u) ? 0u : DT_INF) << (8 * b);
+        w[k] = x;
+    }
+    return make_uint4(w[0], w[1], w[2], w[3]);
 }
 
 // tile t: row r holds the words [gs0 + r * S_row, + len)
@@ -129,48 +165,99 @@ __host__ __device__ inline DtTile dt_tile(const DtPass& p, int64_t t)
 }
 
 // ------------------------------------------------------------------ load
-// LDS dword j of row r = global bytes [gs + 4j, +4); bytes past the row's
-// end, and dwords past them up to the row's slots, are unreached
-__host__ __device__ inline void dt_load(const DtPass& p, const DtTile& T, int tid, int nth, uint32_t* sm,
-                                        const uint32_t* D32, const uint32_t* cwb)
+// Index arithmetic inside a tile is 32-bit; only global addresses are 64-bit.
+// Row r of an outer tile starts at global word gs0 + r S_row, which is
+// (gs0 + r S_row) & 15 = (gs0 + r (S_row & 15)) & 15 bytes past a 16-byte
+// boundary.
+__host__ __device__ inline uint32_t dt_row_shift(const DtPass& p, const DtTile& T, uint32_t r)
 {
-    const int64_t ndp = (int64_t)1 << p.ds;
-    const int64_t nd = (T.len + 3) >> 2;
-    const int64_t nslot = p.nrow << p.ds;
-    for (int64_t e = tid; e < nslot; e += nth) {
-        const int64_t r = e >> p.ds, j = e & (ndp - 1);
-        uint32_t v = DT_INF4;
-        if (j < nd) {
-            const int64_t a = T.gs0 + r * p.S_row + 4 * j;
+    return ((uint32_t)T.gs0 + r * (uint32_t)p.S_row) & 15u;
+}
+
+// the 16-byte global chunks under the tile, as they are, into the raw area:
+// inner: the contiguous range [gs0, gs0 + S_tile); outer: each row's range
+__host__ __device__ inline void dt_load(const DtPass& p, const DtTile& T, int tid, int nth, uint8_t* sm,
+                                        const uint4* D4, const uint32_t* cwb)
+{
+    uint4* raw = reinterpret_cast<uint4*>(sm + p.raw_off);
+    if (p.inner) {
+        const uint32_t s = (uint32_t)T.gs0 & 15u;
+        const int64_t c0 = (T.gs0 - s) >> 4;               // first chunk
+        const uint32_t nk = (s + (uint32_t)p.S_tile + 15) >> 4;
+        for (uint32_t k = tid; k < nk; k += nth) {
             if (p.first) {
-                const uint64_t w = (uint64_t)cwb[a >> 5] | ((uint64_t)cwb[(a >> 5) + 1] << 32);
-                const uint32_t bits = (uint32_t)(w >> (a & 31)) & 0xFu;
-                v = 0;
-                for (int k = 0; k < 4; k++) v |= (((bits >> k) & 1u) ? 0u : DT_INF) << (8 * k);
+                const int64_t A = (c0 + k) << 4;
+                raw[k] = dt_bits_to_bytes((cwb[A >> 5] >> (A & 31)) & 0xFFFFu);
             } else {
-                v = dt_align(D32[(a >> 2) + 1], D32[a >> 2], (uint32_t)(a & 3));
-            }
-            const int64_t keep = T.len - 4 * j;
-            if (keep < 4) {
-                const uint32_t mask = (1u << (8 * keep)) - 1u;
-                v = (v & mask) | (DT_INF4 & ~mask);
+                raw[k] = D4[c0 + k];
             }
         }
-        sm[((r * p.Pr) >> 2) + j] = v;
+    } else {
+        const uint32_t nslot = (uint32_t)p.nrow * (uint32_t)p.nkr;
+        const uint32_t len = (uint32_t)T.len, prc = (uint32_t)p.PR >> 4;
+        for (uint32_t e = tid; e < nslot; e += nth) {
+            const uint32_t r = dt_div(e, (uint32_t)p.nkr, p.mkr);
+            const uint32_t k = e - r * (uint32_t)p.nkr;
+            const uint32_t s = dt_row_shift(p, T, r);
+            if (k < ((s + len + 15) >> 4)) {
+                const int64_t c0 = (T.gs0 + (int64_t)r * p.S_row - s) >> 4;
+                raw[r * prc + k] = D4[c0 + k];
+            }
+        }
+    }
+}
+
+// ---------------------------------------------------------------- realign
+// raw chunks -> rows starting at 16-byte boundaries; bytes past a row's end
+// are unreached.  inner: row r = tile bytes [r q, r q + q) padded to PA
+__host__ __device__ inline void dt_realign(const DtPass& p, const DtTile& T, int tid, int nth, uint8_t* sm)
+{
+    if (p.inner) {
+        const uint32_t* raw = reinterpret_cast<const uint32_t*>(sm + p.raw_off);
+        uint32_t* al = reinterpret_cast<uint32_t*>(sm);
+        const uint32_t s = (uint32_t)T.gs0 & 15u;
+        const uint32_t nd = p.nda, sh = nd == 1 ? 0 : (nd == 2 ? 1 : 2), q = (uint32_t)p.q;
+        const uint32_t nslot = (uint32_t)p.nrow << sh;
+        for (uint32_t e = tid; e < nslot; e += nth) {
+            const uint32_t r = e >> sh, j = e & (nd - 1);
+            const uint32_t o = s + r * q + 4 * j;             // raw byte of row byte 4j
+            uint32_t v = dt_align(raw[(o >> 2) + 1], raw[o >> 2], o & 3);
+            const int32_t keep = (int32_t)q - (int32_t)(4 * j);
+            if (keep < 4) {
+                const uint32_t m = keep > 0 ? (1u << (8 * keep)) - 1u : 0u;
+                v = (v & m) | (DT_INF4 & ~m);
+            }
+            al[e] = v;
+        }
+    } else {
+        const uint4* raw = reinterpret_cast<const uint4*>(sm + p.raw_off);
+        uint4* al = reinterpret_cast<uint4*>(sm);
+        const uint32_t na = (uint32_t)p.nvp + 1, prc = (uint32_t)p.PR >> 4, pac = (uint32_t)p.PA >> 4;
+        const uint32_t nslot = (uint32_t)p.nrow * na;
+        const uint32_t ma = (uint32_t)((((uint64_t)1 << 32) / na) + 1);
+        const int32_t len = (int32_t)T.len;
+        for (uint32_t e = tid; e < nslot; e += nth) {
+            const uint32_t r = dt_div(e, na, na > 1 ? ma : 0u);
+            const uint32_t c = e - r * na;
+            const uint4* rr = raw + r * prc;
+            const uint4 x = dt_funnel(rr[c], rr[c + 1], dt_row_shift(p, T, r));
+            al[r * pac + c] = dt_keep4(x, len - 16 * (int32_t)c);
+        }
     }
 }
 
 // ----------------------------------------------------- innermost axis
 // inner pass: each row is one fiber of the innermost axis (q bytes, the rest
 // of its 4, 8 or 16 bytes unreached)
-__host__ __device__ inline void dt_inrow(const DtPass& p, int tid, int nth, uint32_t* sm)
+__host__ __device__ inline void dt_inrow(const DtPass& p, int tid, int nth, uint8_t* sm)
 {
-    const int nd = 1 << p.ds;
-    for (int64_t r = tid; r < p.nrow; r += nth) {
-        uint32_t* row = sm + ((r * p.Pr) >> 2);
+    const uint32_t nd = p.nda;
+    uint32_t* al = reinterpret_cast<uint32_t*>(sm);
+    for (uint32_t r = tid; r < (uint32_t)p.nrow; r += nth) {
+        uint32_t* row = al + r * nd;
         uint32_t d[4];
         uint32_t m = DT_INF4;
-        for (int j = 0; j < 4; j++) {
+        for (uint32_t j = 0; j < 4; j++) {
             d[j] = j < nd ? row[j] : DT_INF4;
             m = dt_pmin(m, d[j]);
         }
@@ -178,156 +265,264 @@ __host__ __device__ inline void dt_inrow(const DtPass& p, int tid, int nth, uint
         m = dt_pmin(m, m >> 8);
         const uint32_t mb = m & 0xFFu;
         const uint32_t m1 = (mb < DT_INF ? mb + 1 : DT_INF) * 0x01010101u;
-        for (int j = 0; j < 4; j++)
+        for (uint32_t j = 0; j < 4; j++)
             if (j < nd) row[j] = dt_pmin(d[j], m1);
     }
 }
 
 // ------------------------------------------------------------ row axes
 // fibers across rows: q rows u apart, the same column; inner rows by dwords,
-// outer rows by 16-byte vectors
-__host__ __device__ inline void dt_axis(const DtPass& p, const DtTile& T, int ax, int tid, int nth, uint32_t* sm)
+// outer rows by 16-byte chunks
+__host__ __device__ inline void dt_axis(const DtPass& p, const DtTile& T, int ax, int tid, int nth, uint8_t* sm)
 {
-    const uint32_t u = p.u[ax], mg = p.mg[ax];
-    const int64_t nf = p.nrow / p.q;
+    const uint32_t u = p.u[ax], mg = p.mg[ax], q = (uint32_t)p.q;
+    const uint32_t nf = (uint32_t)p.nrow / q;
     if (p.inner) {
-        const int64_t ndp = (int64_t)1 << p.ds;
-        const int64_t nslot = nf << p.ds;
-        for (int64_t e = tid; e < nslot; e += nth) {
-            const int64_t j = e & (ndp - 1);
-            const uint32_t f = (uint32_t)(e >> p.ds);
+        uint32_t* al = reinterpret_cast<uint32_t*>(sm);
+        const uint32_t nd = p.nda, sh = nd == 1 ? 0 : (nd == 2 ? 1 : 2);
+        const uint32_t nslot = nf << sh, step = u * nd;
+        for (uint32_t e = tid; e < nslot; e += nth) {
+            const uint32_t j = e & (nd - 1), f = e >> sh;
             const uint32_t hi = dt_div(f, u, mg), lo = f - hi * u;
-            const int64_t r0 = (int64_t)hi * u * p.q + lo;
+            uint32_t* x = al + (hi * u * q + lo) * nd + j;
             uint32_t m = DT_INF4;
-            for (int v = 0; v < p.q; v++) m = dt_pmin(m, sm[(((r0 + (int64_t)v * u) * p.Pr) >> 2) + j]);
+            for (uint32_t v = 0; v < q; v++) m = dt_pmin(m, x[v * step]);
             const uint32_t m1 = dt_pinc(m);
-            for (int v = 0; v < p.q; v++) {
-                uint32_t* x = sm + (((r0 + (int64_t)v * u) * p.Pr) >> 2) + j;
-                *x = dt_pmin(*x, m1);
-            }
+            for (uint32_t v = 0; v < q; v++) x[v * step] = dt_pmin(x[v * step], m1);
         }
     } else {
-        const int vs = p.ds - 2;                      // 16-byte vectors per row: 1 << vs
-        const int64_t nvec = (T.len + 15) >> 4;
-        const int64_t nslot = nf << vs;
-        uint4* s4 = reinterpret_cast<uint4*>(sm);
-        for (int64_t e = tid; e < nslot; e += nth) {
-            const int64_t c = e & (((int64_t)1 << vs) - 1);
+        uint4* al = reinterpret_cast<uint4*>(sm);
+        const uint32_t pac = (uint32_t)p.PA >> 4, nvp = (uint32_t)p.nvp;
+        const uint32_t nvec = ((uint32_t)T.len + 15) >> 4;
+        const uint32_t nslot = nf * nvp, step = u * pac;
+        for (uint32_t e = tid; e < nslot; e += nth) {
+            const uint32_t f = dt_div(e, nvp, p.mvp);
+            const uint32_t c = e - f * nvp;
             if (c >= nvec) continue;
-            const uint32_t f = (uint32_t)(e >> vs);
             const uint32_t hi = dt_div(f, u, mg), lo = f - hi * u;
-            const int64_t r0 = (int64_t)hi * u * p.q + lo;
+            uint4* x = al + (hi * u * q + lo) * pac + c;
             uint4 m = make_uint4(DT_INF4, DT_INF4, DT_INF4, DT_INF4);
-            for (int v = 0; v < p.q; v++) m = dt_pmin4(m, s4[(((r0 + (int64_t)v * u) * p.Pr) >> 4) + c]);
+            for (uint32_t v = 0; v < q; v++) m = dt_pmin4(m, x[v * step]);
             const uint4 m1 = dt_pinc4(m);
-            for (int v = 0; v < p.q; v++) {
-                uint4* x = s4 + (((r0 + (int64_t)v * u) * p.Pr) >> 4) + c;
-                *x = dt_pmin4(*x, m1);
-            }
+            for (uint32_t v = 0; v < q; v++) x[v * step] = dt_pmin4(x[v * step], m1);
         }
     }
 }
 
 // ----------------------------------------------------------------- store
-__host__ __device__ inline void dt_store(const DtPass& p, const DtTile& T, int tid, int nth, const uint32_t* sm,
-                                         uint8_t* D, DtHist& h)
+// inner: tile byte b sits at row b / q, column b % q of the padded rows
+__host__ __device__ inline uint32_t dt_inner_byte(const DtPass& p, const uint8_t* sm, uint32_t b)
 {
-    const uint8_t* s8 = reinterpret_cast<const uint8_t*>(sm);
-    uint32_t* D32 = reinterpret_cast<uint32_t*>(D);
+    const uint32_t r = dt_div(b, (uint32_t)p.q, p.mq);
+    return sm[r * (uint32_t)p.PA + (b - r * (uint32_t)p.q)];
+}
+// the 4 tile bytes [b, b + 4) (q >= 4: at most two rows)
+__host__ __device__ inline uint32_t dt_inner_dword(const DtPass& p, const uint8_t* sm, uint32_t b)
+{
+    const uint32_t* al = reinterpret_cast<const uint32_t*>(sm);
+    const uint32_t r = dt_div(b, (uint32_t)p.q, p.mq), c = b - r * (uint32_t)p.q;
+    const uint32_t w0 = r * p.nda + (c >> 2);
+    uint32_t v = dt_align(al[w0 + 1], al[w0], c & 3);
+    const uint32_t k = (uint32_t)p.q - c;                  // bytes left in row r
+    if (k < 4) {
+        const uint32_t m = (1u << (8 * k)) - 1u;
+        v = (v & m) | (al[(r + 1) * p.nda] << (8 * k));
+    }
+    return v;
+}
+
+__host__ __device__ inline void dt_store(const DtPass& p, const DtTile& T, int tid, int nth, const uint8_t* sm,
+                                         uint8_t* D)
+{
+    uint4* D4 = reinterpret_cast<uint4*>(D);
     if (p.inner) {
-        // the tile is the contiguous range [gs0, gs0 + S_tile); word x sits at
-        // row (x - gs0) / q, column (x - gs0) % q
         const int64_t b0 = T.gs0, e0 = T.gs0 + p.S_tile;
-        const int64_t g0 = b0 & ~(int64_t)3;
-        const int64_t nw = (e0 - g0 + 3) >> 2;
-        for (int64_t k = tid; k < nw; k += nth) {
-            const int64_t g = g0 + 4 * k;
-            uint32_t v = 0;
-            for (int b = 0; b < 4; b++) {
-                const int64_t x = g + b;
-                if (x < b0 || x >= e0) continue;
-                const uint32_t off = (uint32_t)(x - b0);
-                const uint32_t row = dt_div(off, (uint32_t)p.q, p.mq), col = off - row * (uint32_t)p.q;
-                v |= (uint32_t)s8[(int64_t)row * p.Pr + col] << (8 * b);
-            }
-            if (g >= b0 && g + 4 <= e0) {
-                D32[g >> 2] = v;
-                if (p.last) dt_hist_add4(h, v, p.hn);
-            } else {
-                for (int b = 0; b < 4; b++) {
-                    const int64_t x = g + b;
-                    if (x < b0 || x >= e0) continue;
-                    const uint32_t y = (v >> (8 * b)) & 0xFFu;
-                    D[x] = (uint8_t)y;
-                    if (p.last) dt_hist_add(h, y, p.hn);
+        const uint32_t s = (uint32_t)b0 & 15u;
+        const uint32_t head = (16 - s) & 15u;                   // bytes before the first whole chunk
+        const uint32_t tot = (uint32_t)p.S_tile;
+        const uint32_t nk = tot > head ? (tot - head) >> 4 : 0u; // whole chunks
+        const int64_t c0 = (b0 + head) >> 4;
+        for (uint32_t k = tid; k < nk; k += nth) {
+            const uint32_t b = head + 16 * k;
+            uint32_t w[4];
+#pragma unroll
+            for (int j = 0; j < 4; j++) {
+                if (p.q >= 4) {
+                    w[j] = dt_inner_dword(p, sm, b + 4 * j);
+                } else {
+                    w[j] = 0;
+                    for (uint32_t i = 0; i < 4; i++) w[j] |= dt_inner_byte(p, sm, b + 4 * j + i) << (8 * i);
                 }
             }
+            D4[c0 + k] = make_uint4(w[0], w[1], w[2], w[3]);
         }
+        // the bytes outside whole chunks, at both ends
+        if (tid < 2) {
+            const uint32_t tail0 = head + 16 * nk;
+            const uint32_t x0 = tid == 0 ? 0u : (tail0 > head ? tail0 : head);
+            const uint32_t x1 = tid == 0 ? (head < tot ? head : tot) : tot;
+            for (uint32_t x = x0; x < x1; x++) D[b0 + x] = (uint8_t)dt_inner_byte(p, sm, x);
+        }
+        (void)e0;
     } else {
-        // row r: global dwords from (gs & ~3) covering [gs, gs + len)
-        const int64_t nslot = p.nrow * (int64_t)p.nk;
-        for (int64_t e = tid; e < nslot; e += nth) {
-            const uint32_t r = dt_div((uint32_t)e, p.nk, p.mk);
-            const int64_t k = e - (int64_t)r * p.nk;
-            const int64_t gs = T.gs0 + (int64_t)r * p.S_row, ge = gs + T.len;
-            const int64_t g = (gs & ~(int64_t)3) + 4 * k;
-            if (g >= ge) continue;
-            const uint32_t* row = sm + (((int64_t)r * p.Pr) >> 2);
-            if (g >= gs && g + 4 <= ge) {
-                const int64_t i0 = g - gs;
-                const uint32_t s = (uint32_t)(i0 & 3);
-                const uint32_t v = dt_align(s ? row[(i0 >> 2) + 1] : 0u, row[i0 >> 2], s);
-                D32[g >> 2] = v;
-                if (p.last) dt_hist_add4(h, v, p.hn);
-            } else {
-                const uint8_t* rb = s8 + (int64_t)r * p.Pr;
-                for (int b = 0; b < 4; b++) {
-                    const int64_t x = g + b;
-                    if (x < gs || x >= ge) continue;
-                    const uint32_t y = rb[x - gs];
-                    D[x] = (uint8_t)y;
-                    if (p.last) dt_hist_add(h, y, p.hn);
-                }
-            }
+        const uint4* al = reinterpret_cast<const uint4*>(sm);
+        const uint32_t pac = (uint32_t)p.PA >> 4, len = (uint32_t)T.len;
+        const uint32_t nslot = (uint32_t)p.nrow * (uint32_t)p.nkr;
+        for (uint32_t e = tid; e < nslot; e += nth) {
+            const uint32_t r = dt_div(e, (uint32_t)p.nkr, p.mkr);
+            const uint32_t k = e - r * (uint32_t)p.nkr;
+            const uint32_t s = dt_row_shift(p, T, r);
+            const uint32_t o = ((16 - s) & 15u) + 16 * k;       // row byte of the k-th whole chunk
+            if (o + 16 > len) continue;
+            const uint4* rr = al + r * pac;
+            const int64_t g = (T.gs0 + (int64_t)r * p.S_row + o) >> 4;
+            D4[g] = dt_funnel(rr[o >> 4], rr[(o >> 4) + 1], o & 15);
         }
+        // each row's bytes outside whole chunks: up to 15 at the head and 15
+        // at the tail, one (row, byte) per thread
+        const uint32_t nb = (uint32_t)p.nrow * 32;
+        for (uint32_t e = tid; e < nb; e += nth) {
+            const uint32_t r = e >> 5, i = e & 31;
+            const uint32_t s = dt_row_shift(p, T, r);
+            const uint32_t head = (16 - s) & 15u;
+            const uint32_t h1 = head < len ? head : len;
+            const uint32_t whole = len > head ? (len - head) >> 4 : 0u;
+            const uint32_t t0 = head + 16 * whole > h1 ? head + 16 * whole : h1;
+            const uint32_t x = i < 16 ? i : t0 + (i - 16);      // head byte i, or tail byte i - 16
+            if ((i < 16 && x < h1) || (i >= 16 && x < len))
+                D[T.gs0 + (int64_t)r * p.S_row + x] = sm[r * (uint32_t)p.PA + x];
+        }
+    }
+}
+
+// histogram of the last pass: per thread, c[r] = #{values <= r} for r < 8
+// and the number of values; a byte x <= 127 is <= r exactly when the top bit
+// of (0x80 | r) - x is set, four bytes at a time.  Values of 8 and more
+// (only far words of long codes, and "unreached") are rare and counted one
+// by one in the workgroup's bins.
+constexpr int DT_HL = 8;
+struct DtHist {
+    uint32_t le[DT_HL];
+    uint32_t all;
+};
+__host__ __device__ inline void dt_hist_add4(DtHist& h, uint32_t v, uint32_t nvalid, int n, unsigned int* lh)
+{
+    // bytes [nvalid, 4) of v are not counted: make them 0 and discount them
+    const uint32_t keep = nvalid >= 4 ? 0xFFFFFFFFu : ((1u << (8 * nvalid)) - 1u);
+    v &= keep;
+    h.all += 4;
+#pragma unroll
+    for (int r = 0; r < DT_HL; r++)
+        h.le[r] += (uint32_t)__builtin_popcount(((0x80808080u | ((uint32_t)r * 0x01010101u)) - v) & 0x80808080u);
+    if (nvalid < 4) {                       // the zeroed bytes were counted as 0
+        h.le[0] -= 4 - nvalid;
+#pragma unroll
+        for (int r = 1; r < DT_HL; r++) h.le[r] -= 4 - nvalid;
+        h.all -= 4 - nvalid;
+    }
+    const uint32_t big = ((v | 0x80808080u) - 0x08080808u) & 0x80808080u;   // bytes >= 8
+    if (big) {
+        for (int k = 0; k < 4; k++)
+            if ((big >> (8 * k + 7)) & 1u) {
+                const uint32_t x = (v >> (8 * k)) & 0xFFu;
+                const int bin = (int)x <= n ? (int)x : n + 1;
+#ifdef __HIP_DEVICE_COMPILE__
+                atomicAdd(&lh[bin], 1u);
+#else
+                lh[bin] += 1;
+#endif
+                h.all -= 1;                 // binned here, not from le[]
+            }
+    }
+}
+
+// the last pass without the full histogram: #{values <= rcut} and the
+// maximum, per thread; per-byte max(a, b) = a + b - min(a, b) (bytes <= 127)
+struct DtStat {
+    uint32_t le, mx;
+};
+__host__ __device__ inline void dt_stat_add4(DtStat& st, uint32_t v, uint32_t nvalid, int rcut)
+{
+    const uint32_t keep = nvalid >= 4 ? 0xFFFFFFFFu : ((1u << (8 * nvalid)) - 1u);
+    v &= keep;
+    st.le += (uint32_t)__builtin_popcount(((0x80808080u | ((uint32_t)rcut * 0x01010101u)) - v) & 0x80808080u) -
+             (4 - (nvalid >= 4 ? 4 : nvalid));
+    st.mx = st.mx + v - dt_pmin(st.mx, v);
+}
+
+// the last pass's values, read back from LDS after the store
+__host__ __device__ inline void dt_hist_tile(const DtPass& p, const DtTile& T, int tid, int nth, const uint8_t* sm,
+                                             DtHist& h, DtStat& st, unsigned int* lh)
+{
+    const uint32_t* al = reinterpret_cast<const uint32_t*>(sm);
+    const uint32_t nd = p.nda;                                          // dwords per aligned row
+    const int32_t valid = p.inner ? p.q : (int32_t)T.len;               // bytes per row
+    const uint32_t nslot = (uint32_t)p.nrow * nd;
+    for (uint32_t e = tid; e < nslot; e += nth) {
+        const uint32_t r = dt_div(e, nd, p.mda), j = e - r * nd;
+        const int32_t k = valid - 4 * (int32_t)j;
+        if (k <= 0) continue;
+        if (p.full) dt_hist_add4(h, al[e], (uint32_t)(k < 4 ? k : 4), p.hn, lh);
+        else dt_stat_add4(st, al[e], (uint32_t)(k < 4 ? k : 4), p.rcut);
     }
 }
 
 // ---------------------------------------------------------------- kernels
-__device__ inline void dt_hist_flush(const DtHist& h, int hn, unsigned int* lh)
-{
-#pragma unroll
-    for (int b = 0; b < DT_HB; b++) {
-        const uint32_t c = (uint32_t)((h.w[b >> 2] >> (16 * (b & 3))) & 0xFFFFu);
-        if (b <= hn + 1 && c) atomicAdd(&lh[b], c);
-    }
-}
-
-__global__ void __launch_bounds__(WG) k_dt_pass(const DtPass p, uint8_t* D, const uint32_t* cwb,
+template <int NT>
+__global__ void __launch_bounds__(NT) k_dt_pass(const DtPass p, uint8_t* D, const uint32_t* cwb,
                                                 unsigned long long* hist)
 {
-    extern __shared__ __align__(16) uint32_t sm[];
+    extern __shared__ __align__(16) uint8_t sm[];
     __shared__ unsigned int lh[DT_HB];
     if (threadIdx.x < DT_HB) lh[threadIdx.x] = 0;
-    const uint32_t* D32 = reinterpret_cast<const uint32_t*>(D);
+    const uint4* D4 = reinterpret_cast<const uint4*>(D);
+    DtHist h{};
+    DtStat st{};
     for (int64_t t = blockIdx.x; t < p.ntiles; t += gridDim.x) {
         const DtTile T = dt_tile(p, t);
-        DtHist h{};
         __syncthreads();
-        dt_load(p, T, threadIdx.x, WG, sm, D32, cwb);
+        dt_load(p, T, threadIdx.x, NT, sm, D4, cwb);
+        __syncthreads();
+        dt_realign(p, T, threadIdx.x, NT, sm);
         __syncthreads();
         if (p.inner) {
-            dt_inrow(p, threadIdx.x, WG, sm);
+            dt_inrow(p, threadIdx.x, NT, sm);
             __syncthreads();
         }
         for (int ax = 0; ax < p.nax; ax++) {
-            dt_axis(p, T, ax, threadIdx.x, WG, sm);
+            dt_axis(p, T, ax, threadIdx.x, NT, sm);
             __syncthreads();
         }
-        dt_store(p, T, threadIdx.x, WG, sm, D, h);
-        if (p.last) dt_hist_flush(h, p.hn, lh);
+        dt_store(p, T, threadIdx.x, NT, sm, D);
+        if (p.last) dt_hist_tile(p, T, threadIdx.x, NT, sm, h, st, lh);
     }
-    if (p.last) {
+    if (p.last && !p.full) {
+        // bins: [0] #{values <= rcut}, [1] the maximum
+        __syncthreads();
+        uint32_t mb = 0;
+        for (int k = 0; k < 4; k++) {
+            const uint32_t b = (st.mx >> (8 * k)) & 0xFFu;
+            mb = b > mb ? b : mb;
+        }
+        if (st.le) atomicAdd(&lh[0], st.le);
+        if (mb) atomicMax(&lh[1], mb);
+        __syncthreads();
+        if (threadIdx.x < 2 && lh[threadIdx.x]) {
+            if (threadIdx.x == 0) atomicAdd(&hist[0], (unsigned long long)lh[0]);
+            else atomicMax(&hist[1], (unsigned long long)lh[1]);
+        }
+    }
+    if (p.last && p.full) {
+        __syncthreads();
+        // values < 8 (all of them below bin n + 1 unless n < 7, where no
+        // distance exceeds n): bin r = le[r] - le[r-1]
+        uint32_t prev = 0;
+#pragma unroll
+        for (int r = 0; r < DT_HL; r++) {
+            const uint32_t c = h.le[r] - prev;
+            prev = h.le[r];
+            if (c) atomicAdd(&lh[r <= p.hn ? r : p.hn + 1], c);
+        }
         __syncthreads();
         if (threadIdx.x < p.hn + 2 && lh[threadIdx.x])
             atomicAdd(&hist[threadIdx.x], (unsigned long long)lh[threadIdx.x]);
@@ -347,7 +542,8 @@ __global__ void __launch_bounds__(WG) k_cwb_mark(const Cell C, uint32_t* cwb, in
 
 // ------------------------------------------------------------- planning
 // LDS bytes of a tile (the autotuner may change it)
-constexpr int DT_LDS_DEFAULT = 32768;
+constexpr int DT_LDS_DEFAULT = 40960;
+constexpr int DT_THREADS_DEFAULT = 256;
 constexpr int DT_LDS_MAX = 65536 - 1024;
 
 inline uint32_t dt_magic(uint64_t d)
@@ -379,39 +575,44 @@ inline int dt_plan(int q, int n, int64_t N, int budget, DtPass* out)
 {
     if (budget < 4096 || budget > DT_LDS_MAX || q < 2 || q > 16 || n < 1 || n > MAXN) return 0;
     int np = 0;
-    // inner pass: rows of one innermost fiber, padded to 4, 8 or 16 bytes;
-    // a axes (the innermost + a-1 across rows) in blocks of q^(a-1) rows,
-    // keeping >= 512 tiles on large spaces
-    const int ndp = q <= 4 ? 1 : (q <= 8 ? 2 : 4);
-    const int pr_in = 4 * ndp;
+    // inner pass: rows of one innermost fiber, padded to W = 4, 8 or 16 bytes;
+    // a axes (the innermost + a-1 across rows) in blocks of q^a words,
+    // keeping >= 512 tiles on large spaces.  LDS: rows * W + raw q^a + 48.
+    const int W = q <= 4 ? 4 : (q <= 8 ? 8 : 16);
+    auto inner_lds = [&](int64_t words) {
+        return ((words / q * W + 15) & ~(int64_t)15) + ((words + 47) & ~(int64_t)15);
+    };
     int a = 1;
-    while (a < n && dt_ipow(q, a) * pr_in <= budget &&
+    while (a < n && inner_lds(dt_ipow(q, a + 1)) <= budget &&
            (N / dt_ipow(q, a + 1) >= 512 || N <= 64 * (int64_t)budget))
         a++;
     {
         DtPass& p = out[np++];
         p = DtPass{};
-        const int64_t blk = dt_ipow(q, a - 1);       // rows per block of q^a words
-        int64_t per = budget / (blk * pr_in);
-        const int64_t per_par = N / (blk * q * 512);
+        const int64_t blk = dt_ipow(q, a);          // words per block
+        int64_t per = budget / inner_lds(blk);
+        const int64_t per_par = N / (blk * 512);
         if (per > per_par) per = per_par;
         if (per < 1) per = 1;
-        while (per > 1 && (N / q) % (per * blk)) per--;
+        while (per > 1 && N % (per * blk)) per--;
         p.N = N; p.q = q; p.hn = n; p.inner = 1; p.first = 1;
-        p.nrow = per * blk;
+        p.nrow = per * blk / q;
         p.S_row = q; p.Lc = q; p.nch = 1;
-        p.S_tile = p.nrow * q;
+        p.S_tile = per * blk;
         p.ntiles = N / p.S_tile;
-        p.Pr = pr_in;
-        p.ds = ndp == 1 ? 0 : (ndp == 2 ? 1 : 2);
+        p.PA = W;
+        p.nda = (uint32_t)(W / 4);
+        p.mda = dt_magic(p.nda);
+        p.raw_off = (int32_t)(((p.nrow * W) + 15) & ~(int64_t)15);
+        p.lds = p.raw_off + ((p.S_tile + 47) & ~(int64_t)15);
         p.mq = dt_magic((uint64_t)q);
         dt_fill_axes(p, q, a - 1);
-        if (N % p.S_tile) return 0;
+        if (N % p.S_tile || p.lds > budget) return 0;
     }
-    // outer groups: at most bmax axes each, rows of >= 64 bytes
+    // outer groups: at most bmax axes each, rows of >= 64 words
     const int m = n - a;
     int bmax = 0;
-    for (int64_t B = q; B * (64 + 16) <= budget && bmax < m; B *= q) bmax++;
+    for (int64_t B = q; B * (2 * 64 + 48) <= budget && bmax < m; B *= q) bmax++;
     if (m > 0 && bmax == 0) return 0;
     const int ngroups = m > 0 ? (m + bmax - 1) / bmax : 0;
     int j1 = n - a;
@@ -424,20 +625,28 @@ inline int dt_plan(int q, int n, int64_t N, int budget, DtPass* out)
         const int64_t rows = dt_ipow(q, b);
         const int64_t S_row = dt_ipow(q, n - j1);
         if (rows > 4096) return 0;
-        // Lc = 16 * 2^k: the largest that fits, no longer than the row needs
-        int64_t Lc = 64;
-        while (rows * (2 * Lc + 16) <= budget && Lc < S_row && Lc < 4096) Lc *= 2;
-        int lg = 0;
-        while (((int64_t)4 << lg) < Lc) lg++;     // Lc / 4 = 2^lg dword slots
+        // Lc: a multiple of 16 as large as fits, no longer than the row needs
+        int64_t Lc = ((budget / rows - 48) / 2) & ~(int64_t)15;
+        const int64_t need = (S_row + 15) & ~(int64_t)15;
+        if (Lc > need) Lc = need;
+        if (Lc > 4096) Lc = 4096;
+        if (Lc < 64) return 0;
         p.N = N; p.q = q; p.hn = n;
         p.nrow = rows; p.S_row = S_row; p.S_tile = rows * S_row; p.Lc = Lc;
         p.nch = (S_row + Lc - 1) / Lc;
         p.ntiles = dt_ipow(q, j0) * p.nch;
-        p.Pr = (int32_t)(Lc + 16);
-        p.ds = lg;
-        p.nk = (uint32_t)(Lc / 4 + 1);
-        p.mk = dt_magic(p.nk);
-        if ((uint64_t)rows * p.nk * p.nk >= ((uint64_t)1 << 32)) return 0;   // dt_div exactness
+        p.PA = (int32_t)(Lc + 16);
+        p.nda = (uint32_t)(p.PA / 4);
+        p.mda = dt_magic(p.nda);
+        p.PR = (int32_t)(Lc + 32);
+        p.raw_off = (int32_t)(rows * p.PA);
+        p.lds = p.raw_off + rows * p.PR;
+        p.nkr = (int32_t)(Lc / 16 + 2);
+        p.mkr = dt_magic((uint64_t)p.nkr);
+        p.nvp = (int32_t)(Lc / 16);
+        p.mvp = dt_magic((uint64_t)p.nvp);
+        if (p.lds > budget) return 0;
+        if ((uint64_t)rows * p.nkr * p.nkr >= ((uint64_t)1 << 32)) return 0;   // dt_div exactness
         dt_fill_axes(p, q, b);
         j1 = j0;
     }
@@ -448,7 +657,7 @@ inline int dt_plan(int q, int n, int64_t N, int budget, DtPass* out)
 // LDS bytes a pass's tile needs
 inline int64_t dt_lds_bytes(const DtPass& p)
 {
-    return p.nrow * p.Pr;
+    return p.lds;
 }
 
 }  // namespace cov

@@ -42,7 +42,8 @@ struct cov_cell {
     int64_t tcap = 0;
     // distance transform
     uint32_t* cwb = nullptr;              // codeword bitmap, all zero between calls
-    int dt_lds = 0;                       // LDS bytes per transform tile
+    int dt_lds = 0;                       // LDS bytes per transform tile (0: default)
+    int dt_threads = 0;                   // threads per transform workgroup (0: default)
     bool dist_valid = false;              // dist holds the transform of the current code
     bool cnt_valid = true;                // cnt, planes and counts match the code
 };
@@ -780,9 +781,10 @@ int cov_commit(cov_cell_t* h, int i, int p, int v, int64_t* dunc)
     return COV_OK;
 }
 
-int cov_distance_transform(cov_cell_t* h, uint8_t* dist, int64_t* hist, int* radius)
+// all passes of the transform on the cell's stream; the last pass leaves in
+// hh either the full histogram (full) or {#{dist <= rcut}, max dist}
+static int run_transform(cov_cell_t* h, int full, int rcut, int64_t hh[DT_HB])
 {
-    if (!h) return COV_EPARAM;
     Cell& C = h->h;
     if (!C.dist) {
         void* v = nullptr;
@@ -801,6 +803,8 @@ int cov_distance_transform(cov_cell_t* h, uint8_t* dist, int64_t* hist, int* rad
         h->err = "distance transform: no tile plan for this shape and LDS budget";
         return COV_ELIMIT;
     }
+    pass[np - 1].full = full;
+    pass[np - 1].rcut = rcut;
     unsigned long long* dh = reinterpret_cast<unsigned long long*>(h->d_small);
     HCK(h, hipMemsetAsync(dh, 0, DT_HB * 8, h->stream));
     const int gm = grid_for(C.M, WG, 1, 64);
@@ -811,7 +815,16 @@ int cov_distance_transform(cov_cell_t* h, uint8_t* dist, int64_t* hist, int* rad
         const int g = (int)std::min<int64_t>(p.ntiles, 8192);
         hipEvent_t ev = nullptr;
         RC(timed_begin(h, 2, &ev));
-        hipLaunchKernelGGL(k_dt_pass, dim3(g), dim3(WG), (size_t)dt_lds_bytes(p), h->stream, p, C.dist, h->cwb, dh);
+        const int nt = h->dt_threads ? h->dt_threads : DT_THREADS_DEFAULT;
+        if (nt == 1024)
+            hipLaunchKernelGGL(k_dt_pass<1024>, dim3(g), dim3(1024), (size_t)dt_lds_bytes(p), h->stream, p, C.dist,
+                               h->cwb, dh);
+        else if (nt == 512)
+            hipLaunchKernelGGL(k_dt_pass<512>, dim3(g), dim3(512), (size_t)dt_lds_bytes(p), h->stream, p, C.dist,
+                               h->cwb, dh);
+        else
+            hipLaunchKernelGGL(k_dt_pass<256>, dim3(g), dim3(256), (size_t)dt_lds_bytes(p), h->stream, p, C.dist,
+                               h->cwb, dh);
         HCK(h, hipGetLastError());
         RC(timed_end(h, 2, ev));
         if (i == 0) {
@@ -819,8 +832,16 @@ int cov_distance_transform(cov_cell_t* h, uint8_t* dist, int64_t* hist, int* rad
             HCK(h, hipGetLastError());
         }
     }
-    int64_t hh[DT_HB];
     HCK(h, hipMemcpyAsync(hh, dh, DT_HB * 8, hipMemcpyDeviceToHost, h->stream));
+    return COV_OK;
+}
+
+int cov_distance_transform(cov_cell_t* h, uint8_t* dist, int64_t* hist, int* radius)
+{
+    if (!h) return COV_EPARAM;
+    const Cell& C = h->h;
+    int64_t hh[DT_HB];
+    RC(run_transform(h, 1, 0, hh));
     if (dist) HCK(h, hipMemcpyAsync(dist, C.dist, (size_t)C.N, hipMemcpyDeviceToHost, h->stream));
     HCK(h, hipStreamSynchronize(h->stream));
     int rad = 0;
@@ -829,6 +850,19 @@ int cov_distance_transform(cov_cell_t* h, uint8_t* dist, int64_t* hist, int* rad
     if (hist)
         for (int d = 0; d <= C.n; d++) hist[d] = hh[d];
     if (radius) *radius = rad;
+    h->dist_valid = true;
+    return COV_OK;
+}
+
+int cov_distance_stats(cov_cell_t* h, int r, int64_t* within, int* radius)
+{
+    if (!h || r < 0) return COV_EPARAM;
+    const Cell& C = h->h;
+    int64_t hh[DT_HB];
+    RC(run_transform(h, 0, r > 127 ? 127 : r, hh));
+    HCK(h, hipStreamSynchronize(h->stream));
+    if (within) *within = hh[0];
+    if (radius) *radius = hh[1] > C.n ? C.n + 1 : (int)hh[1];
     h->dist_valid = true;
     return COV_OK;
 }
@@ -1028,9 +1062,11 @@ int cov_get_launch(const cov_cell_t* h, int* eval_groups, int* commit_groups)
     return COV_OK;
 }
 
-int cov_set_transform_tile(cov_cell_t* h, int lds_bytes)
+int cov_set_transform_tile(cov_cell_t* h, int lds_bytes, int threads)
 {
-    if (!h || lds_bytes < 0) return COV_EPARAM;
+    if (!h || lds_bytes < 0 || (threads != 0 && threads != 256 && threads != 512 && threads != 1024))
+        return COV_EPARAM;
+    h->dt_threads = threads;
     DtPass pass[MAXN + 1];
     const Cell& C = h->h;
     if (lds_bytes && dt_plan(C.q, C.n, C.N, lds_bytes, pass) <= 0) return COV_EPARAM;

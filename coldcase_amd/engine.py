@@ -29,7 +29,7 @@ EXPORTS = ['cov_version', 'cov_strerror', 'cov_mem_bytes', 'cov_create', 'cov_de
            'cov_last_error', 'cov_sync', 'cov_set_code', 'cov_set_code_flags', 'cov_get_code', 'cov_get_best',
            'cov_recount', 'cov_ball_update', 'cov_count_uncovered', 'cov_gather_uncovered',
            'cov_first_uncovered_from', 'cov_read_cnt', 'cov_eval_moves', 'cov_commit',
-           'cov_distance_transform', 'cov_gather_far', 'cov_search_init', 'cov_search_params',
+           'cov_distance_transform', 'cov_distance_stats', 'cov_gather_far', 'cov_search_init', 'cov_search_params',
            'cov_search_run', 'cov_search_launch', 'cov_search_state', 'cov_set_launch',
            'cov_get_launch', 'cov_set_transform_tile', 'cov_timing', 'cov_timing_read', 'cov_search_phases',
            'cov_search_bytes', 'cov_batch_create', 'cov_batch_destroy', 'cov_batch_last_error',
@@ -86,6 +86,7 @@ def lib():
         'cov_commit': (i32, [vp, i32, i32, i32, vp]),
         'cov_distance_transform': (i32, [vp, vp, vp, vp]),
         'cov_gather_far': (i64, [vp, i32, vp, i64]),
+        'cov_distance_stats': (i32, [vp, i32, vp, vp]),
         'cov_search_init': (i32, [vp, u64, vp, i32, i32]),
         'cov_search_params': (i32, [vp, i32, i64, i32, i32]),
         'cov_search_run': (i64, [vp, i64, i32, vp]),
@@ -93,7 +94,7 @@ def lib():
         'cov_search_state': (i32, [vp, ctypes.POINTER(State)]),
         'cov_set_launch': (i32, [vp, i32, i32]),
         'cov_get_launch': (i32, [vp, vp, vp]),
-        'cov_set_transform_tile': (i32, [vp, i32]),
+        'cov_set_transform_tile': (i32, [vp, i32, i32]),
         'cov_timing': (i32, [vp, i32]),
         'cov_timing_read': (i32, [vp, vp, vp, i32]),
         'cov_search_phases': (i32, [vp, vp, i32]),
@@ -237,6 +238,13 @@ class Cell:
                     'cov_distance_transform')
         return dist, hist, rad.value
 
+    def distance_stats(self, r):
+        """The transform without its histogram: (#{dist <= r}, radius)."""
+        w = np.zeros(1, np.int64)
+        rad = ctypes.c_int(0)
+        self._check(lib().cov_distance_stats(self.h, int(r), _p(w), ctypes.byref(rad)), 'cov_distance_stats')
+        return int(w[0]), rad.value
+
     def gather_far(self, r, cap):
         """After distance_transform(): (first `cap` words with dist > r, total)."""
         out = np.zeros(max(int(cap), 1), np.int64)
@@ -279,8 +287,8 @@ class Cell:
         self._check(lib().cov_get_launch(self.h, _p(e), _p(c)), 'cov_get_launch')
         return int(e[0]), int(c[0])
 
-    def set_transform_tile(self, lds_bytes):
-        self._check(lib().cov_set_transform_tile(self.h, int(lds_bytes)), 'cov_set_transform_tile')
+    def set_transform_tile(self, lds_bytes, threads=0):
+        self._check(lib().cov_set_transform_tile(self.h, int(lds_bytes), int(threads)), 'cov_set_transform_tile')
 
     def timing(self, on):
         self._check(lib().cov_timing(self.h, int(bool(on))), 'cov_timing')

@@ -141,8 +141,8 @@ def verify_numpy(code, n, q, R, want_witness=True, max_witness=5, n1=None):
     arr = _as_array(code, n)
     with Cell(q, n, R, arr.shape[0]) as cell:
         cell.set_code(arr, recount=False)
-        _, hist, _ = cell.distance_transform()
-        n_unc = int(q ** n - hist[:R + 1].sum())
+        within, _ = cell.distance_stats(R)
+        n_unc = int(q ** n - within)
         wit = []
         if n_unc and want_witness:
             far, _ = cell.gather_far(R, max_witness)

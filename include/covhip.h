@@ -119,7 +119,11 @@ int     cov_commit(cov_cell_t *h, int i, int p, int v, int64_t *dunc);
  * (cov/verify_independent.py:128-185) and covering_radius (cov/verify_cov.py:237). */
 int     cov_distance_transform(cov_cell_t *h, uint8_t *dist, int64_t *hist, int *radius);
 
-/* (d) after cov_distance_transform of the current code: #{w : dist[w] > r},
+/* (c) the same transform without the histogram: #{w : dist[w] <= r} and the
+ * covering radius (max dist) only; dist stays on the device.  The answer of
+ * verify_numpy (uncovered = q^n - within at r = R) and covering_radius. */
+int     cov_distance_stats(cov_cell_t *h, int r, int64_t *within, int *radius);
+/* (d) after cov_distance_transform (or _stats) of the current code: #{w : dist[w] > r},
  * and the first min(cap, total) such words in ascending order; COV_ESTATE if
  * the code changed since the transform.  With r = R these are the uncovered
  * words and verify_numpy's witnesses (cov/verify_cov.py:169-230). */
@@ -146,9 +150,10 @@ int     cov_search_state(cov_cell_t *h, cov_state_t *st);
 /* Workgroups of the evaluator and commit kernels (the autotuner sets them). */
 int     cov_set_launch(cov_cell_t *h, int eval_groups, int commit_groups);
 int     cov_get_launch(const cov_cell_t *h, int *eval_groups, int *commit_groups);
-/* LDS bytes per distance-transform tile (0 = the default, 32768); the tile
+/* LDS bytes per distance-transform tile (0 = the default, 40960) and threads
+ * per transform workgroup (0 = the default, 256; else 512 or 1024); the tile
  * plan groups the n axes into as few HBM passes as tiles of this size allow. */
-int     cov_set_transform_tile(cov_cell_t *h, int lds_bytes);
+int     cov_set_transform_tile(cov_cell_t *h, int lds_bytes, int threads);
 /* Accumulate per-kernel device time with HIP events recorded around each
  * launch on the cell's stream (on = 1; read back in batches, no host sync
  * between launches), read totals in ms and launch counts for kernel kinds

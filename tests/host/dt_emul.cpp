@@ -4,7 +4,7 @@
 // result with a brute-force minimum distance.  Built with AddressSanitizer by
 // tests/test_dt_emul.py, so any access outside a buffer aborts.
 //
-//   dt_emul q n M seed budget  ->  prints "ok passes=P hist=..." or "FAIL ..."
+//   dt_emul q n M seed budget [threads]  ->  prints "ok passes=P hist=..." or "FAIL ..."
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -43,40 +43,65 @@ int main(int argc, char** argv)
     DtPass pass[MAXN + 1];
     const int np = dt_plan(q, n, N, budget, pass);
     if (np <= 0) { printf("FAIL plan\n"); return 1; }
+
     int axes = 1;                       // the innermost axis, inside the inner pass's rows
     for (int i = 0; i < np; i++) axes += pass[i].nax;
     if (axes != n) { printf("FAIL axes %d\n", axes); return 1; }
-    // the device allocation: N + 64 bytes, 4-aligned
-    uint32_t* Dw = new uint32_t[(N + 64) / 4];
+    // the device allocation: N + 64 bytes, 16-aligned
+    uint4* Dw = new uint4[(N + 64) / 16];
     uint8_t* D = reinterpret_cast<uint8_t*>(Dw);
-    memset(D, 0xAB, (N + 64) / 4 * 4);
-    const int64_t nw = (N >> 5) + 2;
+    memset(D, 0xAB, (N + 64) / 16 * 16);
+    const int64_t nw = (N >> 5) + 2;   // as the device allocation
     uint32_t* cwb = new uint32_t[nw];
     memset(cwb, 0, nw * 4);
     for (int i = 0; i < M; i++) cwb[cw[i] >> 5] |= 1u << (cw[i] & 31);
     uint64_t hist[DT_HB] = {0};
-    const int nth = WG;
+    const int nth = argc > 6 ? atoi(argv[6]) : WG;
+    const int light = argc > 7 ? atoi(argv[7]) : -1;     // >= 0: the light statistics at rcut = light
+    pass[np - 1].full = light < 0;
+    pass[np - 1].rcut = light < 0 ? 0 : light;
+    uint64_t lt_le = 0, lt_max = 0;
     for (int pi = 0; pi < np; pi++) {
         const DtPass& p = pass[pi];
         const int64_t lds = dt_lds_bytes(p);
         if (lds + 4 * DT_HB > 65536 || lds > budget) { printf("FAIL lds %lld\n", (long long)lds); return 1; }
         if ((p.inner != 0) != (pi == 0) || (p.first != 0) != (pi == 0)) { printf("FAIL flags\n"); return 1; }
-        uint32_t* sm = new uint32_t[lds / 4];
+        uint4* smv = new uint4[lds / 16];
+        uint8_t* sm = reinterpret_cast<uint8_t*>(smv);
         for (int64_t t = 0; t < p.ntiles; t++) {
             const DtTile T = dt_tile(p, t);
             std::vector<DtHist> h(nth);
-            for (auto& x : h) memset(&x, 0, sizeof x);
-            for (int tid = 0; tid < nth; tid++) dt_load(p, T, tid, nth, sm, reinterpret_cast<const uint32_t*>(D), cwb);
+            for (auto& x : h) memset(&x, 0, sizeof x);     // per tile here; the kernel keeps them across tiles
+            for (int tid = 0; tid < nth; tid++) dt_load(p, T, tid, nth, sm, reinterpret_cast<const uint4*>(D), cwb);
+            for (int tid = 0; tid < nth; tid++) dt_realign(p, T, tid, nth, sm);
             if (p.inner)
                 for (int tid = 0; tid < nth; tid++) dt_inrow(p, tid, nth, sm);
             for (int ax = 0; ax < p.nax; ax++)
                 for (int tid = 0; tid < nth; tid++) dt_axis(p, T, ax, tid, nth, sm);
-            for (int tid = 0; tid < nth; tid++) dt_store(p, T, tid, nth, sm, D, h[tid]);
-            if (p.last)
-                for (int tid = 0; tid < nth; tid++)
-                    for (int b = 0; b <= n + 1; b++) hist[b] += (h[tid].w[b >> 2] >> (16 * (b & 3))) & 0xFFFF;
+            for (int tid = 0; tid < nth; tid++) dt_store(p, T, tid, nth, sm, D);
+            if (p.last && !p.full)
+                for (int tid = 0; tid < nth; tid++) {
+                    unsigned int lh[DT_HB] = {0};
+                    DtStat st{};
+                    dt_hist_tile(p, T, tid, nth, sm, h[tid], st, lh);
+                    lt_le += st.le;
+                    for (int k = 0; k < 4; k++) lt_max = std::max<uint64_t>(lt_max, (st.mx >> (8 * k)) & 0xFF);
+                }
+            if (p.last && p.full)
+                for (int tid = 0; tid < nth; tid++) {
+                    unsigned int lh[DT_HB] = {0};
+                    DtStat st{};
+                    dt_hist_tile(p, T, tid, nth, sm, h[tid], st, lh);
+                    uint32_t prev = 0;
+                    for (int b = 0; b < DT_HL; b++) {
+                        hist[b <= n ? b : n + 1] += h[tid].le[b] - prev;
+                        prev = h[tid].le[b];
+                    }
+                    for (int b = 0; b < DT_HB; b++) hist[b] += lh[b];
+                    if (h[tid].all != prev) { printf("FAIL hist all %u le7 %u\n", h[tid].all, prev); return 1; }
+                }
         }
-        delete[] sm;
+        delete[] smv;
     }
     // brute force on small spaces; on large ones the textbook axis DP, one
     // full-array pass per axis
@@ -114,9 +139,22 @@ int main(int argc, char** argv)
         if (D[w] != best) { if (bad < 5) printf("mismatch w=%lld got %d want %d\n", (long long)w, D[w], best); bad++; }
         bh[best <= n ? best : n + 1]++;
     }
+    if (light >= 0) {
+        uint64_t want_le = 0, want_max = 0;
+        for (int b = 0; b <= n + 1; b++) {
+            if (b <= light) want_le += bh[b];
+            if (bh[b]) want_max = b;
+        }
+        if (want_le != lt_le || want_max != lt_max) {
+            printf("FAIL light le %llu/%llu max %llu/%llu\n", (unsigned long long)lt_le, (unsigned long long)want_le,
+                   (unsigned long long)lt_max, (unsigned long long)want_max);
+            return 1;
+        }
+        for (int b = 0; b <= n + 1; b++) hist[b] = bh[b];
+    }
     for (int b = 0; b <= n + 1; b++)
         if (bh[b] != hist[b]) { printf("FAIL hist bin %d %llu vs %llu\n", b, (unsigned long long)hist[b], (unsigned long long)bh[b]); return 1; }
-    for (int64_t k = N; k < (N + 64) / 4 * 4; k++)
+    for (int64_t k = N; k < (N + 64) / 16 * 16; k++)
         if (D[k] != 0xAB) { printf("FAIL wrote past N at %lld\n", (long long)k); return 1; }
     if (bad) { printf("FAIL %lld mismatches\n", (long long)bad); return 1; }
     printf("ok passes=%d tiles=", np);

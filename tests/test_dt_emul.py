@@ -20,6 +20,10 @@ SHAPES = [  # q n M seed lds
     # the cells of the workloads: K5(7,2), K6(9,4), K7(9,4), K6(10,4) (checked
     # against a full-array axis DP on the large ones)
     (5, 7, 400, 21, 32768), (6, 9, 660, 22, 32768), (7, 9, 1480, 23, 32768), (6, 10, 2784, 24, 32768),
+    # wider workgroups
+    (7, 5, 10, 3, 4096, 512), (3, 10, 25, 6, 8192, 1024), (2, 16, 3, 10, 8192, 512), (6, 9, 660, 25, 40960, 1024),
+    # the light statistics (#{dist <= r} and the maximum) instead of the histogram
+    (5, 7, 40, 31, 32768, 256, 2), (7, 9, 1480, 32, 40960, 512, 4), (2, 16, 3, 33, 8192, 256, 9), (3, 11, 25, 34, 4096, 256, 0),
 ]
 
 
@@ -38,7 +42,7 @@ def emul(tmp_path_factory):
     return out
 
 
-@pytest.mark.parametrize('shape', SHAPES, ids=lambda s: 'K%d(%d)M%d_lds%d' % (s[0], s[1], s[2], s[4]))
+@pytest.mark.parametrize('shape', SHAPES, ids=lambda s: 'K%d(%d)M%d_lds%d_t%d%s' % (s[0], s[1], s[2], s[4], s[5] if len(s) > 5 else 256, '_r%d' % s[6] if len(s) > 6 else ''))
 def test_transform_phases_match_brute_force(emul, shape):
     r = subprocess.run([emul] + [str(x) for x in shape], capture_output=True, text=True, timeout=300)
     assert r.returncode == 0 and r.stdout.startswith('ok'), r.stdout + r.stderr[-3000:]

@@ -122,7 +122,7 @@ def test_distance_transform_tile_plans(q, n, R, M, lds):
     print('case transform plans K%d(%d,%d) M=%d lds=%d' % (q, n, R, M, lds), flush=True)
     rng = np.random.default_rng(lds + q)
     with cell(q, n, R, M) as c:
-        c.set_transform_tile(lds)
+        c.set_transform_tile(lds, 512 if lds < 32768 else 256)
         for rep in range(3):
             code = rng.integers(0, q, size=(M, n)).astype(np.uint8)
             want = mindist(code, n, q)
@@ -156,3 +156,21 @@ def test_recount_matches_oracle_k7_9_4():
         dist, hist, _ = c.distance_transform(want_array=True)
         assert int(hist[5:].sum()) == o.uncovered
         assert ((dist <= 4) == (o.cnt() > 0)).all()
+
+
+@pytest.mark.parametrize('q,n,R,M', [(5, 7, 2, 400), (3, 8, 3, 30), (7, 5, 2, 9), (2, 12, 4, 60)])
+def test_distance_stats(q, n, R, M):
+    """The transform without its histogram: #{dist <= r} and the radius, and
+    the far-word gather after it."""
+    print('case distance stats K%d(%d,%d) M=%d' % (q, n, R, M), flush=True)
+    rng = np.random.default_rng(100 + q)
+    code = rng.integers(0, q, size=(M, n)).astype(np.uint8)
+    want = mindist(code, n, q)
+    with cell(q, n, R, M) as c:
+        c.set_code(code, recount=False)
+        for r in range(0, n + 1):
+            within, rad = c.distance_stats(r)
+            assert within == int((want <= r).sum()) and rad == int(want.max())
+        far, tot = c.gather_far(R, 7)
+        assert tot == int((want > R).sum())
+        assert (far == np.flatnonzero(want > R)[:7]).all()
