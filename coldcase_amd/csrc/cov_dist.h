@@ -542,8 +542,8 @@ __global__ void __launch_bounds__(WG) k_cwb_mark(const Cell C, uint32_t* cwb, in
 
 // ------------------------------------------------------------- planning
 // LDS bytes of a tile (the autotuner may change it)
-constexpr int DT_LDS_DEFAULT = 40960;
-constexpr int DT_THREADS_DEFAULT = 256;
+constexpr int DT_LDS_DEFAULT = 49152;
+constexpr int DT_THREADS_DEFAULT = 512;
 constexpr int DT_LDS_MAX = 65536 - 1024;
 
 inline uint32_t dt_magic(uint64_t d)

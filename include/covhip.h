@@ -150,8 +150,8 @@ int     cov_search_state(cov_cell_t *h, cov_state_t *st);
 /* Workgroups of the evaluator and commit kernels (the autotuner sets them). */
 int     cov_set_launch(cov_cell_t *h, int eval_groups, int commit_groups);
 int     cov_get_launch(const cov_cell_t *h, int *eval_groups, int *commit_groups);
-/* LDS bytes per distance-transform tile (0 = the default, 40960) and threads
- * per transform workgroup (0 = the default, 256; else 512 or 1024); the tile
+/* LDS bytes per distance-transform tile (0 = the default, 49152) and threads
+ * per transform workgroup (0 = the default, 512; else 256 or 1024); the tile
  * plan groups the n axes into as few HBM passes as tiles of this size allow. */
 int     cov_set_transform_tile(cov_cell_t *h, int lds_bytes, int threads);
 /* Accumulate per-kernel device time with HIP events recorded around each
