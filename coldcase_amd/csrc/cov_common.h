@@ -137,6 +137,7 @@ struct Cell {
     GPTR(uint8_t) slot_pos;      // slot -> differing positions (MAXN each)
     GPTR(uint64_t) rbuf;
     GPTR(int64_t) scratch64;     // small reduction outputs
+    GPTR(int32_t) arrive;        // workgroup arrival sub-counters: evaluator, commit
     GPTR(uint8_t) dist;          // distance transform output (N bytes), lazily
     GPTR(Ctl) ctl;
 };

@@ -702,20 +702,35 @@ __device__ __forceinline__ void wg_select(const Cell& C, Ctl* ctl, SelSh& sh)
     }
 }
 
-// One release per workgroup on `counter` (nblk workgroups take part); returns
-// true in the workgroup that completes the count, which then acquires.  All
-// threads.
-__device__ __forceinline__ bool wg_arrive_last(int32_t* counter, int* flag, int nblk)
+// Arrival of the nblk workgroups of one kernel, in two levels, so that the
+// arrivals spread over ARR_G addresses of the L2 instead of queueing on one:
+// workgroup bid releases onto sub[bid % G] (G = min(nblk, ARR_G)); the
+// workgroup that completes a sub-counter acquires it, clears it for the next
+// launch and releases onto `top`; the one that completes `top` (G arrivals)
+// is the last, acquires, and returns true.  Every workgroup's writes before
+// the call happen before the last one's reads after it.  All threads.
+constexpr int ARR_G = 32;
+__device__ __forceinline__ bool wg_arrive_last(int32_t* sub, int32_t* top, int* flag, int bid, int nblk)
 {
     __syncthreads();
     if (threadIdx.x == 0) {
-        const int prev = __hip_atomic_fetch_add(counter, 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-        *flag = (prev == nblk - 1);
+        const int G = nblk < ARR_G ? nblk : ARR_G;
+        const int g = bid % G;
+        const int gsize = nblk / G + (g < nblk % G ? 1 : 0);
+        int last = 0;
+        const int prev = __hip_atomic_fetch_add(&sub[g], 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+        if (prev == gsize - 1) {
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+            __hip_atomic_store(&sub[g], 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            const int pt = __hip_atomic_fetch_add(top, 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+            last = (pt == G - 1);
+        }
+        *flag = last;
     }
     __syncthreads();
     const bool last = *flag != 0;
     if (last) {
-        __threadfence();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
         __syncthreads();
     }
     return last;
@@ -752,7 +767,7 @@ __device__ __forceinline__ void eval_body(const Cell& C, int bid, int nblk)
     } else if (nblk == 1) {
         eval_units(C, ctl, wv, WAVES);
     }
-    if (!wg_arrive_last(&ctl->evdone, &last, nblk)) return;
+    if (!wg_arrive_last(C.arrive, &ctl->evdone, &last, bid, nblk)) return;
     const int64_t ts0 = clk();
     __shared__ Ctl lc;
     ctl_to_lds(&lc, ctl);
@@ -962,7 +977,7 @@ __device__ __forceinline__ void commit_body(const Cell& C, int bid, int nblk, in
     const int64_t d = commit_units(C, ctl, gt, (int64_t)nblk * WG);
     const int64_t ds = wg_sum(d, sh.red);
     if (threadIdx.x == 0 && ds) atomicAdd(reinterpret_cast<unsigned long long*>(&ctl->dunc), (unsigned long long)ds);
-    if (!wg_arrive_last(&ctl->cmdone, &last, nblk)) return;
+    if (!wg_arrive_last(C.arrive + ARR_G, &ctl->cmdone, &last, bid, nblk)) return;
     __shared__ Ctl lc;
     ctl_to_lds(&lc, ctl);
     Ctl* L = &lc;

@@ -139,7 +139,7 @@ int shape_of(int q, int n, int R, int M, Shape* s)
 // device bytes of each array, in allocation order
 struct Sizes {
     int64_t cnt, plane, blk, code, cpk, cidx, tabu, fsub, psub, bsub, ssub, ul_idx, ul_pk,
-        shuf, acc, slot_cw, slot_pos, rbuf, ctl, cell, small;
+        shuf, acc, slot_cw, slot_pos, rbuf, ctl, cell, small, arrive;
 };
 
 Sizes sizes_of(const Shape& s)
@@ -166,6 +166,7 @@ Sizes sizes_of(const Shape& s)
     z.ctl = sizeof(Ctl);
     z.cell = sizeof(Cell);
     z.small = 64 * 8;
+    z.arrive = 2 * 32 * 4;
     return z;
 }
 
@@ -175,7 +176,7 @@ int64_t total_bytes(const Shape& s)
     // plus the distance array and codeword bitmap the transform allocates on first use
     return z.cnt + 3 * z.plane + z.blk + 2 * z.code + z.cpk + z.cidx + z.tabu + z.fsub + z.psub +
            z.bsub + z.ssub + z.ul_idx + z.ul_pk + z.shuf + z.acc + z.slot_cw + z.slot_pos + z.rbuf +
-           z.ctl + z.cell + z.small + (s.N + 64) + ((s.N >> 5) + 2) * 4;
+           z.ctl + z.cell + z.small + z.arrive + (s.N + 64) + ((s.N >> 5) + 2) * 4;
 }
 
 template <typename T>
@@ -496,6 +497,7 @@ int cov_create(int q, int n, int R, int M, cov_cell_t** out)
     AL(C.ctl, z.ctl);
     AL(h->d, z.cell);
     AL(h->d_small, z.small);
+    AL(C.arrive, z.arrive);
 #undef AL
     C.scratch64 = (decltype(C.scratch64))h->d_small;
 
