@@ -32,6 +32,7 @@
 namespace cov {
 
 constexpr int DT_HB = MAXN + 2;      // histogram bins: 0..n, and unreached in bin n+1
+constexpr int DT_SMAX = 6;           // axes a group of at most 64 words can span (q = 2)
 constexpr uint32_t DT_INF = 0x7Fu;   // "not reached yet"
 constexpr uint32_t DT_INF4 = 0x7F7F7F7Fu;
 
@@ -62,6 +63,18 @@ struct DtPass {
     uint32_t u[MAXN];    // per row axis: stride in rows
     uint32_t mg[MAXN];   // magic of u
     uint32_t mq;         // magic of q
+    // column passes (no LDS): one thread per dword column of a row block
+    int32_t col;         // 1: a column pass of nax axes, nrow = q^nax rows
+    uint32_t ncol;       // dword columns per row, ceil(S_row / 4)
+    uint32_t nthr;       // threads: row blocks * ncol
+    uint32_t nbytes;     // bytes the buffer descriptor covers (N + 64)
+    // the seed pass (first pass, from the codeword bitmap; rows are groups)
+    int32_t seed;        // 1: the seed pass of ns + nax axes
+    int32_t ns;          // innermost axes decided inside a group from its bits
+    int32_t G;           // words per group, q^ns <= 64
+    int32_t P;           // LDS bytes per group row: G rounded up to 4
+    uint32_t mG;         // magic of G
+    uint64_t sbase[DT_SMAX]; // digit j < ns of a group: the words whose digit j is 0
 };
 
 // exact floor(f / d) for f * d < 2^32, with m = floor(2^32 / d) + 1 (d > 1)
@@ -467,6 +480,44 @@ __host__ __device__ inline void dt_hist_tile(const DtPass& p, const DtTile& T, i
     }
 }
 
+// the last pass's statistics of a workgroup into the global bins.  Light:
+// [0] #{values <= rcut}, [1] the maximum; full: bin d = #{values == d}, d <= n,
+// and bin n + 1 the unreached.  Wave sums first, then one LDS atomic per wave
+// and bin, then one global atomic per bin.  All threads.
+__device__ inline void dt_stats_flush(const DtPass& p, const DtHist& h, const DtStat& st, unsigned int* lh,
+                                      unsigned long long* hist)
+{
+    __syncthreads();                            // the rare-value bins are complete
+    const bool lead = (threadIdx.x & 63) == 0;
+    if (!p.full) {
+        uint32_t mb = 0;
+        for (int k = 0; k < 4; k++) {
+            const uint32_t b = (st.mx >> (8 * k)) & 0xFFu;
+            mb = b > mb ? b : mb;
+        }
+        const int64_t le = wave_sum64((int64_t)st.le);
+        const int64_t mx = wave_max64((int64_t)mb);
+        if (lead && le) atomicAdd(&lh[0], (unsigned int)le);
+        if (lead && mx) atomicMax(&lh[1], (unsigned int)mx);
+    } else {
+        // values < 8 (all of them below bin n + 1 unless n < 7, where no
+        // distance exceeds n): bin r = le[r] - le[r-1]
+        uint32_t prev = 0;
+#pragma unroll
+        for (int r = 0; r < DT_HL; r++) {
+            const int64_t c = wave_sum64((int64_t)(h.le[r] - prev));
+            prev = h.le[r];
+            if (lead && c) atomicAdd(&lh[r <= p.hn ? r : p.hn + 1], (unsigned int)c);
+        }
+    }
+    __syncthreads();
+    const int nbin = p.full ? p.hn + 2 : 2;
+    if ((int)threadIdx.x < nbin && lh[threadIdx.x]) {
+        if (p.full || threadIdx.x == 0) atomicAdd(&hist[threadIdx.x], (unsigned long long)lh[threadIdx.x]);
+        else atomicMax(&hist[1], (unsigned long long)lh[1]);
+    }
+}
+
 // ---------------------------------------------------------------- kernels
 template <int NT>
 __global__ void __launch_bounds__(NT) k_dt_pass(const DtPass p, uint8_t* D, const uint32_t* cwb,
@@ -496,38 +547,376 @@ __global__ void __launch_bounds__(NT) k_dt_pass(const DtPass p, uint8_t* D, cons
         dt_store(p, T, threadIdx.x, NT, sm, D);
         if (p.last) dt_hist_tile(p, T, threadIdx.x, NT, sm, h, st, lh);
     }
-    if (p.last && !p.full) {
-        // bins: [0] #{values <= rcut}, [1] the maximum
-        __syncthreads();
-        uint32_t mb = 0;
-        for (int k = 0; k < 4; k++) {
-            const uint32_t b = (st.mx >> (8 * k)) & 0xFFu;
-            mb = b > mb ? b : mb;
-        }
-        if (st.le) atomicAdd(&lh[0], st.le);
-        if (mb) atomicMax(&lh[1], mb);
-        __syncthreads();
-        if (threadIdx.x < 2 && lh[threadIdx.x]) {
-            if (threadIdx.x == 0) atomicAdd(&hist[0], (unsigned long long)lh[0]);
-            else atomicMax(&hist[1], (unsigned long long)lh[1]);
-        }
-    }
-    if (p.last && p.full) {
-        __syncthreads();
-        // values < 8 (all of them below bin n + 1 unless n < 7, where no
-        // distance exceeds n): bin r = le[r] - le[r-1]
-        uint32_t prev = 0;
+    if (p.last) dt_stats_flush(p, h, st, lh, hist);
+}
+
+// ------------------------------------------------------------ column pass
+// An outer group of b axes whose q^b rows fit in a thread's registers
+// (q^b <= DT_COL_MAX): thread t owns dword column c of row block a, the 4
+// words [c4, c4 + 4) of each of the q^b rows (c4 = 4 c).  It loads them,
+// applies the b axes in registers and stores them back: no LDS and no
+// barrier, and the 64 lanes of a wave read 256 consecutive bytes of each row.
+// Rows start at any byte, so the dwords are unaligned (the buffer loads and
+// stores take byte offsets).  A row's last column can hold fewer than 4 of
+// its words: its loads read on into the next row (or the allocation's 64
+// bytes of padding) and its stores write only the row's own bytes.  Offsets
+// are 32-bit: the pass needs N + 64 < 2^32.
+constexpr int DT_COL_MAX = 64;
+constexpr int DT_COL_NT = 256;
+
+constexpr int dt_cpow(int q, int b) { return b == 0 ? 1 : q * dt_cpow(q, b - 1); }
+
+// axes per column pass: as many as keep a thread under ~100 VGPRs
+// (2^4, 3^3 and 8^2 rows do; 2^5 and 4^3 rows do not)
+inline int dt_col_bmax(int q)
+{
+    return q == 2 ? 4 : (q == 3 ? 3 : (q <= 8 ? 2 : 1));
+}
+
+template <int Q, int B, class Mem>
+__host__ __device__ __forceinline__ void dt_col_thread(const DtPass& p, uint32_t t, const Mem& m, DtHist& h,
+                                                       DtStat& st, unsigned int* lh)
+{
+    constexpr int NR = dt_cpow(Q, B);
+    static_assert(NR <= DT_COL_MAX, "a column must fit in registers");
+    const uint32_t S = (uint32_t)p.S_row;
+    const uint32_t a = t / p.ncol;
+    const uint32_t c4 = 4 * (t - a * p.ncol);
+    const uint32_t off = a * (uint32_t)p.S_tile + c4;
+    uint32_t v[NR];
 #pragma unroll
-        for (int r = 0; r < DT_HL; r++) {
-            const uint32_t c = h.le[r] - prev;
-            prev = h.le[r];
-            if (c) atomicAdd(&lh[r <= p.hn ? r : p.hn + 1], c);
+    for (int r = 0; r < NR; r++) v[r] = m.ld(off, (uint32_t)r * S);
+    // axis k: fibers of Q rows u = Q^k apart
+    int u = 1;
+#pragma unroll
+    for (int k = 0; k < B; k++) {
+#pragma unroll
+        for (int f = 0; f < NR / Q; f++) {
+            const int r0 = (f / u) * u * Q + f % u;
+            uint32_t mn = v[r0];
+#pragma unroll
+            for (int j = 1; j < Q; j++) mn = dt_pmin(mn, v[r0 + j * u]);
+            mn = dt_pinc(mn);
+#pragma unroll
+            for (int j = 0; j < Q; j++) v[r0 + j * u] = dt_pmin(v[r0 + j * u], mn);
         }
-        __syncthreads();
-        if (threadIdx.x < p.hn + 2 && lh[threadIdx.x])
-            atomicAdd(&hist[threadIdx.x], (unsigned long long)lh[threadIdx.x]);
+        u *= Q;
+    }
+    const uint32_t nb = S - c4;                 // the row's words in this column (>= 1)
+    if (nb >= 4) {
+#pragma unroll
+        for (int r = 0; r < NR; r++) m.st(off, (uint32_t)r * S, v[r]);
+    } else {
+#pragma unroll
+        for (int r = 0; r < NR; r++)
+            for (uint32_t i = 0; i < nb; i++) m.st8(off + i, (uint32_t)r * S, v[r] >> (8 * i));
+    }
+    if (p.last) {
+        const uint32_t nv = nb < 4 ? nb : 4u;
+#pragma unroll
+        for (int r = 0; r < NR; r++) {
+            if (p.full) dt_hist_add4(h, v[r], nv, p.hn, lh);
+            else dt_stat_add4(st, v[r], nv, p.rcut);
+        }
     }
 }
+
+#ifdef __HIP_DEVICE_COMPILE__
+// the distance array through a buffer descriptor: byte offsets, range-checked
+// against nbytes (a load past it reads 0, a store past it is dropped)
+struct DtBufMem {
+    __amdgpu_buffer_rsrc_t rs;
+    __device__ uint32_t ld(uint32_t off, uint32_t soff) const
+    {
+        return __builtin_amdgcn_raw_buffer_load_b32(rs, off, soff, 0);
+    }
+    __device__ void st(uint32_t off, uint32_t soff, uint32_t v) const
+    {
+        __builtin_amdgcn_raw_buffer_store_b32(v, rs, off, soff, 0);
+    }
+    __device__ void st8(uint32_t off, uint32_t soff, uint32_t v) const
+    {
+        __builtin_amdgcn_raw_buffer_store_b8((uint8_t)v, rs, off, soff, 0);
+    }
+};
+#endif
+
+template <int Q, int B>
+__global__ void __launch_bounds__(DT_COL_NT) k_dt_col(const DtPass p, uint8_t* D, unsigned long long* hist)
+{
+    __shared__ unsigned int lh[DT_HB];
+    if (p.last) {
+        if (threadIdx.x < DT_HB) lh[threadIdx.x] = 0;
+        __syncthreads();
+    }
+    DtHist h{};
+    DtStat st{};
+    const uint32_t t = blockIdx.x * DT_COL_NT + threadIdx.x;
+#ifdef __HIP_DEVICE_COMPILE__
+    if (t < p.nthr) {
+        DtBufMem m;
+        m.rs = __builtin_amdgcn_make_buffer_rsrc(D, (short)0, (int)p.nbytes, 0x00020000);
+        dt_col_thread<Q, B>(p, t, m, h, st, lh);
+    }
+#endif
+    if (p.last) dt_stats_flush(p, h, st, lh, hist);
+}
+
+// the (q, b) pairs of column passes, each its own kernel
+template <int Q, int B>
+struct DtQB {
+    static constexpr int q = Q, b = B;
+};
+template <class F>
+inline bool dt_col_dispatch(int q, int b, F&& f)
+{
+#define DT_QB(Q, B) \
+    case Q * 8 + B: f(DtQB<Q, B>{}); return true;
+    switch (q * 8 + b) {
+        DT_QB(2, 1) DT_QB(2, 2) DT_QB(2, 3) DT_QB(2, 4)
+        DT_QB(3, 1) DT_QB(3, 2) DT_QB(3, 3)
+        DT_QB(4, 1) DT_QB(4, 2)
+        DT_QB(5, 1) DT_QB(5, 2) DT_QB(6, 1) DT_QB(6, 2) DT_QB(7, 1) DT_QB(7, 2) DT_QB(8, 1) DT_QB(8, 2)
+        DT_QB(9, 1) DT_QB(10, 1) DT_QB(11, 1) DT_QB(12, 1) DT_QB(13, 1) DT_QB(14, 1) DT_QB(15, 1) DT_QB(16, 1)
+    default: break;
+    }
+#undef DT_QB
+    return false;
+}
+
+// -------------------------------------------------------------- seed pass
+// The first pass, built from the codeword bitmap (one bit per word) instead
+// of a distance array.  Its a innermost axes are split in two:
+//   the ns innermost axes inside a group of G = q^ns <= 64 consecutive words,
+//   decided from the group's G bits alone: with L_0 the codewords and
+//   L_{k+1} = L_k | spread_0(L_k) | ... | spread_{ns-1}(L_k) (spread_j sets a
+//   whole fiber of digit j where any of its words is set), a word's distance
+//   inside the group is the first k with the word in L_k, or "unreached" for a
+//   group without codewords;
+//   the other a - ns axes across the rows of the LDS tile, one row per group
+//   (G bytes padded to P), each axis one phase over q rows u = q^k apart.
+// A tile without any codeword (common on sparse codes) skips the row axes and
+// stores "unreached" everywhere.  The store reads the tile's bytes back in
+// order and writes them with unaligned dword stores; the last pass's
+// statistics are taken from the stored values.
+
+// bytes 0x00 / 0xFF from the low 4 bits of b
+__host__ __device__ inline uint32_t dt_e4(uint32_t b)
+{
+    return ((b * 0x00204081u) & 0x01010101u) * 0xFFu;
+}
+
+// spread along digit j (fibers of Q words u apart) of a group's bit mask
+template <int Q>
+__host__ __device__ inline uint64_t dt_spread(uint64_t x, uint32_t u, uint64_t base)
+{
+    uint64_t y = x;
+#pragma unroll
+    for (int v = 1; v < Q; v++) y |= x >> (v * u);
+    y &= base;
+    uint64_t z = y;
+#pragma unroll
+    for (int v = 1; v < Q; v++) z |= y << (v * u);
+    return z;
+}
+
+// group gi of tile T into its LDS row; returns whether it holds a codeword
+template <int Q>
+__host__ __device__ inline bool dt_seed_group(const DtPass& p, const DtTile& T, uint32_t gi, uint8_t* sm,
+                                              const uint32_t* cwb)
+{
+    const uint32_t G = (uint32_t)p.G;
+    const int64_t W0 = T.gs0 + (int64_t)gi * G;           // first word of the group
+    const int64_t wi = W0 >> 5;
+    const uint32_t sh = (uint32_t)W0 & 31u;
+    const uint64_t lo = (uint64_t)cwb[wi] | ((uint64_t)cwb[wi + 1] << 32);   // wi + 1 <= (N - 1) / 32 + 1
+    uint64_t m = lo >> sh;
+    if (sh + G > 64) m |= (uint64_t)cwb[wi + 2] << (64 - sh);
+    const uint64_t full = G == 64 ? ~(uint64_t)0 : (((uint64_t)1 << G) - 1);
+    m &= full;
+    uint32_t* row = reinterpret_cast<uint32_t*>(sm + (size_t)gi * (uint32_t)p.P);
+    const uint32_t nd = (uint32_t)p.P >> 2;
+    if (!m) {
+        for (uint32_t c = 0; c < nd; c++) row[c] = DT_INF4;
+        return false;
+    }
+    // levels L_0 .. L_{ns-1}; L_ns is the whole group
+    uint64_t L[DT_SMAX];
+    L[0] = m;
+#pragma unroll
+    for (int k = 1; k < DT_SMAX; k++) {
+        if (k < p.ns) {
+            uint64_t x = L[k - 1];
+            uint32_t u = 1;
+            for (int j = 0; j < p.ns; j++, u *= Q) x |= dt_spread<Q>(L[k - 1], u, p.sbase[j]);
+            L[k] = x;
+        }
+    }
+    const uint32_t ks = (uint32_t)p.ns * 0x01010101u;
+    for (uint32_t c = 0; c < nd; c++) {
+        const uint32_t ef = dt_e4((uint32_t)(full >> (4 * c)) & 15u);   // words of the group (not padding)
+        uint32_t v = (ef & ks) | (~ef & DT_INF4);
+#pragma unroll
+        for (int k = DT_SMAX - 1; k >= 0; k--) {
+            if (k < p.ns) {
+                const uint32_t e = dt_e4((uint32_t)(L[k] >> (4 * c)) & 15u);
+                v = (e & ((uint32_t)k * 0x01010101u)) | (~e & v);
+            }
+        }
+        row[c] = v;
+    }
+    return true;
+}
+
+// per-u16 min and sum of two dwords each holding two u16 lanes
+__host__ __device__ inline uint32_t dt_min16(uint32_t a, uint32_t b)
+{
+#ifdef __HIP_DEVICE_COMPILE__
+    typedef uint16_t u16x2 __attribute__((ext_vector_type(2)));
+    return __builtin_bit_cast(uint32_t, __builtin_elementwise_min(__builtin_bit_cast(u16x2, a),
+                                                                  __builtin_bit_cast(u16x2, b)));
+#else
+    const uint32_t l = (a & 0xFFFFu) < (b & 0xFFFFu) ? (a & 0xFFFFu) : (b & 0xFFFFu);
+    const uint32_t h = (a >> 16) < (b >> 16) ? (a >> 16) : (b >> 16);
+    return l | (h << 16);
+#endif
+}
+
+// row axis k of the tile (fibers of Q rows u = q^k apart): each dword is split
+// into its even and odd bytes, two u16 lanes each, so that one packed
+// 16-bit min handles two values; 1 + the fiber minimum is clamped to
+// "unreached", so every value stays a byte <= 127
+template <int Q>
+__host__ __device__ inline void dt_seed_axis(const DtPass& p, int ax, int tid, int nth, uint8_t* sm)
+{
+    const uint32_t u = p.u[ax], mg = p.mg[ax];
+    const uint32_t nd = (uint32_t)p.P >> 2;
+    const uint32_t nf = (uint32_t)p.nrow / Q;
+    const uint32_t nslot = nf * nd, step = u * nd;
+    uint32_t* al = reinterpret_cast<uint32_t*>(sm);
+    for (uint32_t e = tid; e < nslot; e += nth) {
+        const uint32_t f = dt_div(e, nd, p.mda);
+        const uint32_t c = e - f * nd;
+        const uint32_t hi = dt_div(f, u, mg), lo = f - hi * u;
+        uint32_t* x = al + (hi * u * Q + lo) * nd + c;
+        uint32_t ev[Q], od[Q];
+#pragma unroll
+        for (int v = 0; v < Q; v++) {
+            const uint32_t w = x[v * step];
+            ev[v] = w & 0x00FF00FFu;
+            od[v] = (w >> 8) & 0x00FF00FFu;
+        }
+        uint32_t me = ev[0], mo = od[0];
+#pragma unroll
+        for (int v = 1; v < Q; v++) {
+            me = dt_min16(me, ev[v]);
+            mo = dt_min16(mo, od[v]);
+        }
+        me = dt_min16(me + 0x00010001u, 0x007F007Fu);
+        mo = dt_min16(mo + 0x00010001u, 0x007F007Fu);
+#pragma unroll
+        for (int v = 0; v < Q; v++) x[v * step] = dt_min16(ev[v], me) | (dt_min16(od[v], mo) << 8);
+    }
+}
+
+// the tile's bytes in order: byte b sits in group b / G, column b % G
+__host__ __device__ inline uint32_t dt_seed_dword(const DtPass& p, const uint8_t* sm, uint32_t b)
+{
+    const uint32_t G = (uint32_t)p.G, P = (uint32_t)p.P;
+    const uint32_t gi = dt_div(b, G, p.mG), col = b - gi * G;
+    const uint32_t at = gi * P + col;
+    const uint32_t* al = reinterpret_cast<const uint32_t*>(sm);
+    if (col + 4 <= G) {
+        if ((at & 3) == 0) return al[at >> 2];
+        return dt_align(al[(at >> 2) + 1], al[at >> 2], at & 3);
+    }
+    uint32_t v = 0;
+    for (uint32_t i = 0; i < 4; i++) {
+        const uint32_t c = col + i;
+        const uint32_t a2 = c < G ? at + i : (gi + 1) * P + (c - G);
+        v |= (uint32_t)sm[a2] << (8 * i);
+    }
+    return v;
+}
+
+// the store of a seed tile (all of it "unreached" when it has no codeword),
+// and the last pass's statistics
+template <class Mem>
+__host__ __device__ inline void dt_seed_store(const DtPass& p, const DtTile& T, int tid, int nth, const uint8_t* sm,
+                                              bool any, const Mem& m, DtHist& h, DtStat& st, unsigned int* lh)
+{
+    const uint32_t tot = (uint32_t)p.S_tile, nk = tot >> 2;
+    const uint32_t g0 = (uint32_t)T.gs0;
+    for (uint32_t k = tid; k < nk; k += nth) {
+        const uint32_t v = any ? dt_seed_dword(p, sm, 4 * k) : DT_INF4;
+        m.st(4 * k, g0, v);
+        if (p.last) {
+            if (p.full) dt_hist_add4(h, v, 4, p.hn, lh);
+            else dt_stat_add4(st, v, 4, p.rcut);
+        }
+    }
+    const uint32_t tail = tot & 3u;
+    if (tail && tid == nth - 1) {
+        uint32_t v = 0;
+        for (uint32_t i = 0; i < tail; i++) {
+            const uint32_t b = 4 * nk + i;
+            const uint32_t gi = dt_div(b, (uint32_t)p.G, p.mG);
+            const uint32_t x = any ? sm[gi * (uint32_t)p.P + (b - gi * (uint32_t)p.G)] : DT_INF;
+            m.st8(g0 + b, 0, x);
+            v |= x << (8 * i);
+        }
+        if (p.last) {
+            if (p.full) dt_hist_add4(h, v, tail, p.hn, lh);
+            else dt_stat_add4(st, v, tail, p.rcut);
+        }
+    }
+}
+
+template <int Q>
+__global__ void __launch_bounds__(DT_COL_NT) k_dt_seed(const DtPass p, uint8_t* D, const uint32_t* cwb,
+                                                       unsigned long long* hist)
+{
+    extern __shared__ __align__(16) uint8_t sm[];
+    __shared__ unsigned int lh[DT_HB];
+    if (threadIdx.x < DT_HB) lh[threadIdx.x] = 0;
+    DtHist h{};
+    DtStat st{};
+#ifdef __HIP_DEVICE_COMPILE__
+    DtBufMem m;
+    m.rs = __builtin_amdgcn_make_buffer_rsrc(D, (short)0, (int)p.nbytes, 0x00020000);
+    for (int64_t t = blockIdx.x; t < p.ntiles; t += gridDim.x) {
+        const DtTile T = dt_tile(p, t);
+        __syncthreads();
+        int any = 0;
+        for (uint32_t gi = threadIdx.x; gi < (uint32_t)p.nrow; gi += DT_COL_NT)
+            any |= dt_seed_group<Q>(p, T, gi, sm, cwb) ? 1 : 0;
+        any = __syncthreads_or(any);
+        if (any) {
+            for (int ax = 0; ax < p.nax; ax++) {
+                dt_seed_axis<Q>(p, ax, threadIdx.x, DT_COL_NT, sm);
+                __syncthreads();
+            }
+        }
+        dt_seed_store(p, T, threadIdx.x, DT_COL_NT, sm, any != 0, m, h, st, lh);
+    }
+#endif
+    if (p.last) dt_stats_flush(p, h, st, lh, hist);
+}
+
+template <class F>
+inline bool dt_seed_dispatch(int q, F&& f)
+{
+#define DT_SQ(Q) \
+    case Q: f(DtQB<Q, 0>{}); return true;
+    switch (q) {
+        DT_SQ(2) DT_SQ(3) DT_SQ(4) DT_SQ(5) DT_SQ(6) DT_SQ(7) DT_SQ(8) DT_SQ(9)
+        DT_SQ(10) DT_SQ(11) DT_SQ(12) DT_SQ(13) DT_SQ(14) DT_SQ(15) DT_SQ(16)
+    default: break;
+    }
+#undef DT_SQ
+    return false;
+}
+
 
 // codeword bitmap: set (or clear) the bit of every codeword
 __global__ void __launch_bounds__(WG) k_cwb_mark(const Cell C, uint32_t* cwb, int set)
@@ -570,10 +959,94 @@ inline int64_t dt_ipow(int q, int e)
     return r;
 }
 
-// the passes for a cell; returns their number (0 on a bad budget)
-inline int dt_plan(int q, int n, int64_t N, int budget, DtPass* out)
+// axes a seed group spans: q^ns <= 64 words
+inline int dt_seed_smax(int q)
+{
+    return q == 2 ? 6 : (q <= 4 ? 3 : (q <= 8 ? 2 : 1));
+}
+
+// column passes for the outer axes [0, m): dt_col_bmax(q) axes each at
+// most, spread evenly; appends them to out[np..]; returns the new count
+inline int dt_plan_columns(int q, int n, int64_t N, int m, DtPass* out, int np)
+{
+    const int bmax = dt_col_bmax(q);
+    const int ngroups = (m + bmax - 1) / bmax;
+    int j1 = m;
+    for (int g = 0; g < ngroups; g++) {
+        const int left = ngroups - g;
+        const int b = (j1 + left - 1) / left;
+        const int j0 = j1 - b;
+        DtPass& p = out[np++];
+        p = DtPass{};
+        p.N = N; p.q = q; p.hn = n; p.col = 1; p.nax = b;
+        p.nrow = dt_ipow(q, b);
+        p.S_row = dt_ipow(q, n - j1);
+        p.S_tile = p.nrow * p.S_row;
+        p.ncol = (uint32_t)((p.S_row + 3) / 4);
+        p.ntiles = dt_ipow(q, j0) * (int64_t)p.ncol;
+        p.nthr = (uint32_t)p.ntiles;
+        p.nbytes = (uint32_t)(N + 64);
+        j1 = j0;
+    }
+    return np;
+}
+
+// the seed pass over the a innermost axes (tiles of `per` blocks of q^a
+// words, >= 512 tiles on large spaces), then column passes
+inline int dt_plan_seed(int q, int n, int64_t N, int budget, DtPass* out)
+{
+    const int smax = dt_seed_smax(q);
+    auto lds_of = [&](int a) {
+        const int s = a < smax ? a : smax;
+        const int64_t G = dt_ipow(q, s);
+        return dt_ipow(q, a - s) * ((G + 3) & ~(int64_t)3);
+    };
+    int a = 1;
+    while (a < n && lds_of(a + 1) <= budget && (N / dt_ipow(q, a + 1) >= 512 || N <= 64 * (int64_t)budget)) a++;
+    if (lds_of(a) > budget) return 0;
+    const int s = a < smax ? a : smax;
+    const int64_t blk = dt_ipow(q, a);
+    int64_t per = budget / lds_of(a);
+    const int64_t per_par = N / (blk * 512);
+    if (per > per_par) per = per_par;
+    if (per < 1) per = 1;
+    while (per > 1 && N % (per * blk)) per--;
+    DtPass& p = out[0];
+    p = DtPass{};
+    p.N = N; p.q = q; p.hn = n; p.first = 1; p.seed = 1;
+    p.ns = s;
+    p.G = (int32_t)dt_ipow(q, s);
+    p.P = (p.G + 3) & ~3;
+    p.mG = dt_magic((uint64_t)p.G);
+    p.nrow = per * dt_ipow(q, a - s);
+    p.S_row = p.G; p.Lc = p.G; p.nch = 1;
+    p.S_tile = per * blk;
+    p.ntiles = N / p.S_tile;
+    p.nda = (uint32_t)(p.P / 4);
+    p.mda = dt_magic(p.nda);
+    p.lds = p.nrow * p.P;
+    p.nbytes = (uint32_t)(N + 64);
+    dt_fill_axes(p, q, a - s);
+    for (int j = 0; j < s; j++) {
+        const int64_t uj = dt_ipow(q, j);
+        uint64_t b = 0;
+        for (int64_t w = 0; w < p.G; w++)
+            if ((w / uj) % q == 0) b |= (uint64_t)1 << w;
+        p.sbase[j] = b;
+    }
+    if (N % p.S_tile || p.lds > budget) return 0;
+    const int np = dt_plan_columns(q, n, N, n - a, out, 1);
+    out[np - 1].last = 1;
+    return np;
+}
+
+// the passes for a cell; returns their number (0 on a bad budget): a seed
+// pass and column passes, unless `columns` is false or N is too large for
+// 32-bit offsets; then an inner pass and outer passes in LDS tiles.
+inline int dt_plan(int q, int n, int64_t N, int budget, DtPass* out, bool columns = true)
 {
     if (budget < 4096 || budget > DT_LDS_MAX || q < 2 || q > 16 || n < 1 || n > MAXN) return 0;
+    if (columns && N + 64 <= (int64_t)0xFFFFFFFFu) return dt_plan_seed(q, n, N, budget, out);
     int np = 0;
     // inner pass: rows of one innermost fiber, padded to W = 4, 8 or 16 bytes;
     // a axes (the innermost + a-1 across rows) in blocks of q^a words,
@@ -609,8 +1082,8 @@ inline int dt_plan(int q, int n, int64_t N, int budget, DtPass* out)
         dt_fill_axes(p, q, a - 1);
         if (N % p.S_tile || p.lds > budget) return 0;
     }
-    // outer groups: at most bmax axes each, rows of >= 64 words
     const int m = n - a;
+    // outer groups in LDS tiles: at most bmax axes each, rows of >= 64 words
     int bmax = 0;
     for (int64_t B = q; B * (2 * 64 + 48) <= budget && bmax < m; B *= q) bmax++;
     if (m > 0 && bmax == 0) return 0;

@@ -818,7 +818,27 @@ static int run_transform(cov_cell_t* h, int full, int rcut, int64_t hh[DT_HB])
         hipEvent_t ev = nullptr;
         RC(timed_begin(h, 2, &ev));
         const int nt = h->dt_threads ? h->dt_threads : DT_THREADS_DEFAULT;
-        if (nt == 1024)
+        if (p.seed) {
+            const bool ok = dt_seed_dispatch(p.q, [&](auto qb) {
+                using QB = decltype(qb);
+                hipLaunchKernelGGL((k_dt_seed<QB::q>), dim3(g), dim3(DT_COL_NT), (size_t)p.lds, h->stream, p, C.dist,
+                                   h->cwb, dh);
+            });
+            if (!ok) {
+                h->err = "distance transform: no seed kernel for this q";
+                return COV_ELIMIT;
+            }
+        } else if (p.col) {
+            const unsigned gc = (p.nthr + DT_COL_NT - 1) / DT_COL_NT;
+            const bool ok = dt_col_dispatch(p.q, p.nax, [&](auto qb) {
+                using QB = decltype(qb);
+                hipLaunchKernelGGL((k_dt_col<QB::q, QB::b>), dim3(gc), dim3(DT_COL_NT), 0, h->stream, p, C.dist, dh);
+            });
+            if (!ok) {
+                h->err = "distance transform: no column kernel for this (q, axes)";
+                return COV_ELIMIT;
+            }
+        } else if (nt == 1024)
             hipLaunchKernelGGL(k_dt_pass<1024>, dim3(g), dim3(1024), (size_t)dt_lds_bytes(p), h->stream, p, C.dist,
                                h->cwb, dh);
         else if (nt == 512)

@@ -1,10 +1,13 @@
 // Host emulation of the distance-transform kernels: runs the pass plan and the
 // per-thread phases of cov_dist.h thread by thread (phases separated where the
-// kernels place a barrier) over exact-size heap buffers, and compares the
+// kernels place a barrier; column passes one thread after another, through
+// range-checked accessors) over exact-size heap buffers, and compares the
 // result with a brute-force minimum distance.  Built with AddressSanitizer by
 // tests/test_dt_emul.py, so any access outside a buffer aborts.
 //
-//   dt_emul q n M seed budget [threads]  ->  prints "ok passes=P hist=..." or "FAIL ..."
+//   dt_emul q n M seed budget [threads [light_rcut [columns]]]
+//     -> prints "ok passes=P hist=..." or "FAIL ..."; light_rcut < 0 is the full
+//        histogram, columns = 0 keeps every outer pass in LDS tiles
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -13,6 +16,37 @@
 #include "cov_dist.h"
 
 using namespace cov;
+
+// the distance array as the column passes' buffer descriptor sees it: byte
+// offsets, every access checked against the descriptor's range
+struct HostMem {
+    uint8_t* D;
+    uint64_t nbytes;
+    void chk(uint64_t at, int len) const
+    {
+        if (at + len > nbytes) { printf("FAIL column access %llu+%d past %llu\n", (unsigned long long)at, len, (unsigned long long)nbytes); exit(1); }
+    }
+    uint32_t ld(uint32_t off, uint32_t soff) const
+    {
+        const uint64_t at = (uint64_t)off + soff;
+        chk(at, 4);
+        uint32_t v;
+        memcpy(&v, D + at, 4);
+        return v;
+    }
+    void st(uint32_t off, uint32_t soff, uint32_t v) const
+    {
+        const uint64_t at = (uint64_t)off + soff;
+        chk(at, 4);
+        memcpy(D + at, &v, 4);
+    }
+    void st8(uint32_t off, uint32_t soff, uint32_t v) const
+    {
+        const uint64_t at = (uint64_t)off + soff;
+        chk(at, 1);
+        D[at] = (uint8_t)v;
+    }
+};
 
 static uint64_t sm64(uint64_t& x)
 {
@@ -41,11 +75,12 @@ int main(int argc, char** argv)
         cw[i] = w;
     }
     DtPass pass[MAXN + 1];
-    const int np = dt_plan(q, n, N, budget, pass);
+    const bool columns = argc > 8 ? atoi(argv[8]) != 0 : true;
+    const int np = dt_plan(q, n, N, budget, pass, columns);
     if (np <= 0) { printf("FAIL plan\n"); return 1; }
 
-    int axes = 1;                       // the innermost axis, inside the inner pass's rows
-    for (int i = 0; i < np; i++) axes += pass[i].nax;
+    int axes = 0;                       // the inner pass's rows hold the innermost axis, a seed pass's groups ns axes
+    for (int i = 0; i < np; i++) axes += pass[i].nax + (pass[i].seed ? pass[i].ns : (pass[i].inner ? 1 : 0));
     if (axes != n) { printf("FAIL axes %d\n", axes); return 1; }
     // the device allocation: N + 64 bytes, 16-aligned
     uint4* Dw = new uint4[(N + 64) / 16];
@@ -61,11 +96,79 @@ int main(int argc, char** argv)
     pass[np - 1].full = light < 0;
     pass[np - 1].rcut = light < 0 ? 0 : light;
     uint64_t lt_le = 0, lt_max = 0;
+    auto flush = [&](const DtPass& p, const DtHist& h, const DtStat& st, const unsigned int* lh) -> bool {
+        if (!p.full) {
+            lt_le += st.le;
+            for (int k = 0; k < 4; k++) lt_max = std::max<uint64_t>(lt_max, (st.mx >> (8 * k)) & 0xFF);
+            return true;
+        }
+        uint32_t prev = 0;
+        for (int b = 0; b < DT_HL; b++) {
+            hist[b <= n ? b : n + 1] += h.le[b] - prev;
+            prev = h.le[b];
+        }
+        for (int b = 0; b < DT_HB; b++) hist[b] += lh[b];
+        if (h.all != prev) { printf("FAIL hist all %u le7 %u\n", h.all, prev); return false; }
+        return true;
+    };
     for (int pi = 0; pi < np; pi++) {
         const DtPass& p = pass[pi];
+        if (p.col) {
+            if (pi == 0 || p.nthr != (uint32_t)p.ntiles || p.nrow > DT_COL_MAX || p.nbytes != (uint64_t)N + 64 ||
+                (int64_t)p.ncol * 4 < p.S_row || p.S_tile != p.nrow * p.S_row) {
+                printf("FAIL column plan\n");
+                return 1;
+            }
+            const HostMem m{D, (uint64_t)p.nbytes};
+            bool ok = true;
+            const bool known = dt_col_dispatch(p.q, p.nax, [&](auto qb) {
+                using QB = decltype(qb);
+                for (uint32_t t = 0; t < p.nthr && ok; t++) {
+                    DtHist h{};
+                    DtStat st{};
+                    unsigned int lh[DT_HB] = {0};
+                    dt_col_thread<QB::q, QB::b>(p, t, m, h, st, lh);
+                    if (p.last) ok = flush(p, h, st, lh);
+                }
+            });
+            if (!known) { printf("FAIL no column kernel q=%d b=%d\n", p.q, p.nax); return 1; }
+            if (!ok) return 1;
+            continue;
+        }
         const int64_t lds = dt_lds_bytes(p);
         if (lds + 4 * DT_HB > 65536 || lds > budget) { printf("FAIL lds %lld\n", (long long)lds); return 1; }
-        if ((p.inner != 0) != (pi == 0) || (p.first != 0) != (pi == 0)) { printf("FAIL flags\n"); return 1; }
+        if ((p.inner != 0 || p.seed != 0) != (pi == 0) || (p.first != 0) != (pi == 0)) { printf("FAIL flags\n"); return 1; }
+        if (p.seed) {
+            if (p.G > 64 || p.P % 4 || p.P < p.G || p.nbytes != (uint64_t)N + 64 || p.nrow * p.P != p.lds) {
+                printf("FAIL seed plan\n");
+                return 1;
+            }
+            const HostMem m{D, (uint64_t)p.nbytes};
+            std::vector<uint32_t> smv((size_t)lds / 4);
+            uint8_t* sm = reinterpret_cast<uint8_t*>(smv.data());
+            bool ok = true;
+            const bool known = dt_seed_dispatch(p.q, [&](auto qb) {
+                using QB = decltype(qb);
+                for (int64_t t = 0; t < p.ntiles && ok; t++) {
+                    const DtTile T = dt_tile(p, t);
+                    bool any = false;
+                    for (uint32_t gi = 0; gi < (uint32_t)p.nrow; gi++) any |= dt_seed_group<QB::q>(p, T, gi, sm, cwb);
+                    if (any)
+                        for (int ax = 0; ax < p.nax; ax++)
+                            for (int tid = 0; tid < nth; tid++) dt_seed_axis<QB::q>(p, ax, tid, nth, sm);
+                    for (int tid = 0; tid < nth && ok; tid++) {
+                        DtHist h{};
+                        DtStat st{};
+                        unsigned int lh[DT_HB] = {0};
+                        dt_seed_store(p, T, tid, nth, sm, any, m, h, st, lh);
+                        if (p.last) ok = flush(p, h, st, lh);
+                    }
+                }
+            });
+            if (!known) { printf("FAIL no seed kernel q=%d\n", p.q); return 1; }
+            if (!ok) return 1;
+            continue;
+        }
         uint4* smv = new uint4[lds / 16];
         uint8_t* sm = reinterpret_cast<uint8_t*>(smv);
         for (int64_t t = 0; t < p.ntiles; t++) {
@@ -79,26 +182,12 @@ int main(int argc, char** argv)
             for (int ax = 0; ax < p.nax; ax++)
                 for (int tid = 0; tid < nth; tid++) dt_axis(p, T, ax, tid, nth, sm);
             for (int tid = 0; tid < nth; tid++) dt_store(p, T, tid, nth, sm, D);
-            if (p.last && !p.full)
+            if (p.last)
                 for (int tid = 0; tid < nth; tid++) {
                     unsigned int lh[DT_HB] = {0};
                     DtStat st{};
                     dt_hist_tile(p, T, tid, nth, sm, h[tid], st, lh);
-                    lt_le += st.le;
-                    for (int k = 0; k < 4; k++) lt_max = std::max<uint64_t>(lt_max, (st.mx >> (8 * k)) & 0xFF);
-                }
-            if (p.last && p.full)
-                for (int tid = 0; tid < nth; tid++) {
-                    unsigned int lh[DT_HB] = {0};
-                    DtStat st{};
-                    dt_hist_tile(p, T, tid, nth, sm, h[tid], st, lh);
-                    uint32_t prev = 0;
-                    for (int b = 0; b < DT_HL; b++) {
-                        hist[b <= n ? b : n + 1] += h[tid].le[b] - prev;
-                        prev = h[tid].le[b];
-                    }
-                    for (int b = 0; b < DT_HB; b++) hist[b] += lh[b];
-                    if (h[tid].all != prev) { printf("FAIL hist all %u le7 %u\n", h[tid].all, prev); return 1; }
+                    if (!flush(p, h[tid], st, lh)) return 1;
                 }
         }
         delete[] smv;

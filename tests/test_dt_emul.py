@@ -2,8 +2,8 @@
 and every per-thread phase of coldcase_amd/csrc/cov_dist.h run thread by
 thread over exact-size heap buffers under AddressSanitizer, against a
 brute-force minimum distance and its histogram, for shapes that exercise
-one-pass, two-pass and many-pass plans, ragged row chunks, odd strides and
-small LDS budgets."""
+one-pass, two-pass and many-pass plans, column passes and LDS-tile outer
+passes, ragged row chunks, odd strides and small LDS budgets."""
 import os
 import shutil
 import subprocess
@@ -24,6 +24,13 @@ SHAPES = [  # q n M seed lds
     (7, 5, 10, 3, 4096, 512), (3, 10, 25, 6, 8192, 1024), (2, 16, 3, 10, 8192, 512), (6, 9, 660, 25, 40960, 1024),
     # the light statistics (#{dist <= r} and the maximum) instead of the histogram
     (5, 7, 40, 31, 32768, 256, 2), (7, 9, 1480, 32, 40960, 512, 4), (2, 16, 3, 33, 8192, 256, 9), (3, 11, 25, 34, 4096, 256, 0),
+    # seed groups of 64 (q = 4), padded groups of 27, 9 and 11 words, dense codes,
+    # and a seed pass that is also the last pass (light statistics)
+    (4, 9, 50, 51, 32768), (3, 12, 100, 52, 16384), (9, 5, 400, 53, 32768, 512), (11, 4, 300, 54, 8192),
+    (8, 6, 2000, 55, 32768), (5, 9, 3000, 56, 49152, 256, 3), (16, 3, 10, 57, 32768, 256, 1),
+    # outer passes in LDS tiles (the plan for spaces of 2^32 words and more)
+    (7, 9, 1480, 41, 49152, 512, -1, 0), (6, 10, 2784, 42, 49152, 512, 4, 0), (2, 16, 3, 43, 8192, 256, -1, 0),
+    (3, 11, 25, 44, 4096, 1024, 2, 0), (13, 4, 9, 45, 4096, 256, -1, 0),
 ]
 
 
@@ -42,7 +49,13 @@ def emul(tmp_path_factory):
     return out
 
 
-@pytest.mark.parametrize('shape', SHAPES, ids=lambda s: 'K%d(%d)M%d_lds%d_t%d%s' % (s[0], s[1], s[2], s[4], s[5] if len(s) > 5 else 256, '_r%d' % s[6] if len(s) > 6 else ''))
+def _id(s):
+    light = '_r%d' % s[6] if len(s) > 6 and s[6] >= 0 else ''
+    lds = '_ldsouter' if len(s) > 7 and not s[7] else ''
+    return 'K%d(%d)M%d_lds%d_t%d%s%s' % (s[0], s[1], s[2], s[4], s[5] if len(s) > 5 else 256, light, lds)
+
+
+@pytest.mark.parametrize('shape', SHAPES, ids=_id)
 def test_transform_phases_match_brute_force(emul, shape):
     r = subprocess.run([emul] + [str(x) for x in shape], capture_output=True, text=True, timeout=300)
     assert r.returncode == 0 and r.stdout.startswith('ok'), r.stdout + r.stderr[-3000:]
