@@ -169,52 +169,69 @@ def c4_specs(seed=2026, count=64):
 
 def bench_batched(a):
     import torch
-    from coldcase_amd import Batch, Cell
-    from coldcase_amd.autotune import tune_batch
+    from coldcase_amd import Batch, Cell, Fleet
+    from coldcase_amd.autotune import tune_fleet
     from coldcase_amd.engine import MODE_GRAPH
     torch.cuda.set_device(0)
     specs, redrawn = c4_specs()
     moves, prefix = a.steps, a.prefix
+    seeds = [1 * 1000003 + i * 7919 for i in range(len(specs))]
 
     def make(i, s):
         q, n, R, M = s
         c = Cell(q, n, R, M)
         c.search_params(tenure=10, stall=5000, kick=2)
-        c.search_init(1 * 1000003 + i * 7919)
+        c.search_init(seeds[i])
         return c
 
     # plain: one cell at a time, its own launches (hipGraph steps), default shape
     plain_s = 0.0
+    costs = []
     for i, s in enumerate(specs):
         with make(i, s) as c:
             c.run(32, mode=MODE_GRAPH)
             c.sync()
             t0 = time.perf_counter()
             c.run(prefix, mode=MODE_GRAPH)
-            plain_s += time.perf_counter() - t0
+            dt = time.perf_counter() - t0
+            plain_s += dt
+            costs.append(dt)
         print('plain %d/%d' % (i + 1, len(specs)), file=sys.stderr, flush=True)
-    # batched: all cells in one launch pair per step, batch scale autotuned
+    # batched: the cells grouped by their plain step time into concurrent
+    # batches (one evaluator and one commit launch per step and group), each
+    # group's batch scale autotuned
     cells = [make(i, s) for i, s in enumerate(specs)]
-    tuned = tune_batch(cells)
-    for c in cells:                      # restart every cell from its seed after tuning
-        c.search_init(cells.index(c) * 7919 + 1000003)
+    groups, factors, gidx = tune_fleet(cells, costs, ngroups=a.groups)
+
+    def restart():
+        for i, c in enumerate(cells):
+            c.search_init(seeds[i])
+
+    # for reference: the same launch shapes as one batch of all cells
+    restart()
     with Batch(cells) as b:
         b.run(32)
         t0 = time.perf_counter()
         b.run(prefix)
+        one_batch_s = time.perf_counter() - t0
+    restart()
+    with Fleet(groups) as f:
+        f.run(32)
+        t0 = time.perf_counter()
+        f.run(prefix)
         batched_prefix_s = time.perf_counter() - t0
         print('batched prefix done', file=sys.stderr, flush=True)
         done = 32 + prefix
         t0 = time.perf_counter()
         while done < moves:
             k = min(5000, moves - done)
-            b.run(k)
+            f.run(k)
             done += k
             print('batched %d/%d' % (done, moves), file=sys.stderr, flush=True)
         rest_s = time.perf_counter() - t0
     total_batched = batched_prefix_s + rest_s
     st = [c.state() for c in cells]
-    # the timed region holds moves - 32 steps of every cell (32 warmed up the graph)
+    # the timed region holds moves - 32 steps of every cell (32 warmed up the graphs)
     solves_batched = len(cells) / (total_batched * moves / (moves - 32)) if moves > 32 else 0.0
     plain_rate = len(cells) / (plain_s / prefix * moves)             # projected from the prefix
     batched_prefix_rate = len(cells) / (batched_prefix_s / prefix * moves)
@@ -225,12 +242,15 @@ def bench_batched(a):
         'config': {'workload': '64 instances (q,n,R) from {5,6,7,8}x{7,8,9}x{2,3,4}, numpy seed 2026, '
                                'M = ceil(1.3 q^n/|ball|), seeds 1000003 + 7919 i, tenure 10',
                    'redrawn': redrawn, 'prefix_steps': prefix,
-                   'batch_scale': tuned['factor']},
+                   'groups': [len(g) for g in gidx], 'group_batch_scale': factors},
         'batched_full_solves_per_s': round(solves_batched, 4),
+        'plain_solves_per_s_projected': round(plain_rate, 5),
+        'ratio_full_vs_plain': round(solves_batched / plain_rate, 3),
         'prefix': {'plain_s': round(plain_s, 3), 'batched_s': round(batched_prefix_s, 3),
-                   'plain_solves_per_s_projected': round(plain_rate, 5),
+                   'one_batch_s': round(one_batch_s, 3),
                    'batched_solves_per_s_projected': round(batched_prefix_rate, 5),
-                   'ratio': round(plain_s / batched_prefix_s, 3)},
+                   'ratio': round(plain_s / batched_prefix_s, 3),
+                   'ratio_one_batch': round(plain_s / one_batch_s, 3)},
         'covered': sum(1 for x in st if x['done']),
     }
     print(json.dumps(out))
@@ -290,6 +310,7 @@ def main():
     ap.add_argument('--lds', type=int, default=0, help='transform tile LDS bytes (0 = default)')
     ap.add_argument('--threads', type=int, default=0, help='transform workgroup threads (0 = default)')
     ap.add_argument('--prefix', type=int, default=1000, help='batched: steps timed in both modes')
+    ap.add_argument('--groups', type=int, default=3, help='batched: concurrent batches, cells grouped by step cost')
     ap.add_argument('--requests', type=int, default=60, help='serve: requests in the trace')
     ap.add_argument('--tick', type=int, default=256, help='serve: steps per batched tick at most')
     ap.add_argument('--budget-gb', type=float, default=64.0, help='serve: HBM budget')

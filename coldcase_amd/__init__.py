@@ -10,6 +10,6 @@ a C ABI (include/covhip.h); this package is the host side:
   batch       many independent cells in one persistent launch + autotuner
   serve       continuous-batching request loop with HBM-budget admission
 """
-from .engine import Batch, Cell, CovError, lib, lib_path  # noqa: F401
+from .engine import Batch, Cell, CovError, Fleet, lib, lib_path  # noqa: F401
 
-__all__ = ["Batch", "Cell", "CovError", "lib", "lib_path"]
+__all__ = ["Batch", "Cell", "CovError", "Fleet", "lib", "lib_path"]

@@ -10,6 +10,9 @@ What is tuned:
   batch scale    for a batch, one factor applied to every member's tuned
                  counts (a batch shares the GPU, so each cell wants fewer
                  workgroups than alone), timed on the batch itself
+  fleet groups   cells split by step cost into a few batches that run
+                 concurrently on their own streams (engine.Fleet), each
+                 with its own batch scale
   tile           the distance transform's LDS bytes per tile
                  (cov_set_transform_tile), timed on a random code of the shape
 
@@ -115,6 +118,35 @@ def tune_batch(cells, factors=(1.0, 0.5, 0.25, 0.125, 0.0625), steps=64):
     best = min(times, key=times.get)
     setf(best)
     return {'factor': best, 'us_per_step': times[best] / 2 * 1e6}
+
+
+def plan_groups(costs, ngroups=3):
+    """Split cells into at most `ngroups` groups of about equal total cost,
+    the most expensive cells together: indices sorted by decreasing cost,
+    cut where the running sum passes each multiple of total / ngroups.
+    `costs` are per-cell step times (or any proxy); returns lists of indices."""
+    order = sorted(range(len(costs)), key=lambda i: (-costs[i], i))
+    total = float(sum(costs)) or 1.0
+    groups, cur, acc = [], [], 0.0
+    for i in order:
+        cur.append(i)
+        acc += costs[i]
+        if len(groups) < ngroups - 1 and acc >= total * (len(groups) + 1) / ngroups:
+            groups.append(cur)
+            cur = []
+    if cur:
+        groups.append(cur)
+    return groups
+
+
+def tune_fleet(cells, costs, ngroups=3, factors=(1.0, 0.5, 0.25, 0.125), steps=64):
+    """Group the cells by cost (plan_groups) and give each group the batch
+    scale that runs it fastest alone (tune_batch); returns the groups (lists
+    of cells) and their factors."""
+    idx = plan_groups(costs, ngroups)
+    groups = [[cells[i] for i in g] for g in idx]
+    tuned = [tune_batch(g, factors=factors, steps=steps) for g in groups]
+    return groups, [t['factor'] for t in tuned], idx
 
 
 class Table:

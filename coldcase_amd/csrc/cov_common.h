@@ -36,6 +36,7 @@ constexpr int ACCW = 1 + 2 * MAXN;   // per-slot accumulators: t1, hit[16], neg[
 constexpr int LIST_CHUNK = 512;      // uncovered-list entries per evaluator unit
 constexpr int64_t DSEL_RANDOM = 1LL << 60;
 constexpr int TRW = 16;
+constexpr int DIRTY_CAP = 4096;      // logged codeword changes before a full best copy
 constexpr uint64_t NIB1 = 0x1111111111111111ULL;
 
 // Device-resident search state.  Scalars written by the narrow phases (one
@@ -62,6 +63,7 @@ struct alignas(16) Ctl {
     int32_t dtar, ncand, take, nq, use_list;
     int32_t bi, bp, bv, oldv, nties, stamp, ncw;
     int32_t evdone, cmdone;
+    int32_t ndirty;         // codewords logged in dirty since best was last made equal to code
     // wall_clock64 ticks spent per phase (diagnostics): 0 select, 1 post,
     // 2 begin, 3 pick, 4 list upkeep, 5 shuffle helper, 6 rng helper, 7 kick,
     // 8 evaluator span before select, 9 commit span before post, 10 gap commit
@@ -138,6 +140,7 @@ struct Cell {
     GPTR(uint64_t) rbuf;
     GPTR(int64_t) scratch64;     // small reduction outputs
     GPTR(int32_t) arrive;        // workgroup arrival sub-counters: evaluator, commit
+    GPTR(int32_t) dirty;         // codewords changed since best == code (DIRTY_CAP of them)
     GPTR(uint8_t) dist;          // distance transform output (N bytes), lazily
     GPTR(Ctl) ctl;
 };

@@ -44,12 +44,16 @@ namespace cov {
 
 constexpr int SHUF_PAR = 1024;     // draws handled by the parallel shuffle
 
+constexpr int MAX_M_WORDS = (65535 + 63) / 64;   // 64-codeword mask words of the largest code
+
 struct NarrowSh {
     WgRed red;
     WgScan scan;
     int64_t buf[WG];
     int64_t v64[8];
     int32_t v32[8];
+    uint64_t qm[MAX_M_WORDS];   // begin: codewords at the target distance
+    int32_t qfirst[MAX_M_WORDS]; // begin: first slot of each mask word's codewords
 };
 
 __device__ __forceinline__ uint32_t zbit_acq(const Cell& C, int64_t w)
@@ -187,29 +191,49 @@ __device__ __forceinline__ void wg_list_rebuild(const Cell& C, Ctl* ctl, NarrowS
     __syncthreads();
 }
 
-// drop entries that are covered again (their membership bit is cleared)
+// drop entries that are covered again (their membership bit is cleared).
+// In place and in order, LIST_QK consecutive entries per thread per round:
+// every round reads its entries before the scan's barriers and writes the
+// kept ones below its own first entry, so no round overwrites an entry that
+// is still to be read.
+constexpr int LIST_QK = 8;
 __device__ __forceinline__ void wg_list_compact(const Cell& C, Ctl* ctl, NarrowSh& sh)
 {
     const int64_t n0 = ctl->ul_n;
     int64_t out = 0;
-    for (int64_t e0 = 0; e0 < n0; e0 += WG) {
-        const int64_t e = e0 + threadIdx.x;
-        int64_t idx = 0;
-        uint64_t pk = 0;
-        int keep = 0;
-        if (e < n0) {
-            idx = C.ul_idx[e];
-            pk = C.ul_pk[e];
-            keep = zbit_acq(C, idx) != 0;
-            if (!keep) bit_clr(C.inl, idx);
+    for (int64_t e0 = 0; e0 < n0; e0 += (int64_t)WG * LIST_QK) {
+        const int64_t eb = e0 + (int64_t)threadIdx.x * LIST_QK;
+        int64_t idx[LIST_QK];
+        uint64_t pk[LIST_QK];
+        uint32_t keepm = 0;
+        int kept = 0;
+#pragma unroll
+        for (int k = 0; k < LIST_QK; k++) {
+            idx[k] = 0;
+            pk[k] = 0;
+            if (eb + k < n0) {
+                idx[k] = C.ul_idx[eb + k];
+                pk[k] = C.ul_pk[eb + k];
+                if (zbit_acq(C, idx[k])) {
+                    keepm |= 1u << k;
+                    kept++;
+                } else {
+                    bit_clr(C.inl, idx[k]);
+                }
+            }
         }
         int64_t tot;
-        const int64_t incl = wg_incl_sum(keep, sh.scan, &tot);
-        __syncthreads();
-        if (keep) { C.ul_idx[out + incl - 1] = idx; C.ul_pk[out + incl - 1] = pk; }
+        int64_t at = out + wg_incl_sum(kept, sh.scan, &tot) - kept;
+#pragma unroll
+        for (int k = 0; k < LIST_QK; k++)
+            if ((keepm >> k) & 1u) {
+                C.ul_idx[at] = idx[k];
+                C.ul_pk[at] = pk[k];
+                at++;
+            }
         out += tot;
-        __syncthreads();
     }
+    __syncthreads();
     if (threadIdx.x == 0) ctl->ul_n = out;
     __syncthreads();
 }
@@ -270,55 +294,72 @@ __device__ __forceinline__ void wg_begin(const Cell& C, Ctl* ctl, NarrowSh& sh)
     }
     const uint64_t upk = (uint64_t)wg_sum((int64_t)part, sh.red);
 
-    // distances and the target distance (:1342-1351), then the codewords at
-    // distance dtar in index order -> slots; each thread keeps its words'
-    // packed digits in registers across both passes
-    constexpr int KEEP = 4;
-    uint64_t cpk_r[KEEP];
+    // One pass over the codewords, SCAN_ILP loads in flight per thread: the
+    // minimum distance to u, and the codewords at distance R+1 as one bit
+    // each (a ballot per wave, in LDS).  R+1 is the target distance unless no
+    // codeword is that close; then a second pass marks those at the minimum
+    // (:1342-1351).
+    constexpr int SCAN_ILP = 8;
+    const int nwm = (C.M + 63) >> 6;
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     int64_t dmin = INT64_MAX;
+    auto mark_pass = [&](int target, bool with_min) {
+        for (int i0 = 0; i0 < C.M; i0 += SCAN_ILP * WG) {
+            uint64_t c[SCAN_ILP];
 #pragma unroll
-    for (int k = 0; k < KEEP; k++) {
-        const int i = k * WG + threadIdx.x;
-        cpk_r[k] = i < C.M ? C.cpk[i] : 0;
-    }
+            for (int k = 0; k < SCAN_ILP; k++) {
+                const int i = i0 + k * WG + threadIdx.x;
+                c[k] = i < C.M ? C.cpk[i] : 0;
+            }
 #pragma unroll
-    for (int k = 0; k < KEEP; k++) {
-        const int i = k * WG + threadIdx.x;
-        if (i < C.M) { const int64_t d = nib_dist(cpk_r[k], upk); dmin = d < dmin ? d : dmin; }
-    }
-    for (int i = KEEP * WG + threadIdx.x; i < C.M; i += WG) {
-        const int64_t d = nib_dist(C.cpk[i], upk);
-        dmin = d < dmin ? d : dmin;
-    }
+            for (int k = 0; k < SCAN_ILP; k++) {
+                const int i = i0 + k * WG + threadIdx.x;
+                const int64_t d = nib_dist(c[k], upk);
+                if (with_min && i < C.M && d < dmin) dmin = d;
+                const uint64_t m = __ballot(i < C.M && d == target);
+                const int wi = ((i0 + k * WG) >> 6) + wv;
+                if (lane == 0 && wi < nwm) sh.qm[wi] = m;
+            }
+        }
+    };
+    mark_pass(C.R + 1, true);
     dmin = wg_min(dmin, sh.red);
     const int dtar = (dmin <= C.R + 1) ? C.R + 1 : (int)dmin;
-
-    int64_t nq = 0;
-    auto qualify = [&](int i0, uint64_t mine) {
-        const int i = i0 + threadIdx.x;
-        uint64_t ne = 0;
-        int ok = 0;
-        if (i < C.M) {
-            ne = nib_ne(mine, upk);
-            ok = __popcll(ne) == dtar;
+    if (dtar != C.R + 1) {
+        mark_pass(dtar, false);
+        __syncthreads();
+    }
+    // a scan of the mask words' counts gives each word its first slot; each
+    // wave then takes whole words, lane b the word's bit b: slots in index order
+    constexpr int QPT = (MAX_M_WORDS + WG - 1) / WG;   // mask words per thread
+    int cnt = 0;
+#pragma unroll
+    for (int k = 0; k < QPT; k++) {
+        const int j = QPT * threadIdx.x + k;
+        if (j < nwm) cnt += __popcll(sh.qm[j]);
+    }
+    int64_t nq;
+    int run = (int)(wg_incl_sum(cnt, sh.scan, &nq) - cnt);
+#pragma unroll
+    for (int k = 0; k < QPT; k++) {
+        const int j = QPT * threadIdx.x + k;
+        if (j < nwm) {
+            sh.qfirst[j] = run;
+            run += __popcll(sh.qm[j]);
         }
-        int64_t tot;
-        const int64_t incl = wg_incl_sum(ok, sh.scan, &tot);
-        if (ok) {
-            const int64_t slot = nq + incl - 1;
+    }
+    __syncthreads();
+    for (int j = wv; j < nwm; j += WAVES) {
+        const uint64_t m = sh.qm[j];
+        if ((m >> lane) & 1) {
+            const int i = 64 * j + lane;
+            const int64_t slot = sh.qfirst[j] + __popcll(m & ((1ull << lane) - 1));
+            const uint64_t ne = nib_ne(C.cpk[i], upk);
             C.slot_cw[slot] = i;
             int t = 0;
             for (int p = 0; p < C.n; p++)
                 if ((ne >> (4 * p)) & 1) C.slot_pos[slot * MAXN + t++] = (uint8_t)p;
         }
-        nq += tot;
-    };
-#pragma unroll
-    for (int k = 0; k < KEEP; k++)
-        if (k * WG < C.M) qualify(k * WG, cpk_r[k]);
-    for (int i0 = KEEP * WG; i0 < C.M; i0 += WG) {
-        const int i = i0 + threadIdx.x;
-        qualify(i0, i < C.M ? C.cpk[i] : 0);
     }
     const int64_t ncand = nq * dtar;
     const int64_t take = ncand < ctl->cand ? ncand : ctl->cand;
@@ -804,6 +845,14 @@ __device__ __forceinline__ void wg_ball(const Cell& C, Ctl* ctl, int i, int delt
     __syncthreads();
 }
 
+// codeword i changed: log it for the next best copy (thread 0)
+__device__ __forceinline__ void log_dirty(const Cell& C, Ctl* ctl, int i)
+{
+    const int32_t k = ctl->ndirty;
+    if (k < DIRTY_CAP) C.dirty[k] = i;
+    ctl->ndirty = k + 1;
+}
+
 __device__ __forceinline__ void wg_kick(const Cell& C, Ctl* ctl, NarrowSh& sh)
 {
     const int n = C.n, q = C.q;
@@ -844,6 +893,7 @@ __device__ __forceinline__ void wg_kick(const Cell& C, Ctl* ctl, NarrowSh& sh)
             }
             C.cpk[i] = pk;
             C.cidx[i] = idx;
+            log_dirty(C, ctl, i);
         }
         __syncthreads();
         wg_ball(C, ctl, i, +1, sh);
@@ -871,6 +921,7 @@ __device__ __forceinline__ void wg_post(const Cell& C, Ctl* ctl, NarrowSh& sh)
         C.cpk[bi] = nib_put(C.cpk[bi], bp, bv);
         C.cidx[bi] += ctl->dv;
         C.tabu[((int64_t)bi * n + bp) * q + oldv] = ctl->stamp;
+        log_dirty(C, ctl, bi);
         if (ctl->ul_ovf) { ctl->ul_valid = 0; ctl->ul_ovf = 0; ctl->ul_n = 0; }
         ctl->commit_bytes += 8 * (int64_t)C.E * C.P;
         sh.v64[3] = dunc;
@@ -889,11 +940,24 @@ __device__ __forceinline__ void wg_post(const Cell& C, Ctl* ctl, NarrowSh& sh)
     const int64_t dunc = sh.v64[3];
     __syncthreads();
     if (what == 1) {
-        const int64_t nb = (int64_t)C.M * n, n16 = nb / 16;
-        const uint4* src = reinterpret_cast<const uint4*>(C.code);
-        uint4* dst = reinterpret_cast<uint4*>(C.best);
-        for (int64_t x = threadIdx.x; x < n16; x += WG) dst[x] = src[x];
-        for (int64_t x = n16 * 16 + threadIdx.x; x < nb; x += WG) C.best[x] = C.code[x];
+        // best := code: the codewords changed since the last copy, or all of
+        // them when more changed than the log holds
+        const int32_t nd = ctl->ndirty;
+        if (nd <= DIRTY_CAP) {
+            for (int32_t x = threadIdx.x; x < nd * n; x += WG) {
+                const int32_t e = x / n;
+                const int64_t at = (int64_t)C.dirty[e] * n + (x - e * n);
+                C.best[at] = C.code[at];
+            }
+        } else {
+            const int64_t nb = (int64_t)C.M * n, n16 = nb / 16;
+            const uint4* src = reinterpret_cast<const uint4*>(C.code);
+            uint4* dst = reinterpret_cast<uint4*>(C.best);
+            for (int64_t x = threadIdx.x; x < n16; x += WG) dst[x] = src[x];
+            for (int64_t x = n16 * 16 + threadIdx.x; x < nb; x += WG) C.best[x] = C.code[x];
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) ctl->ndirty = 0;
     } else if (what == 2) {
         const int64_t tk0 = clk();
         wg_kick(C, ctl, sh);
@@ -995,6 +1059,7 @@ __device__ __forceinline__ void commit_body(const Cell& C, int bid, int nblk, in
             C.code[(int64_t)L->bi * C.n + L->bp] = (uint8_t)L->bv;
             C.cpk[L->bi] = nib_put(C.cpk[L->bi], L->bp, L->bv);
             C.cidx[L->bi] += L->dv;
+            log_dirty(C, L, L->bi);
             if (L->ul_ovf) { L->ul_valid = 0; L->ul_ovf = 0; L->ul_n = 0; }
             L->begun = 0;
             L->selected = 0;

@@ -139,7 +139,7 @@ int shape_of(int q, int n, int R, int M, Shape* s)
 // device bytes of each array, in allocation order
 struct Sizes {
     int64_t cnt, plane, blk, code, cpk, cidx, tabu, fsub, psub, bsub, ssub, ul_idx, ul_pk,
-        shuf, acc, slot_cw, slot_pos, rbuf, ctl, cell, small, arrive;
+        shuf, acc, slot_cw, slot_pos, rbuf, ctl, cell, small, arrive, dirty;
 };
 
 Sizes sizes_of(const Shape& s)
@@ -167,6 +167,7 @@ Sizes sizes_of(const Shape& s)
     z.cell = sizeof(Cell);
     z.small = 64 * 8;
     z.arrive = 2 * 32 * 4;
+    z.dirty = DIRTY_CAP * 4;
     return z;
 }
 
@@ -176,7 +177,7 @@ int64_t total_bytes(const Shape& s)
     // plus the distance array and codeword bitmap the transform allocates on first use
     return z.cnt + 3 * z.plane + z.blk + 2 * z.code + z.cpk + z.cidx + z.tabu + z.fsub + z.psub +
            z.bsub + z.ssub + z.ul_idx + z.ul_pk + z.shuf + z.acc + z.slot_cw + z.slot_pos + z.rbuf +
-           z.ctl + z.cell + z.small + z.arrive + (s.N + 64) + ((s.N >> 5) + 2) * 4;
+           z.ctl + z.cell + z.small + z.arrive + z.dirty + (s.N + 64) + ((s.N >> 5) + 2) * 4;
 }
 
 template <typename T>
@@ -498,6 +499,7 @@ int cov_create(int q, int n, int R, int M, cov_cell_t** out)
     AL(h->d, z.cell);
     AL(h->d_small, z.small);
     AL(C.arrive, z.arrive);
+    AL(C.dirty, z.dirty);
 #undef AL
     C.scratch64 = (decltype(C.scratch64))h->d_small;
 
@@ -618,6 +620,7 @@ int cov_set_code_flags(cov_cell_t* h, const uint8_t* digits, int flags)
     Ctl c;
     RC(read_ctl(h, &c));
     RC(set_ctl<int64_t>(h, offsetof(Ctl, best_unc), c.unc));
+    RC(set_ctl<int32_t>(h, offsetof(Ctl, ndirty), 0));
     return COV_OK;
 }
 
@@ -994,6 +997,7 @@ int cov_search_init(cov_cell_t* h, uint64_t seed, const uint8_t* seed_code, int 
     Ctl now;
     RC(read_ctl(h, &now));
     RC(set_ctl<int64_t>(h, offsetof(Ctl, best_unc), now.unc));
+    RC(set_ctl<int32_t>(h, offsetof(Ctl, ndirty), 0));
     if (now.unc <= C.ulcap / 2) RC(list_rebuild(h));
     else RC(set_ctl<int32_t>(h, offsetof(Ctl, ul_valid), 0));
     return COV_OK;

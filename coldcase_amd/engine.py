@@ -11,6 +11,7 @@ entry point raises.
 """
 import ctypes
 import os
+import time
 
 import numpy as np
 
@@ -352,6 +353,62 @@ class Batch:
         if getattr(self, 'h', None):
             lib().cov_batch_destroy(self.h)
             self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+
+class Fleet:
+    """Groups of cells, each group a Batch on its own stream, advanced
+    concurrently in chunks of steps: a group of cheap cells runs at its own
+    pace instead of the pace of the most expensive cell of one big batch,
+    and its short dependent phases overlap the long gathers of the others.
+    Every cell keeps the trajectory it has alone."""
+
+    def __init__(self, groups):
+        self.batches = []
+        try:
+            for g in groups:
+                if g:
+                    self.batches.append(Batch(g))
+        except Exception:
+            self.close()
+            raise
+
+    def run(self, nsteps, chunk=2048, mode=MODE_GRAPH, poll=0.0005):
+        """Advance every group by nsteps; returns when all are done."""
+        left = [int(nsteps)] * len(self.batches)
+        busy = [False] * len(self.batches)
+        while True:
+            idle = True
+            for i, b in enumerate(self.batches):
+                if busy[i] and b.query():
+                    b.sync()
+                    busy[i] = False
+                if not busy[i] and left[i] > 0:
+                    k = min(int(chunk), left[i])
+                    b.launch(k, mode=mode)
+                    left[i] -= k
+                    busy[i] = True
+                    idle = False
+            if not any(busy) and not any(left):
+                return
+            if idle:
+                time.sleep(poll)
+
+    def close(self):
+        for b in self.batches:
+            b.close()
+        self.batches = []
 
     def __del__(self):
         try:

@@ -100,3 +100,43 @@ def test_batch_rejects_bad_input():
     with pytest.raises(CovError):
         Batch([])
     c.close()
+
+
+def test_fleet_equals_solo():
+    """Cells split into concurrent batches by cost, run in chunks: every cell
+    as alone."""
+    from coldcase_amd import Fleet
+    from coldcase_amd.autotune import plan_groups
+    import math
+    print('case fleet mix', flush=True)
+    cells = [_cell(s) for s in MIX]
+    costs = [math.comb(s[1], s[2]) * (s[0] - 1) ** s[2] * s[3] for s in MIX]
+    groups = plan_groups(costs, 3)
+    assert sorted(i for g in groups for i in g) == list(range(len(MIX)))
+    with Fleet([[cells[i] for i in g] for g in groups]) as f:
+        f.run(150, chunk=64)
+        f.run(237, chunk=100)
+    for spec, c in zip(MIX, cells):
+        assert _snap(c) == _solo(spec, 387), spec
+        c.close()
+
+
+@pytest.mark.parametrize('spec', [(6, 6, 3, 41, 2, 3, 20), (5, 7, 2, 400, 3, 10, 5000)])
+def test_best_code_after_long_stalls(spec):
+    """Runs whose best improves again after more codeword changes than the
+    device keeps a log of (about 4700 and 6300 here): the best code must still
+    be the oracle's."""
+    q, n, R, M, seed, tenure, stall = spec
+    print('case best after stalls %s' % (spec,), flush=True)
+    c = _cell(spec)
+    c.run(12000, mode=1)
+    o = Oracle(q, n, R, M, seed).init()
+    o.run(12000, tabu=tenure, stall=stall)
+    st = c.state()
+    best, bu = c.best()
+    assert (st['it'], st['uncovered'], st['best_uncovered'], st['kicks']) == \
+        (o.iters, o.uncovered, o.best_uncovered, o.kicks)
+    assert bu == o.best_uncovered
+    assert (best == o.best()).all()
+    assert (c.code() == o.code()).all()
+    c.close()
