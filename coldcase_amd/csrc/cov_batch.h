@@ -41,7 +41,7 @@ __global__ void __launch_bounds__(WG) k_commit_b(const Cell* __restrict__ cells,
 // one workgroup per cell
 __global__ void __launch_bounds__(WG) k_begin_run_b(const Cell* __restrict__ cells, int64_t nsteps)
 {
-    begin_run_body(cells[blockIdx.x], nsteps);
+    begin_run_body(cells[blockIdx.x], nsteps, true);
 }
 
 }  // namespace cov

@@ -116,6 +116,11 @@ struct Cell {
     int64_t N, nzw, nblk, V, ncap, ulcap;
     int64_t pw[MAXN];
     uint32_t magic;         // ceil(2^32 / (q-1)): t / (q-1) = umulhi(t, magic)
+    // sphere walks by tables: pattern t of an R-subset = hi[t / wnl] + lo[t % wnl],
+    // lo over the wa innermost positions of the subset (wnl = (q-1)^wa words),
+    // hi over the other R - wa (wnh words); wok = 0 when they do not fit
+    int32_t wa, wnl, wnh, wok;
+    uint32_t wmag;          // t / wnl = umulhi(t, wmag) (wnl > 1)
     GPTR(uint32_t) cnt32;        // cnt as uint32 pairs
     GPTR(uint32_t) zb;
     GPTR(uint32_t) ob;
@@ -241,6 +246,52 @@ __device__ __forceinline__ int64_t pattern_word(const Cell& C, int64_t base, uin
     }
     if (wpk) *wpk = p2;
     return a;
+}
+
+// Offsets of the patterns of the R-subset `sub` around the word with digits
+// pk, as two tables of one wave: lo over the wa innermost positions of the
+// subset, hi over the rest, so that pattern t is the word
+// base + hi[t / wnl] + lo[t % wnl] (the same word as pattern_word).  The wave
+// builds them in its LDS slice; all 64 lanes.
+constexpr int WT_MAX = 128;
+struct WalkTab {
+    int64_t lo[WT_MAX];
+    int64_t hi[WT_MAX];
+};
+__device__ __forceinline__ int64_t walk_part(const Cell& C, uint64_t pk, uint32_t sub, int i1, int i0, uint32_t x)
+{
+    int64_t s = 0;
+    for (int i = i1; i >= i0; i--) {
+        const uint32_t xq = div_q1(C, x);
+        const int k = (int)(x - xq * (uint32_t)C.q1);
+        x = xq;
+        const int j = (sub >> (4 * i)) & 15;
+        const int c = nib_get(pk, j);
+        s += (int64_t)(k + (k >= c) - c) * C.pw[j];
+    }
+    return s;
+}
+__device__ __forceinline__ void walk_tables(const Cell& C, uint64_t pk, uint32_t sub, WalkTab& tb)
+{
+    const int lane = threadIdx.x & 63;
+    const int r = C.R, a = C.wa;
+    for (int l = lane; l < C.wnl; l += 64) tb.lo[l] = walk_part(C, pk, sub, r - 1, r - a, (uint32_t)l);
+    for (int h = lane; h < C.wnh; h += 64) tb.hi[h] = walk_part(C, pk, sub, r - a - 1, 0, (uint32_t)h);
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+__device__ __forceinline__ int64_t walk_word(const Cell& C, const WalkTab& tb, int64_t base, uint32_t t)
+{
+    const uint32_t h = C.wnl == 1 ? t : __umulhi(t, C.wmag);
+    return base + tb.hi[h] + tb.lo[t - h * (uint32_t)C.wnl];
+}
+// the wave is done with its tables (before the next unit rebuilds them)
+__device__ __forceinline__ void walk_release()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
 // ----------------------------------------------------------------- rng

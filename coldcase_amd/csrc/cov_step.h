@@ -399,13 +399,15 @@ __device__ __forceinline__ void wg_begin(const Cell& C, Ctl* ctl, NarrowSh& sh)
 // ------------------------------------------------------- evaluator units
 // leaving side: words of the distance-R sphere of codeword slot on subset f;
 // four independent gathers in flight per lane
-__device__ __forceinline__ void unit_leave(const Cell& C, int slot, int f)
+__device__ __forceinline__ void unit_leave(const Cell& C, int slot, int f, WalkTab& tb)
 {
     const int lane = threadIdx.x & 63;
     const int i = C.slot_cw[slot];
     const int64_t base = C.cidx[i];
     const uint64_t pk = C.cpk[i];
     const uint32_t sub = C.fsub[f];
+    const bool tab = C.wok != 0;
+    if (tab) walk_tables(C, pk, sub, tb);
     int64_t c1 = 0;
     for (int64_t t0 = 0; t0 < C.P; t0 += 256) {
         uint32_t b[4];
@@ -413,11 +415,14 @@ __device__ __forceinline__ void unit_leave(const Cell& C, int slot, int f)
         for (int k = 0; k < 4; k++) {
             const int64_t t = t0 + k * 64 + lane;
             b[k] = 0;
-            if (t < C.P) b[k] = bit_of(C.ob, pattern_word(C, base, pk, sub, C.R, (uint32_t)t, nullptr));
+            if (t < C.P)
+                b[k] = bit_of(C.ob, tab ? walk_word(C, tb, base, (uint32_t)t)
+                                        : pattern_word(C, base, pk, sub, C.R, (uint32_t)t, nullptr));
         }
 #pragma unroll
         for (int k = 0; k < 4; k++) c1 += __popcll(__ballot(b[k]));
     }
+    if (tab) walk_release();
     if (lane == 0 && c1) {
         int32_t* a = C.acc + (int64_t)slot * ACCW;
         atomicAdd(&a[0], (int)c1);
@@ -427,7 +432,7 @@ __device__ __forceinline__ void unit_leave(const Cell& C, int slot, int f)
 
 // entering side by walks: words of the sphere of c(p:=u_p) on subset e
 // avoiding p, with cnt == 0
-__device__ __forceinline__ void unit_walk(const Cell& C, int slot, int p, int up, int e)
+__device__ __forceinline__ void unit_walk(const Cell& C, int slot, int p, int up, int e, WalkTab& tb)
 {
     const int lane = threadIdx.x & 63;
     const int i = C.slot_cw[slot];
@@ -436,6 +441,8 @@ __device__ __forceinline__ void unit_walk(const Cell& C, int slot, int p, int up
     const int64_t base = C.cidx[i] + (int64_t)(up - cp) * C.pw[p];
     const uint64_t pk = nib_put(pk0, p, up);
     const uint32_t sub = C.psub[(int64_t)p * C.E + e];
+    const bool tab = C.wok != 0;
+    if (tab) walk_tables(C, pk, sub, tb);
     int64_t c0 = 0;
     for (int64_t t0 = 0; t0 < C.P; t0 += 256) {
         uint32_t b[4];
@@ -443,11 +450,14 @@ __device__ __forceinline__ void unit_walk(const Cell& C, int slot, int p, int up
         for (int k = 0; k < 4; k++) {
             const int64_t t = t0 + k * 64 + lane;
             b[k] = 0;
-            if (t < C.P) b[k] = bit_of(C.zb, pattern_word(C, base, pk, sub, C.R, (uint32_t)t, nullptr));
+            if (t < C.P)
+                b[k] = bit_of(C.zb, tab ? walk_word(C, tb, base, (uint32_t)t)
+                                        : pattern_word(C, base, pk, sub, C.R, (uint32_t)t, nullptr));
         }
 #pragma unroll
         for (int k = 0; k < 4; k++) c0 += __popcll(__ballot(b[k]));
     }
+    if (tab) walk_release();
     if (lane == 0 && c0) atomicAdd(&C.acc[(int64_t)slot * ACCW + 1 + MAXN + p], (int)c0);
 }
 
@@ -486,7 +496,7 @@ __device__ __forceinline__ void unit_list(const Cell& C, int slot, int64_t e0, i
 }
 
 // run this wave's share of the step's evaluator units
-__device__ __forceinline__ void eval_units(const Cell& C, const Ctl* ctl, int64_t wid, int64_t nw)
+__device__ __forceinline__ void eval_units(const Cell& C, const Ctl* ctl, int64_t wid, int64_t nw, WalkTab& tb)
 {
     const int64_t nl = ctl->n_leave, ne = ctl->n_enter;
     const int use_list = ctl->use_list, dtar = ctl->dtar;
@@ -496,7 +506,7 @@ __device__ __forceinline__ void eval_units(const Cell& C, const Ctl* ctl, int64_
     for (int64_t g = wid; g < nl + ne; g += nw) {
         if (g < nl) {
             const int slot = (int)(g / C.F);
-            unit_leave(C, slot, (int)(g - (int64_t)slot * C.F));
+            unit_leave(C, slot, (int)(g - (int64_t)slot * C.F), tb);
         } else if (use_list) {
             const int64_t h = g - nl;
             const int slot = (int)(h / nchunk);
@@ -511,7 +521,7 @@ __device__ __forceinline__ void eval_units(const Cell& C, const Ctl* ctl, int64_
             const int t = (int)(r / C.E);
             const int e = (int)(r - (int64_t)t * C.E);
             const int p = C.slot_pos[(int64_t)slot * MAXN + t];
-            unit_walk(C, slot, p, nib_get(upk, p), e);
+            unit_walk(C, slot, p, nib_get(upk, p), e, tb);
         }
     }
 }
@@ -784,6 +794,7 @@ __device__ __forceinline__ void eval_body(const Cell& C, int bid, int nblk)
     Ctl* ctl = C.ctl;
     __shared__ ShufSh shs;
     __shared__ SelSh sel;
+    __shared__ WalkTab wtab[WAVES];
     __shared__ int last;
     if (!ctl->begun || ctl->done || ctl->selected) return;
     const int wv = threadIdx.x >> 6;
@@ -804,9 +815,9 @@ __device__ __forceinline__ void eval_body(const Cell& C, int bid, int nblk)
         // units over all waves of workgroups >= 1, except the rng wave
         const int64_t wid = (int64_t)(bid - 1) * WAVES + wv - (rng_wg == 1 ? 1 : 0);
         const int64_t nw = (int64_t)(nblk - 1) * WAVES - 1;
-        if (!(bid == 1 && wv == 0)) eval_units(C, ctl, wid, nw);
+        if (!(bid == 1 && wv == 0)) eval_units(C, ctl, wid, nw, wtab[wv]);
     } else if (nblk == 1) {
-        eval_units(C, ctl, wv, WAVES);
+        eval_units(C, ctl, wv, WAVES, wtab[wv]);
     }
     if (!wg_arrive_last(C.arrive, &ctl->evdone, &last, bid, nblk)) return;
     const int64_t ts0 = clk();
@@ -992,7 +1003,20 @@ __device__ __forceinline__ void wg_post(const Cell& C, Ctl* ctl, NarrowSh& sh)
 }
 
 // the commit units: every pattern of the subsets avoiding bp, flattened
-__device__ __forceinline__ int64_t commit_units(const Cell& C, const Ctl* ctl, int64_t gt, int64_t nt)
+// offsets of the commit's words around the moving codeword: T[j * 16 + k]
+// = (v - c_j) q^(n-1-j) for the k-th value v != c_j of position j (all threads)
+__device__ __forceinline__ void commit_table(const Cell& C, uint64_t pk, int64_t* T)
+{
+    for (int x = threadIdx.x; x < C.n * C.q1; x += WG) {
+        const int j = x / C.q1, k = x - j * C.q1;
+        const int c = nib_get(pk, j);
+        T[j * 16 + k] = (int64_t)(k + (k >= c) - c) * C.pw[j];
+    }
+    __syncthreads();
+}
+
+__device__ __forceinline__ int64_t commit_units(const Cell& C, const Ctl* ctl, int64_t gt, int64_t nt,
+                                                const int64_t* T)
 {
     const int bi = ctl->bi, bp = ctl->bp;
     const int64_t base = ctl->base, dv = ctl->dv;
@@ -1002,9 +1026,17 @@ __device__ __forceinline__ int64_t commit_units(const Cell& C, const Ctl* ctl, i
     int64_t d = 0;
     for (int64_t g = gt; g < S; g += nt) {
         const int64_t e = g / C.P;
-        const uint32_t t = (uint32_t)(g - e * C.P);
-        uint64_t apk;
-        const int64_t a = pattern_word(C, base, pk, C.psub[(int64_t)bp * C.E + e], C.R, t, &apk);
+        uint32_t t = (uint32_t)(g - e * C.P);
+        const uint32_t sub = C.psub[(int64_t)bp * C.E + e];
+        // the word of pattern t (pattern_word's, through the table)
+        const uint32_t t0 = t;
+        int64_t a = base;
+        for (int i = C.R - 1; i >= 0; i--) {
+            const uint32_t tq = div_q1(C, t);
+            const int k = (int)(t - tq * (uint32_t)C.q1);
+            t = tq;
+            a += T[((sub >> (4 * i)) & 15) * 16 + k];
+        }
         const int64_t b = a + dv;
         const uint16_t ca = cnt_get(C, a), cb = cnt_get(C, b);
         cnt_put(C, a, (uint16_t)(ca - 1));
@@ -1012,7 +1044,11 @@ __device__ __forceinline__ int64_t commit_units(const Cell& C, const Ctl* ctl, i
         if (ca == 1) {
             bit_set(C.zb, a); bit_clr(C.ob, a);
             atomicAdd(&C.blk[a >> BLK_SHIFT], 1u);
-            if (list_on) list_insert(C, C.ctl, a, apk);
+            if (list_on) {
+                uint64_t apk;
+                (void)pattern_word(C, base, pk, sub, C.R, t0, &apk);
+                list_insert(C, C.ctl, a, apk);
+            }
             d++;
         } else if (ca == 2) {
             bit_set(C.ob, a);
@@ -1038,7 +1074,9 @@ __device__ __forceinline__ void commit_body(const Cell& C, int bid, int nblk, in
     if (bid == 0 && threadIdx.x == 0 && ctl->t_eve) ctl->tph[11] += clk() - ctl->t_eve;
     const int64_t tc0 = clk();
     const int64_t gt = (int64_t)bid * WG + threadIdx.x;
-    const int64_t d = commit_units(C, ctl, gt, (int64_t)nblk * WG);
+    __shared__ int64_t ctab[MAXN * 16];
+    commit_table(C, C.cpk[ctl->bi], ctab);
+    const int64_t d = commit_units(C, ctl, gt, (int64_t)nblk * WG, ctab);
     const int64_t ds = wg_sum(d, sh.red);
     if (threadIdx.x == 0 && ds) atomicAdd(reinterpret_cast<unsigned long long*>(&ctl->dunc), (unsigned long long)ds);
     if (!wg_arrive_last(C.arrive + ARR_G, &ctl->cmdone, &last, bid, nblk)) return;
@@ -1079,7 +1117,8 @@ __global__ void __launch_bounds__(WG) k_commit(const Cell C, int api)
 }
 
 // set the run's stop and begin the first step (one workgroup)
-__device__ __forceinline__ void begin_run_body(const Cell& C, int64_t nsteps)
+// start a run of nsteps; notrace: the run records no trace (batched runs)
+__device__ __forceinline__ void begin_run_body(const Cell& C, int64_t nsteps, bool notrace)
 {
     __shared__ NarrowSh sh;
     __shared__ Ctl lc;
@@ -1088,6 +1127,10 @@ __device__ __forceinline__ void begin_run_body(const Cell& C, int64_t nsteps)
     if (threadIdx.x == 0) {
         ctl->it_stop = ctl->it + nsteps - (ctl->begun ? 1 : 0);
         ctl->tpos = 0;
+        if (notrace) {
+            ctl->trace = nullptr;
+            ctl->tcap = 0;
+        }
     }
     __syncthreads();
     wg_begin(C, ctl, sh);
@@ -1097,7 +1140,7 @@ __device__ __forceinline__ void begin_run_body(const Cell& C, int64_t nsteps)
 
 __global__ void __launch_bounds__(WG) k_begin_run(const Cell C, int64_t nsteps)
 {
-    begin_run_body(C, nsteps);
+    begin_run_body(C, nsteps, false);
 }
 
 // --------------------------------------------------------- init kernels
@@ -1229,6 +1272,8 @@ __global__ void __launch_bounds__(WG) k_first_from(const Cell C, int64_t start, 
 // codewords slot_cw[0..k) against word u (ctl->upk), all differing positions
 __global__ void __launch_bounds__(WG) k_eval_probe(const Cell C, int k)
 {
+    __shared__ WalkTab wtab[WAVES];
+    WalkTab& tb = wtab[threadIdx.x >> 6];
     const Ctl* ctl = C.ctl;
     const uint64_t upk = ctl->upk;
     const int64_t wid = (int64_t)blockIdx.x * WAVES + (threadIdx.x >> 6);
@@ -1237,7 +1282,7 @@ __global__ void __launch_bounds__(WG) k_eval_probe(const Cell C, int k)
     for (int64_t g = wid; g < nl + ne; g += nw) {
         if (g < nl) {
             const int slot = (int)(g / C.F);
-            unit_leave(C, slot, (int)(g - (int64_t)slot * C.F));
+            unit_leave(C, slot, (int)(g - (int64_t)slot * C.F), tb);
         } else {
             const int64_t h = g - nl;
             const int slot = (int)(h / ((int64_t)C.n * C.E));
@@ -1246,7 +1291,7 @@ __global__ void __launch_bounds__(WG) k_eval_probe(const Cell C, int k)
             const int e = (int)(r - (int64_t)p * C.E);
             const int up = nib_get(upk, p);
             if (nib_get(C.cpk[C.slot_cw[slot]], p) == up) continue;   // uniform per wave
-            unit_walk(C, slot, p, up, e);
+            unit_walk(C, slot, p, up, e, tb);
         }
     }
 }

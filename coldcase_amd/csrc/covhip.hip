@@ -471,6 +471,18 @@ int cov_create(int q, int n, int R, int M, cov_cell_t** out)
         for (int e = 0; e < n - 1 - j; e++) C.pw[j] *= q;
     }
     C.magic = (q - 1 == 1) ? 0u : (uint32_t)((((uint64_t)1 << 32) + (uint64_t)(q - 2)) / (uint64_t)(q - 1));
+    {
+        // sphere-walk tables: the innermost ceil(R/2) positions in lo
+        const int a = (R + 1) / 2;
+        int64_t nl = 1, nh = 1;
+        for (int i = 0; i < a; i++) nl *= (q - 1);
+        for (int i = a; i < R; i++) nh *= (q - 1);
+        C.wa = a;
+        C.wok = (R >= 1 && nl <= WT_MAX && nh <= WT_MAX) ? 1 : 0;
+        C.wnl = C.wok ? (int32_t)nl : 1;
+        C.wnh = C.wok ? (int32_t)nh : 1;
+        C.wmag = C.wnl > 1 ? (uint32_t)((((uint64_t)1 << 32) / (uint64_t)C.wnl) + 1) : 0u;
+    }
     const Sizes z = sizes_of(s);
     int rc = COV_OK;
 #define AL(ptr, bytes) if ((rc = dalloc(h, &(ptr), (bytes))) != COV_OK) return fail(rc)
@@ -1220,12 +1232,7 @@ int cov_batch_launch(cov_batch_t* b, int64_t nsteps, int mode)
     if (nsteps == 0) return COV_OK;
     const int k = (int)b->cells.size();
     for (cov_cell* c : b->cells) c->dist_valid = false;
-    // no trace records from batched runs
-    const int64_t zero = 0;
-    for (cov_cell* h : b->cells) {
-        HCK(b, hipMemcpyAsync(CTL_PTR(h, trace), &zero, 8, hipMemcpyHostToDevice, b->stream));
-        HCK(b, hipMemcpyAsync(CTL_PTR(h, tcap), &zero, 8, hipMemcpyHostToDevice, b->stream));
-    }
+    // (k_begin_run_b also turns the cells' trace records off)
     hipLaunchKernelGGL(k_begin_run_b, dim3(k), dim3(WG), 0, b->stream, b->d_cells, nsteps);
     HCK(b, hipGetLastError());
     int64_t left = nsteps;

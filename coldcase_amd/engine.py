@@ -392,7 +392,6 @@ class Fleet:
             idle = True
             for i, b in enumerate(self.batches):
                 if busy[i] and b.query():
-                    b.sync()
                     busy[i] = False
                 if not busy[i] and left[i] > 0:
                     k = min(int(chunk), left[i])
@@ -401,6 +400,8 @@ class Fleet:
                     busy[i] = True
                     idle = False
             if not any(busy) and not any(left):
+                for b in self.batches:
+                    b.sync()               # raises on a cell's fault
                 return
             if idle:
                 time.sleep(poll)

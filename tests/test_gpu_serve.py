@@ -11,7 +11,8 @@ pytestmark = pytest.mark.gpu
 SHAPES = [(5, 7, 2, 400), (6, 6, 3, 41), (4, 8, 3, 60), (3, 8, 2, 120), (6, 9, 4, 660)]
 
 
-def _trace(n_req=14, rate=60.0, moves=300, seed=5):
+def _trace(n_req=14, rate=2000.0, moves=2000, seed=5):
+    # arrivals 0.5 ms apart on average and searches of tens of ms: they overlap
     from coldcase_amd.serve import poisson_workload
     return poisson_workload(seed, n_req, rate, SHAPES, verify_frac=0.35, moves=moves)
 
