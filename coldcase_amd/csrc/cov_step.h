@@ -1015,50 +1015,145 @@ __device__ __forceinline__ void commit_table(const Cell& C, uint64_t pk, int64_t
     __syncthreads();
 }
 
+// Runs of contiguous lanes with equal keys (key < 0: no operation) combine
+// their values; the last lane of each run gets the run's total (segmented
+// inclusive scan).  Neighbouring patterns of a sphere walk are neighbouring
+// words, so a wave's plane-word and block updates come in long runs, and one
+// atomic per run replaces one per lane.  All 64 lanes.
+__device__ __forceinline__ bool run_tail(int64_t key, bool* head)
+{
+    const int lane = threadIdx.x & 63;
+    const int64_t kp = __shfl_up(key, 1, 64);
+    const int64_t kn = __shfl_down(key, 1, 64);
+    *head = lane == 0 || kp != key;
+    return lane == 63 || kn != key;
+}
+template <int K>
+__device__ __forceinline__ void run_or(bool head, uint32_t (&v)[K])
+{
+    const int lane = threadIdx.x & 63;
+    int f = head ? 1 : 0;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        uint32_t y[K];
+#pragma unroll
+        for (int k = 0; k < K; k++) y[k] = __shfl_up(v[k], o, 64);
+        const int fy = __shfl_up(f, o, 64);
+        if (lane >= o && !f) {
+#pragma unroll
+            for (int k = 0; k < K; k++) v[k] |= y[k];
+        }
+        if (lane >= o) f |= fy;
+    }
+}
+__device__ __forceinline__ int run_add(bool head, int v)
+{
+    const int lane = threadIdx.x & 63;
+    int f = head ? 1 : 0;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const int y = __shfl_up(v, o, 64);
+        const int fy = __shfl_up(f, o, 64);
+        if (lane >= o && !f) v += y;
+        if (lane >= o) f |= fy;
+    }
+    return v;
+}
+
+// the commit units: every pattern of the subsets avoiding bp, flattened; a
+// wave takes 64 consecutive patterns per round (all lanes take part in every
+// round, those past the end without an operation)
 __device__ __forceinline__ int64_t commit_units(const Cell& C, const Ctl* ctl, int64_t gt, int64_t nt,
                                                 const int64_t* T)
 {
+    const int lane = threadIdx.x & 63;
     const int bi = ctl->bi, bp = ctl->bp;
     const int64_t base = ctl->base, dv = ctl->dv;
     const uint64_t pk = C.cpk[bi];
     const bool list_on = ctl->ul_valid != 0;
     const int64_t S = (int64_t)C.E * C.P;
     int64_t d = 0;
-    for (int64_t g = gt; g < S; g += nt) {
-        const int64_t e = g / C.P;
-        uint32_t t = (uint32_t)(g - e * C.P);
-        const uint32_t sub = C.psub[(int64_t)bp * C.E + e];
-        // the word of pattern t (pattern_word's, through the table)
-        const uint32_t t0 = t;
-        int64_t a = base;
-        for (int i = C.R - 1; i >= 0; i--) {
-            const uint32_t tq = div_q1(C, t);
-            const int k = (int)(t - tq * (uint32_t)C.q1);
-            t = tq;
-            a += T[((sub >> (4 * i)) & 15) * 16 + k];
-        }
-        const int64_t b = a + dv;
-        const uint16_t ca = cnt_get(C, a), cb = cnt_get(C, b);
-        cnt_put(C, a, (uint16_t)(ca - 1));
-        cnt_put(C, b, (uint16_t)(cb + 1));
-        if (ca == 1) {
-            bit_set(C.zb, a); bit_clr(C.ob, a);
-            atomicAdd(&C.blk[a >> BLK_SHIFT], 1u);
-            if (list_on) {
-                uint64_t apk;
-                (void)pattern_word(C, base, pk, sub, C.R, t0, &apk);
-                list_insert(C, C.ctl, a, apk);
+    for (int64_t g0 = gt - lane; g0 < S; g0 += nt) {
+        const int64_t g = g0 + lane;
+        const bool act = g < S;
+        int64_t a = -1, b = -1;
+        uint32_t za = 0, oa_set = 0, oa_clr = 0, zb_clr = 0, ob_set = 0, ob_clr = 0;
+        int da = 0, db = 0;
+        if (act) {
+            const int64_t e = g / C.P;
+            uint32_t t = (uint32_t)(g - e * C.P);
+            const uint32_t sub = C.psub[(int64_t)bp * C.E + e];
+            // the word of pattern t (pattern_word's, through the table)
+            const uint32_t t0 = t;
+            a = base;
+            for (int i = C.R - 1; i >= 0; i--) {
+                const uint32_t tq = div_q1(C, t);
+                const int k = (int)(t - tq * (uint32_t)C.q1);
+                t = tq;
+                a += T[((sub >> (4 * i)) & 15) * 16 + k];
             }
-            d++;
-        } else if (ca == 2) {
-            bit_set(C.ob, a);
+            b = a + dv;
+            const uint16_t ca = cnt_get(C, a), cb = cnt_get(C, b);
+            cnt_put(C, a, (uint16_t)(ca - 1));
+            cnt_put(C, b, (uint16_t)(cb + 1));
+            const uint32_t ma = 1u << (a & 31), mb = 1u << (b & 31);
+            if (ca == 1) {
+                za = ma; oa_clr = ma; da = 1;
+                if (list_on) {
+                    uint64_t apk;
+                    (void)pattern_word(C, base, pk, sub, C.R, t0, &apk);
+                    list_insert(C, C.ctl, a, apk);
+                }
+                d++;
+            } else if (ca == 2) {
+                oa_set = ma;
+            }
+            if (cb == 0) {
+                zb_clr = mb; ob_set = mb; db = -1;
+                d--;
+            } else if (cb == 1) {
+                ob_clr = mb;
+            }
         }
-        if (cb == 0) {
-            bit_clr(C.zb, b); bit_set(C.ob, b);
-            atomicSub(&C.blk[b >> BLK_SHIFT], 1u);
-            d--;
-        } else if (cb == 1) {
-            bit_clr(C.ob, b);
+        // the planes: one atomic per plane word and run
+        {
+            const int64_t key = (za | oa_set | oa_clr) ? (a >> 5) : -1;
+            bool head;
+            const bool tail = run_tail(key, &head);
+            uint32_t v[3] = {za, oa_set, oa_clr};
+            run_or<3>(head, v);
+            if (tail && key >= 0) {
+                if (v[0]) atomicOr(&C.zb[key], v[0]);
+                if (v[1]) atomicOr(&C.ob[key], v[1]);
+                if (v[2]) atomicAnd(&C.ob[key], ~v[2]);
+            }
+        }
+        {
+            const int64_t key = (zb_clr | ob_set | ob_clr) ? (b >> 5) : -1;
+            bool head;
+            const bool tail = run_tail(key, &head);
+            uint32_t v[3] = {zb_clr, ob_set, ob_clr};
+            run_or<3>(head, v);
+            if (tail && key >= 0) {
+                if (v[0]) atomicAnd(&C.zb[key], ~v[0]);
+                if (v[1]) atomicOr(&C.ob[key], v[1]);
+                if (v[2]) atomicAnd(&C.ob[key], ~v[2]);
+            }
+        }
+        // the uncovered-count blocks
+        {
+            const int64_t key = da ? (a >> BLK_SHIFT) : -1;
+            bool head;
+            const bool tail = run_tail(key, &head);
+            const int v = run_add(head, da);
+            if (tail && key >= 0 && v) atomicAdd(&C.blk[key], (uint32_t)v);
+        }
+        {
+            const int64_t key = db ? (b >> BLK_SHIFT) : -1;
+            bool head;
+            const bool tail = run_tail(key, &head);
+            const int v = run_add(head, -db);
+            if (tail && key >= 0 && v) atomicSub(&C.blk[key], (uint32_t)v);
         }
     }
     return d;
