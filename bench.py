@@ -310,7 +310,7 @@ def main():
     ap.add_argument('--lds', type=int, default=0, help='transform tile LDS bytes (0 = default)')
     ap.add_argument('--threads', type=int, default=0, help='transform workgroup threads (0 = default)')
     ap.add_argument('--prefix', type=int, default=1000, help='batched: steps timed in both modes')
-    ap.add_argument('--groups', type=int, default=3, help='batched: concurrent batches, cells grouped by step cost')
+    ap.add_argument('--groups', type=int, default=4, help='batched: concurrent batches, cells grouped by step cost')
     ap.add_argument('--requests', type=int, default=60, help='serve: requests in the trace')
     ap.add_argument('--tick', type=int, default=256, help='serve: steps per batched tick at most')
     ap.add_argument('--budget-gb', type=float, default=64.0, help='serve: HBM budget')

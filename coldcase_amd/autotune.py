@@ -12,7 +12,8 @@ What is tuned:
                  workgroups than alone), timed on the batch itself
   fleet groups   cells split by step cost into a few batches that run
                  concurrently on their own streams (engine.Fleet), each
-                 with its own batch scale
+                 with its own batch scale, chosen with the groups running
+                 together
   tile           the distance transform's LDS bytes per tile
                  (cov_set_transform_tile), timed on a random code of the shape
 
@@ -139,14 +140,45 @@ def plan_groups(costs, ngroups=3):
     return groups
 
 
-def tune_fleet(cells, costs, ngroups=3, factors=(1.0, 0.5, 0.25, 0.125), steps=64):
-    """Group the cells by cost (plan_groups) and give each group the batch
-    scale that runs it fastest alone (tune_batch); returns the groups (lists
-    of cells) and their factors."""
+def tune_fleet(cells, costs, ngroups=4, start=0.25, lo=0.0625, hi=1.0, passes=2, steps=96):
+    """Group the cells by cost (plan_groups), then choose each group's batch
+    scale with the groups running together: coordinate descent over the
+    groups, each trying half and double its current scale, every candidate
+    timed on `steps` steps of the whole fleet.  Leaves the cells at the chosen
+    shapes; returns the groups (lists of cells), their scales and the index
+    lists."""
+    from .engine import Fleet
     idx = plan_groups(costs, ngroups)
     groups = [[cells[i] for i in g] for g in idx]
-    tuned = [tune_batch(g, factors=factors, steps=steps) for g in groups]
-    return groups, [t['factor'] for t in tuned], idx
+    base = [[c.get_launch() for c in g] for g in groups]
+    fs = [start] * len(groups)
+
+    def apply(fv):
+        for g, b, f in zip(groups, base, fv):
+            for c, (e, k) in zip(g, b):
+                c.set_launch(max(2, int(e * f)), max(1, int(k * f)))
+
+    def timed(fv):
+        apply(fv)
+        with Fleet(groups) as fl:
+            fl.run(32)
+            t = time.perf_counter()
+            fl.run(steps)
+            return (time.perf_counter() - t) / steps
+
+    best_t = timed(fs)
+    for _ in range(passes):
+        for gi in range(len(groups)):
+            for cand in (fs[gi] / 2, fs[gi] * 2):
+                if cand < lo or cand > hi:
+                    continue
+                trial = list(fs)
+                trial[gi] = cand
+                t = timed(trial)
+                if t < best_t:
+                    best_t, fs = t, trial
+    apply(fs)
+    return groups, fs, idx
 
 
 class Table:
