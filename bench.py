@@ -281,6 +281,35 @@ def bench_serve(a):
     res = srv.run(reqs, realtime=True)
     el = time.perf_counter() - t0
     s = summarize(res)
+    # plain: the same requests one at a time, each alone on the GPU (graph
+    # steps, default launch shapes); a search's service time is its moves
+    # times the shape's step time over a 256-step prefix, a verify's is timed
+    from coldcase_amd import Cell
+    from coldcase_amd.engine import MODE_GRAPH
+    from coldcase_amd.serve import run_verify
+    step_s, verify_s = {}, {}
+    plain_total = 0.0
+    for r in reqs:
+        key = (r.q, r.n, r.R, r.M)
+        if r.kind == 'search':
+            if key not in step_s:
+                with Cell(*key) as c:
+                    c.search_params(tenure=r.tenure, stall=r.stall, kick=r.kick)
+                    c.search_init(r.seed)
+                    c.run(32, mode=MODE_GRAPH)
+                    c.sync()
+                    t1 = time.perf_counter()
+                    c.run(256, mode=MODE_GRAPH)
+                    step_s[key] = (time.perf_counter() - t1) / 256
+            plain_total += r.moves * step_s[key]
+        else:
+            if key not in verify_s:
+                run_verify(r)
+                t1 = time.perf_counter()
+                run_verify(r)
+                verify_s[key] = time.perf_counter() - t1
+            plain_total += verify_s[key]
+    span = max(x.finished for x in res) - min(x.arrival for x in res)
     out = {'metric': 'p50/p99 solve latency (served)', 'value': round(s['p50_s'], 4), 'unit': 's',
            'n_gpus': 1, 'steps': a.steps, 'warmup': 0, 'ms_per_step': None, 'higher_is_better': False,
            'scaling': 'weak', 'vs_baseline': None, 'dtype': 'u16', 'data': 'synthetic',
@@ -288,7 +317,11 @@ def bench_serve(a):
                                   'bursts of %d moves over K5(7,2)..K6(10,4); record codes as verify inputs'
                                   % (a.requests, a.steps),
                       'hbm_budget_gb': a.budget_gb, 'tick_steps': a.tick},
-           'latency': s, 'makespan_s': round(el, 3), 'peak_cells': srv.peak_cells,
+           'latency': s, 'makespan_s': round(el, 3),
+           'served_rps': round(len(res) / span, 4), 'plain_rps_projected': round(len(res) / plain_total, 4),
+           'ratio_served_vs_plain': round(plain_total / span, 3),
+           'plain_step_us': {'%d,%d,%d,%d' % k: round(v * 1e6, 1) for k, v in step_s.items()},
+           'peak_cells': srv.peak_cells,
            'peak_gb': round(srv.peak_bytes / (1 << 30), 3), 'ticks': srv.ticks,
            'verify_record_uncovered': sorted({r.uncovered for q_, r in zip(reqs, res)
                                               if q_.kind == 'verify' and (q_.q, q_.n, q_.R, q_.M) in codes})}

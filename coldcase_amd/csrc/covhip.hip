@@ -313,6 +313,10 @@ int do_recount(cov_cell* h)
     hipEvent_t a = nullptr;
     RC(timed_begin(h, 3, &a));
     const int g = (int)std::min<int64_t>(h->plan.ntile, 2048);
+    // the kernel's dynamic LDS limit is one value for every cell: set it to
+    // this cell's tile before its launch
+    HCK(h, hipFuncSetAttribute((const void*)k_recount, hipFuncAttributeMaxDynamicSharedMemorySize,
+                               (int)(h->plan.T * 4)));
     hipLaunchKernelGGL(k_recount, dim3(g), dim3(WG), (size_t)h->plan.T * 4, h->stream, h->h, h->plan);
     HCK(h, hipGetLastError());
     RC(timed_end(h, 3, a));
@@ -575,8 +579,7 @@ int cov_create(int q, int n, int R, int M, cov_cell_t** out)
     c.it_stop = 0;
     if ((rc = upload(h, C.ctl, &c, sizeof(Ctl))) != COV_OK) return fail(rc);
     if ((rc = push_cell(h)) != COV_OK) return fail(rc);
-    if (hipFuncSetAttribute((const void*)k_recount, hipFuncAttributeMaxDynamicSharedMemorySize,
-                            (int)(pl.T * 4)) != hipSuccess) {
+    if (pl.T * 4 > 65536) {
         h->err = "recount tile does not fit in LDS";
         return fail(COV_ELIMIT);
     }

@@ -14,12 +14,13 @@ Two request kinds, over cells of any shape (up to the engine's limits):
 Requests are admitted as they arrive while the HBM budget (the sum of
 cov_mem_bytes of resident cells) and the cell slots allow; a request that
 does not fit waits, one larger than the whole budget is refused.  Admitted
-searches stay resident and advance together: every tick is one batched run
+searches stay resident in one of four lanes by the size of their balls; the
+searches of a lane advance together: every tick is one batched run
 (coldcase_amd.engine.Batch, one evaluator and one commit launch per step for
 all of them) of as many steps as the closest-to-done search still needs, up
 to `tick_steps`; searches that finish leave and newly admitted ones join at
-the next tick.  Verifies run on their own streams while a tick is on the
-device.  A request's result does not depend on what else is resident: each
+the lane's next tick.  The lanes tick independently on their own streams, and
+verifies run on theirs while ticks are on the device.  A request's result does not depend on what else is resident: each
 cell keeps its own counters, rng stream and tabu table.
 """
 import collections
@@ -129,6 +130,29 @@ def finish_search(r, c):
     return res
 
 
+def cost_class(r):
+    """The lane of a search request: 0..3 by its ball size, which sets the
+    cost of its steps (a few hundred words to a few hundred thousand)."""
+    v = ball_volume(r.q, r.n, r.R)
+    return 0 if v < 1000 else (1 if v < 30000 else (2 if v < 120000 else 3))
+
+
+NLANES = 4
+
+
+class _Lane:
+    """The resident searches of one cost class: they advance together, one
+    batched tick at a time, on the lane's own stream."""
+
+    def __init__(self):
+        self.active = []                # [request, cell, steps left, admitted time]
+        self.joining = []               # admitted while a tick runs: join at the next
+        self.batch = None
+        self.members = ()
+        self.busy = False
+        self.k = 0
+
+
 class Server:
     """Continuous batching with HBM-budget admission (see the module text).
 
@@ -137,6 +161,10 @@ class Server:
     tick_steps  steps per batched tick at most
     launch      optional callable(Cell) that sets the cell's launch shape
                 (the autotuner's choice for batched runs)
+
+    Resident searches are split by cost class into lanes (cost_class); each
+    lane is a batch on its own stream and ticks at its own pace, so a short
+    search does not wait for the ticks of a long one.
     """
 
     def __init__(self, hbm_budget=64 << 30, max_cells=64, tick_steps=256, mode=MODE_GRAPH, launch=None):
@@ -155,53 +183,77 @@ class Server:
         reqs = sorted(requests, key=lambda r: (r.arrival, r.rid))
         self._results = {}
         self._waiting = collections.deque()
-        self._joining = []              # admitted searches waiting for the next tick
+        self._joining = []              # admitted searches not yet in a lane
         self._used = 0
         self._next = 0
-        active = []                     # [request, cell, steps left, admitted time]
-        batch, members = None, ()
+        lanes = [_Lane() for _ in range(NLANES)]
         self._t0 = time.perf_counter()
+
+        def resident():
+            return sum(len(ln.active) + len(ln.joining) for ln in lanes) + len(self._joining)
+
         try:
-            while self._next < len(reqs) or self._waiting or self._joining or active:
-                self._admit(reqs, realtime, len(active))
-                active.extend(self._joining)
+            while self._next < len(reqs) or self._waiting or self._joining or \
+                    any(ln.active or ln.joining or ln.busy for ln in lanes):
+                did = self._admit(reqs, realtime, resident())
+                pending = {}
+                for a in self._joining:
+                    pending.setdefault(cost_class(a[0]), []).append(a)
                 self._joining = []
-                self.peak_cells = max(self.peak_cells, len(active))
-                if not active:
-                    if self._next < len(reqs) and realtime:
+                for ci, ln in enumerate(lanes):
+                    if ln.busy and ln.batch.query():
+                        ln.batch.sync()
+                        ln.busy = False
+                        self._retire(ln)
+                        did = True
+                    if ci in pending:
+                        ln.joining.extend(pending.pop(ci))
+                    if not ln.busy and ln.joining:
+                        ln.active.extend(ln.joining)
+                        ln.joining = []
+                    if not ln.busy and ln.active:
+                        cells = tuple(a[1] for a in ln.active)
+                        if cells != ln.members:
+                            if ln.batch is not None:
+                                ln.batch.close()
+                            ln.batch, ln.members = Batch(list(cells)), cells
+                        ln.k = min(self.tick_steps, min(a[2] for a in ln.active))
+                        ln.batch.launch(ln.k, mode=self.mode)
+                        ln.busy = True
+                        self.ticks += 1
+                        did = True
+                self.peak_cells = max(self.peak_cells, resident())
+                if not did:
+                    if not any(ln.busy for ln in lanes) and self._next < len(reqs) and realtime:
                         time.sleep(max(0.0, min(reqs[self._next].arrival - self._now(), 0.01)))
-                    continue
-                cells = tuple(a[1] for a in active)
-                if cells != members:
-                    if batch is not None:
-                        batch.close()
-                    batch, members = Batch(list(cells)), cells
-                k = min(self.tick_steps, min(a[2] for a in active))
-                batch.launch(k, mode=self.mode)
-                # arrivals while the tick runs: verifies at once, searches join next tick
-                while not batch.query():
-                    if not self._admit(reqs, realtime, len(active) + len(self._joining)):
-                        time.sleep(0.0005)
-                batch.sync()
-                self.ticks += 1
-                left = []
-                for a in active:
-                    a[2] -= k
-                    if a[2] <= 0 or a[1].state()['done']:
-                        res = finish_search(a[0], a[1])
-                        res.admitted, res.finished = a[3], self._now()
-                        self._results[a[0].rid] = res
-                        self._used -= request_bytes(a[0])
-                        a[1].close()
                     else:
-                        left.append(a)
-                active = left
+                        time.sleep(0.0005)
         finally:
-            if batch is not None:
-                batch.close()
-            for a in active + self._joining:
+            for ln in lanes:
+                if ln.busy:
+                    ln.batch.sync()
+                if ln.batch is not None:
+                    ln.batch.close()
+                for a in ln.active + ln.joining:
+                    a[1].close()
+            for a in self._joining:
                 a[1].close()
         return [self._results[r.rid] for r in sorted(requests, key=lambda r: r.rid)]
+
+    def _retire(self, ln):
+        """After a tick: searches that used their moves or covered leave."""
+        left = []
+        for a in ln.active:
+            a[2] -= ln.k
+            if a[2] <= 0 or a[1].state()['done']:
+                res = finish_search(a[0], a[1])
+                res.admitted, res.finished = a[3], self._now()
+                self._results[a[0].rid] = res
+                self._used -= request_bytes(a[0])
+                a[1].close()
+            else:
+                left.append(a)
+        ln.active = left
 
     def _now(self):
         return time.perf_counter() - self._t0
