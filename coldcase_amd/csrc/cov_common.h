@@ -114,6 +114,7 @@ struct Cell {
     int q, n, R, M, q1, F, E;
     int64_t P;              // (q-1)^R patterns per position subset
     int64_t N, nzw, nblk, V, ncap, ulcap;
+    int64_t ul_use;         // list length below which the evaluator reads the list, (R+1) E P / n
     int64_t pw[MAXN];
     uint32_t magic;         // ceil(2^32 / (q-1)): t / (q-1) = umulhi(t, magic)
     // sphere walks by tables: pattern t of an R-subset = hi[t / wnl] + lo[t % wnl],
@@ -149,6 +150,16 @@ struct Cell {
     GPTR(uint8_t) dist;          // distance transform output (N bytes), lazily
     GPTR(Ctl) ctl;
 };
+
+// The uncovered list only serves the evaluator, which reads it while it is
+// shorter than ul_use: keep it while at most this many words are uncovered
+// (and it fits), rebuild it below half of that.  Speed only: the list never
+// changes a value.
+__host__ __device__ inline int64_t list_keep(const Cell& C)
+{
+    const int64_t cap = C.ulcap / 2;
+    return C.ul_use < cap ? C.ul_use : cap;
+}
 
 // ------------------------------------------------------------- helpers
 __device__ __forceinline__ uint16_t cnt_get(const Cell& C, int64_t w)

@@ -271,8 +271,16 @@ __device__ __forceinline__ void wg_begin(const Cell& C, Ctl* ctl, NarrowSh& sh)
         const int valid = ctl->ul_valid;
         const int64_t ul_n = ctl->ul_n;
         __syncthreads();
-        if (!valid && unc <= C.ulcap / 4) wg_list_rebuild(C, ctl, sh);
-        else if (valid && ul_n > 2 * unc + 64) wg_list_compact(C, ctl, sh);
+        if (!valid && unc <= list_keep(C) / 2) {
+            wg_list_rebuild(C, ctl, sh);
+        } else if (valid && unc > list_keep(C)) {
+            // too long to be read: drop it rather than keep it up to date
+            // (a rebuild rewrites the membership plane)
+            if (threadIdx.x == 0) { ctl->ul_valid = 0; ctl->ul_n = 0; }
+            __syncthreads();
+        } else if (valid && ul_n > 2 * unc + 64) {
+            wg_list_compact(C, ctl, sh);
+        }
     }
 
     const int64_t tp0 = clk();
@@ -303,25 +311,39 @@ __device__ __forceinline__ void wg_begin(const Cell& C, Ctl* ctl, NarrowSh& sh)
     const int nwm = (C.M + 63) >> 6;
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     int64_t dmin = INT64_MAX;
-    auto mark_pass = [&](int target, bool with_min) {
-        for (int i0 = 0; i0 < C.M; i0 += SCAN_ILP * WG) {
-            uint64_t c[SCAN_ILP];
+    // up to SCAN_ILP * WG codewords the thread keeps its words' digits in
+    // registers from the scan to the slot emission
+    uint64_t c[SCAN_ILP];
+    auto load = [&](int i0) {
 #pragma unroll
-            for (int k = 0; k < SCAN_ILP; k++) {
-                const int i = i0 + k * WG + threadIdx.x;
-                c[k] = i < C.M ? C.cpk[i] : 0;
-            }
-#pragma unroll
-            for (int k = 0; k < SCAN_ILP; k++) {
-                const int i = i0 + k * WG + threadIdx.x;
-                const int64_t d = nib_dist(c[k], upk);
-                if (with_min && i < C.M && d < dmin) dmin = d;
-                const uint64_t m = __ballot(i < C.M && d == target);
-                const int wi = ((i0 + k * WG) >> 6) + wv;
-                if (lane == 0 && wi < nwm) sh.qm[wi] = m;
-            }
+        for (int k = 0; k < SCAN_ILP; k++) {
+            const int i = i0 + k * WG + threadIdx.x;
+            c[k] = i < C.M ? C.cpk[i] : 0;
         }
     };
+    auto mark = [&](int i0, int target, bool with_min) {
+#pragma unroll
+        for (int k = 0; k < SCAN_ILP; k++) {
+            const int i = i0 + k * WG + threadIdx.x;
+            const int64_t d = nib_dist(c[k], upk);
+            if (with_min && i < C.M && d < dmin) dmin = d;
+            const uint64_t m = __ballot(i < C.M && d == target);
+            const int wi = ((i0 + k * WG) >> 6) + wv;
+            if (lane == 0 && wi < nwm) sh.qm[wi] = m;
+        }
+    };
+    const bool one_round = C.M <= SCAN_ILP * WG;
+    auto mark_pass = [&](int target, bool with_min) {
+        if (one_round) {
+            mark(0, target, with_min);
+            return;
+        }
+        for (int i0 = 0; i0 < C.M; i0 += SCAN_ILP * WG) {
+            load(i0);
+            mark(i0, target, with_min);
+        }
+    };
+    if (one_round) load(0);
     mark_pass(C.R + 1, true);
     dmin = wg_min(dmin, sh.red);
     const int dtar = (dmin <= C.R + 1) ? C.R + 1 : (int)dmin;
@@ -329,8 +351,7 @@ __device__ __forceinline__ void wg_begin(const Cell& C, Ctl* ctl, NarrowSh& sh)
         mark_pass(dtar, false);
         __syncthreads();
     }
-    // a scan of the mask words' counts gives each word its first slot; each
-    // wave then takes whole words, lane b the word's bit b: slots in index order
+    // a scan of the mask words' counts gives each word its first slot
     constexpr int QPT = (MAX_M_WORDS + WG - 1) / WG;   // mask words per thread
     int cnt = 0;
 #pragma unroll
@@ -349,16 +370,27 @@ __device__ __forceinline__ void wg_begin(const Cell& C, Ctl* ctl, NarrowSh& sh)
         }
     }
     __syncthreads();
-    for (int j = wv; j < nwm; j += WAVES) {
-        const uint64_t m = sh.qm[j];
-        if ((m >> lane) & 1) {
-            const int i = 64 * j + lane;
-            const int64_t slot = sh.qfirst[j] + __popcll(m & ((1ull << lane) - 1));
-            const uint64_t ne = nib_ne(C.cpk[i], upk);
-            C.slot_cw[slot] = i;
-            int t = 0;
-            for (int p = 0; p < C.n; p++)
-                if ((ne >> (4 * p)) & 1) C.slot_pos[slot * MAXN + t++] = (uint8_t)p;
+    // codeword i takes slot first(i / 64) + its rank in its mask word: index order
+    auto emit = [&](int i, uint64_t mine) {
+        const int64_t slot = sh.qfirst[i >> 6] + __popcll(sh.qm[i >> 6] & ((1ull << (i & 63)) - 1));
+        const uint64_t ne = nib_ne(mine, upk);
+        C.slot_cw[slot] = i;
+        int t = 0;
+        for (int p = 0; p < C.n; p++)
+            if ((ne >> (4 * p)) & 1) C.slot_pos[slot * MAXN + t++] = (uint8_t)p;
+    };
+    if (one_round) {
+        // the thread that marked a codeword emits it, its digits still in registers
+#pragma unroll
+        for (int k = 0; k < SCAN_ILP; k++) {
+            const int i = k * WG + threadIdx.x;
+            if (i < C.M && ((sh.qm[i >> 6] >> (i & 63)) & 1)) emit(i, c[k]);
+        }
+    } else {
+        // each wave takes whole mask words, lane b the word's bit b
+        for (int j = wv; j < nwm; j += WAVES) {
+            const uint64_t m = sh.qm[j];
+            if ((m >> lane) & 1) emit(64 * j + lane, C.cpk[64 * j + lane]);
         }
     }
     const int64_t ncand = nq * dtar;

@@ -470,6 +470,7 @@ int cov_create(int q, int n, int R, int M, cov_cell_t** out)
     Cell& C = h->h;
     C.q = q; C.n = n; C.R = R; C.M = M; C.q1 = q - 1; C.F = s.F; C.E = s.E; C.P = s.P;
     C.N = s.N; C.nzw = s.nzw; C.nblk = s.nblk; C.V = s.V; C.ncap = (int64_t)M * n; C.ulcap = s.ulcap;
+    C.ul_use = (int64_t)(R + 1) * s.E * s.P / n;
     for (int j = 0; j < n; j++) {
         C.pw[j] = 1;
         for (int e = 0; e < n - 1 - j; e++) C.pw[j] *= q;
@@ -1013,7 +1014,7 @@ int cov_search_init(cov_cell_t* h, uint64_t seed, const uint8_t* seed_code, int 
     RC(read_ctl(h, &now));
     RC(set_ctl<int64_t>(h, offsetof(Ctl, best_unc), now.unc));
     RC(set_ctl<int32_t>(h, offsetof(Ctl, ndirty), 0));
-    if (now.unc <= C.ulcap / 2) RC(list_rebuild(h));
+    if (now.unc <= list_keep(C)) RC(list_rebuild(h));
     else RC(set_ctl<int32_t>(h, offsetof(Ctl, ul_valid), 0));
     return COV_OK;
 }
