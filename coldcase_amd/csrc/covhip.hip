@@ -46,6 +46,10 @@ struct cov_cell {
     int dt_threads = 0;                   // threads per transform workgroup (0: default)
     bool dist_valid = false;              // dist holds the transform of the current code
     bool cnt_valid = true;                // cnt, planes and counts match the code
+    // recorded on a batch's stream after each batched launch that includes
+    // the cell: the cell's own stream waits for it before touching the cell,
+    // and its memory is not freed before it
+    hipEvent_t used = nullptr;
 };
 
 // Many cells stepped by the same launches (cov_batch.h).
@@ -221,6 +225,7 @@ int set_ctl(cov_cell* h, size_t off, T v)
 
 int read_ctl(cov_cell* h, Ctl* out)
 {
+    if (h->used) HCK(h, hipStreamWaitEvent(h->stream, h->used, 0));
     HCK(h, hipMemcpyAsync(out, h->h.ctl, sizeof(Ctl), hipMemcpyDeviceToHost, h->stream));
     HCK(h, hipStreamSynchronize(h->stream));
     return COV_OK;
@@ -460,12 +465,17 @@ int cov_create(int q, int n, int R, int M, cov_cell_t** out)
     auto fail = [&](int rc) {
         for (void* p : h->allocs) (void)hipFree(p);
         if (h->stream) (void)hipStreamDestroy(h->stream);
+        if (h->used) (void)hipEventDestroy(h->used);
         delete h;
         return rc;
     };
     {
         const hipError_t e = hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking);
         if (e != hipSuccess) { h->stream = nullptr; return fail(-(100 + (int)e)); }
+    }
+    {
+        const hipError_t e = hipEventCreateWithFlags(&h->used, hipEventDisableTiming);
+        if (e != hipSuccess) { h->used = nullptr; return fail(-(100 + (int)e)); }
     }
     Cell& C = h->h;
     C.q = q; C.n = n; C.R = R; C.M = M; C.q1 = q - 1; C.F = s.F; C.E = s.E; C.P = s.P;
@@ -596,6 +606,7 @@ int cov_create(int q, int n, int R, int M, cov_cell_t** out)
 int cov_destroy(cov_cell_t* h)
 {
     if (!h) return COV_EPARAM;
+    if (h->used) (void)hipEventSynchronize(h->used);    // batched launches that include the cell
     (void)hipStreamSynchronize(h->stream);
     if (h->gexec) (void)hipGraphExecDestroy(h->gexec);
     for (void* p : h->allocs) (void)hipFree(p);
@@ -605,6 +616,7 @@ int cov_destroy(cov_cell_t* h)
     for (const auto& p : h->pend) { (void)hipEventDestroy(p.a); (void)hipEventDestroy(p.b); }
     for (hipEvent_t e : h->evs) (void)hipEventDestroy(e);
     (void)hipStreamDestroy(h->stream);
+    if (h->used) (void)hipEventDestroy(h->used);
     delete h;
     return COV_OK;
 }
@@ -1264,6 +1276,7 @@ int cov_batch_launch(cov_batch_t* b, int64_t nsteps, int mode)
         hipLaunchKernelGGL(k_commit_b, dim3(b->commit_total), dim3(WG), 0, b->stream, b->d_cells, b->d_coff, k);
         HCK(b, hipGetLastError());
     }
+    for (cov_cell* c : b->cells) HCK(b, hipEventRecord(c->used, b->stream));
     return COV_OK;
 }
 
