@@ -147,6 +147,7 @@ struct Cell {
     GPTR(int64_t) scratch64;     // small reduction outputs
     GPTR(int32_t) arrive;        // workgroup arrival sub-counters: evaluator, commit
     GPTR(int32_t) dirty;         // codewords changed since best == code (DIRTY_CAP of them)
+    GPTR(uint64_t) jump;         // the rng jump matrices J_1 .. J_{RNG_SPLIT-1}
     GPTR(uint8_t) dist;          // distance transform output (N bytes), lazily
     GPTR(Ctl) ctl;
 };
@@ -306,10 +307,10 @@ __device__ __forceinline__ void walk_release()
 }
 
 // ----------------------------------------------------------------- rng
-__device__ __forceinline__ uint64_t rotl64(uint64_t x, int k) { return (x << k) | (x >> (64 - k)); }
+__host__ __device__ __forceinline__ uint64_t rotl64(uint64_t x, int k) { return (x << k) | (x >> (64 - k)); }
 
 // xoshiro256++ step (Blackman & Vigna)
-__device__ __forceinline__ uint64_t xo_next(uint64_t s[4])
+__host__ __device__ __forceinline__ uint64_t xo_next(uint64_t s[4])
 {
     const uint64_t out = rotl64(s[0] + s[3], 23) + s[0];
     const uint64_t t = s[1] << 17;
@@ -354,6 +355,99 @@ __device__ inline int64_t wave_ring_fill(const Cell& C, Ctl* ctl, int64_t upto, 
         ctl->rgen = g;
     }
     return g - g0;
+}
+
+// Four waves at once.  The generator's state transition is linear over
+// GF(2), so the state RNG_PART * w draws ahead is J_w s for a fixed 256 x 256
+// bit matrix J_w = T^(RNG_PART w).  A matrix is 256 rows of 4 words: bit j of
+// row b is set when bit b of the result depends on bit j of the state, bit b
+// of a state being bit b % 64 of word b / 64.
+constexpr int RNG_SPLIT = 4;                    // waves of the filling workgroup
+constexpr int RNG_PART = 256;                   // draws per wave
+constexpr int RNG_CHUNK = RNG_SPLIT * RNG_PART; // draws per fill
+constexpr int RNG_JUMP_WORDS = (RNG_SPLIT - 1) * 256 * 4;
+
+// J_1 .. J_{RNG_SPLIT-1}, on the host (out: RNG_JUMP_WORDS words)
+inline void rng_jump_tables(uint64_t* out)
+{
+    typedef uint64_t Mat[256][4];
+    static Mat T, P, Q;
+    // T: one step of the state transition (the output function is not needed)
+    for (int j = 0; j < 256; j++) {
+        uint64_t st[4] = {0, 0, 0, 0};
+        st[j >> 6] = (uint64_t)1 << (j & 63);
+        (void)xo_next(st);
+        for (int b = 0; b < 256; b++) {
+            if (j == 0) T[b][0] = T[b][1] = T[b][2] = T[b][3] = 0;
+            if ((st[b >> 6] >> (b & 63)) & 1) T[b][j >> 6] |= (uint64_t)1 << (j & 63);
+        }
+    }
+    // (A B) row b = xor of the rows j of B for the bits j of A's row b
+    auto mul = [](const Mat& A, const Mat& B, Mat& C) {
+        for (int b = 0; b < 256; b++) {
+            uint64_t r[4] = {0, 0, 0, 0};
+            for (int j = 0; j < 256; j++)
+                if ((A[b][j >> 6] >> (j & 63)) & 1)
+                    for (int k = 0; k < 4; k++) r[k] ^= B[j][k];
+            for (int k = 0; k < 4; k++) C[b][k] = r[k];
+        }
+    };
+    // P = T^RNG_PART by squaring (RNG_PART is a power of two)
+    for (int b = 0; b < 256; b++)
+        for (int k = 0; k < 4; k++) P[b][k] = T[b][k];
+    for (int e = 1; e < RNG_PART; e <<= 1) {
+        mul(P, P, Q);
+        for (int b = 0; b < 256; b++)
+            for (int k = 0; k < 4; k++) P[b][k] = Q[b][k];
+    }
+    // J_1 = P, J_w = J_{w-1} P
+    for (int b = 0; b < 256; b++)
+        for (int k = 0; k < 4; k++) Q[b][k] = P[b][k];
+    for (int w = 1; w < RNG_SPLIT; w++) {
+        for (int b = 0; b < 256; b++)
+            for (int k = 0; k < 4; k++) out[((size_t)(w - 1) * 256 + b) * 4 + k] = Q[b][k];
+        if (w + 1 < RNG_SPLIT) {
+            static Mat R;
+            mul(Q, P, R);
+            for (int b = 0; b < 256; b++)
+                for (int k = 0; k < 4; k++) Q[b][k] = R[b][k];
+        }
+    }
+}
+
+// RNG_CHUNK draws from position g0 (state s0) by the RNG_SPLIT waves of one
+// workgroup, wave w the RNG_PART draws from g0 + RNG_PART w, its state J_w s0
+// (lane l computes bits l, l+64, l+128, l+192 of it, one ballot per word).
+// The caller has checked g0 + RNG_CHUNK <= rlow + RB.  All threads.
+__device__ inline void wg_ring_fill4(const Cell& C, Ctl* ctl, const uint64_t* s0, int64_t g0)
+{
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    uint64_t s[4] = {uniform64(s0[0]), uniform64(s0[1]), uniform64(s0[2]), uniform64(s0[3])};
+    if (w > 0) {
+        const uint64_t* M = C.jump + (size_t)(w - 1) * 256 * 4;
+        uint64_t o[4];
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            const uint64_t* row = M + (size_t)(64 * k + lane) * 4;
+            const uint64_t x = (row[0] & s[0]) ^ (row[1] & s[1]) ^ (row[2] & s[2]) ^ (row[3] & s[3]);
+            o[k] = __ballot(__popcll(x) & 1);
+        }
+#pragma unroll
+        for (int k = 0; k < 4; k++) s[k] = uniform64(o[k]);
+    }
+    const int64_t gb = g0 + (int64_t)RNG_PART * w;
+    for (int grp = 0; grp < RNG_PART / 64; grp++) {
+        uint64_t mine = 0;
+        for (int k = 0; k < 64; k++) {
+            const uint64_t r = xo_next(s);
+            if (lane == k) mine = r;
+        }
+        C.rbuf[(gb + 64 * grp + lane) & (RB - 1)] = mine;
+    }
+    if (w == RNG_SPLIT - 1 && lane == 0) {
+        ctl->rst[0] = s[0]; ctl->rst[1] = s[1]; ctl->rst[2] = s[2]; ctl->rst[3] = s[3];
+        ctl->rgen = g0 + RNG_CHUNK;
+    }
 }
 
 __device__ __forceinline__ int64_t clk() { return (int64_t)wall_clock64(); }

@@ -291,16 +291,21 @@ __device__ __forceinline__ void wg_begin(const Cell& C, Ctl* ctl, NarrowSh& sh)
         __syncthreads();
         return;
     }
-    // digits of u, one position per lane
-    uint64_t part = 0;
-    if (threadIdx.x < C.n) {
-        const int j = threadIdx.x;
-        int64_t d;
-        if (C.N <= 0xFFFFFFFFLL) d = (int64_t)(((uint32_t)u / (uint32_t)C.pw[j]) % (uint32_t)C.q);
-        else d = (u / C.pw[j]) % C.q;
-        part = (uint64_t)d << (4 * j);
+    // digits of u, one position per lane of wave 0, summed in the wave
+    if (threadIdx.x < 64) {
+        uint64_t part = 0;
+        if (threadIdx.x < C.n) {
+            const int j = threadIdx.x;
+            int64_t d;
+            if (C.N <= 0xFFFFFFFFLL) d = (int64_t)(((uint32_t)u / (uint32_t)C.pw[j]) % (uint32_t)C.q);
+            else d = (u / C.pw[j]) % C.q;
+            part = (uint64_t)d << (4 * j);
+        }
+        part = (uint64_t)wave_sum64((int64_t)part);
+        if (threadIdx.x == 0) sh.v64[4] = (int64_t)part;
     }
-    const uint64_t upk = (uint64_t)wg_sum((int64_t)part, sh.red);
+    __syncthreads();
+    const uint64_t upk = (uint64_t)sh.v64[4];
 
     // One pass over the codewords, SCAN_ILP loads in flight per thread: the
     // minimum distance to u, and the codewords at distance R+1 as one bit
@@ -345,31 +350,41 @@ __device__ __forceinline__ void wg_begin(const Cell& C, Ctl* ctl, NarrowSh& sh)
     };
     if (one_round) load(0);
     mark_pass(C.R + 1, true);
-    dmin = wg_min(dmin, sh.red);
-    const int dtar = (dmin <= C.R + 1) ? C.R + 1 : (int)dmin;
-    if (dtar != C.R + 1) {
-        mark_pass(dtar, false);
-        __syncthreads();
-    }
+    __syncthreads();
     // a scan of the mask words' counts gives each word its first slot
     constexpr int QPT = (MAX_M_WORDS + WG - 1) / WG;   // mask words per thread
-    int cnt = 0;
-#pragma unroll
-    for (int k = 0; k < QPT; k++) {
-        const int j = QPT * threadIdx.x + k;
-        if (j < nwm) cnt += __popcll(sh.qm[j]);
-    }
     int64_t nq;
-    int run = (int)(wg_incl_sum(cnt, sh.scan, &nq) - cnt);
+    auto slots = [&]() {
+        int cnt = 0;
 #pragma unroll
-    for (int k = 0; k < QPT; k++) {
-        const int j = QPT * threadIdx.x + k;
-        if (j < nwm) {
-            sh.qfirst[j] = run;
-            run += __popcll(sh.qm[j]);
+        for (int k = 0; k < QPT; k++) {
+            const int j = QPT * threadIdx.x + k;
+            if (j < nwm) cnt += __popcll(sh.qm[j]);
+        }
+        int run = (int)(wg_incl_sum(cnt, sh.scan, &nq) - cnt);
+#pragma unroll
+        for (int k = 0; k < QPT; k++) {
+            const int j = QPT * threadIdx.x + k;
+            if (j < nwm) {
+                sh.qfirst[j] = run;
+                run += __popcll(sh.qm[j]);
+            }
+        }
+        __syncthreads();
+    };
+    slots();
+    // u is uncovered, so no codeword is nearer than R+1: with one at R+1 the
+    // target is R+1, else the minimum distance (its own pass)
+    int dtar = C.R + 1;
+    if (nq == 0) {
+        dmin = wg_min(dmin, sh.red);
+        dtar = (dmin <= C.R + 1) ? C.R + 1 : (int)dmin;
+        if (dtar != C.R + 1) {
+            mark_pass(dtar, false);
+            __syncthreads();
+            slots();
         }
     }
-    __syncthreads();
     // codeword i takes slot first(i / 64) + its rank in its mask word: index order
     auto emit = [&](int i, uint64_t mine) {
         const int64_t slot = sh.qfirst[i >> 6] + __popcll(sh.qm[i >> 6] & ((1ull << (i & 63)) - 1));
@@ -832,18 +847,40 @@ __device__ __forceinline__ void eval_body(const Cell& C, int bid, int nblk)
     const int wv = threadIdx.x >> 6;
     const int64_t th0 = clk();
     if (bid == 0 && threadIdx.x == 0 && ctl->t_cme) ctl->tph[10] += th0 - ctl->t_cme;
-    // workgroup 0: the shuffle; wave 0 of workgroup 1 (or of 0 on a one-group
-    // grid, after the shuffle): the rng refill, keeping ~4096 draws ahead
+    // workgroup 0: the shuffle.  The rng refill, keeping ~4096 draws ahead:
+    // on three or more groups all of workgroup 1, RNG_CHUNK draws at once by
+    // its four waves; else wave 0 of workgroup 1 (or of 0 on a one-group grid,
+    // after the shuffle), one draw after another.
+    static_assert(RNG_SPLIT == WAVES, "one wave per rng part");
     const int rng_wg = nblk > 1 ? 1 : 0;
+    const bool rng4 = nblk >= 3;
     if (bid == 0) {
         wg_shuffle(C, ctl, shs);
         if (threadIdx.x == 0) ctl->tph[5] += clk() - th0;
     }
-    if (bid == rng_wg && wv == 0) {
+    if (bid == rng_wg && rng4) {
+        __shared__ int64_t rsh[6];
+        if (threadIdx.x == 0) {
+            const int64_t g0 = ctl->rgen;
+            rsh[0] = g0 < ctl->ruse + 4096 && ctl->budget >= RNG_CHUNK && g0 + RNG_CHUNK <= ctl->rlow + RB;
+            rsh[1] = g0;
+            for (int k = 0; k < 4; k++) rsh[2 + k] = (int64_t)ctl->rst[k];
+        }
+        __syncthreads();
+        if (rsh[0]) {
+            wg_ring_fill4(C, ctl, reinterpret_cast<const uint64_t*>(&rsh[2]), rsh[1]);
+        } else if (wv == 0) {
+            if (ctl->rgen < ctl->ruse + 4096) wave_ring_fill(C, ctl, ctl->ruse + 4096, ctl->budget);
+        }
+        if (threadIdx.x == 0) ctl->tph[6] += clk() - th0;
+    } else if (bid == rng_wg && wv == 0) {
         if (ctl->rgen < ctl->ruse + 4096) wave_ring_fill(C, ctl, ctl->ruse + 4096, ctl->budget);
         if (threadIdx.x == 0) ctl->tph[6] += clk() - th0;
     }
-    if (bid != 0) {
+    if (rng4) {
+        // units over the waves of workgroups >= 2
+        if (bid >= 2) eval_units(C, ctl, (int64_t)(bid - 2) * WAVES + wv, (int64_t)(nblk - 2) * WAVES, wtab[wv]);
+    } else if (bid != 0) {
         // units over all waves of workgroups >= 1, except the rng wave
         const int64_t wid = (int64_t)(bid - 1) * WAVES + wv - (rng_wg == 1 ? 1 : 0);
         const int64_t nw = (int64_t)(nblk - 1) * WAVES - 1;

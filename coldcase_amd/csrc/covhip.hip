@@ -143,7 +143,7 @@ int shape_of(int q, int n, int R, int M, Shape* s)
 // device bytes of each array, in allocation order
 struct Sizes {
     int64_t cnt, plane, blk, code, cpk, cidx, tabu, fsub, psub, bsub, ssub, ul_idx, ul_pk,
-        shuf, acc, slot_cw, slot_pos, rbuf, ctl, cell, small, arrive, dirty;
+        shuf, acc, slot_cw, slot_pos, rbuf, ctl, cell, small, arrive, dirty, jump;
 };
 
 Sizes sizes_of(const Shape& s)
@@ -172,6 +172,7 @@ Sizes sizes_of(const Shape& s)
     z.small = 64 * 8;
     z.arrive = 2 * 32 * 4;
     z.dirty = DIRTY_CAP * 4;
+    z.jump = RNG_JUMP_WORDS * 8;
     return z;
 }
 
@@ -181,7 +182,7 @@ int64_t total_bytes(const Shape& s)
     // plus the distance array and codeword bitmap the transform allocates on first use
     return z.cnt + 3 * z.plane + z.blk + 2 * z.code + z.cpk + z.cidx + z.tabu + z.fsub + z.psub +
            z.bsub + z.ssub + z.ul_idx + z.ul_pk + z.shuf + z.acc + z.slot_cw + z.slot_pos + z.rbuf +
-           z.ctl + z.cell + z.small + z.arrive + z.dirty + (s.N + 64) + ((s.N >> 5) + 2) * 4;
+           z.ctl + z.cell + z.small + z.arrive + z.dirty + z.jump + (s.N + 64) + ((s.N >> 5) + 2) * 4;
 }
 
 template <typename T>
@@ -527,6 +528,7 @@ int cov_create(int q, int n, int R, int M, cov_cell_t** out)
     AL(h->d_small, z.small);
     AL(C.arrive, z.arrive);
     AL(C.dirty, z.dirty);
+    AL(C.jump, z.jump);
 #undef AL
     C.scratch64 = (decltype(C.scratch64))h->d_small;
 
@@ -575,6 +577,15 @@ int cov_create(int q, int n, int R, int M, cov_cell_t** out)
             ss.insert(ss.end(), t.begin(), t.end());
             sacc += (int64_t)t.size() * pr;
         }
+    }
+    {
+        // the rng jump matrices, computed once per process
+        static const std::vector<uint64_t> jt = [] {
+            std::vector<uint64_t> v(RNG_JUMP_WORDS);
+            rng_jump_tables(v.data());
+            return v;
+        }();
+        if ((rc = upload(h, C.jump, jt.data(), (int64_t)jt.size() * 8)) != COV_OK) return fail(rc);
     }
     if ((rc = upload(h, C.fsub, fs.data(), (int64_t)fs.size() * 4)) != COV_OK) return fail(rc);
     if (!ps.empty() && (rc = upload(h, C.psub, ps.data(), (int64_t)ps.size() * 4)) != COV_OK) return fail(rc);
