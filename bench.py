@@ -133,7 +133,7 @@ def bench_transform(a):
                    'hist': [int(x) for x in hist], 'radius': rad, 'covered': within,
                    'uncovered': N - within},
         'roofline': {'bound': 'hbm', 'achieved': round(ach, 1), 'peak': HBM_PEAK_GBS, 'unit': 'GB/s',
-                     'frac': round(ach / HBM_PEAK_GBS, 4), 'traffic': None, 'kernel': 'k_dt_pass+k_dt_col',
+                     'frac': round(ach / HBM_PEAK_GBS, 4), 'traffic': None, 'kernel': 'k_dt_seed+k_dt_col',
                      'avg_transform_us': round(dt_us, 2), 'alg_bytes_per_transform': alg_t,
                      'with_histogram_us': round(dt_hist_us, 2)},
         'recount': {'avg_us': round(rc_us, 2), 'wall_ms': round(el_r * 1e3, 4), 'alg_bytes': alg_r,

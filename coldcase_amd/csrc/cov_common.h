@@ -37,6 +37,10 @@ constexpr int LIST_CHUNK = 512;      // uncovered-list entries per evaluator uni
 constexpr int64_t DSEL_RANDOM = 1LL << 60;
 constexpr int TRW = 16;
 constexpr int DIRTY_CAP = 4096;      // logged codeword changes before a full best copy
+// Workgroup arrival: up to ARR_G sub-counters per kernel, each on its own
+// 256-byte line (atomics to one line are serialised where they are performed)
+constexpr int ARR_G = 32;
+constexpr int ARR_STRIDE = 64;       // int32 slots between sub-counters
 constexpr uint64_t NIB1 = 0x1111111111111111ULL;
 
 // Device-resident search state.  Scalars written by the narrow phases (one
@@ -62,7 +66,7 @@ struct alignas(16) Ctl {
     int32_t ul_valid, ul_ovf;
     int32_t dtar, ncand, take, nq, use_list;
     int32_t bi, bp, bv, oldv, nties, stamp, ncw;
-    int32_t evdone, cmdone;
+    int32_t cmdone;
     int32_t ndirty;         // codewords logged in dirty since best was last made equal to code
     // wall_clock64 ticks spent per phase (diagnostics): 0 select, 1 post,
     // 2 begin, 3 pick, 4 list upkeep, 5 shuffle helper, 6 rng helper, 7 kick,

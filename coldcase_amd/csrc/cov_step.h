@@ -444,6 +444,41 @@ __device__ __forceinline__ void wg_begin(const Cell& C, Ctl* ctl, NarrowSh& sh)
 }
 
 // ------------------------------------------------------- evaluator units
+// Set bits of `plane` over the P words of the sphere of (base, pk) on subset
+// sub, summed over the wave.  K loads per lane are issued before the first is
+// used, so a sphere of up to 64 K words costs one memory round trip.
+template <int K>
+__device__ __forceinline__ int64_t sphere_count_k(const Cell& C, const uint32_t* __restrict__ plane, int64_t base,
+                                                  uint64_t pk, uint32_t sub, const WalkTab& tb, bool tab)
+{
+    const int lane = threadIdx.x & 63;
+    int64_t c = 0;
+    for (int64_t t0 = 0; t0 < C.P; t0 += 64 * K) {
+        uint32_t b[K];
+#pragma unroll
+        for (int k = 0; k < K; k++) {
+            const int64_t t = t0 + k * 64 + lane;
+            b[k] = 0;
+            if (t < C.P)
+                b[k] = bit_of(plane, tab ? walk_word(C, tb, base, (uint32_t)t)
+                                         : pattern_word(C, base, pk, sub, C.R, (uint32_t)t, nullptr));
+        }
+#pragma unroll
+        for (int k = 0; k < K; k++) c += __popcll(__ballot(b[k]));
+    }
+    return c;
+}
+__device__ __forceinline__ int64_t sphere_count(const Cell& C, const uint32_t* __restrict__ plane, int64_t base,
+                                                uint64_t pk, uint32_t sub, WalkTab& tb)
+{
+    const bool tab = C.wok != 0;
+    if (tab) walk_tables(C, pk, sub, tb);
+    const int64_t c = C.P > 256 ? sphere_count_k<16>(C, plane, base, pk, sub, tb, tab)
+                                : sphere_count_k<4>(C, plane, base, pk, sub, tb, tab);
+    if (tab) walk_release();
+    return c;
+}
+
 // leaving side: words of the distance-R sphere of codeword slot on subset f;
 // four independent gathers in flight per lane
 __device__ __forceinline__ void unit_leave(const Cell& C, int slot, int f, WalkTab& tb)
@@ -453,23 +488,7 @@ __device__ __forceinline__ void unit_leave(const Cell& C, int slot, int f, WalkT
     const int64_t base = C.cidx[i];
     const uint64_t pk = C.cpk[i];
     const uint32_t sub = C.fsub[f];
-    const bool tab = C.wok != 0;
-    if (tab) walk_tables(C, pk, sub, tb);
-    int64_t c1 = 0;
-    for (int64_t t0 = 0; t0 < C.P; t0 += 256) {
-        uint32_t b[4];
-#pragma unroll
-        for (int k = 0; k < 4; k++) {
-            const int64_t t = t0 + k * 64 + lane;
-            b[k] = 0;
-            if (t < C.P)
-                b[k] = bit_of(C.ob, tab ? walk_word(C, tb, base, (uint32_t)t)
-                                        : pattern_word(C, base, pk, sub, C.R, (uint32_t)t, nullptr));
-        }
-#pragma unroll
-        for (int k = 0; k < 4; k++) c1 += __popcll(__ballot(b[k]));
-    }
-    if (tab) walk_release();
+    const int64_t c1 = sphere_count(C, C.ob, base, pk, sub, tb);
     if (lane == 0 && c1) {
         int32_t* a = C.acc + (int64_t)slot * ACCW;
         atomicAdd(&a[0], (int)c1);
@@ -488,23 +507,7 @@ __device__ __forceinline__ void unit_walk(const Cell& C, int slot, int p, int up
     const int64_t base = C.cidx[i] + (int64_t)(up - cp) * C.pw[p];
     const uint64_t pk = nib_put(pk0, p, up);
     const uint32_t sub = C.psub[(int64_t)p * C.E + e];
-    const bool tab = C.wok != 0;
-    if (tab) walk_tables(C, pk, sub, tb);
-    int64_t c0 = 0;
-    for (int64_t t0 = 0; t0 < C.P; t0 += 256) {
-        uint32_t b[4];
-#pragma unroll
-        for (int k = 0; k < 4; k++) {
-            const int64_t t = t0 + k * 64 + lane;
-            b[k] = 0;
-            if (t < C.P)
-                b[k] = bit_of(C.zb, tab ? walk_word(C, tb, base, (uint32_t)t)
-                                        : pattern_word(C, base, pk, sub, C.R, (uint32_t)t, nullptr));
-        }
-#pragma unroll
-        for (int k = 0; k < 4; k++) c0 += __popcll(__ballot(b[k]));
-    }
-    if (tab) walk_release();
+    const int64_t c0 = sphere_count(C, C.zb, base, pk, sub, tb);
     if (lane == 0 && c0) atomicAdd(&C.acc[(int64_t)slot * ACCW + 1 + MAXN + p], (int)c0);
 }
 
@@ -801,13 +804,12 @@ __device__ __forceinline__ void wg_select(const Cell& C, Ctl* ctl, SelSh& sh)
 }
 
 // Arrival of the nblk workgroups of one kernel, in two levels, so that the
-// arrivals spread over ARR_G addresses of the L2 instead of queueing on one:
-// workgroup bid releases onto sub[bid % G] (G = min(nblk, ARR_G)); the
+// arrivals spread over ARR_G lines instead of queueing on one: workgroup bid
+// releases onto sub-counter bid % G (G = min(nblk, ARR_G)); the
 // workgroup that completes a sub-counter acquires it, clears it for the next
 // launch and releases onto `top`; the one that completes `top` (G arrivals)
 // is the last, acquires, and returns true.  Every workgroup's writes before
 // the call happen before the last one's reads after it.  All threads.
-constexpr int ARR_G = 32;
 __device__ __forceinline__ bool wg_arrive_last(int32_t* sub, int32_t* top, int* flag, int bid, int nblk)
 {
     __syncthreads();
@@ -816,10 +818,11 @@ __device__ __forceinline__ bool wg_arrive_last(int32_t* sub, int32_t* top, int* 
         const int g = bid % G;
         const int gsize = nblk / G + (g < nblk % G ? 1 : 0);
         int last = 0;
-        const int prev = __hip_atomic_fetch_add(&sub[g], 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+        int32_t* sg = sub + (int64_t)g * ARR_STRIDE;
+        const int prev = __hip_atomic_fetch_add(sg, 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
         if (prev == gsize - 1) {
             __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-            __hip_atomic_store(&sub[g], 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            __hip_atomic_store(sg, 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             const int pt = __hip_atomic_fetch_add(top, 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
             last = (pt == G - 1);
         }
@@ -840,9 +843,7 @@ __device__ __forceinline__ void eval_body(const Cell& C, int bid, int nblk)
 {
     Ctl* ctl = C.ctl;
     __shared__ ShufSh shs;
-    __shared__ SelSh sel;
     __shared__ WalkTab wtab[WAVES];
-    __shared__ int last;
     if (!ctl->begun || ctl->done || ctl->selected) return;
     const int wv = threadIdx.x >> 6;
     const int64_t th0 = clk();
@@ -888,14 +889,22 @@ __device__ __forceinline__ void eval_body(const Cell& C, int bid, int nblk)
     } else if (nblk == 1) {
         eval_units(C, ctl, wv, WAVES, wtab[wv]);
     }
-    if (!wg_arrive_last(C.arrive, &ctl->evdone, &last, bid, nblk)) return;
+}
+
+// The selection of one cell, one workgroup, launched after the evaluator: the
+// kernel boundary orders every evaluator write before it, so the evaluator's
+// workgroups end without a device-scope release of their own.
+__device__ __forceinline__ void select_body(const Cell& C)
+{
+    Ctl* ctl = C.ctl;
+    __shared__ SelSh sel;
+    if (!ctl->begun || ctl->done || ctl->selected) return;
     const int64_t ts0 = clk();
     __shared__ Ctl lc;
     ctl_to_lds(&lc, ctl);
     if (threadIdx.x == 0 && lc.t_cme) lc.tph[8] += ts0 - lc.t_cme;
     wg_select(C, &lc, sel);
     if (threadIdx.x == 0) {
-        lc.evdone = 0;
         const int64_t te = clk();
         lc.tph[0] += te - ts0;
         lc.t_eve = te;
@@ -906,6 +915,11 @@ __device__ __forceinline__ void eval_body(const Cell& C, int bid, int nblk)
 __global__ void __launch_bounds__(WG, 4) k_eval(const Cell C)
 {
     eval_body(C, blockIdx.x, gridDim.x);
+}
+
+__global__ void __launch_bounds__(WG) k_select(const Cell C)
+{
+    select_body(C);
 }
 
 // kick: teleport `kick` codewords near uncovered words (:1553-1578).  All threads.
@@ -1243,7 +1257,7 @@ __device__ __forceinline__ void commit_body(const Cell& C, int bid, int nblk, in
     const int64_t d = commit_units(C, ctl, gt, (int64_t)nblk * WG, ctab);
     const int64_t ds = wg_sum(d, sh.red);
     if (threadIdx.x == 0 && ds) atomicAdd(reinterpret_cast<unsigned long long*>(&ctl->dunc), (unsigned long long)ds);
-    if (!wg_arrive_last(C.arrive + ARR_G, &ctl->cmdone, &last, bid, nblk)) return;
+    if (!wg_arrive_last(C.arrive, &ctl->cmdone, &last, bid, nblk)) return;
     __shared__ Ctl lc;
     ctl_to_lds(&lc, ctl);
     Ctl* L = &lc;

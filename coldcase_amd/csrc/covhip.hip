@@ -170,7 +170,7 @@ Sizes sizes_of(const Shape& s)
     z.ctl = sizeof(Ctl);
     z.cell = sizeof(Cell);
     z.small = 64 * 8;
-    z.arrive = 2 * 32 * 4;
+    z.arrive = ARR_G * ARR_STRIDE * 4;
     z.dirty = DIRTY_CAP * 4;
     z.jump = RNG_JUMP_WORDS * 8;
     return z;
@@ -369,6 +369,8 @@ int launch_step(cov_cell* h)
     hipLaunchKernelGGL(k_eval, dim3(h->eval_groups), dim3(WG), 0, h->stream, h->h);
     HCK(h, hipGetLastError());
     RC(timed_end(h, 0, a));
+    hipLaunchKernelGGL(k_select, dim3(1), dim3(WG), 0, h->stream, h->h);
+    HCK(h, hipGetLastError());
     RC(timed_begin(h, 1, &a));
     hipLaunchKernelGGL(k_commit, dim3(h->commit_groups), dim3(WG), 0, h->stream, h->h, 0);
     HCK(h, hipGetLastError());
@@ -384,6 +386,7 @@ int ensure_graph(cov_cell* h)
     HCK(h, hipStreamBeginCapture(h->stream, hipStreamCaptureModeRelaxed));
     for (int s = 0; s < GRAPH_STEPS; s++) {
         hipLaunchKernelGGL(k_eval, dim3(h->eval_groups), dim3(WG), 0, h->stream, h->h);
+        hipLaunchKernelGGL(k_select, dim3(1), dim3(WG), 0, h->stream, h->h);
         hipLaunchKernelGGL(k_commit, dim3(h->commit_groups), dim3(WG), 0, h->stream, h->h, 0);
     }
     HCK(h, hipStreamEndCapture(h->stream, &g));
@@ -1269,6 +1272,7 @@ int cov_batch_launch(cov_batch_t* b, int64_t nsteps, int mode)
             HCK(b, hipStreamBeginCapture(b->stream, hipStreamCaptureModeRelaxed));
             for (int st = 0; st < GRAPH_STEPS; st++) {
                 hipLaunchKernelGGL(k_eval_b, dim3(b->eval_total), dim3(WG), 0, b->stream, b->d_cells, b->d_eoff, k);
+                hipLaunchKernelGGL(k_select_b, dim3(k), dim3(WG), 0, b->stream, b->d_cells);
                 hipLaunchKernelGGL(k_commit_b, dim3(b->commit_total), dim3(WG), 0, b->stream, b->d_cells, b->d_coff,
                                    k);
             }
@@ -1284,6 +1288,7 @@ int cov_batch_launch(cov_batch_t* b, int64_t nsteps, int mode)
     }
     for (; left > 0; left--) {
         hipLaunchKernelGGL(k_eval_b, dim3(b->eval_total), dim3(WG), 0, b->stream, b->d_cells, b->d_eoff, k);
+        hipLaunchKernelGGL(k_select_b, dim3(k), dim3(WG), 0, b->stream, b->d_cells);
         hipLaunchKernelGGL(k_commit_b, dim3(b->commit_total), dim3(WG), 0, b->stream, b->d_cells, b->d_coff, k);
         HCK(b, hipGetLastError());
     }
