@@ -31,6 +31,13 @@ Secondary workloads (one JSON line each, not the headline):
   --workload serve       Poisson arrivals (seed 7, 20 req/s) of verify and
                          10^5-move search requests over cells up to K6(10,4),
                          continuous batching with HBM-budget admission
+  --workload copy        a plain 256 MiB device-to-device copy: the reference
+                         kernel whose rocprofv3 FETCH_SIZE / WRITE_SIZE
+                         readings scale those counters to bytes
+
+The roofline's `traffic` (HBM bytes per launch of the dominant kernel) is
+read from the counter summary under profiles/ when one is present for the
+workload (a line "traffic_bytes: <bytes>"), else null.
 """
 import argparse
 import json
@@ -45,6 +52,36 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
 Q, N_, R_, M_, SEED, TENURE = 6, 9, 4, 660, 1, 10
+
+
+def measured_traffic(name):
+    """HBM bytes per launch from a committed rocprofv3 counter summary, or None."""
+    p = os.path.join(ROOT, 'profiles', name)
+    if not os.path.exists(p):
+        return None
+    m = re.search(r'traffic_bytes:\s*(\d+)', open(p).read())
+    return int(m.group(1)) if m else None
+
+
+def bench_copy(a):
+    """One device-to-device copy of 256 MiB, four times (torch copy_)."""
+    import torch
+    torch.cuda.set_device(0)
+    nbytes = 256 << 20
+    src = torch.ones(nbytes, dtype=torch.uint8, device='cuda')
+    dst = torch.empty(nbytes, dtype=torch.uint8, device='cuda')
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(4):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    el = (time.perf_counter() - t0) / 4
+    print(json.dumps({'metric': 'device copy (counter calibration)', 'value': round(2 * nbytes / el / 1e9, 1),
+                      'unit': 'GB/s', 'n_gpus': 1, 'steps': 4, 'warmup': 0, 'ms_per_step': round(el * 1e3, 4),
+                      'higher_is_better': True, 'scaling': 'weak', 'vs_baseline': None, 'dtype': 'u8',
+                      'data': 'synthetic', 'config': {'workload': 'copy of %d bytes' % nbytes,
+                                                      'read_bytes': nbytes, 'write_bytes': nbytes}}))
+    return 0
 
 
 def cpu_baseline(cores, sample_iters):
@@ -133,7 +170,8 @@ def bench_transform(a):
                    'hist': [int(x) for x in hist], 'radius': rad, 'covered': within,
                    'uncovered': N - within},
         'roofline': {'bound': 'hbm', 'achieved': round(ach, 1), 'peak': HBM_PEAK_GBS, 'unit': 'GB/s',
-                     'frac': round(ach / HBM_PEAK_GBS, 4), 'traffic': None, 'kernel': 'k_dt_seed+k_dt_col',
+                     'frac': round(ach / HBM_PEAK_GBS, 4),
+                     'traffic': measured_traffic('pmc_k_dt_seed_k_dt_col_K7_9_4.md'), 'kernel': 'k_dt_seed+k_dt_col',
                      'avg_transform_us': round(dt_us, 2), 'alg_bytes_per_transform': alg_t,
                      'with_histogram_us': round(dt_hist_us, 2)},
         'recount': {'avg_us': round(rc_us, 2), 'wall_ms': round(el_r * 1e3, 4), 'alg_bytes': alg_r,
@@ -339,7 +377,7 @@ def main():
     ap.add_argument('--plain-steps', type=int, default=20000)
     ap.add_argument('--cpu-iters', type=int, default=4000)
     ap.add_argument('--no-cpu', action='store_true')
-    ap.add_argument('--workload', choices=['search', 'transform', 'batched', 'serve'], default='search')
+    ap.add_argument('--workload', choices=['search', 'transform', 'batched', 'serve', 'copy'], default='search')
     ap.add_argument('--lds', type=int, default=0, help='transform tile LDS bytes (0 = default)')
     ap.add_argument('--threads', type=int, default=0, help='transform workgroup threads (0 = default)')
     ap.add_argument('--prefix', type=int, default=1000, help='batched: steps timed in both modes')
@@ -353,6 +391,8 @@ def main():
               'path shards by independent cells, not within one (--gpus %d refused)' % a.gpus,
               file=sys.stderr)
         return 2
+    if a.workload == 'copy':
+        return bench_copy(a)
     if a.workload == 'transform':
         if a.steps == ap.get_default('steps'):
             a.steps = 20
@@ -414,7 +454,7 @@ def main():
     bytes_per_launch = (e1 - e0) / max(n_eval, 1)
     achieved = bytes_per_launch / avg_s / 1e9 if avg_s > 0 else 0.0
     roof = {'bound': 'hbm', 'achieved': round(achieved, 1), 'peak': HBM_PEAK_GBS, 'unit': 'GB/s',
-            'frac': round(achieved / HBM_PEAK_GBS, 4), 'traffic': None,
+            'frac': round(achieved / HBM_PEAK_GBS, 4), 'traffic': measured_traffic('pmc_k_eval_K6_9_4.md'),
             'kernel': 'k_eval', 'avg_launch_us': round(avg_s * 1e6, 2),
             'alg_bytes_per_launch': int(bytes_per_launch),
             'commit_avg_us': round(ms[1] / 1e3 / max(int(launches[1]), 1) * 1e6, 2)}

@@ -194,10 +194,12 @@ int dalloc(cov_cell* h, T** p, int64_t bytes)
         h->err = std::string("hipMalloc: ") + hipGetErrorString(e);
         return COV_ENOMEM;
     }
-    e = hipMemset(v, 0, (size_t)bytes);
+    // zeroed on the cell's own stream: a fill on the null stream would not be
+    // ordered before the cell's later work on its non-blocking stream
+    e = hipMemsetAsync(v, 0, (size_t)bytes, h->stream);
     if (e != hipSuccess) {
         (void)hipFree(v);
-        h->err = std::string("hipMemset: ") + hipGetErrorString(e);
+        h->err = std::string("hipMemsetAsync: ") + hipGetErrorString(e);
         return -(100 + (int)e);
     }
     h->allocs.push_back(v);
@@ -326,11 +328,10 @@ int do_recount(cov_cell* h)
     hipLaunchKernelGGL(k_recount, dim3(g), dim3(WG), (size_t)h->plan.T * 4, h->stream, h->h, h->plan);
     HCK(h, hipGetLastError());
     RC(timed_end(h, 3, a));
-    int64_t zero = 0;
-    HCK(h, hipMemcpyAsync(CTL_PTR(h, unc), &zero, 8, hipMemcpyHostToDevice, h->stream));
-    HCK(h, hipMemcpyAsync(CTL_PTR(h, ul_n), &zero, 8, hipMemcpyHostToDevice, h->stream));
-    int32_t zi = 0;
-    HCK(h, hipMemcpyAsync(CTL_PTR(h, ul_valid), &zi, 4, hipMemcpyHostToDevice, h->stream));
+    // (fills, not copies from this frame: the call returns before they run)
+    HCK(h, hipMemsetAsync(CTL_PTR(h, unc), 0, 8, h->stream));
+    HCK(h, hipMemsetAsync(CTL_PTR(h, ul_n), 0, 8, h->stream));
+    HCK(h, hipMemsetAsync(CTL_PTR(h, ul_valid), 0, 4, h->stream));
     hipLaunchKernelGGL(k_derive, dim3(grid_for(h->h.nzw, WG, 1, 4096)), dim3(WG), 0, h->stream, h->h);
     HCK(h, hipGetLastError());
     return COV_OK;
@@ -338,8 +339,7 @@ int do_recount(cov_cell* h)
 
 int do_derive(cov_cell* h)
 {
-    int64_t zero = 0;
-    HCK(h, hipMemcpyAsync(CTL_PTR(h, unc), &zero, 8, hipMemcpyHostToDevice, h->stream));
+    HCK(h, hipMemsetAsync(CTL_PTR(h, unc), 0, 8, h->stream));
     hipLaunchKernelGGL(k_derive, dim3(grid_for(h->h.nzw, WG, 1, 4096)), dim3(WG), 0, h->stream, h->h);
     HCK(h, hipGetLastError());
     return COV_OK;
@@ -347,10 +347,8 @@ int do_derive(cov_cell* h)
 
 int list_rebuild(cov_cell* h)
 {
-    int64_t zero = 0;
-    int32_t zi = 0;
-    HCK(h, hipMemcpyAsync(CTL_PTR(h, ul_n), &zero, 8, hipMemcpyHostToDevice, h->stream));
-    HCK(h, hipMemcpyAsync(CTL_PTR(h, ul_ovf), &zi, 4, hipMemcpyHostToDevice, h->stream));
+    HCK(h, hipMemsetAsync(CTL_PTR(h, ul_n), 0, 8, h->stream));
+    HCK(h, hipMemsetAsync(CTL_PTR(h, ul_ovf), 0, 4, h->stream));
     hipLaunchKernelGGL(k_list_rebuild, dim3(grid_for(h->h.nzw, WG, 1, 4096)), dim3(WG), 0, h->stream, h->h);
     HCK(h, hipGetLastError());
     Ctl c;
@@ -613,6 +611,11 @@ int cov_create(int q, int n, int R, int M, cov_cell_t** out)
                                std::max<int64_t>(1, s.P / 64);
     h->eval_groups = grid_for(eval_waves / 4, 8, 16, 1024);
     h->commit_groups = grid_for((int64_t)s.E * s.P, WG, 1, 2048);
+    // the uploads above read this frame's tables
+    {
+        const hipError_t e = hipStreamSynchronize(h->stream);
+        if (e != hipSuccess) return fail(-(100 + (int)e));
+    }
     *out = h;
     return COV_OK;
 }
@@ -1071,6 +1074,7 @@ int64_t cov_search_run(cov_cell_t* h, int64_t nsteps, int mode, int64_t* trace)
     // the trace buffer and its capacity live in the control block, so captured
     // graphs stay valid whether or not a run records a trace
     if (trace && h->tcap < nsteps) {
+        HCK(h, hipStreamSynchronize(h->stream));    // a launched run may still write the old buffer
         if (h->trace) (void)hipFree(h->trace);
         h->trace = nullptr;
         h->tcap = 0;
@@ -1225,9 +1229,11 @@ int cov_batch_create(cov_cell_t* const* cells, int k, cov_batch_t** out)
     e = hipMalloc(&v, 4 * (k + 1));
     if (e != hipSuccess) return fail(COV_ENOMEM);
     b->d_coff = (int32_t*)v;
-    if ((e = hipMemcpy(b->d_cells, tab.data(), sizeof(Cell) * k, hipMemcpyHostToDevice)) != hipSuccess ||
-        (e = hipMemcpy(b->d_eoff, eo.data(), 4 * (k + 1), hipMemcpyHostToDevice)) != hipSuccess ||
-        (e = hipMemcpy(b->d_coff, co.data(), 4 * (k + 1), hipMemcpyHostToDevice)) != hipSuccess)
+    if ((e = hipMemcpyAsync(b->d_cells, tab.data(), sizeof(Cell) * k, hipMemcpyHostToDevice, b->stream)) !=
+            hipSuccess ||
+        (e = hipMemcpyAsync(b->d_eoff, eo.data(), 4 * (k + 1), hipMemcpyHostToDevice, b->stream)) != hipSuccess ||
+        (e = hipMemcpyAsync(b->d_coff, co.data(), 4 * (k + 1), hipMemcpyHostToDevice, b->stream)) != hipSuccess ||
+        (e = hipStreamSynchronize(b->stream)) != hipSuccess)
         return fail(-(100 + (int)e));
     *out = b;
     return COV_OK;

@@ -140,3 +140,26 @@ def test_best_code_after_long_stalls(spec):
     assert (best == o.best()).all()
     assert (c.code() == o.code()).all()
     c.close()
+
+
+def test_cells_created_while_the_device_is_busy():
+    """Cells created and initialised while a long batch keeps another stream
+    busy: every allocation is zeroed on the new cell's own stream, so its
+    state is the one it has when created on an idle device (a zero fill left
+    on another stream could land after the cell's initialisation)."""
+    from coldcase_amd import Batch
+    print('case create while busy', flush=True)
+    busy = [_cell((6, 9, 4, 660, 11, 10, 5000)), _cell((7, 9, 4, 1480, 12, 10, 5000))]
+    specs = [(5, 7, 2, 400, 21, 10, 5000), (6, 8, 4, 169, 22, 10, 5000), (6, 9, 4, 660, 23, 10, 5000)]
+    with Batch(busy) as b:
+        b.launch(20000)
+        fresh = [_cell(s) for s in specs]
+        assert not b.query(), 'the busy batch ended before the cells were created'
+        for c in fresh:
+            c.run(200, mode=1)
+        b.sync()
+    for spec, c in zip(specs, fresh):
+        assert _snap(c) == _solo(spec, 200), spec
+        c.close()
+    for c in busy:
+        c.close()
