@@ -1,6 +1,6 @@
 // cov_step.h -- the device-resident tabu step (kernel a) and its narrow phases.
 //
-// One search step of cov/engine/covsearch2e.c:1304-1580 is two launches:
+// One search step of cov/engine/covsearch2e.c:1304-1580 is three launches:
 //
 //   k_eval    (wide)  units of work over the candidate codewords:
 //                       leaving side: per (codeword, R-subset of all n
@@ -14,10 +14,11 @@
 //                         and x_p = u_p (:717-753)
 //                     workgroup 0 computes the Fisher-Yates shuffle of the
 //                     candidates meanwhile (the set of codewords to evaluate
-//                     does not depend on the order), one wave of workgroup 1
-//                     refills the rng ring; the last workgroup to finish selects the
-//                     move (tabu, aspiration, reservoir ties) with a parallel
-//                     prefix formulation of the sequential scan (:1503-1518)
+//                     does not depend on the order), workgroup 1 refills the
+//                     rng ring
+//   k_select  (one workgroup)  selects the move (tabu, aspiration, reservoir
+//                     ties) with a parallel prefix formulation of the
+//                     sequential scan (:1503-1518)
 //   k_commit  (wide)  --cnt[a], ++cnt[b=a+dv] over the C(n-1,R)(q-1)^R
 //                     patterns (every word once, no counter atomics, :562-590);
 //                     plane bits, block counts and the uncovered list follow
@@ -30,12 +31,14 @@
 // draws at a cursor in the reference's order, so the trajectory is the
 // reference's bit for bit.
 //
-// Synchronisation: a wide kernel's workgroups publish their results with one
-// agent-scope release atomic each on a counter; only the workgroup that sees
-// the count complete acquires (one L2 invalidate per launch), so workgroups
-// still gathering keep their XCD's L2 warm.  The single-workgroup phases read
-// data that device atomics change (planes, block counts) with agent-scope
-// loads.
+// Synchronisation: kernel boundaries order the evaluator before the selection
+// and the selection before the commit.  Inside the commit, workgroups publish
+// with one agent-scope release atomic each on a counter (a release is an L2
+// write-back on this part, which is why the evaluator, with four to eight
+// times as many workgroups, hands over at a kernel boundary instead); only
+// the workgroup that sees the count complete acquires.  The single-workgroup
+// phases read data that device atomics change (planes, block counts) with
+// agent-scope loads.
 #pragma once
 #include "cov_common.h"
 #include "cov_cover.h"

@@ -16,7 +16,7 @@ cov_mem_bytes of resident cells) and the cell slots allow; a request that
 does not fit waits, one larger than the whole budget is refused.  Admitted
 searches stay resident in one of four lanes by the size of their balls; the
 searches of a lane advance together: every tick is one batched run
-(coldcase_amd.engine.Batch, one evaluator and one commit launch per step for
+(coldcase_amd.engine.Batch, one evaluator, selection and commit launch per step for
 all of them) of as many steps as the closest-to-done search still needs, up
 to `tick_steps`; searches that finish leave and newly admitted ones join at
 the lane's next tick.  The lanes tick independently on their own streams, and

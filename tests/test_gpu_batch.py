@@ -1,4 +1,4 @@
-"""Batched cells (one evaluator and one commit launch per step for all cells)
+"""Batched cells (one evaluator, selection and commit launch per step for all cells)
 against the same cells run alone: every cell's counters, code, best code and
 coverage counts must be identical, whatever the mix of shapes, launch modes,
 run splits, kicks and early covers; one cell is also checked against the
