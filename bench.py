@@ -314,7 +314,8 @@ def bench_serve(a):
             code, _ = parse_code(p, key[0], key[1])
             codes[key] = np.array(code, dtype=np.uint8)
     reqs = poisson_workload(7, a.requests, 20.0, shapes, verify_frac=0.3, moves=a.steps, codes=codes)
-    srv = Server(hbm_budget=int(a.budget_gb * (1 << 30)), max_cells=64, tick_steps=a.tick)
+    srv = Server(hbm_budget=int(a.budget_gb * (1 << 30)), max_cells=64, tick_steps=a.tick,
+                 lane_cap=a.lane_cap or None)
     t0 = time.perf_counter()
     res = srv.run(reqs, realtime=True)
     el = time.perf_counter() - t0
@@ -354,7 +355,7 @@ def bench_serve(a):
            'config': {'workload': 'Poisson arrivals 20 req/s (seed 7), %d requests, 30%% verify / 70%% search '
                                   'bursts of %d moves over K5(7,2)..K6(10,4); record codes as verify inputs'
                                   % (a.requests, a.steps),
-                      'hbm_budget_gb': a.budget_gb, 'tick_steps': a.tick},
+                      'hbm_budget_gb': a.budget_gb, 'tick_steps': a.tick, 'lane_cap': a.lane_cap or None},
            'latency': s, 'makespan_s': round(el, 3),
            'served_rps': round(len(res) / span, 4), 'plain_rps_projected': round(len(res) / plain_total, 4),
            'ratio_served_vs_plain': round(plain_total / span, 3),
@@ -385,6 +386,7 @@ def main():
     ap.add_argument('--requests', type=int, default=60, help='serve: requests in the trace')
     ap.add_argument('--tick', type=int, default=256, help='serve: steps per batched tick at most')
     ap.add_argument('--budget-gb', type=float, default=64.0, help='serve: HBM budget')
+    ap.add_argument('--lane-cap', type=int, default=8, help='serve: searches a lane advances at once (0: all)')
     a = ap.parse_args()
     if a.gpus != 1:
         print('bench.py measures one GPU: every box here has one GPU, and the search '

@@ -161,18 +161,24 @@ class Server:
     tick_steps  steps per batched tick at most
     launch      optional callable(Cell) that sets the cell's launch shape
                 (the autotuner's choice for batched runs)
+    lane_cap    searches a lane advances at once (None: all it holds); the
+                others wait, admitted, in arrival order until a member leaves
 
     Resident searches are split by cost class into lanes (cost_class); each
     lane is a batch on its own stream and ticks at its own pace, so a short
-    search does not wait for the ticks of a long one.
+    search does not wait for the ticks of a long one.  A lane cap trades
+    nothing in results (each search's trajectory is its own) for earlier
+    completions of the first arrivals once a lane's batch fills the device.
     """
 
-    def __init__(self, hbm_budget=64 << 30, max_cells=64, tick_steps=256, mode=MODE_GRAPH, launch=None):
+    def __init__(self, hbm_budget=64 << 30, max_cells=64, tick_steps=256, mode=MODE_GRAPH, launch=None,
+                 lane_cap=None):
         self.budget = int(hbm_budget)
         self.max_cells = int(max_cells)
         self.tick_steps = int(tick_steps)
         self.mode = mode
         self.launch = launch
+        self.lane_cap = None if lane_cap is None else max(1, int(lane_cap))
         self.peak_bytes = 0
         self.peak_cells = 0
         self.ticks = 0
@@ -209,8 +215,10 @@ class Server:
                     if ci in pending:
                         ln.joining.extend(pending.pop(ci))
                     if not ln.busy and ln.joining:
-                        ln.active.extend(ln.joining)
-                        ln.joining = []
+                        room = len(ln.joining) if self.lane_cap is None else self.lane_cap - len(ln.active)
+                        if room > 0:
+                            ln.active.extend(ln.joining[:room])
+                            ln.joining = ln.joining[room:]
                     if not ln.busy and ln.active:
                         cells = tuple(a[1] for a in ln.active)
                         if cells != ln.members:

@@ -22,13 +22,13 @@ def _alone(r):
     return Server(tick_steps=10 ** 9).run([r], realtime=False)[0]
 
 
-@pytest.mark.parametrize('tick', [64, 1000])
-def test_served_equals_alone(tick):
+@pytest.mark.parametrize('tick,cap', [(64, None), (1000, None), (64, 2)])
+def test_served_equals_alone(tick, cap):
     from coldcase_amd.serve import Server, summarize
-    print('case serve mix tick=%d' % tick, flush=True)
+    print('case serve mix tick=%d lane_cap=%s' % (tick, cap), flush=True)
     reqs = _trace()
     assert {r.kind for r in reqs} == {'verify', 'search'}
-    srv = Server(tick_steps=tick)
+    srv = Server(tick_steps=tick, lane_cap=cap)
     res = srv.run(reqs, realtime=True)
     s = summarize(res)
     assert s['errors'] == 0 and s['completed'] == len(reqs)
