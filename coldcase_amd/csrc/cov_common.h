@@ -149,7 +149,7 @@ struct Cell {
     GPTR(uint8_t) slot_pos;      // slot -> differing positions (MAXN each)
     GPTR(uint64_t) rbuf;
     GPTR(int64_t) scratch64;     // small reduction outputs
-    GPTR(int32_t) arrive;        // workgroup arrival sub-counters: evaluator, commit
+    GPTR(int32_t) arrive;        // the commit's workgroup arrival sub-counters
     GPTR(int32_t) dirty;         // codewords changed since best == code (DIRTY_CAP of them)
     GPTR(uint64_t) jump;         // the rng jump matrices J_1 .. J_{RNG_SPLIT-1}
     GPTR(uint8_t) dist;          // distance transform output (N bytes), lazily
