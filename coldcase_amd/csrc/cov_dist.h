@@ -32,6 +32,11 @@
 namespace cov {
 
 constexpr int DT_HB = MAXN + 2;      // histogram bins: 0..n, and unreached in bin n+1
+// the last pass's workgroups add into DT_COPIES copies of the bins, copy
+// blockIdx % DT_COPIES, each copy on its own 256-byte line (atomics to one
+// address from every workgroup would serialise); the host folds the copies
+constexpr int DT_COPIES = 64;
+constexpr int DT_CSTRIDE = 32;       // uint64 per copy
 constexpr int DT_SMAX = 6;           // axes a group of at most 64 words can span (q = 2)
 constexpr uint32_t DT_INF = 0x7Fu;   // "not reached yet"
 constexpr uint32_t DT_INF4 = 0x7F7F7F7Fu;
@@ -480,10 +485,10 @@ __host__ __device__ inline void dt_hist_tile(const DtPass& p, const DtTile& T, i
     }
 }
 
-// the last pass's statistics of a workgroup into the global bins.  Light:
-// [0] #{values <= rcut}, [1] the maximum; full: bin d = #{values == d}, d <= n,
-// and bin n + 1 the unreached.  Wave sums first, then one LDS atomic per wave
-// and bin, then one global atomic per bin.  All threads.
+// the last pass's statistics of a workgroup into its copy of the global bins.
+// Light: [0] #{values <= rcut}, [1] the maximum; full: bin d = #{values == d},
+// d <= n, and bin n + 1 the unreached.  Wave sums first, then one LDS atomic
+// per wave and bin, then one global atomic per bin.  All threads.
 __device__ inline void dt_stats_flush(const DtPass& p, const DtHist& h, const DtStat& st, unsigned int* lh,
                                       unsigned long long* hist)
 {
@@ -512,9 +517,25 @@ __device__ inline void dt_stats_flush(const DtPass& p, const DtHist& h, const Dt
     }
     __syncthreads();
     const int nbin = p.full ? p.hn + 2 : 2;
+    unsigned long long* hc = hist + (size_t)(blockIdx.x % DT_COPIES) * DT_CSTRIDE;
     if ((int)threadIdx.x < nbin && lh[threadIdx.x]) {
-        if (p.full || threadIdx.x == 0) atomicAdd(&hist[threadIdx.x], (unsigned long long)lh[threadIdx.x]);
-        else atomicMax(&hist[1], (unsigned long long)lh[1]);
+        if (p.full || threadIdx.x == 0) atomicAdd(&hc[threadIdx.x], (unsigned long long)lh[threadIdx.x]);
+        else atomicMax(&hc[1], (unsigned long long)lh[1]);
+    }
+}
+
+// the copies folded into hh[DT_HB] (host)
+inline void dt_fold_copies(const int64_t* copies, bool full, int64_t* hh)
+{
+    for (int b = 0; b < DT_HB; b++) hh[b] = 0;
+    for (int c = 0; c < DT_COPIES; c++) {
+        const int64_t* x = copies + (size_t)c * DT_CSTRIDE;
+        if (full) {
+            for (int b = 0; b < DT_HB; b++) hh[b] += x[b];
+        } else {
+            hh[0] += x[0];
+            if (x[1] > hh[1]) hh[1] = x[1];
+        }
     }
 }
 

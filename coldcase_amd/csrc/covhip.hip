@@ -42,6 +42,8 @@ struct cov_cell {
     int64_t tcap = 0;
     // distance transform
     uint32_t* cwb = nullptr;              // codeword bitmap, all zero between calls
+    int64_t* dtstat = nullptr;            // the transform's statistics copies (DT_COPIES x DT_CSTRIDE)
+    std::vector<int64_t> dtcopies;        // their host image
     int dt_lds = 0;                       // LDS bytes per transform tile (0: default)
     int dt_threads = 0;                   // threads per transform workgroup (0: default)
     bool dist_valid = false;              // dist holds the transform of the current code
@@ -630,6 +632,7 @@ int cov_destroy(cov_cell_t* h)
     if (h->trace) (void)hipFree(h->trace);
     if (h->h.dist) (void)hipFree(h->h.dist);
     if (h->cwb) (void)hipFree(h->cwb);
+    if (h->dtstat) (void)hipFree(h->dtstat);
     for (const auto& p : h->pend) { (void)hipEventDestroy(p.a); (void)hipEventDestroy(p.b); }
     for (hipEvent_t e : h->evs) (void)hipEventDestroy(e);
     (void)hipStreamDestroy(h->stream);
@@ -855,8 +858,10 @@ static int run_transform(cov_cell_t* h, int full, int rcut, int64_t hh[DT_HB])
     }
     pass[np - 1].full = full;
     pass[np - 1].rcut = rcut;
-    unsigned long long* dh = reinterpret_cast<unsigned long long*>(h->d_small);
-    HCK(h, hipMemsetAsync(dh, 0, DT_HB * 8, h->stream));
+    const size_t sbytes = (size_t)DT_COPIES * DT_CSTRIDE * 8;
+    if (!h->dtstat) HCK(h, hipMalloc((void**)&h->dtstat, sbytes));
+    unsigned long long* dh = reinterpret_cast<unsigned long long*>(h->dtstat);
+    HCK(h, hipMemsetAsync(dh, 0, sbytes, h->stream));
     const int gm = grid_for(C.M, WG, 1, 64);
     hipLaunchKernelGGL(k_cwb_mark, dim3(gm), dim3(WG), 0, h->stream, h->h, h->cwb, 1);
     HCK(h, hipGetLastError());
@@ -902,7 +907,10 @@ static int run_transform(cov_cell_t* h, int full, int rcut, int64_t hh[DT_HB])
             HCK(h, hipGetLastError());
         }
     }
-    HCK(h, hipMemcpyAsync(hh, dh, DT_HB * 8, hipMemcpyDeviceToHost, h->stream));
+    h->dtcopies.resize((size_t)DT_COPIES * DT_CSTRIDE);
+    HCK(h, hipMemcpyAsync(h->dtcopies.data(), dh, sbytes, hipMemcpyDeviceToHost, h->stream));
+    HCK(h, hipStreamSynchronize(h->stream));
+    dt_fold_copies(h->dtcopies.data(), full != 0, hh);
     return COV_OK;
 }
 
