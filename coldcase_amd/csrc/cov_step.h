@@ -139,31 +139,74 @@ __device__ __forceinline__ void wg_ensure(const Cell& C, Ctl* ctl, int64_t need)
 }
 
 // pick_uncovered: 64 probes, then the cyclic scan (:973-1016).  All threads.
+// The reads go in rounds rather than one after another: first the 65 draws
+// (64 probes and the scan's start); then, together, the probes' plane bits,
+// the rest of the start block and the counts of the 64 blocks after it; then,
+// only when the start block has no uncovered word at or after the start, the
+// words of the first non-empty block among those 64.  Past them, the general
+// scan takes over.  The word returned is the sequential scan's.
+constexpr int PICK_NB = WG - 64 - BLK_WORDS / 32;   // blocks after the start block read at once
+static_assert(PICK_NB > 0, "the pick's second round needs threads for the block counts");
 __device__ __forceinline__ int64_t wg_pick(const Cell& C, Ctl* ctl, NarrowSh& sh)
 {
     wg_ensure(C, ctl, 65);
+    const int tid = threadIdx.x, wv = tid >> 6;
     const int64_t pos = ctl->ruse;
-    if (threadIdx.x < 64) {
-        const uint64_t rv = ring_at(C, pos + threadIdx.x);
+    // round 1: the draws
+    const uint64_t rv = tid < 65 ? ring_at(C, pos + tid) : 0;
+    if (tid == 64) sh.v64[1] = (int64_t)(rv % (uint64_t)C.N);
+    __syncthreads();
+    const int64_t start = sh.v64[1];
+    const int64_t sw = start >> 5, sb = start >> BLK_SHIFT;
+    const int64_t bw0 = sb << (BLK_SHIFT - 5);
+    const int64_t bw1 = (bw0 + (BLK_WORDS / 32) < C.nzw) ? bw0 + (BLK_WORDS / 32) : C.nzw;
+    // round 2
+    int64_t cand_s = INT64_MAX, cand_b = INT64_MAX;
+    if (tid < 64) {
         const int64_t w = (int64_t)(rv % (uint64_t)C.N);
         const bool hit = zbit_acq(C, w) != 0;
         const uint64_t mask = __ballot(hit);
         const int fh = mask ? (__ffsll((unsigned long long)mask) - 1) : -1;
-        if (threadIdx.x == 0) sh.v32[0] = fh;
-        if (fh >= 0 && (int)threadIdx.x == fh) sh.v64[0] = w;
-        if (threadIdx.x == 0 && fh < 0) sh.v64[1] = (int64_t)(ring_at(C, pos + 64) % (uint64_t)C.N);
+        if (tid == 0) sh.v32[0] = fh;
+        if (fh >= 0 && tid == fh) sh.v64[0] = w;
+    } else if (tid < 64 + BLK_WORDS / 32) {
+        const int64_t x = sw + (tid - 64);
+        if (x < bw1) {
+            uint32_t z = ld_acq32(&C.zb[x]);
+            if (x == sw) z &= ~0u << (start & 31);
+            if (z) cand_s = x * 32 + __ffs(z) - 1;
+        }
+    } else {
+        const int64_t b = sb + 1 + (tid - 64 - BLK_WORDS / 32);
+        if (b < C.nblk && ld_acq32(&C.blk[b])) cand_b = b;
     }
+    cand_s = wave_min64(cand_s);
+    cand_b = wave_min64(cand_b);
+    if ((tid & 63) == 0) { sh.buf[wv] = cand_s; sh.buf[WAVES + wv] = cand_b; }
     __syncthreads();
+    int64_t ms = INT64_MAX, mb = INT64_MAX;
+#pragma unroll
+    for (int k = 0; k < WAVES; k++) {
+        ms = sh.buf[k] < ms ? sh.buf[k] : ms;
+        mb = sh.buf[WAVES + k] < mb ? sh.buf[WAVES + k] : mb;
+    }
     const int fh = sh.v32[0];
-    const int64_t w = sh.v64[0], start = sh.v64[1];
+    const int64_t wp = sh.v64[0];
     __syncthreads();
     if (fh >= 0) {
-        if (threadIdx.x == 0) ctl->ruse = pos + fh + 1;
+        if (tid == 0) ctl->ruse = pos + fh + 1;
         __syncthreads();
-        return w;
+        return wp;
     }
-    if (threadIdx.x == 0) ctl->ruse = pos + 65;
+    if (tid == 0) ctl->ruse = pos + 65;
     __syncthreads();
+    if (ms != INT64_MAX) return ms;
+    if (mb != INT64_MAX) {
+        // round 3: the first non-empty block after the start block
+        const int64_t x0 = mb << (BLK_SHIFT - 5);
+        const int64_t x1 = (x0 + (BLK_WORDS / 32) < C.nzw) ? x0 + (BLK_WORDS / 32) : C.nzw;
+        return wg_first_bits(C, x0, x1, ~0u, ~0u, sh);
+    }
     return wg_first_from(C, start, sh);
 }
 
