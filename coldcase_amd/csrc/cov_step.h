@@ -715,11 +715,50 @@ __device__ __forceinline__ void wg_shuffle(const Cell& C, const Ctl* ctl, ShufSh
 }
 
 // ------------------------------------------------------------- select
+constexpr int SEL_PER = 4;                  // candidates per lane in the one-wave selection
+constexpr int SEL_WAVE = 64 * SEL_PER;      // the most candidates it takes
 struct SelSh {
     WgRed red;
     WgScan scan;
     int64_t inc[WG];
+    uint64_t rv[SEL_WAVE + 1];  // one-wave selection: ring draws r0 .. r0 + take
+    int64_t pick[6];            // the selected candidate: i, p, v, old value, word index, delta
 };
+
+// inclusive scans over the 64 lanes of a wave (lane order)
+__device__ __forceinline__ int64_t wave_incl_min(int64_t x)
+{
+    const int lane = threadIdx.x & 63;
+    for (int o = 1; o < 64; o <<= 1) {
+        const int64_t y = __shfl_up(x, o, 64);
+        if (lane >= o && y < x) x = y;
+    }
+    return x;
+}
+__device__ __forceinline__ int64_t wave_incl_max(int64_t x)
+{
+    const int lane = threadIdx.x & 63;
+    for (int o = 1; o < 64; o <<= 1) {
+        const int64_t y = __shfl_up(x, o, 64);
+        if (lane >= o && y > x) x = y;
+    }
+    return x;
+}
+__device__ __forceinline__ int64_t wave_incl_add(int64_t x)
+{
+    const int lane = threadIdx.x & 63;
+    for (int o = 1; o < 64; o <<= 1) {
+        const int64_t y = __shfl_up(x, o, 64);
+        if (lane >= o) x += y;
+    }
+    return x;
+}
+// the value of the lane below (the exclusive form of a scan); `first` on lane 0
+__device__ __forceinline__ int64_t wave_excl(int64_t incl, int64_t first)
+{
+    const int64_t y = __shfl_up(incl, 1, 64);
+    return (threadIdx.x & 63) == 0 ? first : y;
+}
 
 __device__ __forceinline__ int64_t cand_delta(const Cell& C, const Ctl* ctl, uint32_t k,
                                               int* pi, int* pp, int* pv)
@@ -733,6 +772,146 @@ __device__ __forceinline__ int64_t cand_delta(const Cell& C, const Ctl* ctl, uin
     *pp = p;
     *pv = nib_get(ctl->upk, p);
     return (int64_t)(a[0] - a[1 + p]) - (int64_t)a[1 + MAXN + p];
+}
+
+// The selection below for take <= SEL_WAVE candidates, on wave 0 alone: lane l
+// holds candidates SEL_PER*l .. SEL_PER*l + SEL_PER-1 of the shuffled order,
+// so the three prefix scans are a few register steps within the lane plus a
+// wave scan, with no workgroup barrier; the ring draws r0 .. r0 + take sit in
+// LDS, and the lane holding the selected candidate publishes it.  Same
+// results as the workgroup form.  Called by all threads.
+__device__ __forceinline__ void wave_select(const Cell& C, Ctl* ctl, SelSh& sh)
+{
+    const int tid = threadIdx.x;
+    if (tid < 64) {
+        const int lane = tid;
+        const int64_t take = ctl->take, it = ctl->it, unc = ctl->unc, best = ctl->best_unc;
+        const int64_t r0 = ctl->ruse;
+        const int n = C.n, q = C.q;
+        int ci[SEL_PER], cp[SEL_PER], cv[SEL_PER], cold[SEL_PER];
+        int64_t cd[SEL_PER], cbase[SEL_PER];
+        bool valid[SEL_PER];
+        uint64_t rvs[SEL_PER];
+        // the reads in three stages across the lane's candidates, each stage's
+        // loads issued together (a candidate past `take` reads candidate 0's
+        // data, in bounds, and is not used)
+        const int dtar = ctl->dtar;
+        const uint64_t upk = ctl->upk;
+        uint32_t kk[SEL_PER];
+        int slot[SEL_PER];
+        int32_t a0[SEL_PER], ap[SEL_PER], an[SEL_PER], stamp[SEL_PER];
+#pragma unroll
+        for (int k = 0; k < SEL_PER; k++) {
+            const int64_t s = (int64_t)SEL_PER * lane + k;
+            rvs[k] = ring_at(C, r0 + (s <= take ? s : 0));
+            kk[k] = C.shuf[s < take ? s : 0];
+        }
+#pragma unroll
+        for (int k = 0; k < SEL_PER; k++) {
+            slot[k] = (int)(kk[k] / (uint32_t)dtar);
+            const int t = (int)(kk[k] - (uint32_t)slot[k] * (uint32_t)dtar);
+            cp[k] = C.slot_pos[(int64_t)slot[k] * MAXN + t];
+            ci[k] = C.slot_cw[slot[k]];
+            a0[k] = C.acc[(int64_t)slot[k] * ACCW];
+        }
+#pragma unroll
+        for (int k = 0; k < SEL_PER; k++) {
+            const int32_t* a = C.acc + (int64_t)slot[k] * ACCW;
+            cv[k] = nib_get(upk, cp[k]);
+            ap[k] = a[1 + cp[k]];
+            an[k] = a[1 + MAXN + cp[k]];
+            stamp[k] = C.tabu[((int64_t)ci[k] * n + cp[k]) * q + cv[k]];
+            cold[k] = C.code[(int64_t)ci[k] * n + cp[k]];
+            cbase[k] = C.cidx[ci[k]];
+        }
+#pragma unroll
+        for (int k = 0; k < SEL_PER; k++) {
+            const int64_t s = (int64_t)SEL_PER * lane + k;
+            cd[k] = INT64_MAX;
+            valid[k] = false;
+            if (s < take) {
+                cd[k] = (int64_t)(a0[k] - ap[k]) - (int64_t)an[k];
+                valid[k] = !(stamp[k] > it && unc + cd[k] >= best);
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < SEL_PER; k++) {
+            const int64_t s = (int64_t)SEL_PER * lane + k;
+            if (s <= take) sh.rv[s] = rvs[k];
+        }
+        if (lane == 0 && take == SEL_WAVE) sh.rv[SEL_WAVE] = ring_at(C, r0 + SEL_WAVE);
+        // exclusive prefix minimum of the valid deltas
+        int64_t x[SEL_PER], lmin = INT64_MAX;
+#pragma unroll
+        for (int k = 0; k < SEL_PER; k++) {
+            x[k] = valid[k] ? cd[k] : INT64_MAX;
+            lmin = x[k] < lmin ? x[k] : lmin;
+        }
+        int64_t pm = wave_excl(wave_incl_min(lmin), INT64_MAX);
+        bool nb[SEL_PER], tie[SEL_PER];
+        int64_t nt = 0;
+#pragma unroll
+        for (int k = 0; k < SEL_PER; k++) {
+            nb[k] = valid[k] && cd[k] < pm;
+            tie[k] = valid[k] && cd[k] == pm;
+            nt += tie[k] ? 1 : 0;
+            pm = x[k] < pm ? x[k] : pm;
+        }
+        // T: ties up to each candidate; Tb: the last restart's T (at least 0)
+        const int64_t tincl = wave_incl_add(nt);
+        int64_t T = tincl - nt;
+        int64_t Tk[SEL_PER], mk = -1;
+#pragma unroll
+        for (int k = 0; k < SEL_PER; k++) {
+            T += tie[k] ? 1 : 0;
+            Tk[k] = T;
+            if (nb[k]) mk = T;
+        }
+        int64_t tb = wave_excl(wave_incl_max(mk), -1);
+        // the draws written by other lanes of the wave are read below
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        int64_t wl = -1;
+#pragma unroll
+        for (int k = 0; k < SEL_PER; k++) {
+            if (nb[k]) tb = Tk[k];
+            const int64_t Tb = tb < 0 ? 0 : tb;
+            bool ev = nb[k];
+            if (tie[k]) ev = (sh.rv[Tk[k] - 1] % (uint64_t)(1 + Tk[k] - Tb)) == 0;
+            if (ev) wl = (int64_t)SEL_PER * lane + k;
+        }
+        const int64_t winner = wave_max64(wl);
+        const int64_t Tall = __shfl(tincl, 63, 64);
+        int64_t Tball = wave_max64(mk);
+        if (Tball < 0) Tball = 0;
+        const int64_t ssel = winner >= 0 ? winner : (int64_t)(sh.rv[0] % (uint64_t)take);
+#pragma unroll
+        for (int k = 0; k < SEL_PER; k++) {
+            if ((int64_t)SEL_PER * lane + k == ssel) {
+                sh.pick[0] = ci[k]; sh.pick[1] = cp[k]; sh.pick[2] = cv[k]; sh.pick[3] = cold[k];
+                sh.pick[4] = cbase[k]; sh.pick[5] = cd[k];
+            }
+        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        if (lane == 0) {
+            const int64_t draws = winner >= 0 ? Tall : 1;
+            const uint64_t rs = sh.rv[draws];
+            ctl->ruse = r0 + draws + 1;
+            ctl->rlow = ctl->ruse;
+            const int bi = (int)sh.pick[0], bp = (int)sh.pick[1], bv = (int)sh.pick[2], oldv = (int)sh.pick[3];
+            ctl->bi = bi; ctl->bp = bp; ctl->bv = bv; ctl->oldv = oldv;
+            ctl->dsel = winner >= 0 ? sh.pick[5] : DSEL_RANDOM;
+            ctl->nties = winner >= 0 ? (int)(1 + Tall - Tball) : 0;
+            ctl->stamp = (int32_t)(it + ctl->tenure + (int64_t)(rs % 4));
+            ctl->base = sh.pick[4];
+            ctl->dv = (int64_t)(bv - oldv) * C.pw[bp];
+            ctl->ncw = ctl->nq;
+            ctl->selected = 1;
+        }
+    }
 }
 
 // Selection of :1503-1518 in parallel.  In shuffled order, a valid candidate
@@ -753,6 +932,9 @@ __device__ __forceinline__ void wg_select(const Cell& C, Ctl* ctl, SelSh& sh)
     const int64_t take = ctl->take, it = ctl->it, unc = ctl->unc, best = ctl->best_unc;
     const int64_t r0 = ctl->ruse;
     const int n = C.n, q = C.q;
+    if (take <= SEL_WAVE) {
+        wave_select(C, ctl, sh);
+    } else {
     int64_t carry_pm = INT64_MAX, carry_T = 0, carry_Tb = 0, winner = -1;
     for (int64_t s0 = 0; s0 < take; s0 += WG) {
         const int64_t s = s0 + tid;
@@ -824,6 +1006,7 @@ __device__ __forceinline__ void wg_select(const Cell& C, Ctl* ctl, SelSh& sh)
         ctl->dv = (int64_t)(bv - oldv) * C.pw[bp];
         ctl->ncw = ctl->nq;
         ctl->selected = 1;
+    }
     }
     __syncthreads();
     // the accumulators start the next step at zero
