@@ -82,3 +82,12 @@ def test_large_cells_random_init():
     compare(6, 10, 4, 2784, 1, 12, check_cnt=False, threads=8)
     compare(7, 9, 4, 1480, 1, 12, check_cnt=False, threads=8)
     compare(6, 9, 4, 1400, 1, 60, check_cnt=False, threads=8)
+
+
+def test_selection_above_one_wave():
+    """A dense code whose steps offer more than 256 candidate moves: the
+    selection then runs over the whole workgroup in chunks rather than on one
+    wave; both forms must equal the oracle, so the run must take both."""
+    tr = compare(6, 9, 4, 1300, 3, 30, check_cnt=False, threads=8)
+    nc = tr[:, TR_FIELDS.index('ncand')]
+    assert (nc > 256).any() and (nc <= 256).any(), nc
