@@ -48,6 +48,9 @@ __device__ __forceinline__ uint32_t udiv_small(uint32_t x, uint32_t d, float inv
 __global__ void __launch_bounds__(WG) k_recount(const Cell C, RecountPlan plan)
 {
     extern __shared__ uint32_t tile[];
+    __shared__ int64_t rc_off[WAVES][64];
+    __shared__ uint64_t rc_pk[WAVES][64];
+    __shared__ int64_t rc_sb[WAVES][64];
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
     const int t = plan.t;
     const int64_t T = plan.T;
@@ -64,25 +67,56 @@ __global__ void __launch_bounds__(WG) k_recount(const Cell C, RecountPlan plan)
         }
         const uint64_t pmask = t ? ((t >= 16) ? ~0ULL : ((1ULL << (4 * t)) - 1)) : 0ULL;
         __syncthreads();
-        // each wave takes codewords round-robin and walks its suffix ball
-        for (int i = wv; i < C.M; i += WAVES) {
-            const uint64_t cpk = C.cpk[i];
-            const int dpre = __popcll(nib_ne(cpk & pmask, ppk));
-            if (dpre > C.R) continue;
-            const int rr = C.R - dpre;
-            const int64_t vs = plan.spat[rr + 1];
-            const int64_t sbase = C.cidx[i] % T;
-            for (int64_t f = lane; f < vs; f += 64) {
+        // Each wave takes 64 codewords at a time, one per lane; those whose
+        // prefix is within R of the tile's contribute their suffix ball of
+        // radius R - d.  The wave's balls are laid end to end (a wave prefix
+        // sum of their sizes) and the lanes take consecutive items of the
+        // whole, each finding its codeword by a binary search of the starts.
+        for (int i0 = wv * 64; i0 < C.M; i0 += WAVES * 64) {
+            const int i = i0 + lane;
+            uint64_t cpk = 0;
+            int64_t vs = 0, sbase = 0;
+            if (i < C.M) {
+                cpk = C.cpk[i];
+                const int64_t ci = C.cidx[i];
+                const int dpre = __popcll(nib_ne(cpk & pmask, ppk));
+                if (dpre <= C.R) {
+                    vs = plan.spat[C.R - dpre + 1];
+                    sbase = ci % T;
+                }
+            }
+            int64_t incl = vs;
+            for (int o = 1; o < 64; o <<= 1) {
+                const int64_t y = __shfl_up(incl, o, 64);
+                if (lane >= o) incl += y;
+            }
+            const int64_t total = __shfl(incl, 63, 64);
+            rc_off[wv][lane] = incl - vs;
+            rc_pk[wv][lane] = cpk;
+            rc_sb[wv][lane] = sbase;
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+            for (int64_t f = lane; f < total; f += 64) {
+                // the last codeword whose ball starts at or before f (its ball is not empty)
+                int j = 0;
+#pragma unroll
+                for (int step = 32; step; step >>= 1)
+                    if (rc_off[wv][j + step] <= f) j += step;
+                const int64_t loc0 = f - rc_off[wv][j];
                 int r = 0;
-                while (f >= plan.spat[r + 1]) r++;
-                const uint32_t loc = (uint32_t)(f - plan.spat[r]);
+                while (loc0 >= plan.spat[r + 1]) r++;
+                const uint32_t loc = (uint32_t)(loc0 - plan.spat[r]);
                 uint32_t pat;
                 const uint32_t sidx = r ? udiv_small(loc, (uint32_t)plan.spr[r], plan.sinv[r], &pat) : 0;
                 if (!r) pat = 0;
                 const uint32_t sub = plan.ssub[plan.ssub_off[r] + sidx];
-                const int64_t w = pattern_word(C, sbase, cpk, sub, r, pat, nullptr);
+                const int64_t w = pattern_word(C, rc_sb[wv][j], rc_pk[wv][j], sub, r, pat, nullptr);
                 atomicAdd(&tile[w], 1u);
             }
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
         }
         __syncthreads();
         uint16_t* cnt = reinterpret_cast<uint16_t*>(C.cnt32) + tau * T;

@@ -322,7 +322,7 @@ int do_recount(cov_cell* h)
     h->cnt_valid = true;
     hipEvent_t a = nullptr;
     RC(timed_begin(h, 3, &a));
-    const int g = (int)std::min<int64_t>(h->plan.ntile, 2048);
+    const int g = (int)std::min<int64_t>(h->plan.ntile, 8192);
     // the kernel's dynamic LDS limit is one value for every cell: set it to
     // this cell's tile before its launch
     HCK(h, hipFuncSetAttribute((const void*)k_recount, hipFuncAttributeMaxDynamicSharedMemorySize,
