@@ -45,15 +45,33 @@ __device__ __forceinline__ uint32_t udiv_small(uint32_t x, uint32_t d, float inv
     return (uint32_t)qf;
 }
 
+constexpr int RC_SSUB_LDS = 1024;   // suffix subsets the recount keeps in LDS (else read from memory)
+
 __global__ void __launch_bounds__(WG) k_recount(const Cell C, RecountPlan plan)
 {
     extern __shared__ uint32_t tile[];
     __shared__ int64_t rc_off[WAVES][64];
     __shared__ uint64_t rc_pk[WAVES][64];
     __shared__ int64_t rc_sb[WAVES][64];
+    // the per-item tables, indexed per lane: in LDS rather than read from the
+    // kernel arguments with a divergent index
+    __shared__ int64_t rc_spat[MAXR + 2], rc_spr[MAXR + 2], rc_soff[MAXR + 2], rc_pw[MAXN];
+    __shared__ float rc_sinv[MAXR + 2];
+    __shared__ uint32_t rc_ssub[RC_SSUB_LDS];
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
     const int t = plan.t;
     const int64_t T = plan.T;
+    const int64_t nss = plan.ssub_off[MAXR + 1];
+    const bool ss_lds = nss <= RC_SSUB_LDS;
+    if (tid < MAXR + 2) {
+        rc_spat[tid] = plan.spat[tid];
+        rc_spr[tid] = plan.spr[tid];
+        rc_soff[tid] = plan.ssub_off[tid];
+        rc_sinv[tid] = plan.sinv[tid];
+    }
+    if (tid < MAXN) rc_pw[tid] = tid < C.n ? C.pw[tid] : 0;
+    if (ss_lds)
+        for (int64_t x = tid; x < nss; x += WG) rc_ssub[x] = plan.ssub[x];
     for (int64_t tau = blockIdx.x; tau < plan.ntile; tau += gridDim.x) {
         for (int64_t x = tid; x < T; x += WG) tile[x] = 0;
         // packed prefix digits of the tile (positions 0..t-1)
@@ -105,13 +123,26 @@ __global__ void __launch_bounds__(WG) k_recount(const Cell C, RecountPlan plan)
                     if (rc_off[wv][j + step] <= f) j += step;
                 const int64_t loc0 = f - rc_off[wv][j];
                 int r = 0;
-                while (loc0 >= plan.spat[r + 1]) r++;
-                const uint32_t loc = (uint32_t)(loc0 - plan.spat[r]);
+                while (loc0 >= rc_spat[r + 1]) r++;
+                const uint32_t loc = (uint32_t)(loc0 - rc_spat[r]);
                 uint32_t pat;
-                const uint32_t sidx = r ? udiv_small(loc, (uint32_t)plan.spr[r], plan.sinv[r], &pat) : 0;
+                const uint32_t sidx = r ? udiv_small(loc, (uint32_t)rc_spr[r], rc_sinv[r], &pat) : 0;
                 if (!r) pat = 0;
-                const uint32_t sub = plan.ssub[plan.ssub_off[r] + sidx];
-                const int64_t w = pattern_word(C, rc_sb[wv][j], rc_pk[wv][j], sub, r, pat, nullptr);
+                const int64_t si = rc_soff[r] + sidx;
+                const uint32_t sub = ss_lds ? rc_ssub[si] : plan.ssub[si];
+                // the word of pattern pat on subset sub (pattern_word's, with the
+                // place values from LDS)
+                const uint64_t cpk = rc_pk[wv][j];
+                int64_t w = rc_sb[wv][j];
+                uint32_t tt = pat;
+                for (int e = r - 1; e >= 0; e--) {
+                    const uint32_t tq = div_q1(C, tt);
+                    const int k = (int)(tt - tq * (uint32_t)C.q1);
+                    tt = tq;
+                    const int jj = (sub >> (4 * e)) & 15;
+                    const int c = nib_get(cpk, jj);
+                    w += (int64_t)(k + (k >= c) - c) * rc_pw[jj];
+                }
                 atomicAdd(&tile[w], 1u);
             }
             __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
