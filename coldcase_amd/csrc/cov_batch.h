@@ -1,7 +1,7 @@
 // cov_batch.h -- many cells advanced by the same launches.
 //
-// A batched step is the three launches of the single-cell step (evaluator,
-// selection, commit) over the workgroups of every cell at once: the grid is cut into
+// A batched step is the four launches of the single-cell step (evaluator,
+// selection, commit, post) over the workgroups of every cell at once: the grid is cut into
 // consecutive ranges, cell i owning workgroups [off[i], off[i+1]), and each
 // workgroup runs the single-cell body for its cell with its index inside the
 // range.  Every cell keeps its own control block, counters, rng ring and tabu
@@ -41,7 +41,13 @@ __global__ void __launch_bounds__(WG) k_commit_b(const Cell* __restrict__ cells,
                                                  int ncell)
 {
     const int i = batch_cell(off, ncell, blockIdx.x);
-    commit_body(cells[i], blockIdx.x - off[i], off[i + 1] - off[i], 0);
+    commit_body(cells[i], blockIdx.x - off[i], off[i + 1] - off[i]);
+}
+
+// one workgroup per cell
+__global__ void __launch_bounds__(WG) k_post_b(const Cell* __restrict__ cells)
+{
+    post_body(cells[blockIdx.x]);
 }
 
 // one workgroup per cell

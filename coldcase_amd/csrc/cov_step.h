@@ -1,6 +1,6 @@
 // cov_step.h -- the device-resident tabu step (kernel a) and its narrow phases.
 //
-// One search step of cov/engine/covsearch2e.c:1304-1580 is three launches:
+// One search step of cov/engine/covsearch2e.c:1304-1580 is four launches:
 //
 //   k_eval    (wide)  units of work over the candidate codewords:
 //                       leaving side: per (codeword, R-subset of all n
@@ -22,23 +22,23 @@
 //   k_commit  (wide)  --cnt[a], ++cnt[b=a+dv] over the C(n-1,R)(q-1)^R
 //                     patterns (every word once, no counter atomics, :562-590);
 //                     plane bits, block counts and the uncovered list follow
-//                     the transitions; the last workgroup applies the move,
-//                     stamps tabu, tracks the best code, kicks on stagnation,
-//                     writes the trace record and begins the next step: pick
-//                     (:973-1016), distances, candidates (:1339-1376).
+//                     the transitions
+//   k_post    (one workgroup)  applies the move, stamps tabu, tracks the best
+//                     code, kicks on stagnation, writes the trace record and
+//                     begins the next step: pick (:973-1016), distances,
+//                     candidates (:1339-1376).
 //
 // Every decision that consumes randomness reads the ring of xoshiro256++
 // draws at a cursor in the reference's order, so the trajectory is the
 // reference's bit for bit.
 //
-// Synchronisation: kernel boundaries order the evaluator before the selection
-// and the selection before the commit.  Inside the commit, workgroups publish
-// with one agent-scope release atomic each on a counter (a release is an L2
-// write-back on this part, which is why the evaluator, with four to eight
-// times as many workgroups, hands over at a kernel boundary instead); only
-// the workgroup that sees the count complete acquires.  The single-workgroup
-// phases read data that device atomics change (planes, block counts) with
-// agent-scope loads.
+// Synchronisation: kernel boundaries order each launch after the one before.
+// A device-scope release is an L2 write-back on this part, so the wide kernels'
+// workgroups end without one and the narrow phases run as kernels of their
+// own.  Only the single move of the C ABI (k_commit_api) still ends with an
+// arrival counter: its workgroups release onto it and the last one applies
+// the move.  The single-workgroup phases read data that device atomics change
+// (planes, block counts) with agent-scope loads.
 #pragma once
 #include "cov_common.h"
 #include "cov_cover.h"
@@ -1472,19 +1472,53 @@ __device__ __forceinline__ int64_t commit_units(const Cell& C, const Ctl* ctl, i
 }
 
 // The commit of one cell on nblk workgroups; this one is number bid.
-__device__ __forceinline__ void commit_body(const Cell& C, int bid, int nblk, int api)
+// The commit of a search step, on nblk workgroups: the pattern updates only.
+// The step's bookkeeping and the next begin run in post_body, launched after
+// it, so these workgroups end without a device-scope release of their own.
+__device__ __forceinline__ void commit_body(const Cell& C, int bid, int nblk)
 {
     Ctl* ctl = C.ctl;
-    __shared__ NarrowSh sh;
-    __shared__ int last;
+    __shared__ WgRed red;
     if (!ctl->begun || ctl->done || !ctl->selected) return;
     if (bid == 0 && threadIdx.x == 0 && ctl->t_eve) ctl->tph[11] += clk() - ctl->t_eve;
-    const int64_t tc0 = clk();
     const int64_t gt = (int64_t)bid * WG + threadIdx.x;
     __shared__ int64_t ctab[MAXN * 16];
     commit_table(C, C.cpk[ctl->bi], ctab);
     const int64_t d = commit_units(C, ctl, gt, (int64_t)nblk * WG, ctab);
-    const int64_t ds = wg_sum(d, sh.red);
+    const int64_t ds = wg_sum(d, red);
+    if (threadIdx.x == 0 && ds) atomicAdd(reinterpret_cast<unsigned long long*>(&ctl->dunc), (unsigned long long)ds);
+}
+
+// After the commit, one workgroup: apply the move, stamp tabu, track the best
+// code, kick, trace, and begin the next step.
+__device__ __forceinline__ void post_body(const Cell& C)
+{
+    __shared__ NarrowSh sh;
+    __shared__ Ctl lc;
+    const int64_t tp0 = clk();
+    ctl_to_lds(&lc, C.ctl);
+    Ctl* L = &lc;
+    if (!L->begun || L->done || !L->selected) return;
+    if (threadIdx.x == 0 && L->t_eve) L->tph[9] += tp0 - L->t_eve;
+    wg_post(C, L, sh);
+    wg_begin(C, L, sh);
+    if (threadIdx.x == 0) L->t_cme = clk();
+    ctl_from_lds(C.ctl, &lc);
+}
+
+// A single move requested through the C ABI (k_set_move first): the pattern
+// updates, then the last workgroup to arrive applies the move only.
+__device__ __forceinline__ void commit_api_body(const Cell& C, int bid, int nblk)
+{
+    Ctl* ctl = C.ctl;
+    __shared__ WgRed red;
+    __shared__ int last;
+    if (!ctl->begun || ctl->done || !ctl->selected) return;
+    const int64_t gt = (int64_t)bid * WG + threadIdx.x;
+    __shared__ int64_t ctab[MAXN * 16];
+    commit_table(C, C.cpk[ctl->bi], ctab);
+    const int64_t d = commit_units(C, ctl, gt, (int64_t)nblk * WG, ctab);
+    const int64_t ds = wg_sum(d, red);
     if (threadIdx.x == 0 && ds) atomicAdd(reinterpret_cast<unsigned long long*>(&ctl->dunc), (unsigned long long)ds);
     if (!wg_arrive_last(C.arrive, &ctl->cmdone, &last, bid, nblk)) return;
     __shared__ Ctl lc;
@@ -1492,35 +1526,34 @@ __device__ __forceinline__ void commit_body(const Cell& C, int bid, int nblk, in
     Ctl* L = &lc;
     if (threadIdx.x == 0) {
         L->cmdone = 0;
-        L->tph[9] += clk() - tc0;
+        const int64_t dunc = L->dunc;
+        L->dunc = 0;
+        L->unc += dunc;
+        L->dsel = dunc;
+        C.code[(int64_t)L->bi * C.n + L->bp] = (uint8_t)L->bv;
+        C.cpk[L->bi] = nib_put(C.cpk[L->bi], L->bp, L->bv);
+        C.cidx[L->bi] += L->dv;
+        log_dirty(C, L, L->bi);
+        if (L->ul_ovf) { L->ul_valid = 0; L->ul_ovf = 0; L->ul_n = 0; }
+        L->begun = 0;
+        L->selected = 0;
     }
-    if (api) {
-        // a single committed move requested through the C ABI: apply it only
-        if (threadIdx.x == 0) {
-            const int64_t dunc = L->dunc;
-            L->dunc = 0;
-            L->unc += dunc;
-            L->dsel = dunc;
-            C.code[(int64_t)L->bi * C.n + L->bp] = (uint8_t)L->bv;
-            C.cpk[L->bi] = nib_put(C.cpk[L->bi], L->bp, L->bv);
-            C.cidx[L->bi] += L->dv;
-            log_dirty(C, L, L->bi);
-            if (L->ul_ovf) { L->ul_valid = 0; L->ul_ovf = 0; L->ul_n = 0; }
-            L->begun = 0;
-            L->selected = 0;
-        }
-        ctl_from_lds(C.ctl, &lc);
-        return;
-    }
-    wg_post(C, L, sh);
-    wg_begin(C, L, sh);
-    if (threadIdx.x == 0) L->t_cme = clk();
     ctl_from_lds(C.ctl, &lc);
 }
 
-__global__ void __launch_bounds__(WG) k_commit(const Cell C, int api)
+__global__ void __launch_bounds__(WG) k_commit(const Cell C)
 {
-    commit_body(C, blockIdx.x, gridDim.x, api);
+    commit_body(C, blockIdx.x, gridDim.x);
+}
+
+__global__ void __launch_bounds__(WG) k_post(const Cell C)
+{
+    post_body(C);
+}
+
+__global__ void __launch_bounds__(WG) k_commit_api(const Cell C)
+{
+    commit_api_body(C, blockIdx.x, gridDim.x);
 }
 
 // set the run's stop and begin the first step (one workgroup)
@@ -1714,7 +1747,7 @@ __global__ void __launch_bounds__(WG) k_eval_finish(const Cell C, int k, int64_t
     }
 }
 
-// set up a single move for k_commit(api = 1)
+// set up a single move for k_commit_api
 __global__ void k_set_move(const Cell C, int i, int p, int v)
 {
     Ctl* ctl = C.ctl;

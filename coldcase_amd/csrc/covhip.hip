@@ -372,9 +372,11 @@ int launch_step(cov_cell* h)
     hipLaunchKernelGGL(k_select, dim3(1), dim3(WG), 0, h->stream, h->h);
     HCK(h, hipGetLastError());
     RC(timed_begin(h, 1, &a));
-    hipLaunchKernelGGL(k_commit, dim3(h->commit_groups), dim3(WG), 0, h->stream, h->h, 0);
+    hipLaunchKernelGGL(k_commit, dim3(h->commit_groups), dim3(WG), 0, h->stream, h->h);
     HCK(h, hipGetLastError());
     RC(timed_end(h, 1, a));
+    hipLaunchKernelGGL(k_post, dim3(1), dim3(WG), 0, h->stream, h->h);
+    HCK(h, hipGetLastError());
     return COV_OK;
 }
 
@@ -387,7 +389,8 @@ int ensure_graph(cov_cell* h)
     for (int s = 0; s < GRAPH_STEPS; s++) {
         hipLaunchKernelGGL(k_eval, dim3(h->eval_groups), dim3(WG), 0, h->stream, h->h);
         hipLaunchKernelGGL(k_select, dim3(1), dim3(WG), 0, h->stream, h->h);
-        hipLaunchKernelGGL(k_commit, dim3(h->commit_groups), dim3(WG), 0, h->stream, h->h, 0);
+        hipLaunchKernelGGL(k_commit, dim3(h->commit_groups), dim3(WG), 0, h->stream, h->h);
+        hipLaunchKernelGGL(k_post, dim3(1), dim3(WG), 0, h->stream, h->h);
     }
     HCK(h, hipStreamEndCapture(h->stream, &g));
     hipError_t e = hipGraphInstantiate(&h->gexec, g, nullptr, nullptr, 0);
@@ -826,7 +829,7 @@ int cov_commit(cov_cell_t* h, int i, int p, int v, int64_t* dunc)
     if (i < 0 || i >= C.M || p < 0 || p >= C.n || v < 0 || v >= C.q) return COV_EPARAM;
     hipLaunchKernelGGL(k_set_move, dim3(1), dim3(64), 0, h->stream, h->h, i, p, v);
     HCK(h, hipGetLastError());
-    hipLaunchKernelGGL(k_commit, dim3(h->commit_groups), dim3(WG), 0, h->stream, h->h, 1);
+    hipLaunchKernelGGL(k_commit_api, dim3(h->commit_groups), dim3(WG), 0, h->stream, h->h);
     HCK(h, hipGetLastError());
     Ctl c;
     RC(read_ctl(h, &c));
@@ -1289,6 +1292,7 @@ int cov_batch_launch(cov_batch_t* b, int64_t nsteps, int mode)
                 hipLaunchKernelGGL(k_select_b, dim3(k), dim3(WG), 0, b->stream, b->d_cells);
                 hipLaunchKernelGGL(k_commit_b, dim3(b->commit_total), dim3(WG), 0, b->stream, b->d_cells, b->d_coff,
                                    k);
+                hipLaunchKernelGGL(k_post_b, dim3(k), dim3(WG), 0, b->stream, b->d_cells);
             }
             HCK(b, hipStreamEndCapture(b->stream, &g));
             const hipError_t e = hipGraphInstantiate(&b->gexec, g, nullptr, nullptr, 0);
@@ -1304,6 +1308,7 @@ int cov_batch_launch(cov_batch_t* b, int64_t nsteps, int mode)
         hipLaunchKernelGGL(k_eval_b, dim3(b->eval_total), dim3(WG), 0, b->stream, b->d_cells, b->d_eoff, k);
         hipLaunchKernelGGL(k_select_b, dim3(k), dim3(WG), 0, b->stream, b->d_cells);
         hipLaunchKernelGGL(k_commit_b, dim3(b->commit_total), dim3(WG), 0, b->stream, b->d_cells, b->d_coff, k);
+        hipLaunchKernelGGL(k_post_b, dim3(k), dim3(WG), 0, b->stream, b->d_cells);
         HCK(b, hipGetLastError());
     }
     for (cov_cell* c : b->cells) HCK(b, hipEventRecord(c->used, b->stream));
