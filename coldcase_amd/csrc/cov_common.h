@@ -70,8 +70,9 @@ struct alignas(16) Ctl {
     int32_t ndirty;         // codewords logged in dirty since best was last made equal to code
     // wall_clock64 ticks spent per phase (diagnostics): 0 select, 1 post,
     // 2 begin, 3 pick, 4 list upkeep, 5 shuffle helper, 6 rng helper, 7 kick,
-    // 8 evaluator span before select, 9 commit span before post, 10 gap commit
-    // end -> next evaluator start, 11 gap evaluator end -> commit start
+    // 8 previous begin's end -> selection start (the evaluator's span),
+    // 9 selection end -> post start (the commit's span), 10 gap previous
+    // begin's end -> evaluator start, 11 gap selection end -> commit start
     int64_t tph[12];
     int64_t t_eve, t_cme;
     int64_t* trace;         // per-run trace records (tcap of them), or null
@@ -149,7 +150,7 @@ struct Cell {
     GPTR(uint8_t) slot_pos;      // slot -> differing positions (MAXN each)
     GPTR(uint64_t) rbuf;
     GPTR(int64_t) scratch64;     // small reduction outputs
-    GPTR(int32_t) arrive;        // the commit's workgroup arrival sub-counters
+    GPTR(int32_t) arrive;        // workgroup arrival sub-counters of the single-move commit
     GPTR(int32_t) dirty;         // codewords changed since best == code (DIRTY_CAP of them)
     GPTR(uint64_t) jump;         // the rng jump matrices J_1 .. J_{RNG_SPLIT-1}
     GPTR(uint8_t) dist;          // distance transform output (N bytes), lazily
