@@ -162,8 +162,9 @@ int     cov_timing(cov_cell_t *h, int on);
 int     cov_timing_read(cov_cell_t *h, double *ms, int64_t *launches, int nkinds);
 /* Diagnostics: microseconds spent since cov_search_init, from the device wall
  * clock, in: 0 select, 1 post, 2 begin, 3 pick, 4 list upkeep, 5 shuffle,
- * 6 rng refill, 7 kicks, 8 evaluator span before select, 9 commit span
- * before post, 10 gap commit -> next evaluator, 11 gap evaluator -> commit
+ * 6 rng refill, 7 kicks, 8 the evaluator's span (previous begin's end ->
+ * selection start), 9 the commit's span (selection end -> post start),
+ * 10 gap previous begin -> evaluator, 11 gap selection -> commit
  * (n <= 12 entries). */
 int     cov_search_phases(cov_cell_t *h, double *us, int n);
 /* Algorithmic bytes moved by the evaluator and commit kernels since the last
@@ -173,8 +174,10 @@ int     cov_search_bytes(cov_cell_t *h, int64_t *eval_bytes, int64_t *commit_byt
 /* ------------------------------------------------------------ batches */
 /* Many cells, each created, configured and initialised on its own (any
  * shapes), advanced by the same launches: every batched step is one
- * evaluator and one commit launch whose workgroups are split among the cells
- * (each cell's share is its cov_set_launch groups, fixed at creation).  A
+ * evaluator, one selection, one commit and one post launch; the evaluator's
+ * and the commit's workgroups are split among the cells (each cell's share
+ * is its cov_set_launch groups, fixed at creation), the other two run one
+ * workgroup per cell.  A
  * cell's trajectory in a batch is the one it has alone.  The reference runs
  * one cell per process (cov/engine/covsearch2e.c:1304 is a single-cell loop);
  * this has no counterpart there.  While a batch runs its cells belong to it:
