@@ -37,6 +37,13 @@ constexpr int LIST_CHUNK = 512;      // uncovered-list entries per evaluator uni
 constexpr int64_t DSEL_RANDOM = 1LL << 60;
 constexpr int TRW = 16;
 constexpr int DIRTY_CAP = 4096;      // logged codeword changes before a full best copy
+// The leaving side's counts per codeword c, kept up to date instead of
+// recomputed each step: th[c*THW] = T(c) = #{w : d(w,c) = R, cnt(w) = 1} and
+// th[c*THW + 1 + j] = H_j(c), the same words with w_j != c_j.  A word whose
+// cnt crosses 1 changes them for its unique coverer only, which the XOR of
+// its coverers' indices names; codewords that move are recounted whole.
+constexpr int THW = 1 + MAXN;
+constexpr int STALE_MAX = 8;         // codewords recounted per step at most (the move and the kicks)
 // Workgroup arrival: up to ARR_G sub-counters per kernel, each on its own
 // 256-byte line (atomics to one line are serialised where they are performed)
 constexpr int ARR_G = 32;
@@ -67,6 +74,9 @@ struct alignas(16) Ctl {
     int32_t dtar, ncand, take, nq, use_list;
     int32_t bi, bp, bv, oldv, nties, stamp, ncw;
     int32_t cmdone;
+    int32_t th_on;          // the leaving side comes from th (else the per-step walks)
+    int32_t nstale;         // codewords whose th rows the next evaluator recounts
+    int32_t stale[STALE_MAX];
     int32_t ndirty;         // codewords logged in dirty since best was last made equal to code
     // wall_clock64 ticks spent per phase (diagnostics): 0 select, 1 post,
     // 2 begin, 3 pick, 4 list upkeep, 5 shuffle helper, 6 rng helper, 7 kick,
@@ -153,6 +163,8 @@ struct Cell {
     GPTR(int32_t) arrive;        // workgroup arrival sub-counters of the single-move commit
     GPTR(int32_t) dirty;         // codewords changed since best == code (DIRTY_CAP of them)
     GPTR(uint64_t) jump;         // the rng jump matrices J_1 .. J_{RNG_SPLIT-1}
+    GPTR(uint32_t) xor32;        // per word, the XOR of the indices of its covering codewords (uint16 pairs)
+    GPTR(int32_t) th;            // T and H_j per codeword, THW each
     GPTR(uint8_t) dist;          // distance transform output (N bytes), lazily
     GPTR(Ctl) ctl;
 };
@@ -176,6 +188,14 @@ __device__ __forceinline__ void cnt_put(const Cell& C, int64_t w, uint16_t v)
 {
     reinterpret_cast<uint16_t*>(C.cnt32)[w] = v;
 }
+__device__ __forceinline__ uint32_t xor_get(const Cell& C, int64_t w)
+{
+    return reinterpret_cast<const uint16_t*>(C.xor32)[w];
+}
+__device__ __forceinline__ void xor_put(const Cell& C, int64_t w, uint32_t v)
+{
+    reinterpret_cast<uint16_t*>(C.xor32)[w] = (uint16_t)v;
+}
 __device__ __forceinline__ uint32_t bit_of(const uint32_t* plane, int64_t w)
 {
     return (plane[w >> 5] >> (w & 31)) & 1u;
@@ -198,6 +218,23 @@ __device__ __forceinline__ uint64_t nib_ne(uint64_t a, uint64_t b)
 __device__ __forceinline__ int nib_dist(uint64_t a, uint64_t b)
 {
     return __popcll(nib_ne(a, b));
+}
+
+// word w (digits wpk) became, or stopped being, covered by codeword o alone
+// (sign +1 / -1): o's leaving counts change when d(w, o) = R
+__device__ __forceinline__ void th_event(const Cell& C, int o, uint64_t wpk, int sign)
+{
+    if ((unsigned)o >= (unsigned)C.M) return;
+    const uint64_t ne = nib_ne(wpk, C.cpk[o]);
+    if (__popcll(ne) != C.R) return;
+    int32_t* t = C.th + (int64_t)o * THW;
+    atomicAdd(&t[0], sign);
+    uint64_t m = ne;
+    while (m) {
+        const int b = __ffsll((unsigned long long)m) - 1;
+        m &= m - 1;
+        atomicAdd(&t[1 + (b >> 2)], sign);
+    }
 }
 __device__ __forceinline__ int nib_get(uint64_t pk, int j)
 {

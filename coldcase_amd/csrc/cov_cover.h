@@ -229,15 +229,21 @@ __device__ __forceinline__ void list_insert(const Cell& C, Ctl* ctl, int64_t w, 
 
 // one ball word, incremental (distinct words within the call); returns the
 // change in the uncovered count
+// With tr >= 0 the word's coverer XOR takes codeword tr in or out, and the
+// codeword left as the word's only coverer, or no longer its only one, has its
+// leaving counts updated.
 __device__ __forceinline__ int ball_word(const Cell& C, Ctl* ctl, int64_t w, uint64_t wpk,
-                                         int delta, bool list_on)
+                                         int delta, bool list_on, int tr = -1)
 {
     const uint32_t sh = (uint32_t)(w & 1) * 16u;
     uint32_t* p = &C.cnt32[w >> 1];
     int d = 0;
+    uint32_t ox = 0;
+    if (tr >= 0) ox = (atomicXor(&C.xor32[w >> 1], (uint32_t)tr << sh) >> sh) & 0xFFFFu;
     if (delta > 0) {
         const uint32_t old = atomicAdd(p, 1u << sh);
         const uint32_t oc = (old >> sh) & 0xFFFFu;
+        if (tr >= 0 && oc == 1) th_event(C, (int)ox, wpk, -1);
         if (oc == 0) {
             bit_clr(C.zb, w); bit_set(C.ob, w);
             atomicSub(&C.blk[w >> BLK_SHIFT], 1u);
@@ -248,6 +254,7 @@ __device__ __forceinline__ int ball_word(const Cell& C, Ctl* ctl, int64_t w, uin
     } else {
         const uint32_t old = atomicSub(p, 1u << sh);
         const uint32_t oc = (old >> sh) & 0xFFFFu;
+        if (tr >= 0 && oc == 2) th_event(C, (int)(ox ^ (uint32_t)tr), wpk, +1);
         if (oc == 1) {
             bit_set(C.zb, w); bit_clr(C.ob, w);
             atomicAdd(&C.blk[w >> BLK_SHIFT], 1u);

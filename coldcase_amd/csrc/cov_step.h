@@ -475,12 +475,15 @@ __device__ __forceinline__ void wg_begin(const Cell& C, Ctl* ctl, NarrowSh& sh)
         ctl->take = (int32_t)take;
         ctl->use_list = use_list;
         ctl->nchunk = use_list ? nchunk : 0;
-        ctl->n_leave = nq * C.F;
+        // the leaving side: the moved codewords' spheres when the counts are
+        // kept (th), else those of every codeword at distance R+1 from u
+        const int64_t nlv = (ctl->th_on ? ctl->nstale : nq) * (int64_t)C.F;
+        ctl->n_leave = nlv;
         ctl->n_enter = use_list ? nq * nchunk : nq * dtar * C.E;
         ctl->sh_pos = ctl->ruse;
         ctl->rlow = ctl->ruse;
         ctl->ruse += take;
-        ctl->eval_bytes += (int64_t)C.M * C.n + 2 * nq * C.F * C.P +
+        ctl->eval_bytes += (int64_t)C.M * C.n + 2 * nlv * C.P +
                            (use_list ? nq * ctl->ul_n * C.n : 2 * nq * dtar * C.E * C.P);
         ctl->selected = 0;
         ctl->begun = (ncand > 0 && take <= RB - 4096) ? 1 : 0;
@@ -542,6 +545,19 @@ __device__ __forceinline__ void unit_leave(const Cell& C, int slot, int f, WalkT
     }
 }
 
+// the leaving counts of moved codeword i, subset f (its th row starts at zero)
+__device__ __forceinline__ void unit_stale(const Cell& C, int i, int f, WalkTab& tb)
+{
+    const int lane = threadIdx.x & 63;
+    const uint32_t sub = C.fsub[f];
+    const int64_t c1 = sphere_count(C, C.ob, C.cidx[i], C.cpk[i], sub, tb);
+    if (lane == 0 && c1) {
+        int32_t* t = C.th + (int64_t)i * THW;
+        atomicAdd(&t[0], (int)c1);
+        for (int k = 0; k < C.R; k++) atomicAdd(&t[1 + ((sub >> (4 * k)) & 15)], (int)c1);
+    }
+}
+
 // entering side by walks: words of the sphere of c(p:=u_p) on subset e
 // avoiding p, with cnt == 0
 __device__ __forceinline__ void unit_walk(const Cell& C, int slot, int p, int up, int e, WalkTab& tb)
@@ -599,10 +615,12 @@ __device__ __forceinline__ void eval_units(const Cell& C, const Ctl* ctl, int64_
     const int64_t nchunk = ctl->nchunk, ul_n = ctl->ul_n;
     const uint64_t upk = ctl->upk;
     const int64_t per_slot_walk = (int64_t)dtar * C.E;
+    const int th_on = ctl->th_on;
     for (int64_t g = wid; g < nl + ne; g += nw) {
         if (g < nl) {
             const int slot = (int)(g / C.F);
-            unit_leave(C, slot, (int)(g - (int64_t)slot * C.F), tb);
+            if (th_on) unit_stale(C, ctl->stale[slot], (int)(g - (int64_t)slot * C.F), tb);
+            else unit_leave(C, slot, (int)(g - (int64_t)slot * C.F), tb);
         } else if (use_list) {
             const int64_t h = g - nl;
             const int slot = (int)(h / nchunk);
@@ -768,10 +786,12 @@ __device__ __forceinline__ int64_t cand_delta(const Cell& C, const Ctl* ctl, uin
     const int t = (int)(k - (uint32_t)slot * (uint32_t)dtar);
     const int p = C.slot_pos[(int64_t)slot * MAXN + t];
     const int32_t* a = C.acc + (int64_t)slot * ACCW;
-    *pi = C.slot_cw[slot];
+    const int i = C.slot_cw[slot];
+    *pi = i;
     *pp = p;
     *pv = nib_get(ctl->upk, p);
-    return (int64_t)(a[0] - a[1 + p]) - (int64_t)a[1 + MAXN + p];
+    const int32_t* lv = ctl->th_on ? C.th + (int64_t)i * THW : a;
+    return (int64_t)(lv[0] - lv[1 + p]) - (int64_t)a[1 + MAXN + p];
 }
 
 // The selection below for take <= SEL_WAVE candidates, on wave 0 alone: lane l
@@ -797,6 +817,7 @@ __device__ __forceinline__ void wave_select(const Cell& C, Ctl* ctl, SelSh& sh)
         // data, in bounds, and is not used)
         const int dtar = ctl->dtar;
         const uint64_t upk = ctl->upk;
+        const bool th_on = ctl->th_on != 0;
         uint32_t kk[SEL_PER];
         int slot[SEL_PER];
         int32_t a0[SEL_PER], ap[SEL_PER], an[SEL_PER], stamp[SEL_PER];
@@ -812,13 +833,19 @@ __device__ __forceinline__ void wave_select(const Cell& C, Ctl* ctl, SelSh& sh)
             const int t = (int)(kk[k] - (uint32_t)slot[k] * (uint32_t)dtar);
             cp[k] = C.slot_pos[(int64_t)slot[k] * MAXN + t];
             ci[k] = C.slot_cw[slot[k]];
-            a0[k] = C.acc[(int64_t)slot[k] * ACCW];
+            if (!th_on) a0[k] = C.acc[(int64_t)slot[k] * ACCW];
         }
 #pragma unroll
         for (int k = 0; k < SEL_PER; k++) {
             const int32_t* a = C.acc + (int64_t)slot[k] * ACCW;
             cv[k] = nib_get(upk, cp[k]);
-            ap[k] = a[1 + cp[k]];
+            if (th_on) {
+                const int32_t* t = C.th + (int64_t)ci[k] * THW;
+                a0[k] = t[0];
+                ap[k] = t[1 + cp[k]];
+            } else {
+                ap[k] = a[1 + cp[k]];
+            }
             an[k] = a[1 + MAXN + cp[k]];
             stamp[k] = C.tabu[((int64_t)ci[k] * n + cp[k]) * q + cv[k]];
             cold[k] = C.code[(int64_t)ci[k] * n + cp[k]];
@@ -1137,6 +1164,7 @@ __device__ __forceinline__ void select_body(const Cell& C)
         const int64_t te = clk();
         lc.tph[0] += te - ts0;
         lc.t_eve = te;
+        lc.nstale = 0;      // the evaluator has recounted them
     }
     ctl_from_lds(ctl, &lc);
 }
@@ -1157,15 +1185,25 @@ __device__ __forceinline__ void wg_ball(const Cell& C, Ctl* ctl, int i, int delt
     const int64_t base = C.cidx[i];
     const uint64_t pk = C.cpk[i];
     const bool list_on = ctl->ul_valid != 0;
+    const int tr = ctl->th_on ? i : -1;
     int64_t d = 0;
     for (int64_t g = threadIdx.x; g < C.V; g += WG) {
         uint64_t wpk;
         const int64_t w = ball_word_at(C, base, pk, g, &wpk);
-        d += ball_word(C, ctl, w, wpk, delta, list_on);
+        d += ball_word(C, ctl, w, wpk, delta, list_on, tr);
     }
     d = wg_sum(d, sh.red);
     if (threadIdx.x == 0) ctl->unc += d;
     __syncthreads();
+}
+
+// codeword i moved: its leaving counts are recounted by the next evaluator
+// (thread 0; the list holds the move and the kicks, STALE_MAX at most)
+__device__ __forceinline__ void stale_add(Ctl* ctl, int i)
+{
+    for (int k = 0; k < ctl->nstale; k++)
+        if (ctl->stale[k] == i) return;
+    if (ctl->nstale < STALE_MAX) ctl->stale[ctl->nstale++] = i;
 }
 
 // codeword i changed: log it for the next best copy (thread 0)
@@ -1217,6 +1255,7 @@ __device__ __forceinline__ void wg_kick(const Cell& C, Ctl* ctl, NarrowSh& sh)
             C.cpk[i] = pk;
             C.cidx[i] = idx;
             log_dirty(C, ctl, i);
+            if (ctl->th_on) stale_add(ctl, i);
         }
         __syncthreads();
         wg_ball(C, ctl, i, +1, sh);
@@ -1245,8 +1284,10 @@ __device__ __forceinline__ void wg_post(const Cell& C, Ctl* ctl, NarrowSh& sh)
         C.cidx[bi] += ctl->dv;
         C.tabu[((int64_t)bi * n + bp) * q + oldv] = ctl->stamp;
         log_dirty(C, ctl, bi);
+        ctl->nstale = 0;
+        if (ctl->th_on) stale_add(ctl, bi);
         if (ctl->ul_ovf) { ctl->ul_valid = 0; ctl->ul_ovf = 0; ctl->ul_n = 0; }
-        ctl->commit_bytes += 8 * (int64_t)C.E * C.P;
+        ctl->commit_bytes += (ctl->th_on ? 16 : 8) * (int64_t)C.E * C.P;
         sh.v64[3] = dunc;
         sh.v32[2] = 0;
         if (ctl->unc < ctl->best_unc) {
@@ -1285,6 +1326,13 @@ __device__ __forceinline__ void wg_post(const Cell& C, Ctl* ctl, NarrowSh& sh)
         const int64_t tk0 = clk();
         wg_kick(C, ctl, sh);
         if (threadIdx.x == 0) ctl->tph[7] += clk() - tk0;
+    }
+    // the moved codewords' leaving counts restart at zero, after every update
+    // of this step has been applied to them
+    if (ctl->th_on) {
+        __syncthreads();
+        const int ns = ctl->nstale;
+        for (int x = threadIdx.x; x < ns * THW; x += WG) C.th[(int64_t)ctl->stale[x / THW] * THW + x % THW] = 0;
     }
     if (threadIdx.x == 0) {
         if (ctl->trace && ctl->tpos < ctl->tcap) {
@@ -1376,10 +1424,10 @@ __device__ __forceinline__ int run_add(bool head, int v)
 // wave takes 64 consecutive patterns per round (all lanes take part in every
 // round, those past the end without an operation)
 __device__ __forceinline__ int64_t commit_units(const Cell& C, const Ctl* ctl, int64_t gt, int64_t nt,
-                                                const int64_t* T)
+                                                const int64_t* T, bool th)
 {
     const int lane = threadIdx.x & 63;
-    const int bi = ctl->bi, bp = ctl->bp;
+    const int bi = ctl->bi, bp = ctl->bp, bv = ctl->bv;
     const int64_t base = ctl->base, dv = ctl->dv;
     const uint64_t pk = C.cpk[bi];
     const bool list_on = ctl->ul_valid != 0;
@@ -1408,6 +1456,19 @@ __device__ __forceinline__ int64_t commit_units(const Cell& C, const Ctl* ctl, i
             const uint16_t ca = cnt_get(C, a), cb = cnt_get(C, b);
             cnt_put(C, a, (uint16_t)(ca - 1));
             cnt_put(C, b, (uint16_t)(cb + 1));
+            if (th) {
+                // bi leaves a's coverers and joins b's; a word left with one
+                // coverer counts for it, b's former only coverer loses b
+                const uint32_t xa = xor_get(C, a), xb = xor_get(C, b);
+                xor_put(C, a, xa ^ (uint32_t)bi);
+                xor_put(C, b, xb ^ (uint32_t)bi);
+                if (ca == 2 || cb == 1) {
+                    uint64_t apk;
+                    (void)pattern_word(C, base, pk, sub, C.R, t0, &apk);
+                    if (ca == 2) th_event(C, (int)(xa ^ (uint32_t)bi), apk, +1);
+                    if (cb == 1) th_event(C, (int)xb, nib_put(apk, bp, bv), -1);
+                }
+            }
             const uint32_t ma = 1u << (a & 31), mb = 1u << (b & 31);
             if (ca == 1) {
                 za = ma; oa_clr = ma; da = 1;
@@ -1484,7 +1545,7 @@ __device__ __forceinline__ void commit_body(const Cell& C, int bid, int nblk)
     const int64_t gt = (int64_t)bid * WG + threadIdx.x;
     __shared__ int64_t ctab[MAXN * 16];
     commit_table(C, C.cpk[ctl->bi], ctab);
-    const int64_t d = commit_units(C, ctl, gt, (int64_t)nblk * WG, ctab);
+    const int64_t d = commit_units(C, ctl, gt, (int64_t)nblk * WG, ctab, ctl->th_on != 0);
     const int64_t ds = wg_sum(d, red);
     if (threadIdx.x == 0 && ds) atomicAdd(reinterpret_cast<unsigned long long*>(&ctl->dunc), (unsigned long long)ds);
 }
@@ -1517,7 +1578,7 @@ __device__ __forceinline__ void commit_api_body(const Cell& C, int bid, int nblk
     const int64_t gt = (int64_t)bid * WG + threadIdx.x;
     __shared__ int64_t ctab[MAXN * 16];
     commit_table(C, C.cpk[ctl->bi], ctab);
-    const int64_t d = commit_units(C, ctl, gt, (int64_t)nblk * WG, ctab);
+    const int64_t d = commit_units(C, ctl, gt, (int64_t)nblk * WG, ctab, false);
     const int64_t ds = wg_sum(d, red);
     if (threadIdx.x == 0 && ds) atomicAdd(reinterpret_cast<unsigned long long*>(&ctl->dunc), (unsigned long long)ds);
     if (!wg_arrive_last(C.arrive, &ctl->cmdone, &last, bid, nblk)) return;
@@ -1706,6 +1767,48 @@ __global__ void __launch_bounds__(WG) k_first_from(const Cell C, int64_t start, 
     __shared__ NarrowSh sh;
     const int64_t w = wg_first_from(C, start, sh);
     if (threadIdx.x == 0) *out = w;
+}
+
+// ----------------------------------------- coverer XOR and leaving counts
+// xor32 (zeroed first): every codeword's index XORed into the words of its ball
+__global__ void __launch_bounds__(WG) k_xor_build(const Cell C)
+{
+    const int64_t tot = (int64_t)C.M * C.V;
+    for (int64_t x = (int64_t)blockIdx.x * WG + threadIdx.x; x < tot; x += (int64_t)gridDim.x * WG) {
+        const int i = (int)(x / C.V);
+        const int64_t g = x - (int64_t)i * C.V;
+        const int64_t w = ball_word_at(C, C.cidx[i], C.cpk[i], g, nullptr);
+        atomicXor(&C.xor32[w >> 1], (uint32_t)i << (16 * (uint32_t)(w & 1)));
+    }
+}
+
+// th (zeroed first): T and H_j of every codeword, one (codeword, R-subset)
+// sphere per wave
+__global__ void __launch_bounds__(WG) k_th_all(const Cell C)
+{
+    __shared__ WalkTab wtab[WAVES];
+    const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int64_t nu = (int64_t)C.M * C.F;
+    for (int64_t u = (int64_t)blockIdx.x * WAVES + wv; u < nu; u += (int64_t)gridDim.x * WAVES) {
+        const int i = (int)(u / C.F);
+        const int f = (int)(u - (int64_t)i * C.F);
+        const uint32_t sub = C.fsub[f];
+        const int64_t c1 = sphere_count(C, C.ob, C.cidx[i], C.cpk[i], sub, wtab[wv]);
+        if (lane == 0 && c1) {
+            int32_t* t = C.th + (int64_t)i * THW;
+            atomicAdd(&t[0], (int)c1);
+            for (int k = 0; k < C.R; k++) atomicAdd(&t[1 + ((sub >> (4 * k)) & 15)], (int)c1);
+        }
+    }
+}
+
+// the search's use of th from here on: on, no codeword pending a recount
+__global__ void k_th_setup(const Cell C, int on)
+{
+    if (threadIdx.x == 0 && blockIdx.x == 0) {
+        C.ctl->th_on = on;
+        C.ctl->nstale = 0;
+    }
 }
 
 // --------------------------------------------- evaluator through the ABI

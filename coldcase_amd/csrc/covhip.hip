@@ -48,6 +48,10 @@ struct cov_cell {
     int dt_threads = 0;                   // threads per transform workgroup (0: default)
     bool dist_valid = false;              // dist holds the transform of the current code
     bool cnt_valid = true;                // cnt, planes and counts match the code
+    bool th_valid = false;                // the coverer XORs and leaving counts match cnt
+    int kick = 2;                         // the search's kicks per stagnation (as in the control block)
+    int leave_mode = COV_LEAVE_AUTO;      // how the steps find the leaving side (cov_set_leave_mode)
+    int th_on = 0;                        // the control block's th_on as last set from here
     // recorded on a batch's stream after each batched launch that includes
     // the cell: the cell's own stream waits for it before touching the cell,
     // and its memory is not freed before it
@@ -145,7 +149,7 @@ int shape_of(int q, int n, int R, int M, Shape* s)
 // device bytes of each array, in allocation order
 struct Sizes {
     int64_t cnt, plane, blk, code, cpk, cidx, tabu, fsub, psub, bsub, ssub, ul_idx, ul_pk,
-        shuf, acc, slot_cw, slot_pos, rbuf, ctl, cell, small, arrive, dirty, jump;
+        shuf, acc, slot_cw, slot_pos, rbuf, ctl, cell, small, arrive, dirty, jump, xor32, th;
 };
 
 Sizes sizes_of(const Shape& s)
@@ -175,6 +179,8 @@ Sizes sizes_of(const Shape& s)
     z.arrive = ARR_G * ARR_STRIDE * 4;
     z.dirty = DIRTY_CAP * 4;
     z.jump = RNG_JUMP_WORDS * 8;
+    z.xor32 = ((s.N + 1) / 2 + 16) * 4;
+    z.th = (int64_t)s.M * THW * 4;
     return z;
 }
 
@@ -184,7 +190,8 @@ int64_t total_bytes(const Shape& s)
     // plus the distance array and codeword bitmap the transform allocates on first use
     return z.cnt + 3 * z.plane + z.blk + 2 * z.code + z.cpk + z.cidx + z.tabu + z.fsub + z.psub +
            z.bsub + z.ssub + z.ul_idx + z.ul_pk + z.shuf + z.acc + z.slot_cw + z.slot_pos + z.rbuf +
-           z.ctl + z.cell + z.small + z.arrive + z.dirty + z.jump + (s.N + 64) + ((s.N >> 5) + 2) * 4;
+           z.ctl + z.cell + z.small + z.arrive + z.dirty + z.jump + z.xor32 + z.th + (s.N + 64) +
+           ((s.N >> 5) + 2) * 4;
 }
 
 template <typename T>
@@ -320,6 +327,7 @@ int timed_end(cov_cell* h, int kind, hipEvent_t a)
 int do_recount(cov_cell* h)
 {
     h->cnt_valid = true;
+    h->th_valid = false;
     hipEvent_t a = nullptr;
     RC(timed_begin(h, 3, &a));
     const int g = (int)std::min<int64_t>(h->plan.ntile, 8192);
@@ -535,6 +543,8 @@ int cov_create(int q, int n, int R, int M, cov_cell_t** out)
     AL(C.arrive, z.arrive);
     AL(C.dirty, z.dirty);
     AL(C.jump, z.jump);
+    AL(C.xor32, z.xor32);
+    AL(C.th, z.th);
 #undef AL
     C.scratch64 = (decltype(C.scratch64))h->d_small;
 
@@ -653,6 +663,7 @@ int cov_sync(cov_cell_t* h)
 
 int cov_set_code_flags(cov_cell_t* h, const uint8_t* digits, int flags)
 {
+    if (h) h->th_valid = false;
     if (h) h->dist_valid = false;
     if (!h || !digits || (flags & ~COV_NO_RECOUNT)) return COV_EPARAM;
     const Cell& C = h->h;
@@ -708,6 +719,7 @@ int cov_recount(cov_cell_t* h)
 
 int cov_ball_update(cov_cell_t* h, const int64_t* centers, int k, int delta)
 {
+    if (h) h->th_valid = false;
     if (h && !h->cnt_valid) { h->err = "coverage counts not built (cov_recount)"; return COV_ESTATE; }
     if (!h || (k > 0 && !centers) || (delta != 1 && delta != -1) || k < 0) return COV_EPARAM;
     if (k == 0) return COV_OK;
@@ -822,6 +834,7 @@ int cov_eval_moves(cov_cell_t* h, const uint8_t* u, int k, const int32_t* cw, in
 
 int cov_commit(cov_cell_t* h, int i, int p, int v, int64_t* dunc)
 {
+    if (h) h->th_valid = false;
     if (h && !h->cnt_valid) { h->err = "coverage counts not built (cov_recount)"; return COV_ESTATE; }
     if (h) h->dist_valid = false;
     if (!h) return COV_EPARAM;
@@ -951,6 +964,8 @@ int cov_distance_stats(cov_cell_t* h, int r, int64_t* within, int* radius)
 int cov_search_params(cov_cell_t* h, int tenure, int64_t stall, int kick, int cand)
 {
     if (!h || tenure < 0 || stall < 0 || kick < 0 || cand < 1) return COV_EPARAM;
+    h->th_valid = false;
+    h->kick = kick;
     RC(set_ctl<int32_t>(h, offsetof(Ctl, tenure), tenure));
     RC(set_ctl<int64_t>(h, offsetof(Ctl, stall), stall));
     RC(set_ctl<int32_t>(h, offsetof(Ctl, kick), kick));
@@ -960,6 +975,7 @@ int cov_search_params(cov_cell_t* h, int tenure, int64_t stall, int kick, int ca
 
 int cov_search_init(cov_cell_t* h, uint64_t seed, const uint8_t* seed_code, int mseed, int greedy)
 {
+    if (h) h->th_valid = false;
     if (h) h->dist_valid = false;
     if (!h || mseed < 0 || (mseed > 0 && !seed_code)) return COV_EPARAM;
     Cell& C = h->h;
@@ -973,6 +989,7 @@ int cov_search_init(cov_cell_t* h, uint64_t seed, const uint8_t* seed_code, int 
     c.tenure = old.tenure; c.stall = old.stall; c.kick = old.kick; c.cand = old.cand; c.budget = old.budget;
     splitmix_seed(seed, c.rst);
     RC(upload(h, C.ctl, &c, sizeof(Ctl)));
+    h->th_on = 0;
     HCK(h, hipMemsetAsync(C.tabu, 0, (size_t)M * n * C.q * 4, h->stream));
     auto index_of = [&](const uint8_t* w) {
         int64_t x = 0;
@@ -1059,12 +1076,50 @@ int cov_search_init(cov_cell_t* h, uint64_t seed, const uint8_t* seed_code, int 
     return COV_OK;
 }
 
+// Whether the steps keep the leaving counts (th) or walk the candidates'
+// spheres.  Kept, every word whose count crosses 1 costs a few atomics on its
+// coverer's row, so it pays where such words are rare: at an average coverage
+// M*V/N of 4 or more (e^-4 (4 + 8) = 22% of the words have count 1 or 2, and
+// far fewer at the usual 6 to 7).  More kicks per stagnation than the stale
+// list holds also mean walking.
+bool th_wanted(const cov_cell* h)
+{
+    if (h->kick + 1 > STALE_MAX || h->leave_mode == COV_LEAVE_WALK) return false;
+    if (h->leave_mode == COV_LEAVE_KEPT) return true;
+    const Cell& C = h->h;
+    return (double)C.M * (double)C.V >= 4.0 * (double)C.N;
+}
+
+// before a search launch on stream st: (re)build the coverer XOR and the
+// leaving counts unless they are up to date, or switch the steps to walking
+int th_build(cov_cell* h, hipStream_t st)
+{
+    const int on = th_wanted(h) ? 1 : 0;
+    if (on && h->th_valid) return COV_OK;
+    if (!on && h->th_on == 0) return COV_OK;
+    const Cell& C = h->h;
+    if (on) {
+        HCK(h, hipMemsetAsync(C.xor32, 0, (size_t)((C.N + 1) / 2) * 4, st));
+        hipLaunchKernelGGL(k_xor_build, dim3(grid_for((int64_t)C.M * C.V, WG, 1, 8192)), dim3(WG), 0, st, h->h);
+        HCK(h, hipGetLastError());
+        HCK(h, hipMemsetAsync(C.th, 0, (size_t)C.M * THW * 4, st));
+        hipLaunchKernelGGL(k_th_all, dim3(grid_for((int64_t)C.M * C.F, WAVES, 1, 8192)), dim3(WG), 0, st, h->h);
+        HCK(h, hipGetLastError());
+    }
+    hipLaunchKernelGGL(k_th_setup, dim3(1), dim3(64), 0, st, h->h, on);
+    HCK(h, hipGetLastError());
+    h->th_valid = on != 0;
+    h->th_on = on;
+    return COV_OK;
+}
+
 int cov_search_launch(cov_cell_t* h, int64_t nsteps, int mode)
 {
     if (h && !h->cnt_valid) { h->err = "coverage counts not built (cov_recount)"; return COV_ESTATE; }
     if (h) h->dist_valid = false;
     if (!h || nsteps < 0 || (mode != COV_MODE_PLAIN && mode != COV_MODE_GRAPH)) return COV_EPARAM;
     if (nsteps == 0) return COV_OK;
+    RC(th_build(h, h->stream));
     hipLaunchKernelGGL(k_begin_run, dim3(1), dim3(WG), 0, h->stream, h->h, nsteps);
     HCK(h, hipGetLastError());
     int64_t left = nsteps;
@@ -1134,6 +1189,14 @@ int cov_set_launch(cov_cell_t* h, int eval_groups, int commit_groups)
         return COV_EPARAM;
     h->eval_groups = eval_groups;
     h->commit_groups = commit_groups;
+    return COV_OK;
+}
+
+int cov_set_leave_mode(cov_cell_t* h, int mode)
+{
+    if (!h || mode < COV_LEAVE_AUTO || mode > COV_LEAVE_KEPT) return COV_EPARAM;
+    h->leave_mode = mode;
+    h->th_valid = false;
     return COV_OK;
 }
 
@@ -1278,7 +1341,11 @@ int cov_batch_launch(cov_batch_t* b, int64_t nsteps, int mode)
     if (!b || nsteps < 0 || (mode != COV_MODE_PLAIN && mode != COV_MODE_GRAPH)) return COV_EPARAM;
     if (nsteps == 0) return COV_OK;
     const int k = (int)b->cells.size();
-    for (cov_cell* c : b->cells) c->dist_valid = false;
+    for (cov_cell* c : b->cells) {
+        c->dist_valid = false;
+        const int rc = th_build(c, b->stream);
+        if (rc != COV_OK) { b->err = c->err; return rc; }
+    }
     // (k_begin_run_b also turns the cells' trace records off)
     hipLaunchKernelGGL(k_begin_run_b, dim3(k), dim3(WG), 0, b->stream, b->d_cells, nsteps);
     HCK(b, hipGetLastError());

@@ -22,6 +22,7 @@ COV_OK = 0
 COV_TR_W = 16
 COV_DSEL_RANDOM = 1 << 60
 MODE_PLAIN, MODE_GRAPH = 0, 1
+LEAVE_AUTO, LEAVE_WALK, LEAVE_KEPT = 0, 1, 2
 TR_FIELDS = ['it', 'u', 'ncand', 'ncw', 'bi', 'bp', 'bv', 'oldv', 'dsel', 'dunc',
              'unc', 'best', 'kick', 'draws', 'since', 'nties']
 
@@ -32,7 +33,7 @@ EXPORTS = ['cov_version', 'cov_strerror', 'cov_mem_bytes', 'cov_create', 'cov_de
            'cov_first_uncovered_from', 'cov_read_cnt', 'cov_eval_moves', 'cov_commit',
            'cov_distance_transform', 'cov_distance_stats', 'cov_gather_far', 'cov_search_init', 'cov_search_params',
            'cov_search_run', 'cov_search_launch', 'cov_search_state', 'cov_set_launch',
-           'cov_get_launch', 'cov_set_transform_tile', 'cov_timing', 'cov_timing_read', 'cov_search_phases',
+           'cov_get_launch', 'cov_set_leave_mode', 'cov_set_transform_tile', 'cov_timing', 'cov_timing_read', 'cov_search_phases',
            'cov_search_bytes', 'cov_batch_create', 'cov_batch_destroy', 'cov_batch_last_error',
            'cov_batch_size', 'cov_batch_launch', 'cov_batch_sync', 'cov_batch_query', 'cov_batch_run']
 
@@ -94,6 +95,7 @@ def lib():
         'cov_search_launch': (i32, [vp, i64, i32]),
         'cov_search_state': (i32, [vp, ctypes.POINTER(State)]),
         'cov_set_launch': (i32, [vp, i32, i32]),
+        'cov_set_leave_mode': (i32, [vp, i32]),
         'cov_get_launch': (i32, [vp, vp, vp]),
         'cov_set_transform_tile': (i32, [vp, i32, i32]),
         'cov_timing': (i32, [vp, i32]),
@@ -281,6 +283,11 @@ class Cell:
 
     def set_launch(self, eval_groups, commit_groups):
         self._check(lib().cov_set_launch(self.h, int(eval_groups), int(commit_groups)), 'cov_set_launch')
+
+    def set_leave_mode(self, mode):
+        """LEAVE_WALK, LEAVE_KEPT or LEAVE_AUTO: how steps find the leaving
+        side of the deltas (the moves are the same)."""
+        self._check(lib().cov_set_leave_mode(self.h, int(mode)), 'cov_set_leave_mode')
 
     def get_launch(self):
         e = np.zeros(1, np.int32)

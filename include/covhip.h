@@ -150,6 +150,19 @@ int     cov_search_state(cov_cell_t *h, cov_state_t *st);
 /* Workgroups of the evaluator and commit kernels (the autotuner sets them). */
 int     cov_set_launch(cov_cell_t *h, int eval_groups, int commit_groups);
 int     cov_get_launch(const cov_cell_t *h, int *eval_groups, int *commit_groups);
+/* How a step finds the leaving side of each candidate's delta (the words only
+ * the moved codeword covers that the move uncovers).  COV_LEAVE_WALK walks
+ * the candidates' distance-R spheres every step; COV_LEAVE_KEPT keeps per
+ * codeword counts up to date through the commits (2 more bytes per word, a
+ * few atomics per word whose count crosses 1) and recounts only the codewords
+ * that moved; COV_LEAVE_AUTO (the default) keeps them when the average
+ * coverage M*|ball|/q^n is 4 or more.  The moves are the same either way.  No
+ * counterpart in the reference, whose evaluator counts per candidate
+ * (cov/engine/covsearch2e.c). */
+#define COV_LEAVE_AUTO 0
+#define COV_LEAVE_WALK 1
+#define COV_LEAVE_KEPT 2
+int     cov_set_leave_mode(cov_cell_t *h, int mode);
 /* LDS bytes per distance-transform tile (0 = the default, 49152) and threads
  * per transform workgroup (0 = the default, 512; else 256 or 1024); the tile
  * plan groups the n axes into as few HBM passes as tiles of this size allow. */

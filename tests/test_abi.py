@@ -35,8 +35,9 @@ def test_version_and_errors():
 def test_mem_bytes_and_limits():
     from coldcase_amd.engine import CovError, mem_bytes
     b = mem_bytes(6, 9, 4, 660)
-    # cnt (2 B/word) + three bit planes + list + distance array and bitmap
-    assert 3.5 * 6 ** 9 < b < 6.0 * 6 ** 9
+    # cnt and the coverer XOR (2 B/word each) + three bit planes + list +
+    # distance array and bitmap
+    assert 5.5 * 6 ** 9 < b < 8.0 * 6 ** 9
     assert mem_bytes(6, 10, 4, 2784) > 121e6
     for bad in [(1, 5, 1, 3), (17, 3, 1, 3), (3, 5, 5, 3), (3, 5, 1, 70000), (3, 17, 1, 3)]:
         with pytest.raises(CovError):

@@ -18,11 +18,12 @@ def _cell(q, n, R, M):
 
 
 def compare(q, n, R, M, seed, steps, tabu=10, stall=5000, cand=100000, greedy=False,
-            seed_code=None, mode=0, check_cnt=True, threads=1):
-    print('case K%d(%d,%d) M=%d seed=%d steps=%d stall=%d mode=%d' % (q, n, R, M, seed, steps, stall, mode),
-          flush=True)
+            seed_code=None, mode=0, check_cnt=True, threads=1, leave=0):
+    print('case K%d(%d,%d) M=%d seed=%d steps=%d stall=%d mode=%d leave=%d'
+          % (q, n, R, M, seed, steps, stall, mode, leave), flush=True)
     cell = _cell(q, n, R, M)
     cell.search_params(tenure=tabu, stall=stall, kick=2, cand=cand)
+    cell.set_leave_mode(leave)
     cell.search_init(seed, seed_code=seed_code, greedy=greedy)
     o = Oracle(q, n, R, M, seed, threads=threads).init(seed_code, greedy=greedy)
     st0 = cell.state()
@@ -49,30 +50,49 @@ def compare(q, n, R, M, seed, steps, tabu=10, stall=5000, cand=100000, greedy=Fa
     return gpu
 
 
-def test_k5_7_2_random_tabu():
-    compare(5, 7, 2, 400, 1, 2000)
+# the leaving side walked every step, or kept per codeword through the commits
+# (the automatic choice keeps it on the K6(9,4)-sized cells below)
+LEAVES = [1, 2]
 
 
-def test_k6_6_3_kicks():
-    tr = compare(6, 6, 3, 41, 2, 600, tabu=3, stall=20)
+@pytest.mark.parametrize('leave', LEAVES)
+def test_k5_7_2_random_tabu(leave):
+    compare(5, 7, 2, 400, 1, 2000, leave=leave)
+
+
+@pytest.mark.parametrize('leave', LEAVES)
+def test_k6_6_3_kicks(leave):
+    tr = compare(6, 6, 3, 41, 2, 600, tabu=3, stall=20, leave=leave)
     assert tr[:, TR_FIELDS.index('kick')].sum() > 5
 
 
-def test_small_cells_greedy():
+@pytest.mark.parametrize('leave', LEAVES)
+def test_small_cells_greedy(leave):
     for (q, n, R, M, it) in [(3, 4, 1, 9, 200), (2, 7, 2, 8, 300), (7, 3, 1, 20, 300), (5, 4, 3, 3, 50)]:
-        compare(q, n, R, M, 7, it, tabu=0, greedy=True)
+        compare(q, n, R, M, 7, it, tabu=0, greedy=True, leave=leave)
 
 
-def test_cand_sample():
-    compare(6, 6, 3, 41, 5, 300, cand=3)
+@pytest.mark.parametrize('leave', LEAVES)
+def test_cand_sample(leave):
+    compare(6, 6, 3, 41, 5, 300, cand=3, leave=leave)
 
 
-def test_graph_mode_identical():
-    compare(5, 7, 2, 400, 3, 1000, mode=1)
+@pytest.mark.parametrize('leave', LEAVES)
+def test_graph_mode_identical(leave):
+    compare(5, 7, 2, 400, 3, 1000, mode=1, leave=leave)
 
 
-def test_k6_9_4_random_tabu():
-    compare(6, 9, 4, 660, 1, 200, threads=8)
+@pytest.mark.parametrize('leave', [0, 1])
+def test_k6_9_4_random_tabu(leave):
+    compare(6, 9, 4, 660, 1, 200, threads=8, leave=leave)
+
+
+def test_k6_8_4_kicks_kept():
+    """Kicks on a cell where the counts are kept by default (average
+    coverage above 4): the kicked codewords' balls move every count and XOR
+    they touch, and their own rows are recounted."""
+    tr = compare(6, 8, 4, 169, 4, 400, tabu=3, stall=15, threads=8)
+    assert tr[:, TR_FIELDS.index('kick')].sum() > 3
 
 
 def test_large_cells_random_init():
