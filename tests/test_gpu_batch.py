@@ -16,11 +16,12 @@ MIX = [(5, 7, 2, 400, 1, 10, 5000), (6, 6, 3, 41, 2, 3, 20), (4, 8, 3, 60, 3, 10
        (3, 4, 1, 9, 7, 10, 5000), (2, 10, 3, 70, 8, 10, 300)]
 
 
-def _cell(spec):
+def _cell(spec, leave=0):
     from coldcase_amd import Cell
     q, n, R, M, seed, tenure, stall = spec
     c = Cell(q, n, R, M)
     c.search_params(tenure=tenure, stall=stall, kick=2)
+    c.set_leave_mode(leave)
     c.search_init(seed)
     return c
 
@@ -32,8 +33,8 @@ def _snap(c):
             c.code().tobytes(), best.tobytes(), bu, c.cnt().tobytes())
 
 
-def _solo(spec, steps):
-    c = _cell(spec)
+def _solo(spec, steps, leave=0):
+    c = _cell(spec, leave)
     c.run(steps, mode=1)
     s = _snap(c)
     c.close()
@@ -54,6 +55,23 @@ def test_batch_equals_solo(mode):
         got = _snap(c)
         assert got[:6] == want[:6], (spec, got[:6], want[:6])
         assert got[6:] == want[6:], spec
+        c.close()
+
+
+def test_batch_kept_equals_solo_walked():
+    """Every cell of the mix keeps its leaving counts inside the batch
+    (kicks included) and must end exactly where it ends alone with the
+    leaving side walked."""
+    from coldcase_amd import Batch
+    from coldcase_amd.engine import LEAVE_KEPT, LEAVE_WALK
+    print('case batch mix kept vs solo walked', flush=True)
+    cells = [_cell(s, LEAVE_KEPT) for s in MIX]
+    with Batch(cells) as b:
+        b.run(200, mode=1)
+        b.run(187, mode=0)
+    assert cells[1].state()['kicks'] > 5
+    for spec, c in zip(MIX, cells):
+        assert _snap(c) == _solo(spec, 387, LEAVE_WALK), spec
         c.close()
 
 
