@@ -645,6 +645,9 @@ struct ShufSh {
     uint32_t j[SHUF_PAR + 4] __attribute__((aligned(16)));   // swap partner of each draw
     int32_t g[SHUF_PAR];
     int32_t f[SHUF_PAR];
+    int32_t off[SHUF_PAR + 1];   // draws per partner value, then where each value's draws start
+    int32_t item[SHUF_PAR];      // the draws grouped by partner value
+    int32_t wsum[WG / 64];
 };
 
 // Partial Fisher-Yates over the ncand candidates (original index k =
@@ -657,13 +660,100 @@ struct ShufSh {
 // such a swap puts there what slot t held at time t.  So with
 //   g(s) = last t < s with j_t = j_s,   f(t) = last t' < t with j_t' = t,
 // out[s] = j_s if g(s) < 0, else root(g(s)), where root(t) = t if f(t) < 0,
-// else root(f(t)).  All threads of one workgroup: the (draw, chunk of 32
-// earlier draws) pairs are scanned in parallel with LDS maxima, then every
-// thread follows its own chains.
+// else root(f(t)).  All threads of one workgroup; every thread then follows
+// its own chains.  g and f are both "the last earlier draw with a given
+// partner value": with at most SHUF_PAR candidates the draws are grouped by
+// partner value (a count, a prefix sum and a scatter in LDS), and each draw
+// looks through its group, a few entries; with more candidates the (draw,
+// chunk of 32 earlier draws) pairs are scanned with LDS maxima.
 __device__ __forceinline__ void wg_shuffle(const Cell& C, const Ctl* ctl, ShufSh& sh)
 {
     const int tid = threadIdx.x;
     const int64_t ncand = ctl->ncand, take = ctl->take, pos0 = ctl->sh_pos;
+    if (ncand <= SHUF_PAR) {
+        const int nc = (int)ncand, tk = (int)take;
+        for (int v = tid; v <= nc; v += WG) sh.off[v] = 0;
+        for (int s = tid; s < tk; s += WG)
+            sh.j[s] = (uint32_t)(s + (int)(ring_at(C, pos0 + s) % (uint64_t)(nc - s)));
+        __syncthreads();
+        for (int s = tid; s < tk; s += WG) atomicAdd(&sh.off[sh.j[s]], 1);
+        __syncthreads();
+        // exclusive prefix sum of the counts: thread tid owns values
+        // 4 tid .. 4 tid + 3 (SHUF_PAR = 4 WG)
+        static_assert(SHUF_PAR == 4 * WG, "four values per thread");
+        const int lane = tid & 63, wv = tid >> 6;
+        int c4[4], tot = 0;
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            const int v = 4 * tid + k;
+            c4[k] = v < nc ? sh.off[v] : 0;
+            tot += c4[k];
+        }
+        int incl = tot;
+#pragma unroll
+        for (int o = 1; o < 64; o <<= 1) {
+            const int y = __shfl_up(incl, o, 64);
+            if (lane >= o) incl += y;
+        }
+        if (lane == 63) sh.wsum[wv] = incl;
+        __syncthreads();
+        int base = incl - tot;
+        for (int w = 0; w < wv; w++) base += sh.wsum[w];
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            const int v = 4 * tid + k;
+            if (v < nc) {
+                sh.off[v] = base;
+                sh.g[v] = base;        // the scatter's cursor for value v
+            }
+            base += c4[k];
+        }
+        if (tid == 0) sh.off[nc] = tk;
+        __syncthreads();
+        for (int s = tid; s < tk; s += WG) sh.item[atomicAdd(&sh.g[sh.j[s]], 1)] = s;
+        __syncthreads();
+        // the last earlier draw with partner j_s, and with partner s
+        int gs[SHUF_PAR / WG], fs[SHUF_PAR / WG];
+#pragma unroll
+        for (int k = 0; k < SHUF_PAR / WG; k++) {
+            const int s = k * WG + tid;
+            gs[k] = -1;
+            fs[k] = -1;
+            if (s < tk) {
+                const int v = (int)sh.j[s];
+                for (int x = sh.off[v]; x < sh.off[v + 1]; x++) {
+                    const int t = sh.item[x];
+                    if (t < s && t > gs[k]) gs[k] = t;
+                }
+                for (int x = sh.off[s]; x < sh.off[s + 1]; x++) {
+                    const int t = sh.item[x];
+                    if (t < s && t > fs[k]) fs[k] = t;
+                }
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int k = 0; k < SHUF_PAR / WG; k++) {
+            const int s = k * WG + tid;
+            if (s < tk) {
+                sh.g[s] = gs[k];
+                sh.f[s] = fs[k];
+            }
+        }
+        __syncthreads();
+        for (int s = tid; s < tk; s += WG) {
+            int t = sh.g[s];
+            uint32_t out;
+            if (t < 0) {
+                out = sh.j[s];
+            } else {
+                while (sh.f[t] >= 0) t = sh.f[t];
+                out = (uint32_t)t;
+            }
+            C.shuf[s] = out;
+        }
+        return;
+    }
     if (take <= SHUF_PAR) {
         for (int64_t s = tid; s < take; s += WG) {
             sh.j[s] = (uint32_t)(s + (int64_t)(ring_at(C, pos0 + s) % (uint64_t)(ncand - s)));
