@@ -384,14 +384,30 @@ __device__ __forceinline__ void wg_begin(const Cell& C, Ctl* ctl, NarrowSh& sh)
         }
     };
     const bool one_round = C.M <= SCAN_ILP * WG;
+    // larger codes: SCAN_BIG loads in flight per thread per round (the scan
+    // is a chain of load round trips on one workgroup)
+    constexpr int SCAN_BIG = 32;
     auto mark_pass = [&](int target, bool with_min) {
         if (one_round) {
             mark(0, target, with_min);
             return;
         }
-        for (int i0 = 0; i0 < C.M; i0 += SCAN_ILP * WG) {
-            load(i0);
-            mark(i0, target, with_min);
+        for (int i0 = 0; i0 < C.M; i0 += SCAN_BIG * WG) {
+            uint64_t cb[SCAN_BIG];
+#pragma unroll
+            for (int k = 0; k < SCAN_BIG; k++) {
+                const int i = i0 + k * WG + threadIdx.x;
+                cb[k] = i < C.M ? C.cpk[i] : 0;
+            }
+#pragma unroll
+            for (int k = 0; k < SCAN_BIG; k++) {
+                const int i = i0 + k * WG + threadIdx.x;
+                const int64_t d = nib_dist(cb[k], upk);
+                if (with_min && i < C.M && d < dmin) dmin = d;
+                const uint64_t m = __ballot(i < C.M && d == target);
+                const int wi = ((i0 + k * WG) >> 6) + wv;
+                if (lane == 0 && wi < nwm) sh.qm[wi] = m;
+            }
         }
     };
     if (one_round) load(0);
@@ -448,10 +464,42 @@ __device__ __forceinline__ void wg_begin(const Cell& C, Ctl* ctl, NarrowSh& sh)
             if (i < C.M && ((sh.qm[i >> 6] >> (i & 63)) & 1)) emit(i, c[k]);
         }
     } else {
-        // each wave takes whole mask words, lane b the word's bit b
-        for (int j = wv; j < nwm; j += WAVES) {
-            const uint64_t m = sh.qm[j];
-            if ((m >> lane) & 1) emit(64 * j + lane, C.cpk[64 * j + lane]);
+        // each thread takes the QPT mask words it counted and emits their set
+        // bits (a few codewords in all), reading the digits again: the first
+        // two of each word with their loads issued together, any others after
+        uint64_t mk[QPT];
+        int id2[QPT][2];
+        uint64_t pk2[QPT][2];
+#pragma unroll
+        for (int k = 0; k < QPT; k++) {
+            const int j = QPT * threadIdx.x + k;
+            mk[k] = j < nwm ? sh.qm[j] : 0;
+#pragma unroll
+            for (int r = 0; r < 2; r++) {
+                id2[k][r] = -1;
+                if (mk[k]) {
+                    id2[k][r] = 64 * j + __ffsll((unsigned long long)mk[k]) - 1;
+                    mk[k] &= mk[k] - 1;
+                }
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < QPT; k++)
+#pragma unroll
+            for (int r = 0; r < 2; r++) pk2[k][r] = id2[k][r] >= 0 ? C.cpk[id2[k][r]] : 0;
+#pragma unroll
+        for (int k = 0; k < QPT; k++)
+#pragma unroll
+            for (int r = 0; r < 2; r++)
+                if (id2[k][r] >= 0) emit(id2[k][r], pk2[k][r]);
+#pragma unroll
+        for (int k = 0; k < QPT; k++) {
+            const int j = QPT * threadIdx.x + k;
+            while (mk[k]) {
+                const int i = 64 * j + __ffsll((unsigned long long)mk[k]) - 1;
+                mk[k] &= mk[k] - 1;
+                emit(i, C.cpk[i]);
+            }
         }
     }
     const int64_t ncand = nq * dtar;
