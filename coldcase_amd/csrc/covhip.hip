@@ -624,7 +624,7 @@ int cov_create(int q, int n, int R, int M, cov_cell_t** out)
     // default launch shape: about 32 candidate codewords per step
     const int64_t eval_waves = 32 * ((int64_t)s.F + (int64_t)(R + 1) * s.E) *
                                std::max<int64_t>(1, s.P / 64);
-    h->eval_groups = grid_for(eval_waves / 4, 8, 16, 1024);
+    h->eval_groups = grid_for(eval_waves / 4, 4, 16, 1024);
     h->commit_groups = grid_for((int64_t)s.E * s.P, WG, 1, 2048);
     // the uploads above read this frame's tables
     {
