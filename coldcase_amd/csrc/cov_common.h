@@ -165,6 +165,8 @@ struct Cell {
     GPTR(uint64_t) jump;         // the rng jump matrices J_1 .. J_{RNG_SPLIT-1}
     GPTR(uint32_t) xor32;        // per word, the XOR of the indices of its covering codewords (uint16 pairs)
     GPTR(int32_t) th;            // T and H_j per codeword, THW each
+    GPTR(int32_t) cst;           // staged selection reads: codeword, slot, digits, tabu stamp (4 each)
+    GPTR(int64_t) cbs;           // staged selection reads: word index of the candidate's codeword
     GPTR(uint8_t) dist;          // distance transform output (N bytes), lazily
     GPTR(Ctl) ctl;
 };

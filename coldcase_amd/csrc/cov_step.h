@@ -46,6 +46,10 @@
 namespace cov {
 
 constexpr int SHUF_PAR = 1024;     // draws handled by the parallel shuffle
+constexpr int EMIT_LDS = 1024;     // begin: slots listed in LDS before their digits are read
+constexpr int SEL_PER = 4;         // candidates per lane in the one-wave selection
+constexpr int SEL_WAVE = 64 * SEL_PER;   // the most candidates it takes
+static_assert(SEL_WAVE == WG, "one staged candidate per thread");
 
 constexpr int MAX_M_WORDS = (65535 + 63) / 64;   // 64-codeword mask words of the largest code
 
@@ -57,6 +61,7 @@ struct NarrowSh {
     int32_t v32[8];
     uint64_t qm[MAX_M_WORDS];   // begin: codewords at the target distance
     int32_t qfirst[MAX_M_WORDS]; // begin: first slot of each mask word's codewords
+    int32_t elist[EMIT_LDS];     // begin: the codeword of each of the first EMIT_LDS slots
 };
 
 __device__ __forceinline__ uint32_t zbit_acq(const Cell& C, int64_t w)
@@ -384,12 +389,20 @@ __device__ __forceinline__ void wg_begin(const Cell& C, Ctl* ctl, NarrowSh& sh)
         }
     };
     const bool one_round = C.M <= SCAN_ILP * WG;
-    // larger codes: SCAN_BIG loads in flight per thread per round (the scan
-    // is a chain of load round trips on one workgroup)
-    constexpr int SCAN_BIG = 32;
+    // codes above SCAN_DEEP codewords: SCAN_BIG loads in flight per thread
+    // per round (the scan is a chain of load round trips on one workgroup);
+    // below, rounds of SCAN_ILP cost fewer instructions
+    constexpr int SCAN_BIG = 32, SCAN_DEEP = 16384;
     auto mark_pass = [&](int target, bool with_min) {
         if (one_round) {
             mark(0, target, with_min);
+            return;
+        }
+        if (C.M <= SCAN_DEEP) {
+            for (int i0 = 0; i0 < C.M; i0 += SCAN_ILP * WG) {
+                load(i0);
+                mark(i0, target, with_min);
+            }
             return;
         }
         for (int i0 = 0; i0 < C.M; i0 += SCAN_BIG * WG) {
@@ -464,42 +477,28 @@ __device__ __forceinline__ void wg_begin(const Cell& C, Ctl* ctl, NarrowSh& sh)
             if (i < C.M && ((sh.qm[i >> 6] >> (i & 63)) & 1)) emit(i, c[k]);
         }
     } else {
-        // each thread takes the QPT mask words it counted and emits their set
-        // bits (a few codewords in all), reading the digits again: the first
-        // two of each word with their loads issued together, any others after
-        uint64_t mk[QPT];
-        int id2[QPT][2];
-        uint64_t pk2[QPT][2];
+        // each thread lists the set bits of the QPT mask words it counted at
+        // their slots (in LDS, past EMIT_LDS in slot_cw itself); then one slot
+        // per thread reads its codeword's digits, the loads issued together
 #pragma unroll
         for (int k = 0; k < QPT; k++) {
             const int j = QPT * threadIdx.x + k;
-            mk[k] = j < nwm ? sh.qm[j] : 0;
-#pragma unroll
-            for (int r = 0; r < 2; r++) {
-                id2[k][r] = -1;
-                if (mk[k]) {
-                    id2[k][r] = 64 * j + __ffsll((unsigned long long)mk[k]) - 1;
-                    mk[k] &= mk[k] - 1;
+            if (j < nwm) {
+                uint64_t m = sh.qm[j];
+                int r = sh.qfirst[j];
+                while (m) {
+                    const int i = 64 * j + __ffsll((unsigned long long)m) - 1;
+                    m &= m - 1;
+                    if (r < EMIT_LDS) sh.elist[r] = i;
+                    else C.slot_cw[r] = i;
+                    r++;
                 }
             }
         }
-#pragma unroll
-        for (int k = 0; k < QPT; k++)
-#pragma unroll
-            for (int r = 0; r < 2; r++) pk2[k][r] = id2[k][r] >= 0 ? C.cpk[id2[k][r]] : 0;
-#pragma unroll
-        for (int k = 0; k < QPT; k++)
-#pragma unroll
-            for (int r = 0; r < 2; r++)
-                if (id2[k][r] >= 0) emit(id2[k][r], pk2[k][r]);
-#pragma unroll
-        for (int k = 0; k < QPT; k++) {
-            const int j = QPT * threadIdx.x + k;
-            while (mk[k]) {
-                const int i = 64 * j + __ffsll((unsigned long long)mk[k]) - 1;
-                mk[k] &= mk[k] - 1;
-                emit(i, C.cpk[i]);
-            }
+        __syncthreads();
+        for (int64_t sl = threadIdx.x; sl < nq; sl += WG) {
+            const int i = sl < EMIT_LDS ? sh.elist[sl] : C.slot_cw[sl];
+            emit(i, C.cpk[i]);
         }
     }
     const int64_t ncand = nq * dtar;
@@ -870,9 +869,33 @@ __device__ __forceinline__ void wg_shuffle(const Cell& C, const Ctl* ctl, ShufSh
     }
 }
 
+// The one-wave selection's reads that do not depend on the evaluator's
+// counts, made by the shuffle's workgroup once the order is known: for each
+// of the first take <= SEL_WAVE candidates in shuffled order, its codeword,
+// slot, position, new and old digit, tabu stamp and the codeword's word
+// index.  Nothing the evaluator runs changes them.  All threads of the group.
+__device__ __forceinline__ void wg_stage(const Cell& C, const Ctl* ctl)
+{
+    const int take = ctl->take;
+    if (take > SEL_WAVE) return;
+    __syncthreads();    // C.shuf was written by this workgroup
+    const int s = threadIdx.x;
+    if (s < take) {
+        const uint32_t k = C.shuf[s];
+        const int dtar = ctl->dtar;
+        const int slot = (int)(k / (uint32_t)dtar);
+        const int t = (int)(k - (uint32_t)slot * (uint32_t)dtar);
+        const int p = C.slot_pos[(int64_t)slot * MAXN + t];
+        const int i = C.slot_cw[slot];
+        const int v = nib_get(ctl->upk, p);
+        const int32_t stamp = C.tabu[((int64_t)i * C.n + p) * C.q + v];
+        const int cold = C.code[(int64_t)i * C.n + p];
+        reinterpret_cast<int4*>(C.cst)[s] = make_int4(i, slot, p | (v << 8) | (cold << 16), stamp);
+        C.cbs[s] = C.cidx[i];
+    }
+}
+
 // ------------------------------------------------------------- select
-constexpr int SEL_PER = 4;                  // candidates per lane in the one-wave selection
-constexpr int SEL_WAVE = 64 * SEL_PER;      // the most candidates it takes
 struct SelSh {
     WgRed red;
     WgScan scan;
@@ -950,44 +973,40 @@ __device__ __forceinline__ void wave_select(const Cell& C, Ctl* ctl, SelSh& sh)
         int64_t cd[SEL_PER], cbase[SEL_PER];
         bool valid[SEL_PER];
         uint64_t rvs[SEL_PER];
-        // the reads in three stages across the lane's candidates, each stage's
+        // the reads in two stages across the lane's candidates, each stage's
         // loads issued together (a candidate past `take` reads candidate 0's
-        // data, in bounds, and is not used)
-        const int dtar = ctl->dtar;
-        const uint64_t upk = ctl->upk;
+        // data, in bounds, and is not used): the ring draws and what the
+        // evaluator's shuffle group staged, then the counts
         const bool th_on = ctl->th_on != 0;
-        uint32_t kk[SEL_PER];
         int slot[SEL_PER];
         int32_t a0[SEL_PER], ap[SEL_PER], an[SEL_PER], stamp[SEL_PER];
+        int4 st[SEL_PER];
 #pragma unroll
         for (int k = 0; k < SEL_PER; k++) {
             const int64_t s = (int64_t)SEL_PER * lane + k;
+            const int64_t s0 = s < take ? s : 0;
             rvs[k] = ring_at(C, r0 + (s <= take ? s : 0));
-            kk[k] = C.shuf[s < take ? s : 0];
+            st[k] = reinterpret_cast<const int4*>(C.cst)[s0];
+            cbase[k] = C.cbs[s0];
         }
 #pragma unroll
         for (int k = 0; k < SEL_PER; k++) {
-            slot[k] = (int)(kk[k] / (uint32_t)dtar);
-            const int t = (int)(kk[k] - (uint32_t)slot[k] * (uint32_t)dtar);
-            cp[k] = C.slot_pos[(int64_t)slot[k] * MAXN + t];
-            ci[k] = C.slot_cw[slot[k]];
-            if (!th_on) a0[k] = C.acc[(int64_t)slot[k] * ACCW];
-        }
-#pragma unroll
-        for (int k = 0; k < SEL_PER; k++) {
+            ci[k] = st[k].x;
+            slot[k] = st[k].y;
+            cp[k] = st[k].z & 255;
+            cv[k] = (st[k].z >> 8) & 255;
+            cold[k] = st[k].z >> 16;
+            stamp[k] = st[k].w;
             const int32_t* a = C.acc + (int64_t)slot[k] * ACCW;
-            cv[k] = nib_get(upk, cp[k]);
             if (th_on) {
                 const int32_t* t = C.th + (int64_t)ci[k] * THW;
                 a0[k] = t[0];
                 ap[k] = t[1 + cp[k]];
             } else {
+                a0[k] = a[0];
                 ap[k] = a[1 + cp[k]];
             }
             an[k] = a[1 + MAXN + cp[k]];
-            stamp[k] = C.tabu[((int64_t)ci[k] * n + cp[k]) * q + cv[k]];
-            cold[k] = C.code[(int64_t)ci[k] * n + cp[k]];
-            cbase[k] = C.cidx[ci[k]];
         }
 #pragma unroll
         for (int k = 0; k < SEL_PER; k++) {
@@ -1251,6 +1270,7 @@ __device__ __forceinline__ void eval_body(const Cell& C, int bid, int nblk)
     const bool rng4 = nblk >= 3;
     if (bid == 0) {
         wg_shuffle(C, ctl, shs);
+        wg_stage(C, ctl);
         if (threadIdx.x == 0) ctl->tph[5] += clk() - th0;
     }
     if (bid == rng_wg && rng4) {

@@ -149,7 +149,7 @@ int shape_of(int q, int n, int R, int M, Shape* s)
 // device bytes of each array, in allocation order
 struct Sizes {
     int64_t cnt, plane, blk, code, cpk, cidx, tabu, fsub, psub, bsub, ssub, ul_idx, ul_pk,
-        shuf, acc, slot_cw, slot_pos, rbuf, ctl, cell, small, arrive, dirty, jump, xor32, th;
+        shuf, acc, slot_cw, slot_pos, rbuf, ctl, cell, small, arrive, dirty, jump, xor32, th, cst, cbs;
 };
 
 Sizes sizes_of(const Shape& s)
@@ -181,6 +181,8 @@ Sizes sizes_of(const Shape& s)
     z.jump = RNG_JUMP_WORDS * 8;
     z.xor32 = ((s.N + 1) / 2 + 16) * 4;
     z.th = (int64_t)s.M * THW * 4;
+    z.cst = SEL_WAVE * 16;
+    z.cbs = SEL_WAVE * 8;
     return z;
 }
 
@@ -190,7 +192,7 @@ int64_t total_bytes(const Shape& s)
     // plus the distance array and codeword bitmap the transform allocates on first use
     return z.cnt + 3 * z.plane + z.blk + 2 * z.code + z.cpk + z.cidx + z.tabu + z.fsub + z.psub +
            z.bsub + z.ssub + z.ul_idx + z.ul_pk + z.shuf + z.acc + z.slot_cw + z.slot_pos + z.rbuf +
-           z.ctl + z.cell + z.small + z.arrive + z.dirty + z.jump + z.xor32 + z.th + (s.N + 64) +
+           z.ctl + z.cell + z.small + z.arrive + z.dirty + z.jump + z.xor32 + z.th + z.cst + z.cbs + (s.N + 64) +
            ((s.N >> 5) + 2) * 4;
 }
 
@@ -545,6 +547,8 @@ int cov_create(int q, int n, int R, int M, cov_cell_t** out)
     AL(C.jump, z.jump);
     AL(C.xor32, z.xor32);
     AL(C.th, z.th);
+    AL(C.cst, z.cst);
+    AL(C.cbs, z.cbs);
 #undef AL
     C.scratch64 = (decltype(C.scratch64))h->d_small;
 
