@@ -74,6 +74,7 @@ struct alignas(16) Ctl {
     int32_t dtar, ncand, take, nq, use_list;
     int32_t bi, bp, bv, oldv, nties, stamp, ncw;
     int32_t cmdone;
+    uint64_t bpk;           // packed digits of codeword bi before the move (set with bi)
     int32_t th_on;          // the leaving side comes from th (else the per-step walks)
     int32_t nstale;         // codewords whose th rows the next evaluator recounts
     int32_t stale[STALE_MAX];
@@ -167,6 +168,7 @@ struct Cell {
     GPTR(int32_t) th;            // T and H_j per codeword, THW each
     GPTR(int32_t) cst;           // staged selection reads: codeword, slot, digits, tabu stamp (4 each)
     GPTR(int64_t) cbs;           // staged selection reads: word index of the candidate's codeword
+    GPTR(uint64_t) cps;          // staged selection reads: packed digits of the candidate's codeword
     GPTR(uint8_t) dist;          // distance transform output (N bytes), lazily
     GPTR(Ctl) ctl;
 };

@@ -892,6 +892,7 @@ __device__ __forceinline__ void wg_stage(const Cell& C, const Ctl* ctl)
         const int cold = C.code[(int64_t)i * C.n + p];
         reinterpret_cast<int4*>(C.cst)[s] = make_int4(i, slot, p | (v << 8) | (cold << 16), stamp);
         C.cbs[s] = C.cidx[i];
+        C.cps[s] = C.cpk[i];
     }
 }
 
@@ -901,7 +902,7 @@ struct SelSh {
     WgScan scan;
     int64_t inc[WG];
     uint64_t rv[SEL_WAVE + 1];  // one-wave selection: ring draws r0 .. r0 + take
-    int64_t pick[6];            // the selected candidate: i, p, v, old value, word index, delta
+    int64_t pick[7];            // the selected candidate: i, p, v, old value, word index, delta, digits
 };
 
 // inclusive scans over the 64 lanes of a wave (lane order)
@@ -968,9 +969,9 @@ __device__ __forceinline__ void wave_select(const Cell& C, Ctl* ctl, SelSh& sh)
         const int lane = tid;
         const int64_t take = ctl->take, it = ctl->it, unc = ctl->unc, best = ctl->best_unc;
         const int64_t r0 = ctl->ruse;
-        const int n = C.n, q = C.q;
         int ci[SEL_PER], cp[SEL_PER], cv[SEL_PER], cold[SEL_PER];
         int64_t cd[SEL_PER], cbase[SEL_PER];
+        uint64_t cpks[SEL_PER];
         bool valid[SEL_PER];
         uint64_t rvs[SEL_PER];
         // the reads in two stages across the lane's candidates, each stage's
@@ -988,6 +989,7 @@ __device__ __forceinline__ void wave_select(const Cell& C, Ctl* ctl, SelSh& sh)
             rvs[k] = ring_at(C, r0 + (s <= take ? s : 0));
             st[k] = reinterpret_cast<const int4*>(C.cst)[s0];
             cbase[k] = C.cbs[s0];
+            cpks[k] = C.cps[s0];
         }
 #pragma unroll
         for (int k = 0; k < SEL_PER; k++) {
@@ -1074,7 +1076,7 @@ __device__ __forceinline__ void wave_select(const Cell& C, Ctl* ctl, SelSh& sh)
         for (int k = 0; k < SEL_PER; k++) {
             if ((int64_t)SEL_PER * lane + k == ssel) {
                 sh.pick[0] = ci[k]; sh.pick[1] = cp[k]; sh.pick[2] = cv[k]; sh.pick[3] = cold[k];
-                sh.pick[4] = cbase[k]; sh.pick[5] = cd[k];
+                sh.pick[4] = cbase[k]; sh.pick[5] = cd[k]; sh.pick[6] = (int64_t)cpks[k];
             }
         }
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
@@ -1091,6 +1093,7 @@ __device__ __forceinline__ void wave_select(const Cell& C, Ctl* ctl, SelSh& sh)
             ctl->nties = winner >= 0 ? (int)(1 + Tall - Tball) : 0;
             ctl->stamp = (int32_t)(it + ctl->tenure + (int64_t)(rs % 4));
             ctl->base = sh.pick[4];
+            ctl->bpk = (uint64_t)sh.pick[6];
             ctl->dv = (int64_t)(bv - oldv) * C.pw[bp];
             ctl->ncw = ctl->nq;
             ctl->selected = 1;
@@ -1187,6 +1190,7 @@ __device__ __forceinline__ void wg_select(const Cell& C, Ctl* ctl, SelSh& sh)
         ctl->nties = nties;
         ctl->stamp = (int32_t)(it + ctl->tenure + (int64_t)(rs % 4));
         ctl->base = C.cidx[bi];
+        ctl->bpk = C.cpk[bi];
         ctl->dv = (int64_t)(bv - oldv) * C.pw[bp];
         ctl->ncw = ctl->nq;
         ctl->selected = 1;
@@ -1587,7 +1591,7 @@ __device__ __forceinline__ int64_t commit_units(const Cell& C, const Ctl* ctl, i
     const int lane = threadIdx.x & 63;
     const int bi = ctl->bi, bp = ctl->bp, bv = ctl->bv;
     const int64_t base = ctl->base, dv = ctl->dv;
-    const uint64_t pk = C.cpk[bi];
+    const uint64_t pk = ctl->bpk;
     const bool list_on = ctl->ul_valid != 0;
     const int64_t S = (int64_t)C.E * C.P;
     int64_t d = 0;
@@ -1702,7 +1706,7 @@ __device__ __forceinline__ void commit_body(const Cell& C, int bid, int nblk)
     if (bid == 0 && threadIdx.x == 0 && ctl->t_eve) ctl->tph[11] += clk() - ctl->t_eve;
     const int64_t gt = (int64_t)bid * WG + threadIdx.x;
     __shared__ int64_t ctab[MAXN * 16];
-    commit_table(C, C.cpk[ctl->bi], ctab);
+    commit_table(C, ctl->bpk, ctab);
     const int64_t d = commit_units(C, ctl, gt, (int64_t)nblk * WG, ctab, ctl->th_on != 0);
     const int64_t ds = wg_sum(d, red);
     if (threadIdx.x == 0 && ds) atomicAdd(reinterpret_cast<unsigned long long*>(&ctl->dunc), (unsigned long long)ds);
@@ -2016,6 +2020,7 @@ __global__ void k_set_move(const Cell C, int i, int p, int v)
     const int oldv = C.code[(int64_t)i * C.n + p];
     ctl->bi = i; ctl->bp = p; ctl->bv = v; ctl->oldv = oldv;
     ctl->base = C.cidx[i];
+    ctl->bpk = C.cpk[i];
     ctl->dv = (int64_t)(v - oldv) * C.pw[p];
     ctl->begun = 1;
     ctl->selected = 1;

@@ -149,7 +149,7 @@ int shape_of(int q, int n, int R, int M, Shape* s)
 // device bytes of each array, in allocation order
 struct Sizes {
     int64_t cnt, plane, blk, code, cpk, cidx, tabu, fsub, psub, bsub, ssub, ul_idx, ul_pk,
-        shuf, acc, slot_cw, slot_pos, rbuf, ctl, cell, small, arrive, dirty, jump, xor32, th, cst, cbs;
+        shuf, acc, slot_cw, slot_pos, rbuf, ctl, cell, small, arrive, dirty, jump, xor32, th, cst, cbs, cps;
 };
 
 Sizes sizes_of(const Shape& s)
@@ -183,6 +183,7 @@ Sizes sizes_of(const Shape& s)
     z.th = (int64_t)s.M * THW * 4;
     z.cst = SEL_WAVE * 16;
     z.cbs = SEL_WAVE * 8;
+    z.cps = SEL_WAVE * 8;
     return z;
 }
 
@@ -192,7 +193,7 @@ int64_t total_bytes(const Shape& s)
     // plus the distance array and codeword bitmap the transform allocates on first use
     return z.cnt + 3 * z.plane + z.blk + 2 * z.code + z.cpk + z.cidx + z.tabu + z.fsub + z.psub +
            z.bsub + z.ssub + z.ul_idx + z.ul_pk + z.shuf + z.acc + z.slot_cw + z.slot_pos + z.rbuf +
-           z.ctl + z.cell + z.small + z.arrive + z.dirty + z.jump + z.xor32 + z.th + z.cst + z.cbs + (s.N + 64) +
+           z.ctl + z.cell + z.small + z.arrive + z.dirty + z.jump + z.xor32 + z.th + z.cst + z.cbs + z.cps + (s.N + 64) +
            ((s.N >> 5) + 2) * 4;
 }
 
@@ -549,6 +550,7 @@ int cov_create(int q, int n, int R, int M, cov_cell_t** out)
     AL(C.th, z.th);
     AL(C.cst, z.cst);
     AL(C.cbs, z.cbs);
+    AL(C.cps, z.cps);
 #undef AL
     C.scratch64 = (decltype(C.scratch64))h->d_small;
 
