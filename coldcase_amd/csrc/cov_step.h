@@ -157,7 +157,7 @@ __device__ __forceinline__ int64_t wg_pick(const Cell& C, Ctl* ctl, NarrowSh& sh
     wg_ensure(C, ctl, 65);
     const int tid = threadIdx.x, wv = tid >> 6;
     const int64_t pos = ctl->ruse;
-    // round 1: the draws
+    // stage 1: the draws
     const uint64_t rv = tid < 65 ? ring_at(C, pos + tid) : 0;
     if (tid == 64) sh.v64[1] = (int64_t)(rv % (uint64_t)C.N);
     __syncthreads();
@@ -165,7 +165,7 @@ __device__ __forceinline__ int64_t wg_pick(const Cell& C, Ctl* ctl, NarrowSh& sh
     const int64_t sw = start >> 5, sb = start >> BLK_SHIFT;
     const int64_t bw0 = sb << (BLK_SHIFT - 5);
     const int64_t bw1 = (bw0 + (BLK_WORDS / 32) < C.nzw) ? bw0 + (BLK_WORDS / 32) : C.nzw;
-    // round 2
+    // stage 2
     int64_t cand_s = INT64_MAX, cand_b = INT64_MAX;
     if (tid < 64) {
         const int64_t w = (int64_t)(rv % (uint64_t)C.N);
@@ -207,7 +207,7 @@ __device__ __forceinline__ int64_t wg_pick(const Cell& C, Ctl* ctl, NarrowSh& sh
     __syncthreads();
     if (ms != INT64_MAX) return ms;
     if (mb != INT64_MAX) {
-        // round 3: the first non-empty block after the start block
+        // stage 3: the first non-empty block after the start block
         const int64_t x0 = mb << (BLK_SHIFT - 5);
         const int64_t x1 = (x0 + (BLK_WORDS / 32) < C.nzw) ? x0 + (BLK_WORDS / 32) : C.nzw;
         return wg_first_bits(C, x0, x1, ~0u, ~0u, sh);
