@@ -144,7 +144,7 @@ __device__ __forceinline__ void wg_ensure(const Cell& C, Ctl* ctl, int64_t need)
 }
 
 // pick_uncovered: 64 probes, then the cyclic scan (:973-1016).  All threads.
-// The reads go in rounds rather than one after another: first the 65 draws
+// The reads go in three stages rather than one after another: the 65 draws
 // (64 probes and the scan's start); then, together, the probes' plane bits,
 // the rest of the start block and the counts of the 64 blocks after it; then,
 // only when the start block has no uncovered word at or after the start, the
