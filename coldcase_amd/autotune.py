@@ -29,7 +29,7 @@ import time
 
 import numpy as np
 
-from .engine import MODE_GRAPH, Batch, Cell, CovError
+from .engine import GRAPH_STEPS, MODE_GRAPH, Batch, Cell, CovError
 
 TABLE = os.path.join(os.path.dirname(os.path.abspath(__file__)), 'tuned.json')
 
@@ -55,7 +55,7 @@ def _palindrome(cell, cands, setter, steps):
     acc = {c: 0.0 for c in cands}
     for c in list(cands) + list(reversed(cands)):
         setter(c)
-        cell.run(32, mode=MODE_GRAPH)          # capture the graph outside the timing
+        cell.run(GRAPH_STEPS, mode=MODE_GRAPH)  # capture the graph outside the timing
         acc[c] += _time_steps(cell, steps)
     return {c: v / 2 for c, v in acc.items()}
 
@@ -112,7 +112,7 @@ def tune_batch(cells, factors=(1.0, 0.5, 0.25, 0.125, 0.0625), steps=64):
     for f in list(factors) + list(reversed(factors)):
         setf(f)
         with Batch(cells) as b:
-            b.run(32)
+            b.run(GRAPH_STEPS)
             t = time.perf_counter()
             b.run(steps)
             times[f] = times.get(f, 0.0) + (time.perf_counter() - t) / steps
@@ -161,7 +161,7 @@ def tune_fleet(cells, costs, ngroups=4, start=0.25, lo=0.0625, hi=1.0, passes=2,
     def timed(fv):
         apply(fv)
         with Fleet(groups) as fl:
-            fl.run(32)
+            fl.run(GRAPH_STEPS)
             t = time.perf_counter()
             fl.run(steps)
             return (time.perf_counter() - t) / steps

@@ -15,6 +15,17 @@ Flags that only select an implementation in the reference (--preset, --st,
 since every one of them runs the identical search.  Flags that change the
 search (--wide, --early, --fix0, --ucache, --upick, --sa, --noise) are not
 part of this engine and are refused with exit status 2.
+
+Stopping (:1305-1324): the device checks, before every step, the deadline
+(-t seconds of wall clock from process start, or --cpu seconds of this
+process's CPU time when that is given), the iteration budget (--iters) and
+the target (--target T: stop once the best uncovered count is <= T).  The
+time-to-target fields of the RESULT line (:1536-1542) give the iteration
+whose move first reached the target, the wall seconds from the start of the
+search to it (the device clock), and this process's CPU seconds to it.
+While a --cpu budget runs, the host waits for the device by polling, so the
+process's CPU time advances with the wall clock and the budget ends in about
+as many wall seconds.
 """
 import os
 import sys
@@ -24,8 +35,7 @@ import numpy as np
 
 from .verify_cov import DIGITS
 
-IGNORED_VALUE = {'--preset', '--st', '--eval', '--walk', '--threads', '--t0', '--cool', '--tmin', '--cpu',
-                 '--target'}
+IGNORED_VALUE = {'--preset', '--st', '--eval', '--walk', '--threads', '--t0', '--cool', '--tmin'}
 IGNORED_FLAG = {'--hoist', '--pf', '--huge', '--quiet'}
 REFUSED = {'--wide', '--early', '--fix0', '--ucache', '--upick', '--sa'}
 
@@ -86,7 +96,7 @@ def write_code(path, code, q):
 
 def parse_args(argv):
     o = dict(q=0, n=0, R=-1, M=0, t=60.0, s=12345, inp=None, out=None, cand=100000, tabu=0, kick=2,
-             iters=-1, stall=5000, greedy=True, quiet=False, noise=0)
+             iters=-1, stall=5000, greedy=True, quiet=False, noise=0, target=-1, cpu=0.0)
     i = 0
     while i < len(argv):
         a = argv[i]
@@ -96,6 +106,10 @@ def parse_args(argv):
             o[a[1]] = int(val); i += 2
         elif a == '-t' and has:
             o['t'] = float(val); i += 2
+        elif a == '--target' and has:
+            o['target'] = int(val); i += 2
+        elif a == '--cpu' and has:
+            o['cpu'] = float(val); i += 2
         elif a == '-s' and has:
             o['s'] = int(val); i += 2
         elif a == '--in' and has:
@@ -159,9 +173,11 @@ def main(argv=None):
         if o['out']:
             write_code(o['out'], distinct_fill(code, q, n, M), q)
         el = time.monotonic() - t_start
+        hit = o['target'] >= 0
         print('RESULT q=%d n=%d R=%d M=%d uncovered=0 iters=0 kicks=0 time=%.2f rate=0 cpu=%.3f '
-              'cpurate=0 target=-1 ttt_cpu=-1.0000 ttt_wall=-1.0000 ttt_it=-1'
-              % (q, n, R, M, el, time.process_time()))
+              'cpurate=0 target=%d ttt_cpu=%.4f ttt_wall=%.4f ttt_it=%d'
+              % (q, n, R, M, el, time.process_time(), o['target'], 0.0 if hit else -1.0,
+                 0.0 if hit else -1.0, 0 if hit else -1))
         return 0
 
     from .engine import MODE_GRAPH, Cell
@@ -176,22 +192,55 @@ def main(argv=None):
     if verbose:
         print('# init uncovered=%d' % cell.state()['uncovered'])
         sys.stdout.flush()
+    target, cpulimit = o['target'], o['cpu']
     t0 = time.monotonic()
     c0 = time.process_time()
+
+    def budget_left():
+        """Seconds left of the budget: CPU (--cpu) or wall from process start (-t)."""
+        if cpulimit > 0.0:
+            return cpulimit - (time.process_time() - c0)
+        return o['t'] - (time.monotonic() - t_start)
+
+    ttt_it, ttt_wall, ttt_cpu = -1, -1.0, -1.0
+    left = budget_left()
+    cell.limits(target, left if left > 0 else -1.0)
+    if target >= 0 and bu <= target:
+        ttt_it, ttt_wall, ttt_cpu = 0, 0.0, 0.0
     last_report = 0.0
     chunk = 64
     st = cell.state()
     while not st['done']:
-        if time.monotonic() - t_start > o['t']:
+        left = budget_left()
+        if left <= 0:
             break
         if o['iters'] >= 0 and st['it'] >= o['iters']:
             break
         k = chunk if o['iters'] < 0 else min(chunk, o['iters'] - st['it'])
-        prev_best = st['best_uncovered']
-        cell.run(k, mode=MODE_GRAPH)
-        st = cell.state()
+        prev_best, prev_it = st['best_uncovered'], st['it']
+        w0, p0 = time.monotonic(), time.process_time()
+        if cpulimit > 0.0:
+            # the device stops at the CPU budget's worth of wall seconds; the
+            # host polls meanwhile, so its CPU time follows the wall clock
+            cell.limits(target, left)
+            cell.launch(k, mode=MODE_GRAPH)
+            while not cell.query():
+                pass
+        else:
+            cell.launch(k, mode=MODE_GRAPH)
+            cell.sync()
+        st = cell.check_fault()
+        w1, p1 = time.monotonic(), time.process_time()
+        if target >= 0 and ttt_it < 0:
+            it_hit, age = cell.ttt()
+            if it_hit >= 0:
+                ttt_it = it_hit
+                ttt_wall = max(0.0, (time.monotonic() - t0) - age)
+                # this process's CPU time at that moment, interpolated over the chunk
+                f = (ttt_wall - (w0 - t0)) / (w1 - w0) if w1 > w0 else 1.0
+                ttt_cpu = (p0 - c0) + min(max(f, 0.0), 1.0) * (p1 - p0)
         el = time.monotonic() - t0
-        if chunk < 8192 and el < 0.5:
+        if w1 - w0 < 0.25 and chunk < 8192 and st['it'] - prev_it == k:
             chunk *= 2
         if st['best_uncovered'] < prev_best and o['out'] and (time.monotonic() - last_pub >= 1.0
                                                               or st['best_uncovered'] == 0):
@@ -207,9 +256,9 @@ def main(argv=None):
     cpu = time.process_time() - c0
     it = st['it']
     print('RESULT q=%d n=%d R=%d M=%d uncovered=%d iters=%d kicks=%d time=%.2f rate=%.0f cpu=%.3f '
-          'cpurate=%.0f target=-1 ttt_cpu=-1.0000 ttt_wall=-1.0000 ttt_it=-1'
+          'cpurate=%.0f target=%d ttt_cpu=%.4f ttt_wall=%.4f ttt_it=%d'
           % (q, n, R, M, st['best_uncovered'], it, st['kicks'], el, it / (el if el > 0 else 1), cpu,
-             it / (cpu if cpu > 0 else 1)))
+             it / (cpu if cpu > 0 else 1), target, ttt_cpu, ttt_wall, ttt_it))
     sys.stdout.flush()
     if o['out']:
         best, _ = cell.best()

@@ -86,6 +86,17 @@ struct alignas(16) Ctl {
     // begin's end -> evaluator start, 11 gap selection end -> commit start
     int64_t tph[12];
     int64_t t_eve, t_cme;
+    // stop conditions checked before every step, as the reference checks them
+    // at the top of its loop: stop once best_unc <= target (target < 0: none),
+    // and do not begin a step once the device clock has passed deadline (0:
+    // none).  ttt_it is the iteration whose move first brought best_unc to
+    // the target (-1: not yet), t_ttt the device clock then.
+    int64_t target, deadline, ttt_it, t_ttt;
+    // algorithmic bytes of the reference's move per begun step: the scan of
+    // the code (M n), both sides of every candidate codeword's evaluation
+    // walked in full (2 B per counter read) and the commit's read-modify-
+    // write of both spheres (8 B per pattern); and the steps counted
+    int64_t move_bytes, move_steps;
     int64_t* trace;         // per-run trace records (tcap of them), or null
     int64_t tcap;
 };

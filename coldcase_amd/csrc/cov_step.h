@@ -296,9 +296,15 @@ __device__ __forceinline__ void wg_list_compact(const Cell& C, Ctl* ctl, NarrowS
 __device__ __forceinline__ void wg_begin(const Cell& C, Ctl* ctl, NarrowSh& sh)
 {
     const int64_t tb0 = clk();
+    // one reading of the clock for the whole workgroup, so every thread takes
+    // the same branch at the deadline
+    __shared__ int64_t now;
+    if (threadIdx.x == 0) now = tb0;
     __syncthreads();
     const int done = ctl->done, begun = ctl->begun;
     const int64_t unc = ctl->unc, it = ctl->it, it_stop = ctl->it_stop;
+    const int64_t target = ctl->target, best = ctl->best_unc, deadline = ctl->deadline;
+    const int64_t t_now = now;
     __syncthreads();
     if (done || begun) return;
     if (unc <= 0) {
@@ -306,7 +312,15 @@ __device__ __forceinline__ void wg_begin(const Cell& C, Ctl* ctl, NarrowSh& sh)
         __syncthreads();
         return;
     }
+    // the top of the reference's loop (:1305-1324): deadline, iteration
+    // budget, target; each ends the run before the step's rng draws
+    if (deadline != 0 && t_now >= deadline) return;
     if (it >= it_stop) return;
+    if (target >= 0 && best <= target) {
+        if (threadIdx.x == 0) ctl->done = 2;
+        __syncthreads();
+        return;
+    }
     if (threadIdx.x == 0) ctl->it = it + 1;
 
     // uncovered-list upkeep (speed only: the list never changes a value); an
@@ -530,8 +544,10 @@ __device__ __forceinline__ void wg_begin(const Cell& C, Ctl* ctl, NarrowSh& sh)
         ctl->sh_pos = ctl->ruse;
         ctl->rlow = ctl->ruse;
         ctl->ruse += take;
-        ctl->eval_bytes += (int64_t)C.M * C.n + 2 * nlv * C.P +
-                           (use_list ? nq * ctl->ul_n * C.n : 2 * nq * dtar * C.E * C.P);
+        const int64_t enter = use_list ? nq * ctl->ul_n * C.n : 2 * nq * dtar * C.E * C.P;
+        ctl->eval_bytes += (int64_t)C.M * C.n + 2 * nlv * C.P + enter;
+        ctl->move_bytes += (int64_t)C.M * C.n + 2 * nq * C.F * C.P + enter + 8 * (int64_t)C.E * C.P;
+        ctl->move_steps += 1;
         ctl->selected = 0;
         ctl->begun = (ncand > 0 && take <= RB - 4096) ? 1 : 0;
         ctl->tph[2] += clk() - tb0;
@@ -1457,6 +1473,10 @@ __device__ __forceinline__ void wg_post(const Cell& C, Ctl* ctl, NarrowSh& sh)
             ctl->since = 0;
             sh.v32[2] = 1;
             if (ctl->best_unc == 0) ctl->done = 1;
+            if (ctl->target >= 0 && ctl->ttt_it < 0 && ctl->best_unc <= ctl->target) {
+                ctl->ttt_it = ctl->it;
+                ctl->t_ttt = clk();
+            }
         } else if (++ctl->since > ctl->stall) {
             sh.v32[2] = 2;
         }
@@ -2025,6 +2045,35 @@ __global__ void k_set_move(const Cell C, int i, int p, int v)
     ctl->begun = 1;
     ctl->selected = 1;
     ctl->done = 0;
+}
+
+// Stop conditions of the search loop (one thread).  The deadline is `ticks`
+// of the device clock from now (ticks <= 0: none).  A new target restarts the
+// time-to-target record; a target the best code already meets is met now
+// (the reference's check before its first step, covsearch2e.c:1292-1293), and
+// a search stopped at a target it no longer meets may go on.
+__global__ void k_set_limits(const Cell C, int64_t target, int64_t ticks)
+{
+    Ctl* ctl = C.ctl;
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    const int64_t now = clk();
+    if (target != ctl->target) {
+        ctl->ttt_it = -1;
+        ctl->t_ttt = 0;
+    }
+    ctl->target = target;
+    ctl->deadline = ticks > 0 ? now + ticks : 0;
+    if (ctl->done == 2 && (target < 0 || ctl->best_unc > target)) ctl->done = 0;
+    if (target >= 0 && ctl->ttt_it < 0 && ctl->best_unc <= target) {
+        ctl->ttt_it = ctl->it;
+        ctl->t_ttt = now;
+    }
+}
+
+// the device clock, for the host's reading of the time-to-target record
+__global__ void k_clock(int64_t* out)
+{
+    if (threadIdx.x == 0 && blockIdx.x == 0) *out = clk();
 }
 
 }  // namespace cov

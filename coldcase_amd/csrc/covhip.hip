@@ -33,8 +33,8 @@ struct cov_cell {
     int g_steps = 0, g_eg = 0, g_cg = 0;
     // timing
     bool timing = false;
-    double ms[4] = {0, 0, 0, 0};
-    int64_t launches[4] = {0, 0, 0, 0};
+    double ms[6] = {0, 0, 0, 0, 0, 0};
+    int64_t launches[6] = {0, 0, 0, 0, 0, 0};
     std::vector<hipEvent_t> evs;          // event pool
     struct Pend { int kind; hipEvent_t a, b; };
     std::vector<Pend> pend;               // recorded, not yet read
@@ -52,6 +52,7 @@ struct cov_cell {
     int kick = 2;                         // the search's kicks per stagnation (as in the control block)
     int leave_mode = COV_LEAVE_AUTO;      // how the steps find the leaving side (cov_set_leave_mode)
     int th_on = 0;                        // the control block's th_on as last set from here
+    double clk_hz = 1e8;                  // device wall clock rate (hipDeviceAttributeWallClockRate)
     // recorded on a batch's stream after each batched launch that includes
     // the cell: the cell's own stream waits for it before touching the cell,
     // and its memory is not freed before it
@@ -72,7 +73,7 @@ struct cov_batch {
 
 namespace {
 
-constexpr int GRAPH_STEPS = 32;
+constexpr int GRAPH_STEPS = COV_GRAPH_STEPS;
 
 #define HCK(h, x)                                                              \
     do {                                                                       \
@@ -380,14 +381,18 @@ int launch_step(cov_cell* h)
     hipLaunchKernelGGL(k_eval, dim3(h->eval_groups), dim3(WG), 0, h->stream, h->h);
     HCK(h, hipGetLastError());
     RC(timed_end(h, 0, a));
+    RC(timed_begin(h, 4, &a));
     hipLaunchKernelGGL(k_select, dim3(1), dim3(WG), 0, h->stream, h->h);
     HCK(h, hipGetLastError());
+    RC(timed_end(h, 4, a));
     RC(timed_begin(h, 1, &a));
     hipLaunchKernelGGL(k_commit, dim3(h->commit_groups), dim3(WG), 0, h->stream, h->h);
     HCK(h, hipGetLastError());
     RC(timed_end(h, 1, a));
+    RC(timed_begin(h, 5, &a));
     hipLaunchKernelGGL(k_post, dim3(1), dim3(WG), 0, h->stream, h->h);
     HCK(h, hipGetLastError());
+    RC(timed_end(h, 5, a));
     return COV_OK;
 }
 
@@ -494,6 +499,12 @@ int cov_create(int q, int n, int R, int M, cov_cell_t** out)
     {
         const hipError_t e = hipEventCreateWithFlags(&h->used, hipEventDisableTiming);
         if (e != hipSuccess) { h->used = nullptr; return fail(-(100 + (int)e)); }
+    }
+    {
+        int dev = 0, khz = 0;
+        if (hipGetDevice(&dev) == hipSuccess &&
+            hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, dev) == hipSuccess && khz > 0)
+            h->clk_hz = (double)khz * 1e3;
     }
     Cell& C = h->h;
     C.q = q; C.n = n; C.R = R; C.M = M; C.q1 = q - 1; C.F = s.F; C.E = s.E; C.P = s.P;
@@ -621,6 +632,8 @@ int cov_create(int q, int n, int R, int M, cov_cell_t** out)
     c.budget = 1024;
     c.tenure = 0;
     c.it_stop = 0;
+    c.target = -1;
+    c.ttt_it = -1;
     if ((rc = upload(h, C.ctl, &c, sizeof(Ctl))) != COV_OK) return fail(rc);
     if ((rc = push_cell(h)) != COV_OK) return fail(rc);
     if (pl.T * 4 > 65536) {
@@ -993,6 +1006,8 @@ int cov_search_init(cov_cell_t* h, uint64_t seed, const uint8_t* seed_code, int 
     RC(read_ctl(h, &old));
     Ctl c{};
     c.tenure = old.tenure; c.stall = old.stall; c.kick = old.kick; c.cand = old.cand; c.budget = old.budget;
+    c.target = -1;
+    c.ttt_it = -1;
     splitmix_seed(seed, c.rst);
     RC(upload(h, C.ctl, &c, sizeof(Ctl)));
     h->th_on = 0;
@@ -1186,6 +1201,8 @@ int cov_search_state(cov_cell_t* h, cov_state_t* st)
     st->list_n = c.ul_n;
     st->done = c.done;
     st->list_valid = c.ul_valid;
+    st->fault = c.fault;
+    st->reserved = 0;
     return COV_OK;
 }
 
@@ -1231,13 +1248,13 @@ int cov_timing(cov_cell_t* h, int on)
     if (!h) return COV_EPARAM;
     RC(timing_collect(h));
     h->timing = on != 0;
-    for (int k = 0; k < 4; k++) { h->ms[k] = 0; h->launches[k] = 0; }
+    for (int k = 0; k < 6; k++) { h->ms[k] = 0; h->launches[k] = 0; }
     return COV_OK;
 }
 
 int cov_timing_read(cov_cell_t* h, double* ms, int64_t* launches, int nkinds)
 {
-    if (!h || nkinds < 0 || nkinds > 4) return COV_EPARAM;
+    if (!h || nkinds < 0 || nkinds > 6) return COV_EPARAM;
     RC(timing_collect(h));
     for (int k = 0; k < nkinds; k++) {
         if (ms) ms[k] = h->ms[k];
@@ -1251,17 +1268,63 @@ int cov_search_phases(cov_cell_t* h, double* us, int n)
     if (!h || n < 0 || n > 12 || (n && !us)) return COV_EPARAM;
     Ctl c;
     RC(read_ctl(h, &c));
-    for (int k = 0; k < n; k++) us[k] = (double)c.tph[k] * 0.01;   // 100 MHz wall clock
+    for (int k = 0; k < n; k++) us[k] = (double)c.tph[k] * 1e6 / h->clk_hz;
     return COV_OK;
 }
 
-int cov_search_bytes(cov_cell_t* h, int64_t* eval_bytes, int64_t* commit_bytes)
+int cov_search_limits(cov_cell_t* h, int64_t target, double seconds)
+{
+    if (!h || !(seconds == seconds)) return COV_EPARAM;
+    if (target < -1) target = -1;
+    int64_t ticks = 0;
+    if (seconds > 0) {
+        const double t = seconds * h->clk_hz;
+        ticks = t >= 9.0e18 ? (int64_t)9e18 : std::max<int64_t>(1, (int64_t)t);
+    } else if (seconds < 0) {
+        ticks = 1;                        // already past: no further step begins
+    }
+    if (h->used) HCK(h, hipStreamWaitEvent(h->stream, h->used, 0));
+    hipLaunchKernelGGL(k_set_limits, dim3(1), dim3(64), 0, h->stream, h->h, target, ticks);
+    HCK(h, hipGetLastError());
+    HCK(h, hipStreamSynchronize(h->stream));
+    return COV_OK;
+}
+
+int cov_search_ttt(cov_cell_t* h, int64_t* ttt_it, double* age_s)
+{
+    if (!h) return COV_EPARAM;
+    Ctl c;
+    RC(read_ctl(h, &c));
+    hipLaunchKernelGGL(k_clock, dim3(1), dim3(64), 0, h->stream, h->d_small);
+    HCK(h, hipGetLastError());
+    int64_t now = 0;
+    HCK(h, hipMemcpyAsync(&now, h->d_small, 8, hipMemcpyDeviceToHost, h->stream));
+    HCK(h, hipStreamSynchronize(h->stream));
+    if (ttt_it) *ttt_it = c.ttt_it;
+    if (age_s) *age_s = c.ttt_it >= 0 ? (double)(now - c.t_ttt) / h->clk_hz : -1.0;
+    return COV_OK;
+}
+
+int cov_query(cov_cell_t* h)
+{
+    if (!h) return COV_EPARAM;
+    const hipError_t e = hipStreamQuery(h->stream);
+    if (e == hipSuccess) return 1;
+    if (e == hipErrorNotReady) return 0;
+    HCK(h, e);
+    return 0;
+}
+
+int cov_search_bytes(cov_cell_t* h, int64_t* eval_bytes, int64_t* commit_bytes, int64_t* move_bytes,
+                     int64_t* move_steps)
 {
     if (!h) return COV_EPARAM;
     Ctl c;
     RC(read_ctl(h, &c));
     if (eval_bytes) *eval_bytes = c.eval_bytes;
     if (commit_bytes) *commit_bytes = c.commit_bytes;
+    if (move_bytes) *move_bytes = c.move_bytes;
+    if (move_steps) *move_steps = c.move_steps;
     return COV_OK;
 }
 

@@ -22,6 +22,7 @@ COV_OK = 0
 COV_TR_W = 16
 COV_DSEL_RANDOM = 1 << 60
 MODE_PLAIN, MODE_GRAPH = 0, 1
+GRAPH_STEPS = 40          # COV_GRAPH_STEPS: steps per captured graph
 LEAVE_AUTO, LEAVE_WALK, LEAVE_KEPT = 0, 1, 2
 TR_FIELDS = ['it', 'u', 'ncand', 'ncw', 'bi', 'bp', 'bv', 'oldv', 'dsel', 'dunc',
              'unc', 'best', 'kick', 'draws', 'since', 'nties']
@@ -32,7 +33,8 @@ EXPORTS = ['cov_version', 'cov_strerror', 'cov_mem_bytes', 'cov_create', 'cov_de
            'cov_recount', 'cov_ball_update', 'cov_count_uncovered', 'cov_gather_uncovered',
            'cov_first_uncovered_from', 'cov_read_cnt', 'cov_eval_moves', 'cov_commit',
            'cov_distance_transform', 'cov_distance_stats', 'cov_gather_far', 'cov_search_init', 'cov_search_params',
-           'cov_search_run', 'cov_search_launch', 'cov_search_state', 'cov_set_launch',
+           'cov_search_run', 'cov_search_launch', 'cov_search_state', 'cov_query', 'cov_search_limits',
+           'cov_search_ttt', 'cov_set_launch',
            'cov_get_launch', 'cov_set_leave_mode', 'cov_set_transform_tile', 'cov_timing', 'cov_timing_read', 'cov_search_phases',
            'cov_search_bytes', 'cov_batch_create', 'cov_batch_destroy', 'cov_batch_last_error',
            'cov_batch_size', 'cov_batch_launch', 'cov_batch_sync', 'cov_batch_query', 'cov_batch_run']
@@ -47,7 +49,8 @@ class State(ctypes.Structure):
                 ('best_uncovered', ctypes.c_int64), ('kicks', ctypes.c_int64),
                 ('since_improve', ctypes.c_int64), ('draws', ctypes.c_int64),
                 ('list_n', ctypes.c_int64), ('done', ctypes.c_int32),
-                ('list_valid', ctypes.c_int32)]
+                ('list_valid', ctypes.c_int32), ('fault', ctypes.c_int32),
+                ('reserved', ctypes.c_int32)]
 
 
 _lib = None
@@ -94,6 +97,9 @@ def lib():
         'cov_search_run': (i64, [vp, i64, i32, vp]),
         'cov_search_launch': (i32, [vp, i64, i32]),
         'cov_search_state': (i32, [vp, ctypes.POINTER(State)]),
+        'cov_query': (i32, [vp]),
+        'cov_search_limits': (i32, [vp, i64, ctypes.c_double]),
+        'cov_search_ttt': (i32, [vp, vp, vp]),
         'cov_set_launch': (i32, [vp, i32, i32]),
         'cov_set_leave_mode': (i32, [vp, i32]),
         'cov_get_launch': (i32, [vp, vp, vp]),
@@ -101,7 +107,7 @@ def lib():
         'cov_timing': (i32, [vp, i32]),
         'cov_timing_read': (i32, [vp, vp, vp, i32]),
         'cov_search_phases': (i32, [vp, vp, i32]),
-        'cov_search_bytes': (i32, [vp, vp, vp]),
+        'cov_search_bytes': (i32, [vp, vp, vp, vp, vp]),
         'cov_batch_create': (i32, [ctypes.POINTER(vp), i32, ctypes.POINTER(vp)]),
         'cov_batch_destroy': (i32, [vp]),
         'cov_batch_last_error': (ctypes.c_char_p, [vp]),
@@ -276,10 +282,35 @@ class Cell:
     def launch(self, nsteps, mode=MODE_GRAPH):
         self._check(lib().cov_search_launch(self.h, int(nsteps), int(mode)), 'cov_search_launch')
 
+    def query(self):
+        """True when every launch on the cell's stream has finished (no wait)."""
+        return self._check(lib().cov_query(self.h), 'cov_query') == 1
+
+    def limits(self, target=-1, seconds=0.0):
+        """Stop before a step once best_uncovered <= target (target < 0: none)
+        or once `seconds` from now have passed on the device clock (0: none)."""
+        self._check(lib().cov_search_limits(self.h, int(target), float(seconds)), 'cov_search_limits')
+
+    def ttt(self):
+        """(iteration that first reached the target or -1, seconds since then or -1)."""
+        it = np.zeros(1, np.int64)
+        age = np.zeros(1, np.float64)
+        self._check(lib().cov_search_ttt(self.h, _p(it), _p(age)), 'cov_search_ttt')
+        return int(it[0]), float(age[0])
+
     def state(self):
         s = State()
         self._check(lib().cov_search_state(self.h, ctypes.byref(s)), 'cov_search_state')
-        return {k: getattr(s, k) for k, _ in State._fields_}
+        return {k: getattr(s, k) for k, _ in State._fields_ if k != 'reserved'}
+
+    def check_fault(self, st=None):
+        """Raise CovError if the search stopped on a device-detected error."""
+        st = self.state() if st is None else st
+        if st['fault']:
+            raise CovError({1: 'inconsistent counters: no candidate move',
+                            2: 'candidate sample larger than the rng ring allows'}.get(st['fault'], 'fault %d'
+                                                                                         % st['fault']))
+        return st
 
     def set_launch(self, eval_groups, commit_groups):
         self._check(lib().cov_set_launch(self.h, int(eval_groups), int(commit_groups)), 'cov_set_launch')
@@ -302,9 +333,11 @@ class Cell:
         self._check(lib().cov_timing(self.h, int(bool(on))), 'cov_timing')
 
     def timing_read(self):
-        ms = np.zeros(4, np.float64)
-        n = np.zeros(4, np.int64)
-        self._check(lib().cov_timing_read(self.h, _p(ms), _p(n), 4), 'cov_timing_read')
+        """(ms, launches) per kind: 0 evaluator, 1 commit, 2 transform pass,
+        3 recount, 4 selection, 5 post."""
+        ms = np.zeros(6, np.float64)
+        n = np.zeros(6, np.int64)
+        self._check(lib().cov_timing_read(self.h, _p(ms), _p(n), 6), 'cov_timing_read')
         return ms, n
 
     def phases(self):
@@ -315,10 +348,11 @@ class Cell:
                          'eval_span', 'commit_span', 'gap_c2e', 'gap_e2c'], us.tolist()))
 
     def search_bytes(self):
-        e = np.zeros(1, np.int64)
-        c = np.zeros(1, np.int64)
-        self._check(lib().cov_search_bytes(self.h, _p(e), _p(c)), 'cov_search_bytes')
-        return int(e[0]), int(c[0])
+        """(evaluator bytes, commit bytes, reference move bytes B_move, steps counted)."""
+        v = np.zeros(4, np.int64)
+        self._check(lib().cov_search_bytes(self.h, _p(v[0:]), _p(v[1:]), _p(v[2:]), _p(v[3:])),
+                    'cov_search_bytes')
+        return tuple(int(x) for x in v)
 
 
 class Batch:

@@ -70,6 +70,7 @@ class Result:
     iters: int = 0
     kicks: int = 0
     best_sha256: str = ''
+    best_code: object = None       # search: the best code found (M x n digits)
 
     @property
     def latency(self):
@@ -127,6 +128,7 @@ def finish_search(r, c):
     res.iters = int(st['it'])
     res.kicks = int(st['kicks'])
     res.best_sha256 = hashlib.sha256(best.tobytes()).hexdigest()
+    res.best_code = best
     return res
 
 
@@ -208,7 +210,10 @@ class Server:
                 self._joining = []
                 for ci, ln in enumerate(lanes):
                     if ln.busy and ln.batch.query():
-                        ln.batch.sync()
+                        try:
+                            ln.batch.sync()
+                        except CovError:
+                            self._fail_faulted(ln)
                         ln.busy = False
                         self._retire(ln)
                         did = True
@@ -222,8 +227,7 @@ class Server:
                     if not ln.busy and ln.active:
                         cells = tuple(a[1] for a in ln.active)
                         if cells != ln.members:
-                            if ln.batch is not None:
-                                ln.batch.close()
+                            self._drop_batch(ln)
                             ln.batch, ln.members = Batch(list(cells)), cells
                         ln.k = min(self.tick_steps, min(a[2] for a in ln.active))
                         ln.batch.launch(ln.k, mode=self.mode)
@@ -239,18 +243,51 @@ class Server:
         finally:
             for ln in lanes:
                 if ln.busy:
-                    ln.batch.sync()
-                if ln.batch is not None:
-                    ln.batch.close()
+                    try:
+                        ln.batch.sync()
+                    except CovError:
+                        pass
+                self._drop_batch(ln)
                 for a in ln.active + ln.joining:
                     a[1].close()
             for a in self._joining:
                 a[1].close()
         return [self._results[r.rid] for r in sorted(requests, key=lambda r: r.rid)]
 
+    def _fail_faulted(self, ln):
+        """A tick ended with a device-detected error in some of the lane's
+        cells: those requests fail alone; the others' cells are unaffected
+        (each has its own control block) and stay in the lane."""
+        keep = []
+        gone = []
+        for a in ln.active:
+            st = a[1].state()
+            if st['fault']:
+                try:
+                    a[1].check_fault(st)
+                except CovError as e:
+                    err = str(e)
+                self._results[a[0].rid] = Result(a[0].rid, a[0].kind, a[0].arrival, admitted=a[3],
+                                                 finished=self._now(), error=err)
+                self._used -= request_bytes(a[0])
+                gone.append(a[1])
+            else:
+                keep.append(a)
+        self._drop_batch(ln)
+        for c in gone:
+            c.close()
+        ln.active = keep
+
+    @staticmethod
+    def _drop_batch(ln):
+        # a batch holds its member cells: it goes before any of them is closed
+        if ln.batch is not None:
+            ln.batch.close()
+        ln.batch, ln.members = None, ()
+
     def _retire(self, ln):
         """After a tick: searches that used their moves or covered leave."""
-        left = []
+        left, gone = [], []
         for a in ln.active:
             a[2] -= ln.k
             if a[2] <= 0 or a[1].state()['done']:
@@ -258,9 +295,13 @@ class Server:
                 res.admitted, res.finished = a[3], self._now()
                 self._results[a[0].rid] = res
                 self._used -= request_bytes(a[0])
-                a[1].close()
+                gone.append(a[1])
             else:
                 left.append(a)
+        if gone:
+            self._drop_batch(ln)
+            for c in gone:
+                c.close()
         ln.active = left
 
     def _now(self):

@@ -42,6 +42,9 @@ extern "C" {
 /* search-loop modes of cov_search_run */
 #define COV_MODE_PLAIN 0   /* one launch sequence per step                 */
 #define COV_MODE_GRAPH 1   /* steps replayed from a captured hipGraph      */
+/* steps per captured graph: graph mode replays whole graphs and runs the
+ * remainder of a launch (nsteps mod this) as plain launches */
+#define COV_GRAPH_STEPS 40
 
 typedef struct cov_cell cov_cell_t;
 typedef struct cov_batch cov_batch_t;
@@ -54,8 +57,13 @@ typedef struct {
     int64_t since_improve;
     int64_t draws;            /* rng draws consumed so far                    */
     int64_t list_n;           /* uncovered-list length (incl. stale entries)  */
-    int32_t done;             /* 1 once the search stopped (covered)          */
+    int32_t done;             /* 1 once the search stopped covered (or on a
+                                 fault), 2 once it stopped at its target      */
     int32_t list_valid;
+    int32_t fault;            /* 0, or why the search stopped on an error:
+                                 1 no candidate move (inconsistent counts),
+                                 2 candidate sample beyond the rng ring       */
+    int32_t reserved;
 } cov_state_t;
 
 /* ------------------------------------------------------------- lifecycle */
@@ -145,6 +153,21 @@ int64_t cov_search_run(cov_cell_t *h, int64_t nsteps, int mode, int64_t *trace);
 /* Non-blocking form: enqueue the steps and return; cov_sync() waits. */
 int     cov_search_launch(cov_cell_t *h, int64_t nsteps, int mode);
 int     cov_search_state(cov_cell_t *h, cov_state_t *st);
+/* 1 when every launch on the cell's stream has finished, 0 while any runs. */
+int     cov_query(cov_cell_t *h);
+/* Stop conditions, checked on the device before every step as the reference
+ * checks them at the top of its loop (cov/engine/covsearch2e.c:1305-1324):
+ * no step begins once best_uncovered <= target (--target; target < 0: none;
+ * the state's done then reads 2), nor once `seconds` from now have passed on
+ * the device clock (-t / --cpu; seconds == 0: none, seconds < 0: at once).
+ * A new target restarts the time-to-target record; a target the best code
+ * already meets is met at the current iteration (:1292-1293).  Reset to none
+ * by cov_search_init. */
+int     cov_search_limits(cov_cell_t *h, int64_t target, double seconds);
+/* The time-to-target record (:1536-1542): the iteration whose move first
+ * brought best_uncovered to the target (-1 if none yet), and the seconds of
+ * device clock since that moment (-1 if none). */
+int     cov_search_ttt(cov_cell_t *h, int64_t *ttt_it, double *age_s);
 
 /* -------------------------------------------------- launch shape, timing */
 /* Workgroups of the evaluator and commit kernels (the autotuner sets them). */
@@ -169,8 +192,9 @@ int     cov_set_leave_mode(cov_cell_t *h, int mode);
 int     cov_set_transform_tile(cov_cell_t *h, int lds_bytes, int threads);
 /* Accumulate per-kernel device time with HIP events recorded around each
  * launch on the cell's stream (on = 1; read back in batches, no host sync
- * between launches), read totals in ms and launch counts for kernel kinds
- * 0 = evaluator, 1 = commit, 2 = distance transform pass, 3 = recount. */
+ * between launches; plain-mode search steps only), read totals in ms and
+ * launch counts for kernel kinds 0 = evaluator, 1 = commit, 2 = distance
+ * transform pass, 3 = recount, 4 = selection, 5 = post (nkinds <= 6). */
 int     cov_timing(cov_cell_t *h, int on);
 int     cov_timing_read(cov_cell_t *h, double *ms, int64_t *launches, int nkinds);
 /* Diagnostics: microseconds spent since cov_search_init, from the device wall
@@ -180,9 +204,14 @@ int     cov_timing_read(cov_cell_t *h, double *ms, int64_t *launches, int nkinds
  * 10 gap previous begin -> evaluator, 11 gap selection -> commit
  * (n <= 12 entries). */
 int     cov_search_phases(cov_cell_t *h, double *us, int n);
-/* Algorithmic bytes moved by the evaluator and commit kernels since the last
- * cov_search_init (2 B per counter read or update, as the uint16 cnt). */
-int     cov_search_bytes(cov_cell_t *h, int64_t *eval_bytes, int64_t *commit_bytes);
+/* Algorithmic bytes since the last cov_search_init, 2 B per counter read or
+ * update (the uint16 cnt): what this engine's evaluator and commit kernels
+ * read and write (eval_bytes, commit_bytes), and the reference's move as it
+ * runs it, B_move = M n + 2 sum_cw (S_f + E_cw) + 8 S per step, every
+ * candidate codeword's full distance-R sphere walked and E_cw its entering
+ * side (walks, or the uncovered list), summed over move_steps steps. */
+int     cov_search_bytes(cov_cell_t *h, int64_t *eval_bytes, int64_t *commit_bytes,
+                         int64_t *move_bytes, int64_t *move_steps);
 
 /* ------------------------------------------------------------ batches */
 /* Many cells, each created, configured and initialised on its own (any

@@ -51,6 +51,8 @@ struct cso {
     int64_t it, since, nkicks;
     int kick_order;
     int threads;
+    int64_t target;      /* stop once best_unc <= target (< 0: none)        */
+    int64_t ttt_it;      /* iteration that first reached the target, or -1  */
 
     /* position subsets: subs[r] lists all r-subsets of 0..n-1 (lexicographic),
      * nsub[r] of them; psub[p] lists the R-subsets avoiding p. */
@@ -308,6 +310,8 @@ cso_t *cso_create(int q, int n, int R, int M, uint64_t seed)
         s->pw[p] = 1;
         for (int e = 0; e < n - 1 - p; e++) s->pw[p] *= q;
     }
+    s->target = -1;
+    s->ttt_it = -1;
     s->cnt = calloc((size_t)s->N, sizeof(uint16_t));
     s->code = calloc((size_t)M * n, 1);
     s->best = calloc((size_t)M * n, 1);
@@ -584,7 +588,9 @@ int64_t cso_run(cso_t *s, int64_t iters, int tabu_len, int64_t stall,
     const int64_t TOTPAT = (int64_t)s->npsub * P;   /* S, the per-move sphere */
 
     while (s->uncovered > 0) {
+        /* the top of the loop (:1322-1324): iteration budget, then target */
         if (iters >= 0 && done >= iters) break;
+        if (s->target >= 0 && s->best_unc <= s->target) break;
         s->it++;
         done++;
         const int64_t it = s->it;
@@ -681,6 +687,8 @@ int64_t cso_run(cso_t *s, int64_t iters, int tabu_len, int64_t stall,
             s->best_unc = s->uncovered;
             memcpy(s->best, s->code, (size_t)M * n);
             s->since = 0;
+            /* time-to-target record (:1536-1541) */
+            if (s->target >= 0 && s->ttt_it < 0 && s->best_unc <= s->target) s->ttt_it = it;
             if (s->best_unc == 0) stop = 1;
         } else if (++s->since > stall) {
             /* kick: teleport `kick` random codewords near uncovered words (:1553-1578) */
@@ -740,6 +748,15 @@ int64_t cso_run(cso_t *s, int64_t iters, int tabu_len, int64_t stall,
 
 /* -------------------------------------------------------- accessors */
 void cso_set_kick_order(cso_t *s, int order) { s->kick_order = order; }
+/* --target (:1039,1078): a target the best code already meets is met at the
+ * current iteration (:1291-1293) */
+void cso_set_target(cso_t *s, int64_t target)
+{
+    s->target = target < 0 ? -1 : target;
+    s->ttt_it = -1;
+    if (s->target >= 0 && s->best_unc <= s->target) s->ttt_it = s->it;
+}
+int64_t cso_ttt_it(const cso_t *s) { return s->ttt_it; }
 void cso_set_threads(cso_t *s, int threads) { s->threads = threads > 0 ? threads : 1; }
 int64_t cso_uncovered(const cso_t *s) { return s->uncovered; }
 int64_t cso_best_uncovered(const cso_t *s) { return s->best_unc; }

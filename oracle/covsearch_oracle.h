@@ -60,6 +60,11 @@ int64_t cso_run(cso_t *s, int64_t iters, int tabu_len, int64_t stall,
 /* order of the two draws in a kick perturbation: 0 = value then position,
  * 1 = position then value */
 void    cso_set_kick_order(cso_t *s, int order);
+/* --target T: runs stop before a step once the best uncovered count is <= T
+ * (T < 0: no target); cso_ttt_it gives the iteration whose move first
+ * reached it (-1: not yet; the current iteration if already met when set). */
+void    cso_set_target(cso_t *s, int64_t target);
+int64_t cso_ttt_it(const cso_t *s);
 void    cso_set_threads(cso_t *s, int threads);
 
 int64_t cso_uncovered(const cso_t *s);

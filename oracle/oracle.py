@@ -37,6 +37,9 @@ def lib():
         L.cso_run.argtypes = [vp, i64, i32, i64, i32, i32, vp]
         L.cso_set_kick_order.argtypes = [vp, i32]
         L.cso_set_threads.argtypes = [vp, i32]
+        L.cso_set_target.argtypes = [vp, i64]
+        L.cso_ttt_it.restype = i64
+        L.cso_ttt_it.argtypes = [vp]
         for f in ('cso_uncovered', 'cso_best_uncovered', 'cso_iters', 'cso_kicks', 'cso_space'):
             getattr(L, f).restype = i64
             getattr(L, f).argtypes = [vp]
@@ -102,6 +105,13 @@ class Oracle:
         done = lib().cso_run(self.h, iters, tabu, stall, kick, cand,
                              _ptr(tr) if trace else None)
         return tr[:done] if trace else done
+
+    def set_target(self, target):
+        lib().cso_set_target(self.h, int(target))
+
+    @property
+    def ttt_it(self):
+        return lib().cso_ttt_it(self.h)
 
     def set_code(self, code):
         c = np.ascontiguousarray(code, dtype=np.uint8).reshape(self.M, self.n)

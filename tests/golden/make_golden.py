@@ -8,6 +8,13 @@ kicks) and the sha256 of the code file it publishes (plus the text when it is
 small).  A run is a pure function of its arguments, so these pin the whole
 trajectory: rng consumption, candidate order, tie-breaks, tabu jitter, kicks.
 
+Runs tagged `target` stop at --target T and also record the iteration that
+first reached it (the RESULT line's ttt_it).  Runs listed in prof_runs() are
+made with the -DPROF build (oracle/_ref/covsearch2e_prof) and also record its
+per-step aggregates (mean candidates, sample size, uncovered count before
+the step, selected delta and distinct codewords; cov/engine/covsearch2e.c:
+1590-1618), which pin sums over every step of a long run.
+
 Usage:  make -C oracle ref && python3 tests/golden/make_golden.py
 Writes: tests/golden/trajectories.json
 """
@@ -24,8 +31,10 @@ from concurrent.futures import ThreadPoolExecutor
 HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(os.path.dirname(HERE))
 REF = os.path.join(ROOT, 'oracle', '_ref', 'covsearch2e')
+REF_PROF = os.path.join(ROOT, 'oracle', '_ref', 'covsearch2e_prof')
 DATA = os.path.join(HERE, 'data')
-RESULT = re.compile(r'RESULT .*?uncovered=(-?\d+) iters=(\d+) kicks=(\d+)')
+RESULT = re.compile(r'RESULT .*?uncovered=(-?\d+) iters=(\d+) kicks=(\d+) .*?target=(-?\d+) .*?ttt_it=(-?\d+)')
+PROF_KEYS = ('mean_ncand', 'mean_take', 'mean_uncovered', 'mean_bestd', 'mean_ncw')
 
 
 def ball(q, n, R):
@@ -58,11 +67,11 @@ def runs():
         for it in (1, 10, 100, 1000, 10000):
             add(5, 7, 2, 400, s, it, T, tag='C1')
     for s in (1, 2, 3):
-        for it in (1, 3, 10, 30, 100, 300):
+        for it in (1, 3, 10, 30, 100, 300, 1000, 3000, 10000):
             add(6, 9, 4, 660, s, it, T, threads=8, tag='C2')
-    for it in (1, 3, 10, 30):
+    for it in (1, 3, 10, 30, 100, 300):
         add(7, 9, 4, 1480, 1, it, T, threads=8, tag='C3')
-    for it in (1, 3):
+    for it in (1, 3, 10, 30):
         add(6, 10, 4, 2784, 1, it, T, threads=8, tag='C5')
     # batched-grid shapes at M = ceil(1.3 q^n / V)
     for q, n, R in [(5, 7, 3), (6, 7, 2), (7, 7, 4), (8, 7, 3), (5, 8, 2), (6, 8, 3),
@@ -88,13 +97,51 @@ def runs():
     # greedy init on a benchmark shape
     add(6, 9, 4, 660, 2, 10, ['--tabu', '10'], threads=8, tag='greedy')
     add(5, 7, 2, 400, 2, 1000, ['--tabu', '10'], tag='greedy')
+    # --target: stop once the best uncovered count reaches T (the time-to-target
+    # harnesses cov/opt/ttt.py:62 and cov/gemm/run_bench.py:68 run this way)
+    for s, tg in ((1, 5000), (2, 4500), (3, 4200)):
+        add(5, 7, 2, 400, s, 10000, T + ['--target', str(tg)], tag='target')
+    add(5, 7, 2, 400, 4, 300, T + ['--target', '10'], tag='target')          # never reached
+    for s in (1, 2, 3):
+        add(6, 6, 3, 41, s, 20000, ['--tabu', '10', '--target', '30'], infile='seeds_K6_6_3_M41.txt',
+            tag='target')
+    add(6, 6, 3, 41, 4, 20000, ['--tabu', '3', '--stall', '50', '--target', '25'], infile='seeds_K6_6_3_M41.txt',
+        tag='target')
+    add(6, 8, 4, 169, 1, 3000, ['--tabu', '10', '--target', '200'], infile='seeds_K6_8_4_M169.txt', threads=8,
+        tag='target')
+    add(6, 9, 4, 660, 1, 10, ['--target', '0'], infile='K6_9_4_M660.txt', tag='target')   # met at the start
+    # exact instances of the batched configuration (coldcase_amd/workloads.py:
+    # c4_specs() index, its seed 1000003 + 7919 i), spanning the grid: the
+    # largest space, the largest M, every radius
+    sys.path.insert(0, ROOT)
+    from coldcase_amd.workloads import c4_seed, c4_specs
+    specs, _ = c4_specs()
+    for i in (4, 6, 7, 10, 15, 20, 24, 36, 39, 59):
+        q, n, R, M = specs[i]
+        add(q, n, R, M, c4_seed(i), 40, T, threads=8, tag='C4-exact')
     return out
 
 
-def run_one(r):
+def prof_runs():
+    """Runs also made with the -DPROF build, keyed like runs()."""
+    T = ['--random-init', '--tabu', '10']
+    out = []
+    for s in (1, 2, 3, 4, 5):
+        out.append(dict(q=5, n=7, R=2, M=400, seed=s, iters=10000, flags=T, infile=None, threads=1))
+    out.append(dict(q=6, n=9, R=4, M=660, seed=1, iters=10000, flags=T, infile=None, threads=8))
+    out.append(dict(q=5, n=7, R=2, M=400, seed=1, iters=2000, flags=T[:1] + ['--tabu', '3', '--stall', '10'],
+                    infile=None, threads=1))
+    return out
+
+
+def prof_key(r):
+    return (r['q'], r['n'], r['R'], r['M'], r['seed'], r['iters'], tuple(r['flags']), r['infile'])
+
+
+def run_one(r, exe=REF):
     with tempfile.TemporaryDirectory() as td:
         outp = os.path.join(td, 'out.txt')
-        cmd = [REF, '-q', str(r['q']), '-n', str(r['n']), '-R', str(r['R']), '-M', str(r['M']),
+        cmd = [exe, '-q', str(r['q']), '-n', str(r['n']), '-R', str(r['R']), '-M', str(r['M']),
                '-s', str(r['seed']), '--iters', str(r['iters']), '--preset', 'p5b',
                '--threads', str(r['threads']), '-t', '1e9', '--quiet', '--out', outp] + r['flags']
         if r['infile']:
@@ -109,6 +156,18 @@ def run_one(r):
     rec['uncovered'] = int(m.group(1))
     rec['iters_done'] = int(m.group(2))
     rec['kicks'] = int(m.group(3))
+    if int(m.group(4)) >= 0:
+        rec['ttt_it'] = int(m.group(5))
+    if exe == REF_PROF:
+        prof = {}
+        for line in p.stdout.splitlines():
+            if line.startswith('# PROF'):
+                for tok in line.split():
+                    if '=' in tok:
+                        k, v = tok.split('=', 1)
+                        if k in PROF_KEYS:
+                            prof[k] = float(v)
+        rec['prof'] = prof
     rec['out_sha256'] = hashlib.sha256(text.encode()).hexdigest()
     if len(text) <= 2048:
         rec['out_text'] = text
@@ -118,13 +177,33 @@ def run_one(r):
 def main():
     if not os.path.exists(REF):
         sys.exit('build the reference first: make -C oracle ref')
+    # --only TAG[,TAG]: regenerate those tags, keep every other record
+    only = set(sys.argv[sys.argv.index('--only') + 1].split(',')) if '--only' in sys.argv else None
     todo = runs()
+    kept = []
+    if only is not None:
+        todo = [r for r in todo if r['tag'] in only]
+        kept = [r for r in json.load(open(os.path.join(HERE, 'trajectories.json')))['runs']
+                if r['tag'] not in only]
     heavy = [r for r in todo if r['threads'] > 1]
     light = [r for r in todo if r['threads'] == 1]
     with ThreadPoolExecutor(max_workers=8) as ex:
         res_light = list(ex.map(run_one, light))
     res_heavy = [run_one(r) for r in heavy]
     res = res_light + res_heavy
+    # the -DPROF aggregates, attached to the matching run (added if absent)
+    byk = {prof_key(r): r for r in res}
+    for pr in (prof_runs() if only is None else []):
+        rec = run_one(pr, exe=REF_PROF)
+        k = prof_key(pr)
+        if k in byk:
+            for f in ('uncovered', 'iters_done', 'kicks', 'out_sha256'):
+                assert byk[k][f] == rec[f], ('the -DPROF build diverged', k, f)
+            byk[k]['prof'] = rec['prof']
+        else:
+            rec['tag'] = 'prof'
+            res.append(rec)
+    res += kept
     res.sort(key=lambda r: (r['tag'], r['q'], r['n'], r['R'], r['M'], r['seed'], r['iters']))
     json.dump({'generator': 'tests/golden/make_golden.py',
                'solver': 'cov/engine/covsearch2e.c (reference, gcc -O3, --preset p5b)',

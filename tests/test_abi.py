@@ -53,3 +53,9 @@ def test_create_without_gpu_fails_loudly():
     from coldcase_amd import Cell, CovError
     with pytest.raises(CovError):
         Cell(5, 7, 2, 400)
+
+
+def test_graph_steps_constant_matches_the_header():
+    from coldcase_amd import engine
+    src = open(os.path.join(ROOT, 'include', 'covhip.h')).read()
+    assert int(re.search(r'#define COV_GRAPH_STEPS (\d+)', src).group(1)) == engine.GRAPH_STEPS

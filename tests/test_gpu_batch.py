@@ -3,10 +3,14 @@ against the same cells run alone: every cell's counters, code, best code and
 coverage counts must be identical, whatever the mix of shapes, launch modes,
 run splits, kicks and early covers; one cell is also checked against the
 oracle step for step."""
+import hashlib
+import math
+
 import numpy as np
 import pytest
 
 from oracle.oracle import Oracle
+from tests._util import cli_code_sha, golden_runs
 
 pytestmark = pytest.mark.gpu
 
@@ -180,4 +184,58 @@ def test_cells_created_while_the_device_is_busy():
         assert _snap(c) == _solo(spec, 200), spec
         c.close()
     for c in busy:
+        c.close()
+
+
+def _light(c):
+    st = c.state()
+    best, bu = c.best()
+    return (st['it'], st['uncovered'], st['best_uncovered'], st['kicks'], st['draws'], st['done'], st['fault'],
+            c.code().tobytes(), best.tobytes(), bu, hashlib.sha256(c.cnt().tobytes()).hexdigest())
+
+
+def test_c4_grid_fleet():
+    """The batched configuration itself: its 64 instances (the bench's own
+    draw, coldcase_amd.workloads.c4_specs, seeds 1000003 + 7919 i), grouped
+    by cost and scaled by the autotuner, advanced as a fleet of concurrent
+    batches in chunks.  After 40 steps the instances with reference fixtures
+    match them (best uncovered, iterations, kicks, published code); after 80
+    every cell equals its solo run."""
+    from coldcase_amd import Fleet
+    from coldcase_amd.autotune import tune_fleet
+    from coldcase_amd.workloads import c4_seed, c4_specs
+    specs, redrawn = c4_specs()
+    assert len(specs) == 64 and not redrawn
+    print('case c4 fleet: 64 instances', flush=True)
+    cells = [_cell((q, n, R, M, c4_seed(i), 10, 5000)) for i, (q, n, R, M) in enumerate(specs)]
+    costs = [math.comb(n, R) * (q - 1) ** R + M for (q, n, R, M) in specs]
+    groups, factors, idx = tune_fleet(cells, costs, ngroups=4, passes=1, steps=40)
+    assert len(groups) == 4 and sorted(sum(idx, [])) == list(range(64))
+    for i, c in enumerate(cells):
+        c.search_init(c4_seed(i))
+    gold = {(r['q'], r['n'], r['R'], r['M'], r['seed']): r for r in golden_runs() if r['tag'] == 'C4-exact'}
+    assert len(gold) == 10
+    with Fleet(groups) as f:
+        f.run(40, chunk=20)
+        seen = 0
+        for i, (spec, c) in enumerate(zip(specs, cells)):
+            r = gold.get(spec + (c4_seed(i),))
+            if r is None:
+                continue
+            st = c.check_fault()
+            best, bu = c.best()
+            assert (bu, st['it'], st['kicks']) == (r['uncovered'], r['iters_done'], r['kicks']), spec
+            assert cli_code_sha(best, spec[0], spec[3]) == r['out_sha256'], spec
+            seen += 1
+        assert seen == 10
+        f.run(40, chunk=20)
+    for i, (spec, c) in enumerate(zip(specs, cells)):
+        solo = _cell(spec + (c4_seed(i), 10, 5000))
+        solo.run(80, mode=1)
+        want = _light(solo)
+        solo.close()
+        got = _light(c)
+        assert got[:7] == want[:7], (i, spec, got[:7], want[:7])
+        assert got[7:] == want[7:], (i, spec)
+    for c in cells:
         c.close()

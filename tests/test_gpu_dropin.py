@@ -10,6 +10,7 @@ the covering radius, and the CLI's exit codes.
 import hashlib
 import os
 import re
+import time
 
 import numpy as np
 import pytest
@@ -37,11 +38,105 @@ def test_covsearch_cli_matches_reference(r, tmp_path, capsys):
         argv += ['--in', os.path.join(DATA, r['infile'])]
     rc = covsearch.main(argv)
     text = capsys.readouterr().out
-    m = re.search(r'RESULT q=\d+ n=\d+ R=\d+ M=\d+ uncovered=(\d+) iters=(\d+) kicks=(\d+)', text)
+    m = re.search(r'RESULT q=\d+ n=\d+ R=\d+ M=\d+ uncovered=(\d+) iters=(\d+) kicks=(\d+) .*target=(-?\d+) '
+                  r'ttt_cpu=(\S+) ttt_wall=(\S+) ttt_it=(-?\d+)', text)
     assert m, text
     assert (int(m.group(1)), int(m.group(2)), int(m.group(3))) == (r['uncovered'], r['iters_done'], r['kicks'])
+    f = r['flags']
+    target = int(f[f.index('--target') + 1]) if '--target' in f else -1
+    assert int(m.group(4)) == target
+    assert int(m.group(7)) == r.get('ttt_it', -1)
+    if r.get('ttt_it', -1) >= 0:
+        assert 0.0 <= float(m.group(6)) < 600 and 0.0 <= float(m.group(5)) < 600
+    else:
+        assert float(m.group(5)) == -1.0 and float(m.group(6)) == -1.0
     assert rc == (0 if r['uncovered'] == 0 else 1)
     assert hashlib.sha256(open(out, 'rb').read()).hexdigest() == r['out_sha256']
+
+
+def _result(text):
+    m = re.search(r'RESULT .*uncovered=(\d+) iters=(\d+) kicks=(\d+) time=(\S+) .*cpu=(\S+) cpurate', text)
+    assert m, text
+    return int(m.group(1)), int(m.group(2)), int(m.group(3)), float(m.group(4)), float(m.group(5))
+
+
+@pytest.mark.parametrize('budget', [['-t', '1.5'], ['--cpu', '0.6', '-t', '999999']], ids=['wall', 'cpu'])
+def test_covsearch_cli_stops_at_its_budget(budget, tmp_path, capsys):
+    """The device reads the clock before every step (covsearch2e.c:1305-1315):
+    a run far longer than its budget ends within it, publishes M distinct
+    codewords, and is a clean prefix of the unbounded trajectory."""
+    from coldcase_amd import covsearch
+    print('case cli budget %s' % budget, flush=True)
+    out = str(tmp_path / 'code.txt')
+    base = ['-q', '6', '-n', '9', '-R', '4', '-M', '660', '-s', '5', '--random-init', '--tabu', '10', '--quiet']
+    t0 = time.monotonic()
+    rc = covsearch.main(base + ['--out', out] + budget)
+    wall = time.monotonic() - t0
+    unc, iters, kicks, el, cpu = _result(capsys.readouterr().out)
+    assert rc == 1 and 100 < iters < 10 ** 6
+    lim = float(budget[1])
+    assert wall < lim + 2.0, wall
+    if budget[0] == '--cpu':
+        assert lim - 0.05 <= cpu < lim + 0.25, cpu
+    lines = open(out).read().split()
+    assert len(lines) == 660 and len(set(lines)) == 660
+    # the same run with its iteration count as the budget publishes the same code
+    out2 = str(tmp_path / 'code2.txt')
+    rc = covsearch.main(base + ['--out', out2, '--iters', str(iters), '-t', '1e9'])
+    assert _result(capsys.readouterr().out)[:3] == (unc, iters, kicks)
+    assert open(out2).read() == open(out).read()
+
+
+def test_cell_target_matches_oracle():
+    """--target on the device: the trace, the stop (done = 2) and the
+    iteration that reached the target equal the oracle's."""
+    from coldcase_amd import Cell
+    from oracle.oracle import Oracle
+    q, n, R, M, seed, target = 5, 7, 2, 400, 3, 4300
+    print('case target K%d(%d,%d) M=%d seed %d target %d' % (q, n, R, M, seed, target), flush=True)
+    with Cell(q, n, R, M) as c:
+        c.search_params(tenure=10)
+        c.search_init(seed)
+        c.limits(target)
+        tr = c.run(20000, trace=True)
+        st = c.state()
+        ttt_it, age = c.ttt()
+    o = Oracle(q, n, R, M, seed).init(greedy=False)
+    o.set_target(target)
+    ref = o.run(20000, tabu=10, trace=True)
+    assert tr.shape == ref.shape and (tr == ref).all()
+    assert st['done'] == 2 and st['best_uncovered'] <= target
+    assert ttt_it == o.ttt_it == st['it'] and age >= 0.0
+    # lifting the target resumes the same trajectory
+    with Cell(q, n, R, M) as c:
+        c.search_params(tenure=10)
+        c.search_init(seed)
+        c.limits(target)
+        c.run(20000)
+        c.limits(-1)
+        tr2 = c.run(50, trace=True)
+    o.set_target(-1)
+    ref2 = o.run(50, tabu=10, trace=True)
+    assert (tr2 == ref2).all()
+
+
+def test_cell_deadline_stops_before_a_step():
+    from coldcase_amd import Cell
+    print('case deadline K6(9,4)', flush=True)
+    with Cell(6, 9, 4, 660) as c:
+        c.search_params(tenure=10)
+        c.search_init(1)
+        c.limits(-1, -1.0)               # already past: no step begins
+        assert c.run(100) == 0 and c.state()['it'] == 0
+        # 40,000 graph-replayed steps would take about 1.6 s; the steps
+        # launched after the deadline return at once
+        c.limits(-1, 0.2)
+        t0 = time.monotonic()
+        ran = c.run(40000, mode=1)
+        el = time.monotonic() - t0
+        assert 0 < ran < 40000 and el < 1.2, (ran, el)
+        c.limits(-1, 0.0)                # no deadline
+        assert c.run(10) == 10
 
 
 def _word(w, q, n):

@@ -8,6 +8,7 @@ import numpy as np
 import pytest
 
 from oracle.oracle import TR_FIELDS, Oracle
+from tests._util import golden_runs
 
 pytestmark = pytest.mark.gpu
 
@@ -111,3 +112,31 @@ def test_selection_above_one_wave():
     tr = compare(6, 9, 4, 1300, 3, 30, check_cnt=False, threads=8)
     nc = tr[:, TR_FIELDS.index('ncand')]
     assert (nc > 256).any() and (nc <= 256).any(), nc
+
+
+def _prof_id(r):
+    return 'K%d(%d,%d)M%d-s%d-it%d-%s' % (r['q'], r['n'], r['R'], r['M'], r['seed'], r['iters'],
+                                         '_'.join(r['flags']))
+
+
+@pytest.mark.parametrize('r', [r for r in golden_runs() if 'prof' in r], ids=_prof_id)
+def test_step_sums_match_reference_profile(r):
+    """Over whole runs of up to 10^4 moves (K6(9,4) seed 1 included), the
+    engine's per-step records sum to the reference's -DPROF aggregates:
+    mean candidates, sample size, uncovered count before each step, selected
+    delta and distinct codewords (covsearch2e.c:1590-1618)."""
+    from tests._util import prof_aggregates, prof_matches
+    f = r['flags']
+    tabu = int(f[f.index('--tabu') + 1])
+    stall = int(f[f.index('--stall') + 1]) if '--stall' in f else 5000
+    print('case prof %s' % _prof_id(r), flush=True)
+    cell = _cell(r['q'], r['n'], r['R'], r['M'])
+    cell.search_params(tenure=tabu, stall=stall, kick=2)
+    cell.search_init(r['seed'], greedy='--random-init' not in f)
+    unc0 = cell.state()['uncovered']
+    tr = cell.run(r['iters'], mode=1, trace=True)
+    st = cell.check_fault()
+    _, bu = cell.best()
+    cell.close()
+    assert (bu, st['it'], st['kicks']) == (r['uncovered'], r['iters_done'], r['kicks'])
+    assert prof_matches(prof_aggregates(tr, unc0), r['prof']) == {}

@@ -1,10 +1,13 @@
 """The serving loop on the GPU: every served result equals the same request
 run alone (tolerance 0), verify answers match the oracle, admission keeps
 resident memory within the budget, and oversized requests are refused."""
+import os
+
 import numpy as np
 import pytest
 
 from oracle.verify_oracle import mindist
+from tests._util import DATA, cli_code_sha, golden_runs, read_code
 
 pytestmark = pytest.mark.gpu
 
@@ -70,3 +73,33 @@ def test_budget_admission_and_refusal():
     for r, got in zip(small, res[:5]):
         assert not got.error and got.answer() == _alone(r).answer()
     assert res[5].error
+
+
+def test_serve_k6_10_4_search_and_record_verify():
+    """The serving configuration's largest cell, K6(10,4) M=2784 (121 MB of
+    counts): a search from the fixture's seed and a verify of the record
+    code, in a mix with smaller requests.  The search equals its run alone
+    and the reference's fixture (best uncovered, iterations, kicks, the
+    published code); the verify finds the record code covering."""
+    from coldcase_amd.serve import Request, Server
+    print('case serve K6(10,4) search + record verify', flush=True)
+    gold = [r for r in golden_runs() if r['tag'] == 'C5' and r['iters'] == 30][0]
+    rec = read_code(os.path.join(DATA, 'K6_10_4_M2784.txt'), 10)
+    reqs = [Request(0, 'search', 6, 10, 4, 2784, arrival=0.0, seed=gold['seed'], moves=30),
+            Request(1, 'verify', 6, 10, 4, 2784, arrival=0.001, code=rec),
+            Request(2, 'search', 6, 9, 4, 660, arrival=0.002, seed=11, moves=500),
+            Request(3, 'search', 5, 7, 2, 400, arrival=0.003, seed=12, moves=2000),
+            Request(4, 'verify', 6, 9, 4, 660, arrival=0.004,
+                    code=read_code(os.path.join(DATA, 'K6_9_4_M660.txt'), 9)),
+            Request(5, 'search', 6, 10, 4, 2784, arrival=0.005, seed=gold['seed'] + 1, moves=20)]
+    srv = Server(tick_steps=8)
+    res = srv.run(reqs, realtime=True)
+    assert all(not r.error for r in res)
+    assert srv.peak_cells >= 2
+    for r, got in zip(reqs, res):
+        assert got.answer() == _alone(r).answer(), r.rid
+    s0 = res[0]
+    assert (s0.best_uncovered, s0.iters, s0.kicks) == (gold['uncovered'], gold['iters_done'], gold['kicks'])
+    assert cli_code_sha(s0.best_code, 6, 2784) == gold['out_sha256']
+    assert res[1].uncovered == 0 and res[1].radius == 4 and res[1].witnesses == []
+    assert res[4].uncovered == 0 and res[4].radius == 4

@@ -1,8 +1,10 @@
 """The serving loop's host logic on the CPU, with stand-in cells and batches
 that record what the loop does to them: no batch is built from or launched
 with a closed cell, no cell is closed or read while a batch holding it is in
-flight, a lane never advances more searches than its cap, every request gets
-exactly one answer, and every admitted cell is closed at the end."""
+flight, no cell is closed while a batch that holds it is open, a lane never
+advances more searches than its cap, a cell that faults fails its own
+request only, every request gets exactly one answer, and every admitted cell
+is closed at the end."""
 import random
 import time
 
@@ -21,6 +23,8 @@ class _Cell:
         self.inflight = 0
         self.it = 0
         self.done = False
+        self.fault = 0
+        self.holders = 0
         _Cell.live.append(self)
 
     def search_params(self, **k):
@@ -42,7 +46,13 @@ class _Cell:
 
     def state(self):
         assert not self.closed and self.inflight == 0, 'state read while in flight'
-        return {'done': self.done, 'uncovered': 5, 'it': self.it, 'kicks': 0}
+        return {'done': self.done, 'uncovered': 5, 'it': self.it, 'kicks': 0, 'fault': self.fault}
+
+    def check_fault(self, st=None):
+        st = self.state() if st is None else st
+        if st['fault']:
+            raise S.CovError('fault %d' % st['fault'])
+        return st
 
     def best(self):
         assert not self.closed and self.inflight == 0, 'best read while in flight'
@@ -68,6 +78,8 @@ class _Batch:
         if _Batch.cap is not None:
             assert len(cells) <= _Batch.cap, 'lane above its cap'
         self.cells = list(cells)
+        for c in self.cells:
+            c.holders += 1
         self.busy_until = None
         self.closed = False
         self.k = 0
@@ -83,17 +95,30 @@ class _Batch:
     def query(self):
         return self.busy_until is None or time.perf_counter() >= self.busy_until
 
+    fault_rate = 0.0
+
     def sync(self):
         if self.busy_until is not None:
+            faulted = False
             for c in self.cells:
                 c.inflight -= 1
+                if c.done:
+                    continue
+                if random.random() < _Batch.fault_rate:
+                    c.fault, c.done, faulted = 2, True, True
+                    continue
                 c.it += self.k
                 if random.random() < 0.01:
                     c.done = True
             self.busy_until = None
+            if faulted:
+                raise S.CovError('cov_batch_sync: a cell faulted')
 
     def close(self):
         assert self.busy_until is None, 'closing a busy batch'
+        if not self.closed:
+            for c in self.cells:
+                c.holders -= 1
         self.closed = True
 
 
@@ -111,13 +136,18 @@ def test_lane_logic(monkeypatch, seed):
     reqs = S.poisson_workload(seed, 40, 2000.0, SHAPES, verify_frac=0.3, moves=moves)
     cap = rnd.choice([None, 1, 3])
     _Batch.cap = cap
+    _Batch.fault_rate = 0.02 if seed % 3 == 2 else 0.0
     _Cell.live = []
     srv = S.Server(hbm_budget=rnd.choice([40 * 1000 * 2784, 10 ** 12]), max_cells=rnd.choice([8, 64]),
                    tick_steps=rnd.choice([64, 256]), lane_cap=cap)
     res = srv.run(reqs, realtime=rnd.random() < 0.5)
     assert [r.rid for r in res] == sorted(r.rid for r in reqs)
+    failed = 0
     for q, r in zip(sorted(reqs, key=lambda x: x.rid), res):
-        assert not r.error
-        if q.kind == 'search':
+        if r.error:
+            assert _Batch.fault_rate > 0 and q.kind == 'search' and 'fault' in r.error
+            failed += 1
+        elif q.kind == 'search':
             assert 0 < r.iters <= q.moves       # its moves, or fewer once covered
+    assert failed == sum(1 for c in _Cell.live if c.fault)
     assert all(c.closed for c in _Cell.live), 'leaked cell'
