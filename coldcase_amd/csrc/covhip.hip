@@ -1,6 +1,7 @@
 // covhip.hip -- host side of libcovhip.so: cell handles, tables, launches and
 // the C ABI declared in include/covhip.h.
 #include <hip/hip_runtime.h>
+#include <hip/hip_ext.h>
 
 #include <algorithm>
 #include <cstddef>
@@ -17,6 +18,7 @@
 #include "cov_step.h"
 
 using namespace cov;
+
 
 struct cov_cell {
     Cell h{};                 // host copy of the device descriptor
@@ -308,23 +310,25 @@ int pool_event(cov_cell* h, hipEvent_t* e)
     return COV_OK;
 }
 
-int timed_begin(cov_cell* h, int kind, hipEvent_t* a)
+// Launch on the cell's stream.  With timing on, the launch records a start
+// and a stop event around the kernel's own execution (hipExtLaunchKernelGGL),
+// so a kind accumulates kernel durations as a kernel trace reports them,
+// without the gaps of separately recorded events.
+template <typename K, typename... A>
+int launch_t(cov_cell* h, int kind, K kern, dim3 g, dim3 b, uint32_t lds, A... args)
 {
-    (void)kind;
-    if (!h->timing) return COV_OK;
+    if (!h->timing) {
+        hipLaunchKernelGGL(kern, g, b, lds, h->stream, args...);
+        HCK(h, hipGetLastError());
+        return COV_OK;
+    }
     if (h->pend.size() >= 4096) RC(timing_collect(h));
-    RC(pool_event(h, a));
-    HCK(h, hipEventRecord(*a, h->stream));
-    return COV_OK;
-}
-
-int timed_end(cov_cell* h, int kind, hipEvent_t a)
-{
-    if (!h->timing) return COV_OK;
-    hipEvent_t b;
-    RC(pool_event(h, &b));
-    HCK(h, hipEventRecord(b, h->stream));
-    h->pend.push_back({kind, a, b});
+    hipEvent_t a = nullptr, e = nullptr;
+    RC(pool_event(h, &a));
+    RC(pool_event(h, &e));
+    hipExtLaunchKernelGGL(kern, g, b, lds, h->stream, a, e, 0, args...);
+    HCK(h, hipGetLastError());
+    h->pend.push_back({kind, a, e});
     return COV_OK;
 }
 
@@ -332,16 +336,12 @@ int do_recount(cov_cell* h)
 {
     h->cnt_valid = true;
     h->th_valid = false;
-    hipEvent_t a = nullptr;
-    RC(timed_begin(h, 3, &a));
     const int g = (int)std::min<int64_t>(h->plan.ntile, 8192);
     // the kernel's dynamic LDS limit is one value for every cell: set it to
     // this cell's tile before its launch
     HCK(h, hipFuncSetAttribute((const void*)k_recount, hipFuncAttributeMaxDynamicSharedMemorySize,
                                (int)(h->plan.T * 4)));
-    hipLaunchKernelGGL(k_recount, dim3(g), dim3(WG), (size_t)h->plan.T * 4, h->stream, h->h, h->plan);
-    HCK(h, hipGetLastError());
-    RC(timed_end(h, 3, a));
+    RC(launch_t(h, 3, k_recount, dim3(g), dim3(WG), (uint32_t)(h->plan.T * 4), h->h, h->plan));
     // (fills, not copies from this frame: the call returns before they run)
     HCK(h, hipMemsetAsync(CTL_PTR(h, unc), 0, 8, h->stream));
     HCK(h, hipMemsetAsync(CTL_PTR(h, ul_n), 0, 8, h->stream));
@@ -376,23 +376,10 @@ int list_rebuild(cov_cell* h)
 
 int launch_step(cov_cell* h)
 {
-    hipEvent_t a = nullptr;
-    RC(timed_begin(h, 0, &a));
-    hipLaunchKernelGGL(k_eval, dim3(h->eval_groups), dim3(WG), 0, h->stream, h->h);
-    HCK(h, hipGetLastError());
-    RC(timed_end(h, 0, a));
-    RC(timed_begin(h, 4, &a));
-    hipLaunchKernelGGL(k_select, dim3(1), dim3(WG), 0, h->stream, h->h);
-    HCK(h, hipGetLastError());
-    RC(timed_end(h, 4, a));
-    RC(timed_begin(h, 1, &a));
-    hipLaunchKernelGGL(k_commit, dim3(h->commit_groups), dim3(WG), 0, h->stream, h->h);
-    HCK(h, hipGetLastError());
-    RC(timed_end(h, 1, a));
-    RC(timed_begin(h, 5, &a));
-    hipLaunchKernelGGL(k_post, dim3(1), dim3(WG), 0, h->stream, h->h);
-    HCK(h, hipGetLastError());
-    RC(timed_end(h, 5, a));
+    RC(launch_t(h, 0, k_eval, dim3(h->eval_groups), dim3(WG), 0, h->h));
+    RC(launch_t(h, 4, k_select, dim3(1), dim3(WG), 0, h->h));
+    RC(launch_t(h, 1, k_commit, dim3(h->commit_groups), dim3(WG), 0, h->h));
+    RC(launch_t(h, 5, k_post, dim3(1), dim3(WG), 0, h->h));
     return COV_OK;
 }
 
@@ -903,14 +890,13 @@ static int run_transform(cov_cell_t* h, int full, int rcut, int64_t hh[DT_HB])
     for (int i = 0; i < np; i++) {
         const DtPass& p = pass[i];
         const int g = (int)std::min<int64_t>(p.ntiles, 8192);
-        hipEvent_t ev = nullptr;
-        RC(timed_begin(h, 2, &ev));
         const int nt = h->dt_threads ? h->dt_threads : DT_THREADS_DEFAULT;
+        int lrc = COV_OK;
         if (p.seed) {
             const bool ok = dt_seed_dispatch(p.q, [&](auto qb) {
                 using QB = decltype(qb);
-                hipLaunchKernelGGL((k_dt_seed<QB::q>), dim3(g), dim3(DT_COL_NT), (size_t)p.lds, h->stream, p, C.dist,
-                                   h->cwb, dh);
+                lrc = launch_t(h, 2, (k_dt_seed<QB::q>), dim3(g), dim3(DT_COL_NT), (uint32_t)p.lds, p, C.dist,
+                               h->cwb, dh);
             });
             if (!ok) {
                 h->err = "distance transform: no seed kernel for this q";
@@ -920,23 +906,19 @@ static int run_transform(cov_cell_t* h, int full, int rcut, int64_t hh[DT_HB])
             const unsigned gc = (p.nthr + DT_COL_NT - 1) / DT_COL_NT;
             const bool ok = dt_col_dispatch(p.q, p.nax, [&](auto qb) {
                 using QB = decltype(qb);
-                hipLaunchKernelGGL((k_dt_col<QB::q, QB::b>), dim3(gc), dim3(DT_COL_NT), 0, h->stream, p, C.dist, dh);
+                lrc = launch_t(h, 2, (k_dt_col<QB::q, QB::b>), dim3(gc), dim3(DT_COL_NT), 0, p, C.dist, dh);
             });
             if (!ok) {
                 h->err = "distance transform: no column kernel for this (q, axes)";
                 return COV_ELIMIT;
             }
         } else if (nt == 1024)
-            hipLaunchKernelGGL(k_dt_pass<1024>, dim3(g), dim3(1024), (size_t)dt_lds_bytes(p), h->stream, p, C.dist,
-                               h->cwb, dh);
+            lrc = launch_t(h, 2, k_dt_pass<1024>, dim3(g), dim3(1024), (uint32_t)dt_lds_bytes(p), p, C.dist, h->cwb, dh);
         else if (nt == 512)
-            hipLaunchKernelGGL(k_dt_pass<512>, dim3(g), dim3(512), (size_t)dt_lds_bytes(p), h->stream, p, C.dist,
-                               h->cwb, dh);
+            lrc = launch_t(h, 2, k_dt_pass<512>, dim3(g), dim3(512), (uint32_t)dt_lds_bytes(p), p, C.dist, h->cwb, dh);
         else
-            hipLaunchKernelGGL(k_dt_pass<256>, dim3(g), dim3(256), (size_t)dt_lds_bytes(p), h->stream, p, C.dist,
-                               h->cwb, dh);
-        HCK(h, hipGetLastError());
-        RC(timed_end(h, 2, ev));
+            lrc = launch_t(h, 2, k_dt_pass<256>, dim3(g), dim3(256), (uint32_t)dt_lds_bytes(p), p, C.dist, h->cwb, dh);
+        RC(lrc);
         if (i == 0) {
             hipLaunchKernelGGL(k_cwb_mark, dim3(gm), dim3(WG), 0, h->stream, h->h, h->cwb, 0);
             HCK(h, hipGetLastError());

@@ -190,9 +190,10 @@ int     cov_set_leave_mode(cov_cell_t *h, int mode);
  * per transform workgroup (0 = the default, 512; else 256 or 1024); the tile
  * plan groups the n axes into as few HBM passes as tiles of this size allow. */
 int     cov_set_transform_tile(cov_cell_t *h, int lds_bytes, int threads);
-/* Accumulate per-kernel device time with HIP events recorded around each
- * launch on the cell's stream (on = 1; read back in batches, no host sync
- * between launches; plain-mode search steps only), read totals in ms and
+/* Accumulate per-kernel device time (on = 1): each launch on the cell's
+ * stream records a start and a stop event around its own execution
+ * (hipExtLaunchKernelGGL), read back in batches with no host sync between
+ * launches; search steps are timed in plain mode only.  Read totals in ms and
  * launch counts for kernel kinds 0 = evaluator, 1 = commit, 2 = distance
  * transform pass, 3 = recount, 4 = selection, 5 = post (nkinds <= 6). */
 int     cov_timing(cov_cell_t *h, int on);
