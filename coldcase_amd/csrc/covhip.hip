@@ -7,6 +7,7 @@
 #include <cstddef>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -19,6 +20,12 @@
 
 using namespace cov;
 
+// The allocation-free gather's scratch (cov_gather_uncovered_noalloc): NA_BLK
+// block counts, the total on its own 256-byte line, NA_CAP output indices.
+constexpr int64_t NA_BLK = 16384;
+constexpr int64_t NA_CAP = 65536;
+constexpr int64_t NA_WORDS = NA_BLK + 32 + NA_CAP;
+static __device__ __align__(256) int64_t scratch[NA_WORDS];
 
 struct cov_cell {
     Cell h{};                 // host copy of the device descriptor
@@ -435,6 +442,62 @@ int64_t ordered_gather(cov_cell* h, Sel sel, int64_t* out, int64_t cap)
     return total;
 }
 
+// ------------------------------------------- the gather without allocation
+// cov_gather_uncovered_noalloc keeps its block counts, their total and its
+// output staging in one file-scope device array instead of allocating them
+// per call: NA_BLK block counts (words up to NA_BLK * 4096 = 2^26), the
+// total, then NA_CAP indices.  Each device holds its own copy for the life
+// of the process; host code reaches it only through the address that
+// hipGetSymbolAddress gives on that device (na_scratch_ptr).
+// (NA_BLK, NA_CAP and the array are declared above the anonymous namespace.)
+
+// exclusive prefix sum of v[0..nb) in place and its total, one workgroup
+__global__ void __launch_bounds__(WG) k_scan_excl(int64_t* v, int64_t nb, int64_t* total)
+{
+    __shared__ WgScan sc;
+    const int64_t per = (nb + WG - 1) / WG;
+    const int64_t b0 = (int64_t)threadIdx.x * per;
+    int64_t s = 0;
+    for (int64_t k = 0; k < per; k++)
+        if (b0 + k < nb) s += v[b0 + k];
+    int64_t tot;
+    int64_t run = wg_incl_sum(s, sc, &tot) - s;
+    for (int64_t k = 0; k < per; k++)
+        if (b0 + k < nb) {
+            const int64_t t = v[b0 + k];
+            v[b0 + k] = run;
+            run += t;
+        }
+    if (threadIdx.x == 0) *total = tot;
+}
+
+constexpr int NA_DEVICES = 64;
+
+// this device's address of `scratch`, resolved once per device (with that
+// device current, as the calling thread's is); nullptr on failure
+int64_t* na_scratch_ptr(cov_cell* h)
+{
+    static std::mutex mu;
+    static int64_t* addr[NA_DEVICES] = {};
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess || dev < 0 || dev >= NA_DEVICES) {
+        h->err = "no device for the scratch array";
+        return nullptr;
+    }
+    std::lock_guard<std::mutex> lk(mu);
+    if (!addr[dev]) {
+        void* p = nullptr;
+        e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p) {
+            h->err = std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e);
+            return nullptr;
+        }
+        addr[dev] = (int64_t*)p;
+    }
+    return addr[dev];
+}
+
 }  // namespace
 
 // =============================================================== the ABI
@@ -773,6 +836,40 @@ int64_t cov_gather_uncovered(cov_cell_t* h, int64_t* out, int64_t cap)
     if (h && !h->cnt_valid) { h->err = "coverage counts not built (cov_recount)"; return COV_ESTATE; }
     if (!h || cap < 0 || (cap > 0 && !out)) return COV_EPARAM;
     return ordered_gather(h, SelZero{reinterpret_cast<const uint16_t*>(h->h.cnt32)}, out, cap);
+}
+
+int64_t cov_gather_uncovered_noalloc(cov_cell_t* h, int64_t* out, int64_t cap)
+{
+    if (h && !h->cnt_valid) { h->err = "coverage counts not built (cov_recount)"; return COV_ESTATE; }
+    if (!h || cap < 0 || (cap > 0 && !out)) return COV_EPARAM;
+    const Cell& C = h->h;
+    if (C.nblk > NA_BLK) { h->err = "more than 2^26 words for the allocation-free gather"; return COV_ELIMIT; }
+    if (cap > NA_CAP) { h->err = "cap above 65536 for the allocation-free gather"; return COV_ELIMIT; }
+    int64_t* sp = na_scratch_ptr(h);
+    if (!sp) return COV_ESTATE;
+    int64_t* bz = sp;                   // block counts, then their offsets
+    int64_t* tot = sp + NA_BLK;         // total selected
+    int64_t* stage = sp + NA_BLK + 32;  // the first min(cap, total) indices
+    if (h->used) HCK(h, hipStreamWaitEvent(h->stream, h->used, 0));
+    const SelZero sel{reinterpret_cast<const uint16_t*>(C.cnt32)};
+    const int g = grid_for(C.nblk, 1, 1, 8192);
+    hipLaunchKernelGGL(k_block_count<SelZero>, dim3(g), dim3(WG), 0, h->stream, h->h, sel, bz);
+    HCK(h, hipGetLastError());
+    hipLaunchKernelGGL(k_scan_excl, dim3(1), dim3(WG), 0, h->stream, bz, C.nblk, tot);
+    HCK(h, hipGetLastError());
+    if (cap > 0) {
+        hipLaunchKernelGGL(k_block_emit<SelZero>, dim3(g), dim3(WG), 0, h->stream, h->h, sel, bz, stage, cap);
+        HCK(h, hipGetLastError());
+    }
+    int64_t total = 0;
+    HCK(h, hipMemcpyAsync(&total, tot, 8, hipMemcpyDeviceToHost, h->stream));
+    HCK(h, hipStreamSynchronize(h->stream));
+    const int64_t m = std::min(cap, total);
+    if (m > 0) {
+        HCK(h, hipMemcpyAsync(out, stage, (size_t)m * 8, hipMemcpyDeviceToHost, h->stream));
+        HCK(h, hipStreamSynchronize(h->stream));
+    }
+    return total;
 }
 
 int64_t cov_gather_far(cov_cell_t* h, int r, int64_t* out, int64_t cap)

@@ -31,7 +31,7 @@ TR_FIELDS = ['it', 'u', 'ncand', 'ncw', 'bi', 'bp', 'bv', 'oldv', 'dsel', 'dunc'
 EXPORTS = ['cov_version', 'cov_strerror', 'cov_mem_bytes', 'cov_create', 'cov_destroy',
            'cov_last_error', 'cov_sync', 'cov_set_code', 'cov_set_code_flags', 'cov_get_code', 'cov_get_best',
            'cov_recount', 'cov_ball_update', 'cov_count_uncovered', 'cov_gather_uncovered',
-           'cov_first_uncovered_from', 'cov_read_cnt', 'cov_eval_moves', 'cov_commit',
+           'cov_first_uncovered_from', 'cov_gather_uncovered_noalloc', 'cov_read_cnt', 'cov_eval_moves', 'cov_commit',
            'cov_distance_transform', 'cov_distance_stats', 'cov_gather_far', 'cov_search_init', 'cov_search_params',
            'cov_search_run', 'cov_search_launch', 'cov_search_state', 'cov_query', 'cov_search_limits',
            'cov_search_ttt', 'cov_set_launch',
@@ -86,6 +86,7 @@ def lib():
         'cov_count_uncovered': (i64, [vp]),
         'cov_gather_uncovered': (i64, [vp, vp, i64]),
         'cov_first_uncovered_from': (i64, [vp, i64]),
+        'cov_gather_uncovered_noalloc': (i64, [vp, vp, i64]),
         'cov_read_cnt': (i32, [vp, i64, i64, vp]),
         'cov_eval_moves': (i32, [vp, vp, i32, vp, vp]),
         'cov_commit': (i32, [vp, i32, i32, i32, vp]),
@@ -214,6 +215,15 @@ class Cell:
     def gather_uncovered(self, cap):
         out = np.zeros(max(int(cap), 1), np.int64)
         tot = self._check(lib().cov_gather_uncovered(self.h, _p(out), int(cap)), 'cov_gather_uncovered')
+        return out[:min(int(cap), tot)], tot
+
+    def gather_uncovered_noalloc(self, cap):
+        """gather_uncovered without device allocation (a static per-GPU
+        scratch array; q^n <= 2^26, cap <= 65536; calls on one GPU must not
+        overlap)."""
+        out = np.zeros(max(int(cap), 1), np.int64)
+        tot = self._check(lib().cov_gather_uncovered_noalloc(self.h, _p(out), int(cap)),
+                          'cov_gather_uncovered_noalloc')
         return out[:min(int(cap), tot)], tot
 
     def first_uncovered_from(self, start):

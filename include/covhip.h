@@ -102,6 +102,16 @@ int64_t cov_count_uncovered(cov_cell_t *h);
  * returns the total.  Replaces gather_uncovered_kernel (cov/search/covsearch.cu:213)
  * with a deterministic order, and verify_pure's witnesses (cov/verify_cov.py:142-151). */
 int64_t cov_gather_uncovered(cov_cell_t *h, int64_t *out, int64_t cap);
+/* (d) cov_gather_uncovered without any allocation.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops).  The block counts, their total and
+ * the output staging live in one static device array per GPU (640 KB, held
+ * until the process ends) instead of per-call allocations.  Limits: q^n <=
+ * 2^26 words and cap <= 65536, else COV_ELIMIT.  Every call on one GPU uses
+ * that GPU's array and leaves its contents behind (each call rewrites what
+ * it reads); calls that overlap on one GPU, from any cells or threads, race
+ * and must be serialised by the caller. */
+int64_t cov_gather_uncovered_noalloc(cov_cell_t *h, int64_t *out, int64_t cap);
 /* (d) first uncovered word at or after start, cyclically; -1 if none.
  * The scan of pick_uncovered (cov/engine/covsearch2e.c:1003-1015). */
 int64_t cov_first_uncovered_from(cov_cell_t *h, int64_t start);
