@@ -120,14 +120,24 @@ def bench_search(a):
     cell = Cell(q, n, R, M)
     cell.search_params(tenure=TENURE, stall=5000, kick=2)
     cell.search_init(SEED)
-    cell.run(a.warmup * mps, mode=mode)
+
+    def go(moves, md):
+        # one launch of all the moves, or pieces of --chunk moves with a wait
+        # between (counter collection cannot hold tens of thousands of
+        # dispatches in flight)
+        k = a.chunk if a.chunk > 0 else moves
+        while moves > 0:
+            cell.launch(min(k, moves), mode=md)
+            cell.sync()
+            moves -= k
+
+    go(a.warmup * mps, mode)
     st0 = cell.state()
     b0 = cell.search_bytes()
     torch.cuda.synchronize()
     cell.sync()
     t0 = time.perf_counter()
-    cell.launch(a.steps * mps, mode=mode)
-    cell.sync()
+    go(a.steps * mps, mode)
     torch.cuda.synchronize()
     t1 = time.perf_counter()
     st1 = cell.check_fault()
@@ -141,8 +151,7 @@ def bench_search(a):
     if a.plain_steps > 0 and mode == MODE_GRAPH and not st1['done']:
         p0 = cell.state()['it']
         t0 = time.perf_counter()
-        cell.launch(a.plain_steps * mps, mode=MODE_PLAIN)
-        cell.sync()
+        go(a.plain_steps * mps, MODE_PLAIN)
         el_p = time.perf_counter() - t0
         pm = cell.check_fault()['it'] - p0
         plain = {'value': round(pm / el_p, 1), 'moves': pm, 'us_per_move': round(el_p * 1e6 / max(pm, 1), 3)}
@@ -164,7 +173,7 @@ def bench_search(a):
     ach_step = bm / (step_us * 1e-6) / 1e9
     roof = {'bound': 'hbm', 'achieved': round(ach, 1), 'peak': HBM_PEAK_GBS, 'unit': 'GB/s',
             'frac': round(ach / HBM_PEAK_GBS, 5),
-            'traffic': measured_traffic('pmc_step_K6_9_4.md', dom),
+            'traffic': measured_traffic('step_kernels_K6_9_4.md', dom),
             'kernel': dom, 'avg_launch_us': round(avg[dom], 3),
             'alg_bytes_per_launch': int(bm),
             'alg_bytes_note': 'B_move = M n + 2 sum_cw (S_f + E_cw) + 8 S per move, the reference move with every '
@@ -559,6 +568,7 @@ def main():
     ap.add_argument('--warmup', type=int, default=5)
     ap.add_argument('--moves-per-step', type=int, default=50000)
     ap.add_argument('--mode', choices=['graph', 'plain'], default='graph')
+    ap.add_argument('--chunk', type=int, default=0, help='launch search windows in pieces of this many moves')
     ap.add_argument('--timing-steps', type=int, default=300, help='plain moves timed with per-launch events')
     ap.add_argument('--plain-steps', type=int, default=1, help='steps of plain-mode moves after the window')
     ap.add_argument('--parts', default='search,cpu,batched,serve,transform')
