@@ -319,8 +319,12 @@ def cpu_portfolio(specs, seeds, moves, procs, total_moves):
     that rate, then packed longest-first onto `procs` processes."""
     def one(i):
         q, n, R, M = specs[i]
-        it, wall, cpu = run_ref(q, n, R, M, seeds[i], moves, 1)
-        return it / wall if wall > 0 else float('inf')
+        m = moves
+        while True:
+            it, wall, cpu = run_ref(q, n, R, M, seeds[i], m, 1)
+            if wall >= 0.2 or m >= 100 * moves:
+                return it / max(wall, 0.01)
+            m *= 10
 
     t0 = time.perf_counter()
     with ThreadPoolExecutor(max_workers=procs) as ex:
@@ -334,7 +338,8 @@ def cpu_portfolio(specs, seeds, moves, procs, total_moves):
     makespan = max(load)
     return {'value': round(len(specs) / makespan, 6), 'unit': 'solves/s', 'cores': procs,
             'kind': ref_exe()[1], 'projected': True,
-            'sample': 'each instance %d moves single-threaded, %d processes at once (%.1f s); 10^5-move time '
+            'sample': 'each instance %d moves or more (at least 0.2 s) single-threaded, %d processes at once '
+                      '(%.1f s); 10^5-move time '
                       'per instance from its rate, packed longest first' % (moves, procs, el),
             'makespan_s': round(makespan, 1), 'slowest_instance_moves_per_s': round(min(rates), 2)}
 
@@ -452,8 +457,14 @@ def cpu_serial(reqs, codes, threads, sample_moves):
     for r in reqs:
         key = (r.q, r.n, r.R, r.M)
         if r.kind == 'search' and key not in rate:
-            it, wall, _ = run_ref(*key, r.seed, sample_moves, threads)
-            rate[key] = it / wall
+            # grow the sample of a fast shape until it runs a measurable time
+            moves = sample_moves
+            while True:
+                it, wall, _ = run_ref(*key, r.seed, moves, threads)
+                if wall >= 0.2 or moves >= 100 * sample_moves:
+                    break
+                moves *= 10
+            rate[key] = it / max(wall, 0.01)
         if r.kind == 'verify' and key not in vt:
             t0 = time.perf_counter()
             dilate_counts(np.asarray(r.code), r.n, r.q, r.R)
@@ -473,7 +484,8 @@ def cpu_serial(reqs, codes, threads, sample_moves):
             'rps': round(len(reqs) / span, 6), 'cores': threads, 'kind': ref_exe()[1], 'projected': True,
             'search_moves_per_s': {'%d,%d,%d,%d' % k: round(v, 1) for k, v in rate.items()},
             'verify_s': {'%d,%d,%d,%d' % k: round(v, 3) for k, v in vt.items()},
-            'sample': 'searches at the CPU solver rate over each shape\'s first %d moves (%d threads); verifies '
+            'sample': 'searches at the CPU solver rate over each shape\'s first %d moves or more (at least 0.2 s; '
+                      '%d threads); verifies '
                       'by numpy axis dilation on one core; FIFO over the same arrivals' % (sample_moves, threads)}
 
 

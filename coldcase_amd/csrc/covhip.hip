@@ -17,6 +17,7 @@
 #include "cov_dist.h"
 #include "cov_batch.h"
 #include "cov_step.h"
+#include "cov_count.h"
 
 using namespace cov;
 
@@ -42,8 +43,8 @@ struct cov_cell {
     int g_steps = 0, g_eg = 0, g_cg = 0;
     // timing
     bool timing = false;
-    double ms[6] = {0, 0, 0, 0, 0, 0};
-    int64_t launches[6] = {0, 0, 0, 0, 0, 0};
+    double ms[7] = {0, 0, 0, 0, 0, 0, 0};
+    int64_t launches[7] = {0, 0, 0, 0, 0, 0, 0};
     std::vector<hipEvent_t> evs;          // event pool
     struct Pend { int kind; hipEvent_t a, b; };
     std::vector<Pend> pend;               // recorded, not yet read
@@ -62,6 +63,11 @@ struct cov_cell {
     int leave_mode = COV_LEAVE_AUTO;      // how the steps find the leaving side (cov_set_leave_mode)
     int th_on = 0;                        // the control block's th_on as last set from here
     double clk_hz = 1e8;                  // device wall clock rate (hipDeviceAttributeWallClockRate)
+    // count transform: (R+1) int32 layers of N words (allocated on first
+    // use); layer 0 holds the map after a sum transform
+    int32_t* ct = nullptr;
+    int ct_lds = 0;                       // LDS bytes per transform tile (0: the default)
+    int map_kind = -1;                    // source of the map in layer 0, -1: none
     // recorded on a batch's stream after each batched launch that includes
     // the cell: the cell's own stream waits for it before touching the cell,
     // and its memory is not freed before it
@@ -498,6 +504,82 @@ int64_t* na_scratch_ptr(cov_cell* h)
     return addr[dev];
 }
 
+// ------------------------------------------------------- count transform
+constexpr int CT_LDS_DEFAULT = 65536;
+
+// The passes of a count transform of R + 1 layers: one inner pass over the a
+// fastest axes (q^a words per tile), then outer passes of b axes each on
+// tiles of q^b rows by W words, as many axes per pass as the LDS budget
+// holds with W >= 64 (or the whole row segment when it is shorter).
+int ct_plan(int q, int n, int R, int64_t N, int lds, std::vector<CtPass>& out)
+{
+    out.clear();
+    const int R1 = R + 1;
+    const int64_t wb = 4LL * R1;
+    auto pw = [&](int e) { int64_t v = 1; for (int i = 0; i < e; i++) v *= q; return v; };
+    int a = 0;
+    while (a < n && pw(a + 1) * wb <= lds) a++;
+    if (a == 0) return COV_ELIMIT;
+    CtPass p{};
+    p.N = N; p.q = q; p.R1 = R1;
+    p.k = a; p.wsh = 0; p.rows = pw(a); p.rs = 1; p.span = pw(a); p.ncb = 1; p.ntiles = N / pw(a);
+    out.push_back(p);
+    int j = a;
+    while (j < n) {
+        const int64_t colspan = pw(j);
+        int wmin = 0;
+        while ((2LL << wmin) <= std::min<int64_t>(64, colspan)) wmin++;
+        int b = n - j;
+        while (b > 1 && pw(b) * (1LL << wmin) * wb > lds) b--;
+        if (pw(b) * (1LL << wmin) * wb > lds) {
+            while (wmin > 0 && pw(b) * (1LL << wmin) * wb > lds) wmin--;
+            if (pw(b) * (1LL << wmin) * wb > lds) return COV_ELIMIT;
+        }
+        int wsh = wmin;
+        while (wsh < 8 && (2LL << wsh) <= colspan && pw(b) * (2LL << wsh) * wb <= lds) wsh++;
+        CtPass o{};
+        o.N = N; o.q = q; o.R1 = R1; o.k = b; o.wsh = wsh; o.rows = pw(b); o.rs = colspan; o.span = pw(j + b);
+        o.ncb = (colspan + (1LL << wsh) - 1) >> wsh;
+        o.ntiles = (N / o.span) * o.ncb;
+        out.push_back(o);
+        j += b;
+    }
+    out.front().first = 1;
+    out.back().last = 1;
+    return COV_OK;
+}
+
+int run_count_transform(cov_cell* h, int source, int out_mode)
+{
+    const Cell& C = h->h;
+    if (!h->ct) {
+        HCK(h, hipMalloc((void**)&h->ct, (size_t)(C.R + 1) * (size_t)C.N * 4));
+    }
+    std::vector<CtPass> passes;
+    const int lds = h->ct_lds ? h->ct_lds : CT_LDS_DEFAULT;
+    if (ct_plan(C.q, C.n, C.R, C.N, lds, passes) != COV_OK) {
+        h->err = "count transform: no tile plan for this shape and LDS budget";
+        return COV_ELIMIT;
+    }
+    if (source == 0) {
+        HCK(h, hipMemsetAsync(h->ct, 0, (size_t)C.N * 4, h->stream));
+        hipLaunchKernelGGL(k_ct_scatter, dim3(grid_for(C.M, WG, 1, 256)), dim3(WG), 0, h->stream, h->h, h->ct);
+        HCK(h, hipGetLastError());
+    }
+    for (CtPass p : passes) {
+        p.src = source == 0 ? CT_SRC_LAYER : CT_SRC_CNT;
+        p.target = source == 2 ? 1 : 0;
+        p.out = out_mode;
+        const size_t lb = (size_t)p.R1 * (size_t)(p.rows << p.wsh) * 4;
+        HCK(h, hipFuncSetAttribute((const void*)k_ct_pass, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lb));
+        const int g = (int)std::min<int64_t>(p.ntiles, 16384);
+        RC(launch_t(h, 6, k_ct_pass, dim3(g), dim3(CT_NT), (uint32_t)lb, p, h->ct,
+                    (const uint16_t*)reinterpret_cast<const uint16_t*>(C.cnt32)));
+    }
+    h->map_kind = out_mode == CT_OUT_SUM ? source : -1;
+    return COV_OK;
+}
+
 }  // namespace
 
 // =============================================================== the ABI
@@ -715,6 +797,7 @@ int cov_destroy(cov_cell_t* h)
     if (h->h.dist) (void)hipFree(h->h.dist);
     if (h->cwb) (void)hipFree(h->cwb);
     if (h->dtstat) (void)hipFree(h->dtstat);
+    if (h->ct) (void)hipFree(h->ct);
     for (const auto& p : h->pend) { (void)hipEventDestroy(p.a); (void)hipEventDestroy(p.b); }
     for (hipEvent_t e : h->evs) (void)hipEventDestroy(e);
     (void)hipStreamDestroy(h->stream);
@@ -1059,6 +1142,140 @@ int cov_distance_stats(cov_cell_t* h, int r, int64_t* within, int* radius)
     return COV_OK;
 }
 
+int cov_count_transform(cov_cell_t* h, int source, int32_t* layers, int32_t* map)
+{
+    if (!h || source < 0 || source > 2) return COV_EPARAM;
+    if (source > 0 && !h->cnt_valid) { h->err = "coverage counts not built (cov_recount)"; return COV_ESTATE; }
+    const Cell& C = h->h;
+    if ((C.R + 1) * C.N > ((int64_t)1 << 34)) { h->err = "count transform layers above 64 GB"; return COV_ELIMIT; }
+    RC(run_count_transform(h, source, layers ? CT_OUT_LAYERS : CT_OUT_SUM));
+    if (layers)
+        HCK(h, hipMemcpyAsync(layers, h->ct, (size_t)(C.R + 1) * (size_t)C.N * 4, hipMemcpyDeviceToHost, h->stream));
+    else if (map)
+        HCK(h, hipMemcpyAsync(map, h->ct, (size_t)C.N * 4, hipMemcpyDeviceToHost, h->stream));
+    HCK(h, hipStreamSynchronize(h->stream));
+    return COV_OK;
+}
+
+int cov_set_count_tile(cov_cell_t* h, int lds_bytes)
+{
+    if (!h || lds_bytes < 0 || lds_bytes > 160 * 1024) return COV_EPARAM;
+    std::vector<CtPass> ps;
+    if (lds_bytes && ct_plan(h->h.q, h->h.n, h->h.R, h->h.N, lds_bytes, ps) != COV_OK) return COV_EPARAM;
+    h->ct_lds = lds_bytes;
+    return COV_OK;
+}
+
+int cov_map_max(cov_cell_t* h, int32_t* max)
+{
+    if (!h || !max) return COV_EPARAM;
+    if (h->map_kind < 0) { h->err = "no map (cov_count_transform)"; return COV_ESTATE; }
+    HCK(h, hipMemsetAsync(h->d_small, 0, 8, h->stream));
+    hipLaunchKernelGGL(k_map_max, dim3(grid_for(h->h.N, WG * 8, 1, 4096)), dim3(WG), 0, h->stream, h->h, h->ct,
+                       h->d_small);
+    HCK(h, hipGetLastError());
+    int64_t r = 0;
+    HCK(h, hipMemcpyAsync(&r, h->d_small, 8, hipMemcpyDeviceToHost, h->stream));
+    HCK(h, hipStreamSynchronize(h->stream));
+    *max = (int32_t)r;
+    return COV_OK;
+}
+
+int cov_map_hist(cov_cell_t* h, int nbins, int32_t vmax, int64_t* hist)
+{
+    if (!h || !hist || nbins < 1 || nbins > 8192 || vmax < 1) return COV_EPARAM;
+    if (h->map_kind < 0) { h->err = "no map (cov_count_transform)"; return COV_ESTATE; }
+    unsigned long long* dh = nullptr;
+    HCK(h, hipMallocAsync((void**)&dh, (size_t)nbins * 8, h->stream));
+    HCK(h, hipMemsetAsync(dh, 0, (size_t)nbins * 8, h->stream));
+    hipLaunchKernelGGL(k_map_hist, dim3(grid_for(h->h.N, WG * 16, 1, 2048)), dim3(WG), (size_t)nbins * 4, h->stream,
+                       h->h, h->ct, nbins, vmax, dh);
+    HCK(h, hipGetLastError());
+    HCK(h, hipMemcpyAsync(hist, dh, (size_t)nbins * 8, hipMemcpyDeviceToHost, h->stream));
+    HCK(h, hipFreeAsync(dh, h->stream));
+    HCK(h, hipStreamSynchronize(h->stream));
+    return COV_OK;
+}
+
+int64_t cov_map_extract(cov_cell_t* h, int32_t thr, int64_t* idx, int32_t* val, int64_t cap)
+{
+    if (!h || cap < 0 || (cap > 0 && (!idx || !val))) return COV_EPARAM;
+    if (h->map_kind < 0) { h->err = "no map (cov_count_transform)"; return COV_ESTATE; }
+    const int64_t tot = ordered_gather(h, SelMapGe{h->ct, thr}, idx, cap);
+    if (tot < 0) return tot;
+    const int64_t m = std::min(cap, tot);
+    if (m > 0) RC(cov_map_read(h, idx, (int)m, val));
+    return tot;
+}
+
+int cov_map_read(cov_cell_t* h, const int64_t* idx, int k, int32_t* out)
+{
+    if (!h || k < 0 || (k > 0 && (!idx || !out))) return COV_EPARAM;
+    if (h->map_kind < 0) { h->err = "no map (cov_count_transform)"; return COV_ESTATE; }
+    if (k == 0) return COV_OK;
+    for (int i = 0; i < k; i++)
+        if (idx[i] < 0 || idx[i] >= h->h.N) { h->err = "word index out of range"; return COV_EPARAM; }
+    int64_t* di = nullptr;
+    int32_t* dv = nullptr;
+    HCK(h, hipMallocAsync((void**)&di, (size_t)k * 8, h->stream));
+    HCK(h, hipMallocAsync((void**)&dv, (size_t)k * 4, h->stream));
+    HCK(h, hipMemcpyAsync(di, idx, (size_t)k * 8, hipMemcpyHostToDevice, h->stream));
+    hipLaunchKernelGGL(k_map_read, dim3(grid_for(k, WG, 1, 1024)), dim3(WG), 0, h->stream, h->ct, di, k, dv);
+    HCK(h, hipGetLastError());
+    HCK(h, hipMemcpyAsync(out, dv, (size_t)k * 4, hipMemcpyDeviceToHost, h->stream));
+    HCK(h, hipFreeAsync(di, h->stream));
+    HCK(h, hipFreeAsync(dv, h->stream));
+    HCK(h, hipStreamSynchronize(h->stream));
+    return COV_OK;
+}
+
+int cov_ball_gather(cov_cell_t* h, const int64_t* centers, int k, int target, int32_t* out)
+{
+    if (h && !h->cnt_valid) { h->err = "coverage counts not built (cov_recount)"; return COV_ESTATE; }
+    if (!h || k < 0 || target < 0 || target > 65535 || (k > 0 && (!centers || !out))) return COV_EPARAM;
+    if (k == 0) return COV_OK;
+    for (int i = 0; i < k; i++)
+        if (centers[i] < 0 || centers[i] >= h->h.N) { h->err = "centre out of range"; return COV_EPARAM; }
+    int64_t* dc = nullptr;
+    int32_t* dv = nullptr;
+    HCK(h, hipMallocAsync((void**)&dc, (size_t)k * 8, h->stream));
+    HCK(h, hipMallocAsync((void**)&dv, (size_t)k * 4, h->stream));
+    HCK(h, hipMemcpyAsync(dc, centers, (size_t)k * 8, hipMemcpyHostToDevice, h->stream));
+    hipLaunchKernelGGL(k_ball_gather, dim3(std::min(k, 4096)), dim3(WG), 0, h->stream, h->h, dc, k, target, dv);
+    HCK(h, hipGetLastError());
+    HCK(h, hipMemcpyAsync(out, dv, (size_t)k * 4, hipMemcpyDeviceToHost, h->stream));
+    HCK(h, hipFreeAsync(dc, h->stream));
+    HCK(h, hipFreeAsync(dv, h->stream));
+    HCK(h, hipStreamSynchronize(h->stream));
+    return COV_OK;
+}
+
+int cov_load_words(cov_cell_t* h, const int64_t* words, int m)
+{
+    if (!h || m < 0 || (m > 0 && !words)) return COV_EPARAM;
+    const Cell& C = h->h;
+    for (int i = 0; i < m; i++)
+        if (words[i] < 0 || words[i] >= C.N) { h->err = "word index out of range"; return COV_EPARAM; }
+    h->th_valid = false;
+    h->dist_valid = false;
+    if (h->used) HCK(h, hipStreamWaitEvent(h->stream, h->used, 0));
+    HCK(h, hipMemsetAsync(C.cnt32, 0, (size_t)((C.N + 1) / 2) * 4, h->stream));
+    if (m > 0) {
+        int64_t* dc = nullptr;
+        HCK(h, hipMallocAsync((void**)&dc, (size_t)m * 8, h->stream));
+        HCK(h, hipMemcpyAsync(dc, words, (size_t)m * 8, hipMemcpyHostToDevice, h->stream));
+        hipLaunchKernelGGL(k_ball_multi, dim3(grid_for((int64_t)m * C.V, WG, 1, 4096)), dim3(WG), 0, h->stream,
+                           h->h, dc, m, 1);
+        HCK(h, hipGetLastError());
+        HCK(h, hipFreeAsync(dc, h->stream));
+    }
+    h->cnt_valid = true;
+    RC(do_derive(h));
+    RC(set_ctl<int32_t>(h, offsetof(Ctl, ul_valid), 0));
+    HCK(h, hipStreamSynchronize(h->stream));
+    return COV_OK;
+}
+
 int cov_search_params(cov_cell_t* h, int tenure, int64_t stall, int kick, int cand)
 {
     if (!h || tenure < 0 || stall < 0 || kick < 0 || cand < 1) return COV_EPARAM;
@@ -1327,13 +1544,13 @@ int cov_timing(cov_cell_t* h, int on)
     if (!h) return COV_EPARAM;
     RC(timing_collect(h));
     h->timing = on != 0;
-    for (int k = 0; k < 6; k++) { h->ms[k] = 0; h->launches[k] = 0; }
+    for (int k = 0; k < 7; k++) { h->ms[k] = 0; h->launches[k] = 0; }
     return COV_OK;
 }
 
 int cov_timing_read(cov_cell_t* h, double* ms, int64_t* launches, int nkinds)
 {
-    if (!h || nkinds < 0 || nkinds > 6) return COV_EPARAM;
+    if (!h || nkinds < 0 || nkinds > 7) return COV_EPARAM;
     RC(timing_collect(h));
     for (int k = 0; k < nkinds; k++) {
         if (ms) ms[k] = h->ms[k];

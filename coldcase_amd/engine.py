@@ -24,6 +24,7 @@ COV_DSEL_RANDOM = 1 << 60
 MODE_PLAIN, MODE_GRAPH = 0, 1
 GRAPH_STEPS = 40          # COV_GRAPH_STEPS: steps per captured graph
 LEAVE_AUTO, LEAVE_WALK, LEAVE_KEPT = 0, 1, 2
+CT_CODE, CT_UNCOVERED, CT_PRIVATE = 0, 1, 2
 TR_FIELDS = ['it', 'u', 'ncand', 'ncw', 'bi', 'bp', 'bv', 'oldv', 'dsel', 'dunc',
              'unc', 'best', 'kick', 'draws', 'since', 'nties']
 
@@ -36,7 +37,8 @@ EXPORTS = ['cov_version', 'cov_strerror', 'cov_mem_bytes', 'cov_create', 'cov_de
            'cov_search_run', 'cov_search_launch', 'cov_search_state', 'cov_query', 'cov_search_limits',
            'cov_search_ttt', 'cov_set_launch',
            'cov_get_launch', 'cov_set_leave_mode', 'cov_set_transform_tile', 'cov_timing', 'cov_timing_read', 'cov_search_phases',
-           'cov_search_bytes', 'cov_batch_create', 'cov_batch_destroy', 'cov_batch_last_error',
+           'cov_search_bytes', 'cov_count_transform', 'cov_set_count_tile', 'cov_map_max', 'cov_map_hist',
+           'cov_map_extract', 'cov_map_read', 'cov_ball_gather', 'cov_load_words', 'cov_batch_create', 'cov_batch_destroy', 'cov_batch_last_error',
            'cov_batch_size', 'cov_batch_launch', 'cov_batch_sync', 'cov_batch_query', 'cov_batch_run']
 
 
@@ -109,6 +111,14 @@ def lib():
         'cov_timing_read': (i32, [vp, vp, vp, i32]),
         'cov_search_phases': (i32, [vp, vp, i32]),
         'cov_search_bytes': (i32, [vp, vp, vp, vp, vp]),
+        'cov_count_transform': (i32, [vp, i32, vp, vp]),
+        'cov_set_count_tile': (i32, [vp, i32]),
+        'cov_map_max': (i32, [vp, vp]),
+        'cov_map_hist': (i32, [vp, i32, i32, vp]),
+        'cov_map_extract': (i64, [vp, i32, vp, vp, i64]),
+        'cov_map_read': (i32, [vp, vp, i32, vp]),
+        'cov_ball_gather': (i32, [vp, vp, i32, i32, vp]),
+        'cov_load_words': (i32, [vp, vp, i32]),
         'cov_batch_create': (i32, [ctypes.POINTER(vp), i32, ctypes.POINTER(vp)]),
         'cov_batch_destroy': (i32, [vp]),
         'cov_batch_last_error': (ctypes.c_char_p, [vp]),
@@ -344,11 +354,59 @@ class Cell:
 
     def timing_read(self):
         """(ms, launches) per kind: 0 evaluator, 1 commit, 2 transform pass,
-        3 recount, 4 selection, 5 post."""
-        ms = np.zeros(6, np.float64)
-        n = np.zeros(6, np.int64)
-        self._check(lib().cov_timing_read(self.h, _p(ms), _p(n), 6), 'cov_timing_read')
+        3 recount, 4 selection, 5 post, 6 count-transform pass."""
+        ms = np.zeros(7, np.float64)
+        n = np.zeros(7, np.int64)
+        self._check(lib().cov_timing_read(self.h, _p(ms), _p(n), 7), 'cov_timing_read')
         return ms, n
+
+    # ------------------------------------------------ count transform, maps
+    def count_transform(self, source, layers=False, want_map=False):
+        """N_d for d = 0..R of the source set (CT_CODE, CT_UNCOVERED,
+        CT_PRIVATE): the (R+1, q^n) layers if `layers`, else the map sum_d
+        N_d is kept on the device (and returned if `want_map`)."""
+        L = np.zeros((self.R + 1, self.N), np.int32) if layers else None
+        m = np.zeros(self.N, np.int32) if (want_map and not layers) else None
+        self._check(lib().cov_count_transform(self.h, int(source), _p(L), _p(m)), 'cov_count_transform')
+        return L if layers else m
+
+    def set_count_tile(self, lds_bytes):
+        self._check(lib().cov_set_count_tile(self.h, int(lds_bytes)), 'cov_set_count_tile')
+
+    def map_max(self):
+        v = np.zeros(1, np.int32)
+        self._check(lib().cov_map_max(self.h, _p(v)), 'cov_map_max')
+        return int(v[0])
+
+    def map_hist(self, nbins, vmax):
+        h = np.zeros(int(nbins), np.int64)
+        self._check(lib().cov_map_hist(self.h, int(nbins), int(vmax), _p(h)), 'cov_map_hist')
+        return h
+
+    def map_extract(self, thr, cap):
+        """(words with map value >= thr in ascending order, their values, total)."""
+        idx = np.zeros(max(int(cap), 1), np.int64)
+        val = np.zeros(max(int(cap), 1), np.int32)
+        tot = self._check(lib().cov_map_extract(self.h, int(thr), _p(idx), _p(val), int(cap)), 'cov_map_extract')
+        m = min(int(cap), tot)
+        return idx[:m], val[:m], tot
+
+    def map_read(self, words):
+        w = np.ascontiguousarray(words, dtype=np.int64).ravel()
+        out = np.zeros(max(len(w), 1), np.int32)
+        self._check(lib().cov_map_read(self.h, _p(w), len(w), _p(out)), 'cov_map_read')
+        return out[:len(w)]
+
+    def ball_gather(self, centers, target):
+        c = np.ascontiguousarray(centers, dtype=np.int64).ravel()
+        out = np.zeros(max(len(c), 1), np.int32)
+        self._check(lib().cov_ball_gather(self.h, _p(c), len(c), int(target), _p(out)), 'cov_ball_gather')
+        return out[:len(c)]
+
+    def load_words(self, words):
+        """Coverage counts of a word list of any length (the LNS engine's code)."""
+        w = np.ascontiguousarray(words, dtype=np.int64).ravel()
+        self._check(lib().cov_load_words(self.h, _p(w), len(w)), 'cov_load_words')
 
     def phases(self):
         """Microseconds in the narrow phases since search_init."""

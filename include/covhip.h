@@ -147,6 +147,42 @@ int     cov_distance_stats(cov_cell_t *h, int r, int64_t *within, int *radius);
  * words and verify_numpy's witnesses (cov/verify_cov.py:169-230). */
 int64_t cov_gather_far(cov_cell_t *h, int r, int64_t *out, int64_t cap);
 
+/* --------------------------------------------- count transform and maps */
+/* (a22) The distance-count transform: for every word x and d = 0..R,
+ * N_d(x) = #{u in S : d_H(x, u) = d}, by n axis steps
+ * A_d <- A_d + T_{d-1} - A_{d-1} on (R+1) int32 layers, several axes per
+ * pass in LDS tiles.  Replaces dct_transform (cov/gemm2/dct.cu:360-411) with
+ * its three sources: COV_CT_CODE the cell's code with multiplicities,
+ * COV_CT_UNCOVERED {x : cnt[x] == 0}, COV_CT_PRIVATE {x : cnt[x] == 1}.
+ * layers (host, (R+1) x q^n int32, layer-major) receives every N_d; else the
+ * map sum_d N_d stays on the device for the map calls below (the gain map
+ * for COV_CT_UNCOVERED, the loss map for COV_CT_PRIVATE) and is also copied
+ * to map (host, q^n int32) when that is not NULL. */
+#define COV_CT_CODE      0
+#define COV_CT_UNCOVERED 1
+#define COV_CT_PRIVATE   2
+int     cov_count_transform(cov_cell_t *h, int source, int32_t *layers, int32_t *map);
+/* LDS bytes per count-transform tile (0 = the default, 65536). */
+int     cov_set_count_tile(cov_cell_t *h, int lds_bytes);
+/* The map of the last sum transform: its maximum (dct_map_max, dct.cu:450);
+ * the histogram of its positive values in nbins bins, value v in bin
+ * floor((v-1) nbins / vmax) (dct_map_hist, :459); the first min(cap, total)
+ * words with a value >= thr, in ascending index order, and their values,
+ * returning the total (dct_map_extract, :469; the reference's order is that
+ * of its atomics); the values at given words (dct_map_read_at, :484). */
+int     cov_map_max(cov_cell_t *h, int32_t *max);
+int     cov_map_hist(cov_cell_t *h, int nbins, int32_t vmax, int64_t *hist);
+int64_t cov_map_extract(cov_cell_t *h, int32_t thr, int64_t *idx, int32_t *val, int64_t cap);
+int     cov_map_read(cov_cell_t *h, const int64_t *idx, int k, int32_t *out);
+/* Per centre word, #{x in its radius-R ball : cnt[x] == target}: the exact
+ * gain (target 0) or loss (target 1) of a codeword there.  Replaces
+ * dct_ball_gather (cov/gemm2/dct.cu:427). */
+int     cov_ball_gather(cov_cell_t *h, const int64_t *centers, int k, int target, int32_t *out);
+/* Coverage counts of a word list of any length (not the cell's code rows):
+ * zero, then +1 over the radius-R ball of each of the m words.  The LNS
+ * engine's recount (cov/gemm2/dctlib.py:108-113 with dct_set_code). */
+int     cov_load_words(cov_cell_t *h, const int64_t *words, int m);
+
 /* --------------------------------------------------------------- search */
 /* Seed the rng (splitmix64 -> xoshiro256++, cov/engine/covsearch2e.c:211-219)
  * and build the initial code: seed_code (mseed words, may be NULL/0), then
@@ -205,7 +241,8 @@ int     cov_set_transform_tile(cov_cell_t *h, int lds_bytes, int threads);
  * (hipExtLaunchKernelGGL), read back in batches with no host sync between
  * launches; search steps are timed in plain mode only.  Read totals in ms and
  * launch counts for kernel kinds 0 = evaluator, 1 = commit, 2 = distance
- * transform pass, 3 = recount, 4 = selection, 5 = post (nkinds <= 6). */
+ * transform pass, 3 = recount, 4 = selection, 5 = post, 6 = count transform
+ * pass (nkinds <= 7). */
 int     cov_timing(cov_cell_t *h, int on);
 int     cov_timing_read(cov_cell_t *h, double *ms, int64_t *launches, int nkinds);
 /* Diagnostics: microseconds spent since cov_search_init, from the device wall

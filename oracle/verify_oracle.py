@@ -10,6 +10,9 @@ never imported by the product.
   dilate_counts     covered count after r rounds of axis dilation
                     (cov/verify_independent.py:128-141)
   ball_volume       cov/verify_cov.py:101-102
+  count_layers      N_d(x) = #{u in S : d(x,u) = d}, d = 0..R, by comparing
+                    every word with every member of S (the brute force of
+                    cov/gemm2/test_dct.py:26-39, MSB-first indices)
 """
 import itertools
 from math import comb
@@ -103,4 +106,16 @@ def dilate_counts(code, n, q, R):
             nxt.reshape(q ** p, q, q ** (n - 1 - p))[:] |= v.any(axis=1, keepdims=True)
         cov = nxt
         out.append(int(cov.sum()))
+    return out
+
+
+def count_layers(S_idx, n, q, R):
+    """(R+1, q^n) int32: N_d(x) for the multiset S of word indices."""
+    N = q ** n
+    digs = digits_table(q, n)
+    out = np.zeros((R + 1, N), dtype=np.int32)
+    for u in np.asarray(S_idx, dtype=np.int64):
+        d = (digs != digs[u][None, :]).sum(axis=1)
+        for r in range(R + 1):
+            out[r] += (d == r)
     return out

@@ -172,3 +172,27 @@ def test_verify_cli_record_codes(fname, q, n, R, capsys):
     rc = verify_cov.main([os.path.join(DATA, fname), '-q', str(q), '-n', str(n), '-R', str(R - 1),
                           '--method', 'both', '--quiet'])
     assert rc == 1
+
+
+def test_verify_cli_output_as_the_reference_scripts_read_it(tmp_path, capsys):
+    """The CLI's per-method lines read with the reference scripts' own
+    patterns: the regex of cov/gemm/run_bench.py:56 over `--method both`, and
+    the awk of cov/engine/selftest.sh:61-62 over `--method pure`."""
+    from coldcase_amd import verify_cov
+    print('case verify cli patterns', flush=True)
+    rng = np.random.default_rng(4)
+    code = rng.integers(0, 5, size=(300, 7))
+    p = tmp_path / 'c.txt'
+    p.write_text(''.join(''.join(str(int(v)) for v in w) + '\n' for w in code))
+    d = mindist(np.unique(code.astype(np.uint8), axis=0), 7, 5)
+    want = int((d > 2).sum())
+    rc = verify_cov.main([str(p), '-q', '5', '-n', '7', '-R', '2', '--method', 'both'])
+    out = capsys.readouterr().out
+    vals = re.findall(r"^method [a-z]+ *: uncovered=(\d+)", out, re.M)
+    assert rc == 1 and len(vals) == 2 and {int(v) for v in vals} == {want}
+    rc = verify_cov.main([str(p), '-q', '5', '-n', '7', '-R', '2', '--method', 'pure', '--radius'])
+    out = capsys.readouterr().out
+    # awk -F'uncovered=' '/method pure/{print $2+0}'
+    got = [line.split('uncovered=')[1] for line in out.splitlines() if 'method pure' in line]
+    assert len(got) == 1 and int(re.match(r'\s*(\d+)', got[0]).group(1)) == want
+    assert 'radius  %d' % int(d.max()) in out
