@@ -12,9 +12,10 @@ The search itself runs on the device (coldcase_amd.engine.Cell): the same
 trajectory as the reference for the same arguments, step for step.
 Flags that only select an implementation in the reference (--preset, --st,
 --eval, --walk, --hoist, --pf, --huge, --threads) are accepted and ignored,
-since every one of them runs the identical search.  Flags that change the
-search (--wide, --early, --fix0, --ucache, --upick, --sa, --noise) are not
-part of this engine and are refused with exit status 2.
+since every one of them runs the identical search.  --wide (:1353-1376) is
+supported, as the same search.  Flags that change the search otherwise
+(--early, --fix0, --ucache, --upick, --sa, --noise) are not part of this
+engine and are refused with exit status 2.
 
 Stopping (:1305-1324): the device checks, before every step, the deadline
 (-t seconds of wall clock from process start, or --cpu seconds of this
@@ -37,7 +38,7 @@ from .verify_cov import DIGITS
 
 IGNORED_VALUE = {'--preset', '--st', '--eval', '--walk', '--threads', '--t0', '--cool', '--tmin'}
 IGNORED_FLAG = {'--hoist', '--pf', '--huge', '--quiet'}
-REFUSED = {'--wide', '--early', '--fix0', '--ucache', '--upick', '--sa'}
+REFUSED = {'--early', '--fix0', '--ucache', '--upick', '--sa'}
 
 
 def read_code(path, n):
@@ -96,7 +97,7 @@ def write_code(path, code, q):
 
 def parse_args(argv):
     o = dict(q=0, n=0, R=-1, M=0, t=60.0, s=12345, inp=None, out=None, cand=100000, tabu=0, kick=2,
-             iters=-1, stall=5000, greedy=True, quiet=False, noise=0, target=-1, cpu=0.0)
+             iters=-1, stall=5000, greedy=True, quiet=False, noise=0, target=-1, cpu=0.0, wide=False)
     i = 0
     while i < len(argv):
         a = argv[i]
@@ -118,6 +119,8 @@ def parse_args(argv):
             o['out'] = val; i += 2
         elif a in ('--cand', '--tabu', '--kick', '--stall', '--iters', '--noise') and has:
             o[a[2:]] = int(val); i += 2
+        elif a == '--wide':
+            o['wide'] = True; i += 1
         elif a == '--random-init':
             o['greedy'] = False; i += 1
         elif a == '--quiet':
@@ -180,9 +183,11 @@ def main(argv=None):
                  0.0 if hit else -1.0, 0 if hit else -1))
         return 0
 
-    from .engine import MODE_GRAPH, Cell
+    from .engine import GRAPH_STEPS, MODE_GRAPH, Cell
     cell = Cell(q, n, R, M)
     cell.search_params(tenure=o['tabu'], stall=o['stall'], kick=o['kick'], cand=o['cand'])
+    if o['wide']:
+        cell.search_wide(True)
     cell.search_init(o['s'], seed_code=seed_code, greedy=o['greedy'])
     best, bu = cell.best()
     last_pub = -1e9
@@ -209,6 +214,7 @@ def main(argv=None):
         ttt_it, ttt_wall, ttt_cpu = 0, 0.0, 0.0
     last_report = 0.0
     chunk = 64
+    step_s = None               # seconds per step, from the last chunk
     st = cell.state()
     while not st['done']:
         left = budget_left()
@@ -217,6 +223,10 @@ def main(argv=None):
         if o['iters'] >= 0 and st['it'] >= o['iters']:
             break
         k = chunk if o['iters'] < 0 else min(chunk, o['iters'] - st['it'])
+        if step_s:
+            # no more steps than the budget can hold, plus one graph: the
+            # device stops at the deadline, and launches past it cost time too
+            k = min(k, int(left / step_s) + GRAPH_STEPS)
         prev_best, prev_it = st['best_uncovered'], st['it']
         w0, p0 = time.monotonic(), time.process_time()
         if cpulimit > 0.0:
@@ -231,6 +241,8 @@ def main(argv=None):
             cell.sync()
         st = cell.check_fault()
         w1, p1 = time.monotonic(), time.process_time()
+        if st['it'] - prev_it >= GRAPH_STEPS:
+            step_s = (w1 - w0) / (st['it'] - prev_it)
         if target >= 0 and ttt_it < 0:
             it_hit, age = cell.ttt()
             if it_hit >= 0:

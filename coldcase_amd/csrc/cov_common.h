@@ -49,6 +49,8 @@ constexpr int STALE_MAX = 8;         // codewords recounted per step at most (th
 constexpr int ARR_G = 32;
 constexpr int ARR_STRIDE = 64;       // int32 slots between sub-counters
 constexpr uint64_t NIB1 = 0x1111111111111111ULL;
+constexpr int WIDE_W = MAXN * MAXQ;  // --wide entering counts per slot: [position][value]
+constexpr int64_t UL_REF_HALF = (int64_t)1 << 19;   // the reference's list rebuild threshold (ULCAP / 2)
 
 // Device-resident search state.  Scalars written by the narrow phases (one
 // workgroup) and read by the wide kernels of the same step.
@@ -97,6 +99,10 @@ struct alignas(16) Ctl {
     // walked in full (2 B per counter read) and the commit's read-modify-
     // write of both spheres (8 B per pattern); and the steps counted
     int64_t move_bytes, move_steps;
+    // --wide: every move of a candidate codeword is a candidate while the
+    // uncovered list is valid and short (wnow, decided per step); per =
+    // candidates per candidate codeword (n (q-1) when wnow, else dtar)
+    int32_t wide, wnow, per, pad_w;
     int64_t* trace;         // per-run trace records (tcap of them), or null
     int64_t tcap;
 };
@@ -181,6 +187,7 @@ struct Cell {
     GPTR(int64_t) cbs;           // staged selection reads: word index of the candidate's codeword
     GPTR(uint64_t) cps;          // staged selection reads: packed digits of the candidate's codeword
     GPTR(uint8_t) dist;          // distance transform output (N bytes), lazily
+    GPTR(int32_t) accw;          // --wide: entering counts per slot, position and value (WIDE_W each), lazily
     GPTR(Ctl) ctl;
 };
 

@@ -336,7 +336,13 @@ __device__ __forceinline__ void wg_begin(const Cell& C, Ctl* ctl, NarrowSh& sh)
         const int valid = ctl->ul_valid;
         const int64_t ul_n = ctl->ul_n;
         __syncthreads();
-        if (!valid && unc <= list_keep(C) / 2) {
+        if (ctl->wide) {
+            // the wide neighbourhood's availability depends on the list's
+            // own state, so it follows the reference's rules (:1326-1329):
+            // rebuilt below 2^19 uncovered words, compacted, never dropped
+            if (!valid && unc <= UL_REF_HALF) wg_list_rebuild(C, ctl, sh);
+            else if (valid && ul_n > 2 * unc + 64) wg_list_compact(C, ctl, sh);
+        } else if (!valid && unc <= list_keep(C) / 2) {
             wg_list_rebuild(C, ctl, sh);
         } else if (valid && unc > list_keep(C)) {
             // too long to be read: drop it rather than keep it up to date
@@ -515,12 +521,14 @@ __device__ __forceinline__ void wg_begin(const Cell& C, Ctl* ctl, NarrowSh& sh)
             emit(i, C.cpk[i]);
         }
     }
-    const int64_t ncand = nq * dtar;
-    const int64_t take = ncand < ctl->cand ? ncand : ctl->cand;
-
-    // list decision and unit counts
+    // list decision and unit counts; with --wide every move of the candidate
+    // codewords is a candidate while the list is used (:1353-1376)
     const int use_list = ctl->ul_valid &&
                          (ctl->ul_n * C.n < (int64_t)(C.R + 1) * C.E * C.P);
+    const int wnow = ctl->wide && use_list;
+    const int per = wnow ? C.n * C.q1 : dtar;
+    const int64_t ncand = nq * per;
+    const int64_t take = ncand < ctl->cand ? ncand : ctl->cand;
     const int64_t nchunk = (ctl->ul_n + LIST_CHUNK - 1) / LIST_CHUNK;
     // the shuffle's draws (:1384-1389) are reserved now; the evaluator's
     // helper wave computes the permutation while the units run
@@ -535,6 +543,8 @@ __device__ __forceinline__ void wg_begin(const Cell& C, Ctl* ctl, NarrowSh& sh)
         ctl->ncand = (int32_t)ncand;
         ctl->take = (int32_t)take;
         ctl->use_list = use_list;
+        ctl->wnow = wnow;
+        ctl->per = per;
         ctl->nchunk = use_list ? nchunk : 0;
         // the leaving side: the moved codewords' spheres when the counts are
         // kept (th), else those of every codeword at distance R+1 from u
@@ -637,6 +647,42 @@ __device__ __forceinline__ void unit_walk(const Cell& C, int slot, int p, int up
 }
 
 // entering side from the uncovered list: entries e0..e1 against codeword slot
+// --wide form: for every uncovered x at distance R+1 from the slot's
+// codeword c, one count at (j, x_j) for each position j where x and c
+// differ: the entering count of every move (j, v) of c (:720-734).  The
+// counters (position j, value v) are spread over the lanes, lane l owning
+// j * MAXQ + v = l, l + 64, l + 128, l + 192, and each takes the ballot
+// counts of its pair.
+__device__ __forceinline__ void unit_list_wide(const Cell& C, int slot, int64_t e0, int64_t e1, int64_t ul_n)
+{
+    const int lane = threadIdx.x & 63;
+    const uint64_t cpk = C.cpk[C.slot_cw[slot]];
+    int32_t own[WIDE_W / 64] = {0, 0, 0, 0};
+    for (int64_t x0 = e0; x0 < e1; x0 += 64) {
+        const int64_t x = x0 + lane;
+        uint64_t ne = 0, xpk = 0;
+        if (x < e1 && x < ul_n) {
+            const int64_t xi = C.ul_idx[x];
+            xpk = C.ul_pk[x];
+            if (bit_of(C.zb, xi) && nib_dist(xpk, cpk) == C.R + 1) ne = nib_ne(xpk, cpk);
+        }
+        for (int j = 0; j < C.n; j++) {
+            const bool bj = (ne >> (4 * j)) & 1;
+            if (!__any(bj)) continue;
+            const int xj = nib_get(xpk, j);
+            for (int v = 0; v < C.q; v++) {
+                const int c = __popcll(__ballot(bj && xj == v));
+                const int id = j * MAXQ + v;
+                if (c && lane == (id & 63)) own[id >> 6] += c;
+            }
+        }
+    }
+    int32_t* a = C.accw + (int64_t)slot * WIDE_W;
+#pragma unroll
+    for (int k = 0; k < WIDE_W / 64; k++)
+        if (own[k]) atomicAdd(&a[lane + 64 * k], own[k]);
+}
+
 __device__ __forceinline__ void unit_list(const Cell& C, int slot, int64_t e0, int64_t e1,
                                           uint64_t upk, int64_t ul_n)
 {
@@ -674,7 +720,7 @@ __device__ __forceinline__ void unit_list(const Cell& C, int slot, int64_t e0, i
 __device__ __forceinline__ void eval_units(const Cell& C, const Ctl* ctl, int64_t wid, int64_t nw, WalkTab& tb)
 {
     const int64_t nl = ctl->n_leave, ne = ctl->n_enter;
-    const int use_list = ctl->use_list, dtar = ctl->dtar;
+    const int use_list = ctl->use_list, dtar = ctl->dtar, wnow = ctl->wnow;
     const int64_t nchunk = ctl->nchunk, ul_n = ctl->ul_n;
     const uint64_t upk = ctl->upk;
     const int64_t per_slot_walk = (int64_t)dtar * C.E;
@@ -690,7 +736,8 @@ __device__ __forceinline__ void eval_units(const Cell& C, const Ctl* ctl, int64_
             const int64_t ch = h - (int64_t)slot * nchunk;
             const int64_t e0 = ch * LIST_CHUNK;
             const int64_t e1 = (e0 + LIST_CHUNK < ul_n) ? e0 + LIST_CHUNK : ul_n;
-            unit_list(C, slot, e0, e1, upk, ul_n);
+            if (wnow) unit_list_wide(C, slot, e0, e1, ul_n);
+            else unit_list(C, slot, e0, e1, upk, ul_n);
         } else {
             const int64_t h = g - nl;
             const int slot = (int)(h / per_slot_walk);
@@ -885,6 +932,36 @@ __device__ __forceinline__ void wg_shuffle(const Cell& C, const Ctl* ctl, ShufSh
     }
 }
 
+// Candidate k of the unshuffled order: its slot (candidate codeword),
+// position and new value.  Standard: k = slot * dtar + t, the t-th position
+// where the codeword differs from u, new value u_p.  --wide: k = slot * n(q-1)
+// + p(q-1) + t, the t-th value other than c_p (:1359-1376 order).
+__device__ __forceinline__ void cand_move(const Cell& C, const Ctl* ctl, uint32_t k, int* pslot, int* pp, int* pv)
+{
+    const int per = ctl->per;
+    const int slot = (int)(k / (uint32_t)per);
+    const int r = (int)(k - (uint32_t)slot * (uint32_t)per);
+    int p, v;
+    if (ctl->wnow) {
+        p = r / C.q1;
+        const int t = r - p * C.q1;
+        const int cp = nib_get(C.cpk[C.slot_cw[slot]], p);
+        v = t + (t >= cp);
+    } else {
+        p = C.slot_pos[(int64_t)slot * MAXN + r];
+        v = nib_get(ctl->upk, p);
+    }
+    *pslot = slot;
+    *pp = p;
+    *pv = v;
+}
+
+// the entering count of move (p, v) of the slot's codeword
+__device__ __forceinline__ int32_t cand_enter(const Cell& C, const Ctl* ctl, int slot, int p, int v)
+{
+    return ctl->wnow ? C.accw[(int64_t)slot * WIDE_W + p * MAXQ + v] : C.acc[(int64_t)slot * ACCW + 1 + MAXN + p];
+}
+
 // The one-wave selection's reads that do not depend on the evaluator's
 // counts, made by the shuffle's workgroup once the order is known: for each
 // of the first take <= SEL_WAVE candidates in shuffled order, its codeword,
@@ -898,12 +975,9 @@ __device__ __forceinline__ void wg_stage(const Cell& C, const Ctl* ctl)
     const int s = threadIdx.x;
     if (s < take) {
         const uint32_t k = C.shuf[s];
-        const int dtar = ctl->dtar;
-        const int slot = (int)(k / (uint32_t)dtar);
-        const int t = (int)(k - (uint32_t)slot * (uint32_t)dtar);
-        const int p = C.slot_pos[(int64_t)slot * MAXN + t];
+        int slot, p, v;
+        cand_move(C, ctl, k, &slot, &p, &v);
         const int i = C.slot_cw[slot];
-        const int v = nib_get(ctl->upk, p);
         const int32_t stamp = C.tabu[((int64_t)i * C.n + p) * C.q + v];
         const int cold = C.code[(int64_t)i * C.n + p];
         reinterpret_cast<int4*>(C.cst)[s] = make_int4(i, slot, p | (v << 8) | (cold << 16), stamp);
@@ -959,17 +1033,15 @@ __device__ __forceinline__ int64_t wave_excl(int64_t incl, int64_t first)
 __device__ __forceinline__ int64_t cand_delta(const Cell& C, const Ctl* ctl, uint32_t k,
                                               int* pi, int* pp, int* pv)
 {
-    const int dtar = ctl->dtar;
-    const int slot = (int)(k / (uint32_t)dtar);
-    const int t = (int)(k - (uint32_t)slot * (uint32_t)dtar);
-    const int p = C.slot_pos[(int64_t)slot * MAXN + t];
+    int slot, p, v;
+    cand_move(C, ctl, k, &slot, &p, &v);
     const int32_t* a = C.acc + (int64_t)slot * ACCW;
     const int i = C.slot_cw[slot];
     *pi = i;
     *pp = p;
-    *pv = nib_get(ctl->upk, p);
+    *pv = v;
     const int32_t* lv = ctl->th_on ? C.th + (int64_t)i * THW : a;
-    return (int64_t)(lv[0] - lv[1 + p]) - (int64_t)a[1 + MAXN + p];
+    return (int64_t)(lv[0] - lv[1 + p]) - (int64_t)cand_enter(C, ctl, slot, p, v);
 }
 
 // The selection below for take <= SEL_WAVE candidates, on wave 0 alone: lane l
@@ -1024,7 +1096,7 @@ __device__ __forceinline__ void wave_select(const Cell& C, Ctl* ctl, SelSh& sh)
                 a0[k] = a[0];
                 ap[k] = a[1 + cp[k]];
             }
-            an[k] = a[1 + MAXN + cp[k]];
+            an[k] = cand_enter(C, ctl, slot[k], cp[k], cv[k]);
         }
 #pragma unroll
         for (int k = 0; k < SEL_PER; k++) {
@@ -1215,6 +1287,8 @@ __device__ __forceinline__ void wg_select(const Cell& C, Ctl* ctl, SelSh& sh)
     __syncthreads();
     // the accumulators start the next step at zero
     for (int64_t x = tid; x < (int64_t)ctl->nq * ACCW; x += WG) C.acc[x] = 0;
+    if (ctl->wnow)
+        for (int64_t x = tid; x < (int64_t)ctl->nq * WIDE_W; x += WG) C.accw[x] = 0;
     // distinct codewords among the first `take` (differs from nq only when
     // --cand cuts the sample)
     if (take < ctl->ncand) {
@@ -1223,10 +1297,10 @@ __device__ __forceinline__ void wg_select(const Cell& C, Ctl* ctl, SelSh& sh)
             const int64_t s = s0 + tid;
             int first = 0;
             if (s < take) {
-                const uint32_t slot = C.shuf[s] / (uint32_t)ctl->dtar;
+                const uint32_t slot = C.shuf[s] / (uint32_t)ctl->per;
                 first = 1;
                 for (int64_t s2 = 0; s2 < s; s2++)
-                    if (C.shuf[s2] / (uint32_t)ctl->dtar == slot) { first = 0; break; }
+                    if (C.shuf[s2] / (uint32_t)ctl->per == slot) { first = 0; break; }
             }
             cntd += first;
         }

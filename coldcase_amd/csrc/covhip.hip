@@ -84,6 +84,7 @@ struct cov_batch {
     int eval_total = 0, commit_total = 0;
     hipGraphExec_t gexec = nullptr;
     std::string err;
+    std::vector<const void*> accw;        // each cell's wide counts when the batch was made
 };
 
 namespace {
@@ -1288,6 +1289,17 @@ int cov_search_params(cov_cell_t* h, int tenure, int64_t stall, int kick, int ca
     return COV_OK;
 }
 
+int cov_search_wide(cov_cell_t* h, int on)
+{
+    if (!h || (on != 0 && on != 1)) return COV_EPARAM;
+    if (on && !h->h.accw) {
+        RC(dalloc(h, &h->h.accw, (int64_t)h->h.M * WIDE_W * 4));
+        RC(push_cell(h));
+    }
+    RC(set_ctl<int32_t>(h, offsetof(Ctl, wide), on));
+    return COV_OK;
+}
+
 int cov_search_init(cov_cell_t* h, uint64_t seed, const uint8_t* seed_code, int mseed, int greedy)
 {
     if (h) h->th_valid = false;
@@ -1302,6 +1314,7 @@ int cov_search_init(cov_cell_t* h, uint64_t seed, const uint8_t* seed_code, int 
     RC(read_ctl(h, &old));
     Ctl c{};
     c.tenure = old.tenure; c.stall = old.stall; c.kick = old.kick; c.cand = old.cand; c.budget = old.budget;
+    c.wide = old.wide;
     c.target = -1;
     c.ttt_it = -1;
     splitmix_seed(seed, c.rst);
@@ -1652,6 +1665,7 @@ int cov_batch_create(cov_cell_t* const* cells, int k, cov_batch_t** out)
     std::vector<int32_t> eo(k + 1, 0), co(k + 1, 0);
     for (int i = 0; i < k; i++) {
         tab[i] = b->cells[i]->h;
+        b->accw.push_back((const void*)b->cells[i]->h.accw);
         eo[i + 1] = eo[i] + b->cells[i]->eval_groups;
         co[i + 1] = co[i] + b->cells[i]->commit_groups;
     }
@@ -1706,6 +1720,11 @@ int cov_batch_launch(cov_batch_t* b, int64_t nsteps, int mode)
     if (!b || nsteps < 0 || (mode != COV_MODE_PLAIN && mode != COV_MODE_GRAPH)) return COV_EPARAM;
     if (nsteps == 0) return COV_OK;
     const int k = (int)b->cells.size();
+    for (int i = 0; i < k; i++)
+        if ((const void*)b->cells[i]->h.accw != b->accw[i]) {
+            b->err = "cell " + std::to_string(i) + " switched to --wide after the batch was made";
+            return COV_ESTATE;
+        }
     for (cov_cell* c : b->cells) {
         c->dist_valid = false;
         const int rc = th_build(c, b->stream);

@@ -34,7 +34,7 @@ EXPORTS = ['cov_version', 'cov_strerror', 'cov_mem_bytes', 'cov_create', 'cov_de
            'cov_recount', 'cov_ball_update', 'cov_count_uncovered', 'cov_gather_uncovered',
            'cov_first_uncovered_from', 'cov_gather_uncovered_noalloc', 'cov_read_cnt', 'cov_eval_moves', 'cov_commit',
            'cov_distance_transform', 'cov_distance_stats', 'cov_gather_far', 'cov_search_init', 'cov_search_params',
-           'cov_search_run', 'cov_search_launch', 'cov_search_state', 'cov_query', 'cov_search_limits',
+           'cov_search_wide', 'cov_search_run', 'cov_search_launch', 'cov_search_state', 'cov_query', 'cov_search_limits',
            'cov_search_ttt', 'cov_set_launch',
            'cov_get_launch', 'cov_set_leave_mode', 'cov_set_transform_tile', 'cov_timing', 'cov_timing_read', 'cov_search_phases',
            'cov_search_bytes', 'cov_count_transform', 'cov_set_count_tile', 'cov_map_max', 'cov_map_hist',
@@ -97,6 +97,7 @@ def lib():
         'cov_distance_stats': (i32, [vp, i32, vp, vp]),
         'cov_search_init': (i32, [vp, u64, vp, i32, i32]),
         'cov_search_params': (i32, [vp, i32, i64, i32, i32]),
+        'cov_search_wide': (i32, [vp, i32]),
         'cov_search_run': (i64, [vp, i64, i32, vp]),
         'cov_search_launch': (i32, [vp, i64, i32]),
         'cov_search_state': (i32, [vp, ctypes.POINTER(State)]),
@@ -293,6 +294,11 @@ class Cell:
     def search_params(self, tenure=0, stall=5000, kick=2, cand=100000):
         self._check(lib().cov_search_params(self.h, int(tenure), int(stall), int(kick), int(cand)),
                     'cov_search_params')
+
+    def search_wide(self, on=True):
+        """--wide: every move of a candidate codeword is a candidate while the
+        uncovered list is valid and short (set before search_init)."""
+        self._check(lib().cov_search_wide(self.h, int(bool(on))), 'cov_search_wide')
 
     def run(self, nsteps, mode=MODE_PLAIN, trace=False):
         tr = np.zeros((max(int(nsteps), 1), COV_TR_W), np.int64) if trace else None

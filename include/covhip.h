@@ -192,6 +192,14 @@ int     cov_search_init(cov_cell_t *h, uint64_t seed, const uint8_t *seed_code,
                         int mseed, int greedy);
 /* Search knobs: --tabu, --stall, --kick, --cand (:1030-1066). */
 int     cov_search_params(cov_cell_t *h, int tenure, int64_t stall, int kick, int cand);
+/* --wide (:1068, :1353-1376): while the uncovered-word list is valid and
+ * short, every move (p, v != c_p) of a candidate codeword is a candidate,
+ * not only the moves towards u; the uncovered list then follows the
+ * reference's own rules (rebuilt below 2^19 uncovered words, compacted when
+ * it holds twice the uncovered count plus 64, never dropped), since its
+ * state decides when the wide neighbourhood applies.  Set before
+ * cov_search_init and before the cell joins a batch (allocates M KB). */
+int     cov_search_wide(cov_cell_t *h, int on);
 /* Run up to nsteps more tabu steps (:1304-1580) on the device.  trace, if not
  * NULL, receives COV_TR_W int64 per step actually run.  Returns the number of
  * steps run (fewer when the code covers). */

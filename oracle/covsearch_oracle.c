@@ -54,6 +54,18 @@ struct cso {
     int64_t target;      /* stop once best_unc <= target (< 0: none)        */
     int64_t ttt_it;      /* iteration that first reached the target, or -1  */
 
+    /* --wide (:1353-1376): every move of a candidate codeword is a
+     * candidate while the reference's uncovered-word list is valid and
+     * short.  Whether it is depends on that list's own bookkeeping (valid
+     * flag, length counting stale entries), which rl_* restate
+     * (:286-338, :1326-1329); the entries themselves are not needed. */
+    int wide;
+    int rl_valid;
+    int64_t rl_n;
+    uint64_t *rl_bit;    /* N bits: the word is in the list          */
+    int64_t *rl_idx;     /* RL_CAP entries                           */
+    int64_t *dvecw;      /* M * n * q deltas of every move           */
+
     /* position subsets: subs[r] lists all r-subsets of 0..n-1 (lexicographic),
      * nsub[r] of them; psub[p] lists the R-subsets avoiding p. */
     uint8_t *subs[MAXN + 1];
@@ -183,6 +195,41 @@ static inline void set_del(cso_t *s, int64_t w)
     s->upos[w] = -1;
 }
 
+/* the reference's list bookkeeping: insert on cnt -> 0 (:294-302),
+ * compaction (:305-320) and rebuild in ascending order (:323-338) */
+#define RL_CAP ((int64_t)1 << 20)
+static inline void rl_insert(cso_t *s, int64_t w)
+{
+    if (!s->wide || !s->rl_valid || ((s->rl_bit[w >> 6] >> (w & 63)) & 1)) return;
+    if (s->rl_n >= RL_CAP) { s->rl_valid = 0; return; }
+    s->rl_bit[w >> 6] |= 1ULL << (w & 63);
+    s->rl_idx[s->rl_n++] = w;
+}
+
+static void rl_compact(cso_t *s)
+{
+    int64_t m = 0;
+    for (int64_t e = 0; e < s->rl_n; e++) {
+        const int64_t w = s->rl_idx[e];
+        if (s->cnt[w] == 0) s->rl_idx[m++] = w;
+        else s->rl_bit[w >> 6] &= ~(1ULL << (w & 63));
+    }
+    s->rl_n = m;
+}
+
+static void rl_rebuild(cso_t *s)
+{
+    memset(s->rl_bit, 0, (size_t)((s->N + 63) / 64) * 8);
+    s->rl_n = 0;
+    s->rl_valid = 1;
+    for (int64_t w = 0; w < s->N; w++) {
+        if (s->cnt[w]) continue;
+        if (s->rl_n >= RL_CAP) { s->rl_valid = 0; return; }
+        s->rl_bit[w >> 6] |= 1ULL << (w & 63);
+        s->rl_idx[s->rl_n++] = w;
+    }
+}
+
 /* cnt transitions: every +-1 on cnt goes through these two */
 static inline int64_t cnt_inc(cso_t *s, int64_t w)
 {
@@ -192,7 +239,7 @@ static inline int64_t cnt_inc(cso_t *s, int64_t w)
 
 static inline int64_t cnt_dec(cso_t *s, int64_t w)
 {
-    if (--s->cnt[w] == 0) { set_add(s, w); return 1; }
+    if (--s->cnt[w] == 0) { set_add(s, w); rl_insert(s, w); return 1; }
     return 0;
 }
 
@@ -366,6 +413,7 @@ void cso_destroy(cso_t *s)
     free(s->upos); free(s->uidx); free(s->udig);
     free(s->cand_i); free(s->cand_p); free(s->cand_v); free(s->cand_d);
     free(s->dist); free(s->cw_list); free(s->cw_mark); free(s->dvec);
+    free(s->rl_bit); free(s->rl_idx); free(s->dvecw);
     free(s);
 }
 
@@ -552,6 +600,54 @@ void cso_eval_codeword(cso_t *s, int i, const uint8_t *u, int64_t *dvec, int use
         if (c[p] != u[p]) dvec[p] = (t1 - hit[p]) - neg[p];
 }
 
+/* Every move (p, v != c_p) of codeword i from one evaluation (:696-760):
+ * dvec[p*q + v] = (T - H_p) - neg[p][v], the entering counts neg[j][x_j]
+ * over the uncovered words x at distance R+1 (j running over the
+ * positions where x and c differ), or walked per move without the set. */
+void cso_eval_codeword_wide(cso_t *s, int i, int64_t *dvec)
+{
+    const int n = s->n, R = s->R, q = s->q;
+    const uint8_t *c = s->code + (size_t)i * n;
+    const int64_t base = s->cidx[i];
+    int64_t offs[MAXN * 64];
+    make_offs(s, c, offs);
+    int64_t t1 = 0, hit[MAXN] = {0};
+    for (int k = 0; k < s->nsub[R]; k++) {
+        const uint8_t *ps = s->subs[R] + (size_t)k * (R ? R : 1);
+        int64_t c1 = 0;
+        WALK(s, offs, ps, R, base, c1 += (s->cnt[w_] == 1));
+        t1 += c1;
+        for (int t = 0; t < R; t++) hit[ps[t]] += c1;
+    }
+    int64_t neg[MAXN * 64];
+    memset(neg, 0, sizeof(int64_t) * (size_t)n * q);
+    if (s->use_set) {
+        for (int64_t e = 0; e < s->nu; e++) {
+            const uint8_t *xd = s->udig + (size_t)e * n;
+            int d = 0;
+            for (int j = 0; j < n && d <= R + 1; j++) d += (xd[j] != c[j]);
+            if (d != R + 1) continue;
+            for (int j = 0; j < n; j++)
+                if (xd[j] != c[j]) neg[j * q + xd[j]]++;
+        }
+    } else {
+        for (int p = 0; p < n; p++)
+            for (int v = 0; v < q; v++) {
+                if (v == c[p]) continue;
+                const int64_t dv = (int64_t)(v - c[p]) * s->pw[p];
+                int64_t nn = 0;
+                for (int k = 0; k < s->npsub; k++) {
+                    const uint8_t *ps = s->psub + ((size_t)p * s->npsub + k) * (R ? R : 1);
+                    WALK(s, offs, ps, R, base + dv, nn += (s->cnt[w_] == 0));
+                }
+                neg[p * q + v] = nn;
+            }
+    }
+    for (int p = 0; p < n; p++)
+        for (int v = 0; v < q; v++)
+            dvec[p * q + v] = v == c[p] ? CSO_DSEL_RANDOM : (t1 - hit[p]) - neg[p * q + v];
+}
+
 int64_t cso_commit(cso_t *s, int i, int p, int v)
 {
     /* --cnt[a], ++cnt[b] with b = a + dv over the S patterns; a and b range
@@ -594,6 +690,10 @@ int64_t cso_run(cso_t *s, int64_t iters, int tabu_len, int64_t stall,
         s->it++;
         done++;
         const int64_t it = s->it;
+        if (s->wide) {
+            if (!s->rl_valid && s->uncovered <= RL_CAP / 2) rl_rebuild(s);
+            else if (s->rl_valid && s->rl_n > 2 * s->uncovered + 64) rl_compact(s);
+        }
 
         int64_t uw = cso_pick(s);
         if (uw < 0) break;
@@ -609,11 +709,24 @@ int64_t cso_run(cso_t *s, int64_t iters, int tabu_len, int64_t stall,
             if (d < dmin) dmin = d;
         }
         const int dtar = (dmin <= R + 1) ? R + 1 : dmin;
+        /* --wide takes every move of the candidate codewords while the
+         * reference's list is valid and short (:1353-1357) */
+        const int wide_now = s->wide && s->rl_valid && s->rl_n * n < (int64_t)(R + 1) * TOTPAT;
         int ncand = 0;
         for (int i = 0; i < M; i++) {
             if (s->dist[i] != dtar) continue;
             const uint8_t *c = s->code + (size_t)i * n;
             for (int p = 0; p < n; p++) {
+                if (wide_now) {
+                    for (int v = 0; v < q; v++) {
+                        if (v == c[p]) continue;
+                        s->cand_i[ncand] = i;
+                        s->cand_p[ncand] = p;
+                        s->cand_v[ncand] = v;
+                        ncand++;
+                    }
+                    continue;
+                }
                 if (c[p] == u[p]) continue;
                 s->cand_i[ncand] = i;
                 s->cand_p[ncand] = p;
@@ -646,13 +759,23 @@ int64_t cso_run(cso_t *s, int64_t iters, int tabu_len, int64_t stall,
             }
         }
         const int use_list = s->use_set && (s->nu * n < (int64_t)(R + 1) * TOTPAT);
+        if (wide_now) {
 #pragma omp parallel for schedule(dynamic, 1) num_threads(s->threads) if (ncw > 1 && s->threads > 1)
-        for (int z = 0; z < ncw; z++) {
-            int i = s->cw_list[z];
-            cso_eval_codeword(s, i, u, s->dvec + (size_t)i * n, use_list);
+            for (int z = 0; z < ncw; z++) {
+                int i = s->cw_list[z];
+                cso_eval_codeword_wide(s, i, s->dvecw + (size_t)i * n * q);
+            }
+            for (int t = 0; t < take; t++)
+                s->cand_d[t] = s->dvecw[((size_t)s->cand_i[t] * n + s->cand_p[t]) * q + s->cand_v[t]];
+        } else {
+#pragma omp parallel for schedule(dynamic, 1) num_threads(s->threads) if (ncw > 1 && s->threads > 1)
+            for (int z = 0; z < ncw; z++) {
+                int i = s->cw_list[z];
+                cso_eval_codeword(s, i, u, s->dvec + (size_t)i * n, use_list);
+            }
+            for (int t = 0; t < take; t++)
+                s->cand_d[t] = s->dvec[(size_t)s->cand_i[t] * n + s->cand_p[t]];
         }
-        for (int t = 0; t < take; t++)
-            s->cand_d[t] = s->dvec[(size_t)s->cand_i[t] * n + s->cand_p[t]];
 
         /* selection: tabu with aspiration, reservoir tie-break (:1503-1518) */
         int64_t bestd = CSO_DSEL_RANDOM;
@@ -748,6 +871,29 @@ int64_t cso_run(cso_t *s, int64_t iters, int tabu_len, int64_t stall,
 
 /* -------------------------------------------------------- accessors */
 void cso_set_kick_order(cso_t *s, int order) { s->kick_order = order; }
+/* --wide (:1068): switch the wide neighbourhood on before cso_init */
+int cso_set_wide(cso_t *s, int on)
+{
+    if (!on) { s->wide = 0; return 0; }
+    if (!s->rl_bit) {
+        s->rl_bit = calloc((size_t)((s->N + 63) / 64), 8);
+        s->rl_idx = malloc(sizeof(int64_t) * (size_t)RL_CAP);
+        s->dvecw = malloc(sizeof(int64_t) * (size_t)s->M * s->n * s->q);
+        const int64_t cap = (int64_t)s->M * s->n * s->q + 1;
+        s->cand_i = realloc(s->cand_i, sizeof(int) * cap);
+        s->cand_p = realloc(s->cand_p, sizeof(int) * cap);
+        s->cand_v = realloc(s->cand_v, sizeof(int) * cap);
+        s->cand_d = realloc(s->cand_d, sizeof(int64_t) * cap);
+        s->ncap = cap;
+        if (!s->rl_bit || !s->rl_idx || !s->dvecw || !s->cand_i || !s->cand_p || !s->cand_v || !s->cand_d)
+            return -1;
+    }
+    s->wide = 1;
+    s->rl_valid = 0;
+    s->rl_n = 0;
+    return 0;
+}
+
 /* --target (:1039,1078): a target the best code already meets is met at the
  * current iteration (:1291-1293) */
 void cso_set_target(cso_t *s, int64_t target)

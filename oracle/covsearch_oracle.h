@@ -64,6 +64,12 @@ void    cso_set_kick_order(cso_t *s, int order);
  * (T < 0: no target); cso_ttt_it gives the iteration whose move first
  * reached it (-1: not yet; the current iteration if already met when set). */
 void    cso_set_target(cso_t *s, int64_t target);
+/* --wide: every move of a candidate codeword is a candidate while the
+ * reference's uncovered-word list is valid and short; set before cso_init.
+ * Returns 0, or -1 when its buffers cannot be allocated. */
+int     cso_set_wide(cso_t *s, int on);
+/* every move's delta of codeword i: dvec[p*q + v] (v == c_p: CSO_DSEL_RANDOM) */
+void    cso_eval_codeword_wide(cso_t *s, int i, int64_t *dvec);
 int64_t cso_ttt_it(const cso_t *s);
 void    cso_set_threads(cso_t *s, int threads);
 

@@ -38,6 +38,9 @@ def lib():
         L.cso_set_kick_order.argtypes = [vp, i32]
         L.cso_set_threads.argtypes = [vp, i32]
         L.cso_set_target.argtypes = [vp, i64]
+        L.cso_set_wide.restype = i32
+        L.cso_set_wide.argtypes = [vp, i32]
+        L.cso_eval_codeword_wide.argtypes = [vp, i32, vp]
         L.cso_ttt_it.restype = i64
         L.cso_ttt_it.argtypes = [vp]
         for f in ('cso_uncovered', 'cso_best_uncovered', 'cso_iters', 'cso_kicks', 'cso_space'):
@@ -105,6 +108,16 @@ class Oracle:
         done = lib().cso_run(self.h, iters, tabu, stall, kick, cand,
                              _ptr(tr) if trace else None)
         return tr[:done] if trace else done
+
+    def set_wide(self, on=True):
+        if lib().cso_set_wide(self.h, int(bool(on))) != 0:
+            raise MemoryError('cso_set_wide')
+        return self
+
+    def eval_codeword_wide(self, i):
+        d = np.zeros(self.n * self.q, np.int64)
+        lib().cso_eval_codeword_wide(self.h, int(i), _ptr(d))
+        return d.reshape(self.n, self.q)
 
     def set_target(self, target):
         lib().cso_set_target(self.h, int(target))

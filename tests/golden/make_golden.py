@@ -110,6 +110,22 @@ def runs():
     add(6, 8, 4, 169, 1, 3000, ['--tabu', '10', '--target', '200'], infile='seeds_K6_8_4_M169.txt', threads=8,
         tag='target')
     add(6, 9, 4, 660, 1, 10, ['--target', '0'], infile='K6_9_4_M660.txt', tag='target')   # met at the start
+    # --wide: every move of a candidate codeword while the uncovered list is
+    # valid and short (:1353-1376; cov/opt/ttt.py's p5b+wide variant)
+    W = ['--wide', '--tabu', '10']
+    for sd in (1, 2, 3):
+        for it in (200, 2000):
+            add(6, 6, 3, 41, sd, it, W, infile='seeds_K6_6_3_M41.txt', tag='wide')
+        add(6, 6, 3, 41, sd, 20000, W + ['--target', '30'], infile='seeds_K6_6_3_M41.txt', tag='wide')
+    for sd in (1, 2):
+        add(6, 8, 4, 169, sd, 2000, W + ['--target', '20'], infile='seeds_K6_8_4_M169.txt', threads=8, tag='wide')
+        add(6, 6, 3, 41, sd, 3000, ['--wide', '--tabu', '3', '--stall', '20'], infile='seeds_K6_6_3_M41.txt',
+            tag='wide')
+        add(5, 6, 2, 70, sd, 2000, ['--random-init'] + W, tag='wide')
+    for it in (100, 1000, 3000):
+        add(6, 9, 4, 660, 1, it, ['--random-init'] + W, threads=8, tag='wide')
+    add(6, 6, 3, 41, 5, 2000, W + ['--cand', '50'], infile='seeds_K6_6_3_M41.txt', tag='wide')
+    add(5, 7, 2, 400, 1, 3000, ['--random-init'] + W, tag='wide')        # the list never short enough
     # exact instances of the batched configuration (coldcase_amd/workloads.py:
     # c4_specs() index, its seed 1000003 + 7919 i), spanning the grid: the
     # largest space, the largest M, every radius

@@ -239,3 +239,39 @@ def test_c4_grid_fleet():
         assert got[7:] == want[7:], (i, spec)
     for c in cells:
         c.close()
+
+
+def test_wide_cells_in_a_batch_equal_solo():
+    """--wide cells (and a standard one) in one batch end where they end alone."""
+    import os
+    from coldcase_amd import Batch, Cell
+    from tests._util import DATA, read_code
+    print('case batch wide', flush=True)
+    seed41 = read_code(os.path.join(DATA, 'seeds_K6_6_3_M41.txt'), 6)
+    specs = [(6, 6, 3, 41, 1, True), (6, 6, 3, 41, 2, True), (5, 6, 2, 70, 3, True), (6, 6, 3, 41, 4, False)]
+
+    def make(sp):
+        q, n, R, M, seed, wide = sp
+        c = Cell(q, n, R, M)
+        c.search_params(tenure=10)
+        if wide:
+            c.search_wide(True)
+        c.search_init(seed, seed_code=seed41 if M == 41 else None)
+        return c
+
+    cells = [make(sp) for sp in specs]
+    with Batch(cells) as b:
+        b.run(700, mode=1)
+        b.run(300, mode=0)
+    for sp, c in zip(specs, cells):
+        solo = make(sp)
+        solo.run(1000, mode=1)
+        assert _light(c) == _light(solo), sp
+        solo.close()
+        c.close()
+    late = make(specs[3])
+    with Batch([late]) as b:
+        late.search_wide(True)
+        with pytest.raises(Exception):
+            b.run(10)
+    late.close()

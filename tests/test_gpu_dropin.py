@@ -128,13 +128,11 @@ def test_cell_deadline_stops_before_a_step():
         c.search_init(1)
         c.limits(-1, -1.0)               # already past: no step begins
         assert c.run(100) == 0 and c.state()['it'] == 0
-        # 40,000 graph-replayed steps would take about 1.6 s; the steps
-        # launched after the deadline return at once
+        # at about 40 us a step, 0.2 s of deadline is about 5,000 of the
+        # 40,000 steps launched; the launches after it begin no step
         c.limits(-1, 0.2)
-        t0 = time.monotonic()
         ran = c.run(40000, mode=1)
-        el = time.monotonic() - t0
-        assert 0 < ran < 40000 and el < 1.2, (ran, el)
+        assert 1000 < ran < 20000, ran
         c.limits(-1, 0.0)                # no deadline
         assert c.run(10) == 10
 

@@ -19,14 +19,19 @@ def _cell(q, n, R, M):
 
 
 def compare(q, n, R, M, seed, steps, tabu=10, stall=5000, cand=100000, greedy=False,
-            seed_code=None, mode=0, check_cnt=True, threads=1, leave=0):
-    print('case K%d(%d,%d) M=%d seed=%d steps=%d stall=%d mode=%d leave=%d'
-          % (q, n, R, M, seed, steps, stall, mode, leave), flush=True)
+            seed_code=None, mode=0, check_cnt=True, threads=1, leave=0, wide=False):
+    print('case K%d(%d,%d) M=%d seed=%d steps=%d stall=%d mode=%d leave=%d wide=%d'
+          % (q, n, R, M, seed, steps, stall, mode, leave, wide), flush=True)
     cell = _cell(q, n, R, M)
     cell.search_params(tenure=tabu, stall=stall, kick=2, cand=cand)
     cell.set_leave_mode(leave)
+    if wide:
+        cell.search_wide(True)
     cell.search_init(seed, seed_code=seed_code, greedy=greedy)
-    o = Oracle(q, n, R, M, seed, threads=threads).init(seed_code, greedy=greedy)
+    o = Oracle(q, n, R, M, seed, threads=threads)
+    if wide:
+        o.set_wide()
+    o.init(seed_code, greedy=greedy)
     st0 = cell.state()
     assert st0['uncovered'] == o.uncovered, (st0, o.uncovered)
     assert st0['draws'] == o.draws
@@ -140,3 +145,28 @@ def test_step_sums_match_reference_profile(r):
     cell.close()
     assert (bu, st['it'], st['kicks']) == (r['uncovered'], r['iters_done'], r['kicks'])
     assert prof_matches(prof_aggregates(tr, unc0), r['prof']) == {}
+
+
+# --wide: every move of the candidate codewords while the list is short
+def _seed41():
+    import os
+    from tests._util import DATA, read_code
+    return read_code(os.path.join(DATA, 'seeds_K6_6_3_M41.txt'), 6)
+
+
+@pytest.mark.parametrize('leave', LEAVES)
+@pytest.mark.parametrize('mode', [0, 1])
+def test_wide_k6_6_3_seed_file(leave, mode):
+    tr = compare(6, 6, 3, 41, 1, 3000, seed_code=_seed41(), leave=leave, mode=mode, wide=True)
+    assert (tr[:, 2] > 200).any()          # steps with the wide candidate set (n(q-1) = 30 per codeword)
+
+
+def test_wide_kicks_and_sampling():
+    compare(6, 6, 3, 41, 2, 2000, tabu=3, stall=20, seed_code=_seed41(), wide=True)
+    compare(6, 6, 3, 41, 5, 1500, cand=50, seed_code=_seed41(), wide=True)
+
+
+@pytest.mark.parametrize('leave', LEAVES)
+def test_wide_k6_9_4(leave):
+    tr = compare(6, 9, 4, 660, 1, 400, leave=leave, wide=True, threads=8)
+    assert tr[-1, 2] > 300

@@ -25,6 +25,8 @@ def replay(r, threads=None):
         threads = 1 if r["q"] ** r["n"] < 3_000_000 else 4
     o = Oracle(r['q'], r['n'], r['R'], r['M'], r['seed'], threads=threads)
     f = r['flags']
+    if '--wide' in f:
+        o.set_wide()
     greedy = '--random-init' not in f
     tabu = int(f[f.index('--tabu') + 1]) if '--tabu' in f else 0
     stall = int(f[f.index('--stall') + 1]) if '--stall' in f else 5000
