@@ -2095,6 +2095,52 @@ __global__ void __launch_bounds__(WG) k_eval_probe(const Cell C, int k)
     }
 }
 
+// every move of the codewords slot_cw[0..k): the leaving units as above, and
+// per (codeword, position p, value v != c_p, R-subset avoiding p) the words
+// with cnt == 0 of that subset's sphere around the moved codeword, counted
+// into negw[slot][p][v]
+__global__ void __launch_bounds__(WG) k_eval_probe_all(const Cell C, int k, int32_t* negw)
+{
+    __shared__ WalkTab wtab[WAVES];
+    WalkTab& tb = wtab[threadIdx.x >> 6];
+    const int lane = threadIdx.x & 63;
+    const int64_t wid = (int64_t)blockIdx.x * WAVES + (threadIdx.x >> 6);
+    const int64_t nw = (int64_t)gridDim.x * WAVES;
+    const int64_t per = (int64_t)C.n * C.q1 * C.E;
+    const int64_t nl = (int64_t)k * C.F, ne = (int64_t)k * per;
+    for (int64_t g = wid; g < nl + ne; g += nw) {
+        if (g < nl) {
+            const int slot = (int)(g / C.F);
+            unit_leave(C, slot, (int)(g - (int64_t)slot * C.F), tb);
+            continue;
+        }
+        const int64_t h = g - nl;
+        const int slot = (int)(h / per);
+        const int64_t r = h - (int64_t)slot * per;
+        const int pv = (int)(r / C.E);
+        const int e = (int)(r - (int64_t)pv * C.E);
+        const int p = pv / C.q1, t = pv - p * C.q1;
+        const int i = C.slot_cw[slot];
+        const uint64_t pk0 = C.cpk[i];
+        const int cp = nib_get(pk0, p);
+        const int v = t + (t >= cp);
+        const int64_t base = C.cidx[i] + (int64_t)(v - cp) * C.pw[p];
+        const int64_t c0 = sphere_count(C, C.zb, base, nib_put(pk0, p, v), C.psub[(int64_t)p * C.E + e], tb);
+        if (lane == 0 && c0) atomicAdd(&negw[(int64_t)slot * WIDE_W + p * MAXQ + v], (int)c0);
+    }
+}
+
+__global__ void __launch_bounds__(WG) k_eval_finish_all(const Cell C, int k, const int32_t* negw, int64_t* dvec)
+{
+    const int nq = C.n * C.q;
+    for (int x = blockIdx.x * WG + threadIdx.x; x < k * nq; x += gridDim.x * WG) {
+        const int slot = x / nq, pv = x - slot * nq, p = pv / C.q, v = pv - p * C.q;
+        const int32_t* a = C.acc + (int64_t)slot * ACCW;
+        if (nib_get(C.cpk[C.slot_cw[slot]], p) == v) dvec[x] = DSEL_RANDOM;
+        else dvec[x] = (int64_t)(a[0] - a[1 + p]) - (int64_t)negw[(int64_t)slot * WIDE_W + p * MAXQ + v];
+    }
+}
+
 __global__ void __launch_bounds__(WG) k_eval_finish(const Cell C, int k, int64_t* dvec)
 {
     const uint64_t upk = C.ctl->upk;

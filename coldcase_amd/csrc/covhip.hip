@@ -1019,6 +1019,37 @@ int cov_eval_moves(cov_cell_t* h, const uint8_t* u, int k, const int32_t* cw, in
     return COV_OK;
 }
 
+int cov_eval_codewords(cov_cell_t* h, int k, const int32_t* cw, int64_t* dvec)
+{
+    if (h && !h->cnt_valid) { h->err = "coverage counts not built (cov_recount)"; return COV_ESTATE; }
+    if (!h || k < 0 || (k > 0 && (!cw || !dvec))) return COV_EPARAM;
+    const Cell& C = h->h;
+    if (k > C.M) return COV_ELIMIT;
+    if (k == 0) return COV_OK;
+    for (int t = 0; t < k; t++)
+        if (cw[t] < 0 || cw[t] >= C.M) return COV_EPARAM;
+    if (h->used) HCK(h, hipStreamWaitEvent(h->stream, h->used, 0));
+    RC(upload(h, C.slot_cw, cw, (int64_t)k * 4));
+    HCK(h, hipMemsetAsync(C.acc, 0, (size_t)k * ACCW * 4, h->stream));
+    int32_t* negw = nullptr;
+    int64_t* dd = nullptr;
+    HCK(h, hipMallocAsync((void**)&negw, (size_t)k * WIDE_W * 4, h->stream));
+    HCK(h, hipMemsetAsync(negw, 0, (size_t)k * WIDE_W * 4, h->stream));
+    const int64_t units = (int64_t)k * (C.F + (int64_t)C.n * C.q1 * C.E);
+    hipLaunchKernelGGL(k_eval_probe_all, dim3(grid_for(units, 4 * 8, 1, 4096)), dim3(WG), 0, h->stream, h->h, k,
+                       negw);
+    HCK(h, hipGetLastError());
+    HCK(h, hipMallocAsync((void**)&dd, (size_t)k * C.n * C.q * 8, h->stream));
+    hipLaunchKernelGGL(k_eval_finish_all, dim3(grid_for((int64_t)k * C.n * C.q, WG, 1, 1024)), dim3(WG), 0,
+                       h->stream, h->h, k, negw, dd);
+    HCK(h, hipGetLastError());
+    HCK(h, hipMemcpyAsync(dvec, dd, (size_t)k * C.n * C.q * 8, hipMemcpyDeviceToHost, h->stream));
+    HCK(h, hipFreeAsync(dd, h->stream));
+    HCK(h, hipFreeAsync(negw, h->stream));
+    HCK(h, hipStreamSynchronize(h->stream));
+    return COV_OK;
+}
+
 int cov_commit(cov_cell_t* h, int i, int p, int v, int64_t* dunc)
 {
     if (h) h->th_valid = false;

@@ -32,7 +32,8 @@ TR_FIELDS = ['it', 'u', 'ncand', 'ncw', 'bi', 'bp', 'bv', 'oldv', 'dsel', 'dunc'
 EXPORTS = ['cov_version', 'cov_strerror', 'cov_mem_bytes', 'cov_create', 'cov_destroy',
            'cov_last_error', 'cov_sync', 'cov_set_code', 'cov_set_code_flags', 'cov_get_code', 'cov_get_best',
            'cov_recount', 'cov_ball_update', 'cov_count_uncovered', 'cov_gather_uncovered',
-           'cov_first_uncovered_from', 'cov_gather_uncovered_noalloc', 'cov_read_cnt', 'cov_eval_moves', 'cov_commit',
+           'cov_first_uncovered_from', 'cov_gather_uncovered_noalloc', 'cov_read_cnt', 'cov_eval_moves',
+           'cov_eval_codewords', 'cov_commit',
            'cov_distance_transform', 'cov_distance_stats', 'cov_gather_far', 'cov_search_init', 'cov_search_params',
            'cov_search_wide', 'cov_search_run', 'cov_search_launch', 'cov_search_state', 'cov_query', 'cov_search_limits',
            'cov_search_ttt', 'cov_set_launch',
@@ -91,6 +92,7 @@ def lib():
         'cov_gather_uncovered_noalloc': (i64, [vp, vp, i64]),
         'cov_read_cnt': (i32, [vp, i64, i64, vp]),
         'cov_eval_moves': (i32, [vp, vp, i32, vp, vp]),
+        'cov_eval_codewords': (i32, [vp, i32, vp, vp]),
         'cov_commit': (i32, [vp, i32, i32, i32, vp]),
         'cov_distance_transform': (i32, [vp, vp, vp, vp]),
         'cov_gather_far': (i64, [vp, i32, vp, i64]),
@@ -252,6 +254,13 @@ class Cell:
         cw = np.ascontiguousarray(codewords, dtype=np.int32).ravel()
         d = np.zeros((len(cw), self.n), np.int64)
         self._check(lib().cov_eval_moves(self.h, _p(u), len(cw), _p(cw), _p(d)), 'cov_eval_moves')
+        return d
+
+    def eval_codewords(self, codewords):
+        """(k, n, q) deltas of every move of the given codewords."""
+        cw = np.ascontiguousarray(codewords, dtype=np.int32).ravel()
+        d = np.zeros((len(cw), self.n, self.q), np.int64)
+        self._check(lib().cov_eval_codewords(self.h, len(cw), _p(cw), _p(d)), 'cov_eval_codewords')
         return d
 
     def commit(self, i, p, v):

@@ -125,6 +125,13 @@ int     cov_read_cnt(cov_cell_t *h, int64_t off, int64_t len, uint16_t *out);
  * Replaces eval_codeword (cov/engine/covsearch2e.c:696-760). */
 int     cov_eval_moves(cov_cell_t *h, const uint8_t *u, int k, const int32_t *cw,
                        int64_t *dvec);
+/* (a) Every single-coordinate move of each of the k codewords cw[t]:
+ * dvec[(t*n + p)*q + v] = the change in the uncovered count of c_p := v, from
+ * one enumeration of the codeword's distance-R sphere (leaving side T - H_p)
+ * and the entering sphere of each move; COV_DSEL_RANDOM where v == c_p.  The
+ * n x q vector of eval_codeword (cov/engine/covsearch2e.c:696-760), which the
+ * --wide neighbourhood reads (:1353-1376). */
+int     cov_eval_codewords(cov_cell_t *h, int k, const int32_t *cw, int64_t *dvec);
 /* Commit the move c_i[p] := v; *dunc receives the change in uncovered.
  * Replaces move_commit (cov/engine/covsearch2e.c:562-590). */
 int     cov_commit(cov_cell_t *h, int i, int p, int v, int64_t *dunc);
