@@ -657,7 +657,9 @@ __device__ __forceinline__ void unit_list_wide(const Cell& C, int slot, int64_t 
 {
     const int lane = threadIdx.x & 63;
     const uint64_t cpk = C.cpk[C.slot_cw[slot]];
-    int32_t own[WIDE_W / 64] = {0, 0, 0, 0};
+    // four named counters, not an array: a runtime index would put it in
+    // private memory
+    int32_t o0 = 0, o1 = 0, o2 = 0, o3 = 0;
     for (int64_t x0 = e0; x0 < e1; x0 += 64) {
         const int64_t x = x0 + lane;
         uint64_t ne = 0, xpk = 0;
@@ -673,14 +675,21 @@ __device__ __forceinline__ void unit_list_wide(const Cell& C, int slot, int64_t 
             for (int v = 0; v < C.q; v++) {
                 const int c = __popcll(__ballot(bj && xj == v));
                 const int id = j * MAXQ + v;
-                if (c && lane == (id & 63)) own[id >> 6] += c;
+                if (c && lane == (id & 63)) {
+                    const int kq = id >> 6;
+                    o0 += kq == 0 ? c : 0;
+                    o1 += kq == 1 ? c : 0;
+                    o2 += kq == 2 ? c : 0;
+                    o3 += kq == 3 ? c : 0;
+                }
             }
         }
     }
     int32_t* a = C.accw + (int64_t)slot * WIDE_W;
-#pragma unroll
-    for (int k = 0; k < WIDE_W / 64; k++)
-        if (own[k]) atomicAdd(&a[lane + 64 * k], own[k]);
+    if (o0) atomicAdd(&a[lane], o0);
+    if (o1) atomicAdd(&a[lane + 64], o1);
+    if (o2) atomicAdd(&a[lane + 128], o2);
+    if (o3) atomicAdd(&a[lane + 192], o3);
 }
 
 __device__ __forceinline__ void unit_list(const Cell& C, int slot, int64_t e0, int64_t e1,
