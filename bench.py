@@ -289,6 +289,18 @@ def bench_transform(a, reps=20, warmup=2):
     cell.timing(False)
     dt_hist_us = ms[2] * 1e3 / reps
     rc_us = ms[3] * 1e3 / max(int(launches[3]), 1)
+    # the count transform of the uncovered set (the gain map), kernel time of
+    # all its passes per transform
+    from coldcase_amd.engine import CT_UNCOVERED
+    cell.count_transform(CT_UNCOVERED)
+    cell.timing(True)
+    for _ in range(max(reps // 4, 2)):
+        cell.count_transform(CT_UNCOVERED)
+    ms, launches = cell.timing_read()
+    cell.timing(False)
+    nct = max(reps // 4, 2)
+    ct_us = ms[6] * 1e3 / nct
+    ct_passes = int(launches[6]) // nct
     cell.close()
     alg_t = 2 * n * N
     alg_r = 2 * N + 4 * M * V
@@ -304,6 +316,19 @@ def bench_transform(a, reps=20, warmup=2):
            'recount': {'avg_us': round(rc_us, 2), 'alg_bytes': alg_r,
                        'achieved_GBs': round(alg_r / (rc_us * 1e-6) / 1e9, 1),
                        'atomics_per_s': round(M * V / (rc_us * 1e-6), 1)}}
+    # count transform: the reference's formulation moves (R+1) int32 layers
+    # in and out once per axis, 8 n (R+1) bytes per word; the passes here
+    # read 2 B of counts, write and re-read the layers once per pass and
+    # write the 4 B map
+    alg_c = 8 * n * (R + 1) * N
+    lay = 4 * (R + 1)
+    own_c = N * (2 + lay + 2 * lay * (ct_passes - 2) + lay + 4) if ct_passes >= 2 else N * (2 + 4)
+    out['count_transform'] = {'source': 'uncovered words (the gain map)', 'avg_us': round(ct_us, 2),
+                              'passes': ct_passes, 'alg_bytes': alg_c,
+                              'achieved_GBs': round(alg_c / (ct_us * 1e-6) / 1e9, 1),
+                              'pass_bytes': own_c,
+                              'pass_bytes_GBs': round(own_c / (ct_us * 1e-6) / 1e9, 1),
+                              'pass_bytes_frac': round(own_c / (ct_us * 1e-6) / 1e9 / HBM_PEAK_GBS, 4)}
     if traffic:
         tr = traffic / (dt_us * 1e-6) / 1e9
         out['roofline']['traffic_achieved'] = round(tr, 1)
