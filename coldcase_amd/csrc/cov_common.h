@@ -231,15 +231,24 @@ __device__ __forceinline__ void bit_clr(uint32_t* plane, int64_t w)
     atomicAnd(&plane[w >> 5], ~(1u << (w & 31)));
 }
 
+// The word-index arithmetic below is also compiled for the host, where a
+// bounds-checked emulation runs it over the same plan (cov_plan.h).
+
+// high half of a 32 x 32-bit product (v_mul_hi_u32 on the device)
+__host__ __device__ __forceinline__ uint32_t mulhi32(uint32_t a, uint32_t b)
+{
+    return (uint32_t)(((uint64_t)a * (uint64_t)b) >> 32);
+}
+
 // nibble-wise "differs" mask: bit 4j set iff digit j of a and b differ
-__device__ __forceinline__ uint64_t nib_ne(uint64_t a, uint64_t b)
+__host__ __device__ __forceinline__ uint64_t nib_ne(uint64_t a, uint64_t b)
 {
     uint64_t x = a ^ b;
     return (x | (x >> 1) | (x >> 2) | (x >> 3)) & NIB1;
 }
-__device__ __forceinline__ int nib_dist(uint64_t a, uint64_t b)
+__host__ __device__ __forceinline__ int nib_dist(uint64_t a, uint64_t b)
 {
-    return __popcll(nib_ne(a, b));
+    return __builtin_popcountll(nib_ne(a, b));
 }
 
 // word w (digits wpk) became, or stopped being, covered by codeword o alone
@@ -258,17 +267,17 @@ __device__ __forceinline__ void th_event(const Cell& C, int o, uint64_t wpk, int
         atomicAdd(&t[1 + (b >> 2)], sign);
     }
 }
-__device__ __forceinline__ int nib_get(uint64_t pk, int j)
+__host__ __device__ __forceinline__ int nib_get(uint64_t pk, int j)
 {
     return (int)((pk >> (4 * j)) & 15u);
 }
-__device__ __forceinline__ uint64_t nib_put(uint64_t pk, int j, int v)
+__host__ __device__ __forceinline__ uint64_t nib_put(uint64_t pk, int j, int v)
 {
     return (pk & ~(15ULL << (4 * j))) | ((uint64_t)v << (4 * j));
 }
 
 // digits of a word index, packed (32-bit arithmetic when q^n fits)
-__device__ inline uint64_t pack_index(const Cell& C, int64_t w)
+__host__ __device__ inline uint64_t pack_index(const Cell& C, int64_t w)
 {
     uint64_t pk = 0;
     if (C.N <= 0xFFFFFFFFLL) {
@@ -295,16 +304,16 @@ __device__ __forceinline__ uint32_t ld_acq32(const uint32_t* p)
     return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-__device__ __forceinline__ uint32_t div_q1(const Cell& C, uint32_t t)
+__host__ __device__ __forceinline__ uint32_t div_q1(const Cell& C, uint32_t t)
 {
-    return C.q1 == 1 ? t : __umulhi(t, C.magic);
+    return C.q1 == 1 ? t : mulhi32(t, C.magic);
 }
 
 // Word of pattern t (0 <= t < (q-1)^r) on the position subset `sub` (r
 // positions, 4 bits each, increasing) around the word with digits pk at base:
 // the last position varies fastest, and the k-th value of a position skips
 // the centre digit (v = k + (k >= c)).  Also returns the packed digits.
-__device__ __forceinline__ int64_t pattern_word(const Cell& C, int64_t base, uint64_t pk,
+__host__ __device__ __forceinline__ int64_t pattern_word(const Cell& C, int64_t base, uint64_t pk,
                                                 uint32_t sub, int r, uint32_t t,
                                                 uint64_t* wpk)
 {
@@ -334,7 +343,8 @@ struct WalkTab {
     int64_t lo[WT_MAX];
     int64_t hi[WT_MAX];
 };
-__device__ __forceinline__ int64_t walk_part(const Cell& C, uint64_t pk, uint32_t sub, int i1, int i0, uint32_t x)
+__host__ __device__ __forceinline__ int64_t walk_part(const Cell& C, uint64_t pk, uint32_t sub, int i1, int i0,
+                                                     uint32_t x)
 {
     int64_t s = 0;
     for (int i = i1; i >= i0; i--) {
@@ -347,20 +357,30 @@ __device__ __forceinline__ int64_t walk_part(const Cell& C, uint64_t pk, uint32_
     }
     return s;
 }
+// entries first, first + stride, ... of both tables
+__host__ __device__ __forceinline__ void walk_fill(const Cell& C, uint64_t pk, uint32_t sub, int64_t* lo, int64_t* hi,
+                                                   int first, int stride)
+{
+    const int r = C.R, a = C.wa;
+    for (int l = first; l < C.wnl; l += stride) lo[l] = walk_part(C, pk, sub, r - 1, r - a, (uint32_t)l);
+    for (int h = first; h < C.wnh; h += stride) hi[h] = walk_part(C, pk, sub, r - a - 1, 0, (uint32_t)h);
+}
 __device__ __forceinline__ void walk_tables(const Cell& C, uint64_t pk, uint32_t sub, WalkTab& tb)
 {
-    const int lane = threadIdx.x & 63;
-    const int r = C.R, a = C.wa;
-    for (int l = lane; l < C.wnl; l += 64) tb.lo[l] = walk_part(C, pk, sub, r - 1, r - a, (uint32_t)l);
-    for (int h = lane; h < C.wnh; h += 64) tb.hi[h] = walk_part(C, pk, sub, r - a - 1, 0, (uint32_t)h);
+    walk_fill(C, pk, sub, tb.lo, tb.hi, threadIdx.x & 63, 64);
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
+__host__ __device__ __forceinline__ int64_t walk_word(const Cell& C, const int64_t* lo, const int64_t* hi,
+                                                     int64_t base, uint32_t t)
+{
+    const uint32_t h = C.wnl == 1 ? t : mulhi32(t, C.wmag);
+    return base + hi[h] + lo[t - h * (uint32_t)C.wnl];
+}
 __device__ __forceinline__ int64_t walk_word(const Cell& C, const WalkTab& tb, int64_t base, uint32_t t)
 {
-    const uint32_t h = C.wnl == 1 ? t : __umulhi(t, C.wmag);
-    return base + tb.hi[h] + tb.lo[t - h * (uint32_t)C.wnl];
+    return walk_word(C, tb.lo, tb.hi, base, t);
 }
 // the wave is done with its tables (before the next unit rebuilds them)
 __device__ __forceinline__ void walk_release()

@@ -268,7 +268,7 @@ __device__ __forceinline__ int ball_word(const Cell& C, Ctl* ctl, int64_t w, uin
 }
 
 // decode ball word g (0 <= g < V) around (base, pk)
-__device__ __forceinline__ int64_t ball_word_at(const Cell& C, int64_t base, uint64_t pk,
+__host__ __device__ __forceinline__ int64_t ball_word_at(const Cell& C, int64_t base, uint64_t pk,
                                                 int64_t g, uint64_t* wpk)
 {
     int r = 0;

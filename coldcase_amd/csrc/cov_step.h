@@ -1627,17 +1627,35 @@ __device__ __forceinline__ void wg_post(const Cell& C, Ctl* ctl, NarrowSh& sh)
     __syncthreads();
 }
 
-// the commit units: every pattern of the subsets avoiding bp, flattened
 // offsets of the commit's words around the moving codeword: T[j * 16 + k]
-// = (v - c_j) q^(n-1-j) for the k-th value v != c_j of position j (all threads)
+// = (v - c_j) q^(n-1-j) for the k-th value v != c_j of position j
+__host__ __device__ __forceinline__ int64_t commit_offset(const Cell& C, uint64_t pk, int j, int k)
+{
+    const int c = nib_get(pk, j);
+    return (int64_t)(k + (k >= c) - c) * C.pw[j];
+}
+// (all threads)
 __device__ __forceinline__ void commit_table(const Cell& C, uint64_t pk, int64_t* T)
 {
     for (int x = threadIdx.x; x < C.n * C.q1; x += WG) {
         const int j = x / C.q1, k = x - j * C.q1;
-        const int c = nib_get(pk, j);
-        T[j * 16 + k] = (int64_t)(k + (k >= c) - c) * C.pw[j];
+        T[j * 16 + k] = commit_offset(C, pk, j, k);
     }
     __syncthreads();
+}
+// the word of pattern t on subset sub around base (pattern_word's word,
+// through the table)
+__host__ __device__ __forceinline__ int64_t commit_word(const Cell& C, const int64_t* T, int64_t base, uint32_t sub,
+                                                       uint32_t t)
+{
+    int64_t a = base;
+    for (int i = C.R - 1; i >= 0; i--) {
+        const uint32_t tq = div_q1(C, t);
+        const int k = (int)(t - tq * (uint32_t)C.q1);
+        t = tq;
+        a += T[((sub >> (4 * i)) & 15) * 16 + k];
+    }
+    return a;
 }
 
 // Runs of contiguous lanes with equal keys (key < 0: no operation) combine
@@ -1706,17 +1724,9 @@ __device__ __forceinline__ int64_t commit_units(const Cell& C, const Ctl* ctl, i
         int da = 0, db = 0;
         if (act) {
             const int64_t e = g / C.P;
-            uint32_t t = (uint32_t)(g - e * C.P);
+            const uint32_t t = (uint32_t)(g - e * C.P);
             const uint32_t sub = C.psub[(int64_t)bp * C.E + e];
-            // the word of pattern t (pattern_word's, through the table)
-            const uint32_t t0 = t;
-            a = base;
-            for (int i = C.R - 1; i >= 0; i--) {
-                const uint32_t tq = div_q1(C, t);
-                const int k = (int)(t - tq * (uint32_t)C.q1);
-                t = tq;
-                a += T[((sub >> (4 * i)) & 15) * 16 + k];
-            }
+            a = commit_word(C, T, base, sub, t);
             b = a + dv;
             const uint16_t ca = cnt_get(C, a), cb = cnt_get(C, b);
             cnt_put(C, a, (uint16_t)(ca - 1));
@@ -1729,7 +1739,7 @@ __device__ __forceinline__ int64_t commit_units(const Cell& C, const Ctl* ctl, i
                 xor_put(C, b, xb ^ (uint32_t)bi);
                 if (ca == 2 || cb == 1) {
                     uint64_t apk;
-                    (void)pattern_word(C, base, pk, sub, C.R, t0, &apk);
+                    (void)pattern_word(C, base, pk, sub, C.R, t, &apk);
                     if (ca == 2) th_event(C, (int)(xa ^ (uint32_t)bi), apk, +1);
                     if (cb == 1) th_event(C, (int)xb, nib_put(apk, bp, bv), -1);
                 }
@@ -1739,7 +1749,7 @@ __device__ __forceinline__ int64_t commit_units(const Cell& C, const Ctl* ctl, i
                 za = ma; oa_clr = ma; da = 1;
                 if (list_on) {
                     uint64_t apk;
-                    (void)pattern_word(C, base, pk, sub, C.R, t0, &apk);
+                    (void)pattern_word(C, base, pk, sub, C.R, t, &apk);
                     list_insert(C, C.ctl, a, apk);
                 }
                 d++;

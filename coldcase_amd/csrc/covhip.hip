@@ -18,6 +18,7 @@
 #include "cov_batch.h"
 #include "cov_step.h"
 #include "cov_count.h"
+#include "cov_plan.h"
 
 using namespace cov;
 
@@ -100,69 +101,6 @@ constexpr int GRAPH_STEPS = COV_GRAPH_STEPS;
         }                                                                      \
     } while (0)
 
-int64_t binom(int a, int b)
-{
-    if (b < 0 || b > a) return 0;
-    if (b > a - b) b = a - b;
-    int64_t r = 1;
-    for (int i = 0; i < b; i++) r = r * (a - i) / (i + 1);
-    return r;
-}
-
-// all r-subsets of `pos`, lexicographic, packed 4 bits per position
-std::vector<uint32_t> subsets(const std::vector<int>& pos, int r)
-{
-    std::vector<uint32_t> out;
-    const int np = (int)pos.size();
-    if (r > np) return out;
-    std::vector<int> c(r);
-    for (int i = 0; i < r; i++) c[i] = i;
-    for (;;) {
-        uint32_t s = 0;
-        for (int i = 0; i < r; i++) s |= (uint32_t)pos[c[i]] << (4 * i);
-        out.push_back(s);
-        int i = r - 1;
-        while (i >= 0 && c[i] == np - r + i) i--;
-        if (i < 0) break;
-        c[i]++;
-        for (int j = i + 1; j < r; j++) c[j] = c[j - 1] + 1;
-    }
-    return out;
-}
-
-struct Shape {
-    int q, n, R, M, q1, F, E;
-    int64_t P, N, V, nzw, nblk, ulcap;
-};
-
-int shape_of(int q, int n, int R, int M, Shape* s)
-{
-    if (q < 2 || q > MAXQ || n < 1 || n > MAXN || R < 0 || R > MAXR || R >= n || M < 1 || M > 65535)
-        return COV_EPARAM;
-    int64_t N = 1;
-    for (int i = 0; i < n; i++) {
-        if (N > (int64_t)4e10 / q) return COV_ESPACE;
-        N *= q;
-    }
-    s->q = q; s->n = n; s->R = R; s->M = M; s->q1 = q - 1;
-    s->F = (int)binom(n, R);
-    s->E = (int)binom(n - 1, R);
-    s->P = 1;
-    for (int i = 0; i < R; i++) s->P *= (q - 1);
-    if (s->P >= (1 << 24)) return COV_ELIMIT;
-    s->N = N;
-    s->V = 0;
-    for (int r = 0; r <= R; r++) {
-        int64_t t = binom(n, r);
-        for (int i = 0; i < r; i++) t *= (q - 1);
-        s->V += t;
-    }
-    s->nzw = (N + 31) / 32;
-    s->nblk = (N + BLK_WORDS - 1) / BLK_WORDS;
-    s->ulcap = std::min<int64_t>(N, (int64_t)1 << 20);
-    return COV_OK;
-}
-
 // device bytes of each array, in allocation order
 struct Sizes {
     int64_t cnt, plane, blk, code, cpk, cidx, tabu, fsub, psub, bsub, ssub, ul_idx, ul_pk,
@@ -237,10 +175,15 @@ int dalloc(cov_cell* h, T** p, int64_t bytes)
     return COV_OK;
 }
 
+// host -> device on the cell's stream, complete on return: callers pass stack
+// values and vectors that die with their scope, and pageable sources are not
+// guaranteed to be staged by the time an asynchronous copy returns (set-up
+// paths only; the step loop uploads nothing)
 template <typename T>
 int upload(cov_cell* h, T* dst, const void* src, int64_t bytes)
 {
     HCK(h, hipMemcpyAsync(dst, src, (size_t)bytes, hipMemcpyHostToDevice, h->stream));
+    HCK(h, hipStreamSynchronize(h->stream));
     return COV_OK;
 }
 
@@ -640,26 +583,8 @@ int cov_create(int q, int n, int R, int M, cov_cell_t** out)
             h->clk_hz = (double)khz * 1e3;
     }
     Cell& C = h->h;
-    C.q = q; C.n = n; C.R = R; C.M = M; C.q1 = q - 1; C.F = s.F; C.E = s.E; C.P = s.P;
-    C.N = s.N; C.nzw = s.nzw; C.nblk = s.nblk; C.V = s.V; C.ncap = (int64_t)M * n; C.ulcap = s.ulcap;
-    C.ul_use = (int64_t)(R + 1) * s.E * s.P / n;
-    for (int j = 0; j < n; j++) {
-        C.pw[j] = 1;
-        for (int e = 0; e < n - 1 - j; e++) C.pw[j] *= q;
-    }
-    C.magic = (q - 1 == 1) ? 0u : (uint32_t)((((uint64_t)1 << 32) + (uint64_t)(q - 2)) / (uint64_t)(q - 1));
-    {
-        // sphere-walk tables: the innermost ceil(R/2) positions in lo
-        const int a = (R + 1) / 2;
-        int64_t nl = 1, nh = 1;
-        for (int i = 0; i < a; i++) nl *= (q - 1);
-        for (int i = a; i < R; i++) nh *= (q - 1);
-        C.wa = a;
-        C.wok = (R >= 1 && nl <= WT_MAX && nh <= WT_MAX) ? 1 : 0;
-        C.wnl = C.wok ? (int32_t)nl : 1;
-        C.wnh = C.wok ? (int32_t)nh : 1;
-        C.wmag = C.wnl > 1 ? (uint32_t)((((uint64_t)1 << 32) / (uint64_t)C.wnl) + 1) : 0u;
-    }
+    SubsetTables tabs;
+    plan_cell(s, C, tabs);
     const Sizes z = sizes_of(s);
     int rc = COV_OK;
 #define AL(ptr, bytes) if ((rc = dalloc(h, &(ptr), (bytes))) != COV_OK) return fail(rc)
@@ -698,29 +623,6 @@ int cov_create(int q, int n, int R, int M, cov_cell_t** out)
 #undef AL
     C.scratch64 = (decltype(C.scratch64))h->d_small;
 
-    // position-subset tables
-    std::vector<int> all(n);
-    for (int j = 0; j < n; j++) all[j] = j;
-    std::vector<uint32_t> fs = subsets(all, R), ps;
-    for (int p = 0; p < n; p++) {
-        std::vector<int> rest;
-        for (int j = 0; j < n; j++) if (j != p) rest.push_back(j);
-        std::vector<uint32_t> t = subsets(rest, R);
-        ps.insert(ps.end(), t.begin(), t.end());
-    }
-    std::vector<uint32_t> bs;
-    for (int r = 0; r <= MAXR + 1; r++) { C.bsub_off[r] = 0; C.bpat_off[r] = 0; }
-    int64_t patacc = 0;
-    for (int r = 0; r <= R; r++) {
-        C.bsub_off[r] = (int64_t)bs.size();
-        C.bpat_off[r] = patacc;
-        std::vector<uint32_t> t = subsets(all, r);
-        bs.insert(bs.end(), t.begin(), t.end());
-        int64_t pr = 1;
-        for (int i = 0; i < r; i++) pr *= (q - 1);
-        patacc += (int64_t)t.size() * pr;
-    }
-    for (int r = R + 1; r <= MAXR + 1; r++) { C.bsub_off[r] = (int64_t)bs.size(); C.bpat_off[r] = patacc; }
     // recount plan: tiles of q^(n-t) words with a common t-digit prefix
     RecountPlan& pl = h->plan;
     pl.t = 0;
@@ -753,9 +655,10 @@ int cov_create(int q, int n, int R, int M, cov_cell_t** out)
         }();
         if ((rc = upload(h, C.jump, jt.data(), (int64_t)jt.size() * 8)) != COV_OK) return fail(rc);
     }
-    if ((rc = upload(h, C.fsub, fs.data(), (int64_t)fs.size() * 4)) != COV_OK) return fail(rc);
-    if (!ps.empty() && (rc = upload(h, C.psub, ps.data(), (int64_t)ps.size() * 4)) != COV_OK) return fail(rc);
-    if ((rc = upload(h, C.bsub, bs.data(), (int64_t)bs.size() * 4)) != COV_OK) return fail(rc);
+    if ((rc = upload(h, C.fsub, tabs.fs.data(), (int64_t)tabs.fs.size() * 4)) != COV_OK) return fail(rc);
+    if (!tabs.ps.empty() && (rc = upload(h, C.psub, tabs.ps.data(), (int64_t)tabs.ps.size() * 4)) != COV_OK)
+        return fail(rc);
+    if ((rc = upload(h, C.bsub, tabs.bs.data(), (int64_t)tabs.bs.size() * 4)) != COV_OK) return fail(rc);
     if ((rc = upload(h, pl.ssub, ss.data(), (int64_t)ss.size() * 4)) != COV_OK) return fail(rc);
 
     Ctl c{};
