@@ -33,7 +33,8 @@ constexpr int BLK_SHIFT = 12;        // words per uncovered-count block = 4096
 constexpr int BLK_WORDS = 1 << BLK_SHIFT;
 constexpr int RB = 1 << 14;          // rng ring size (draws)
 constexpr int ACCW = 1 + 2 * MAXN;   // per-slot accumulators: t1, hit[16], neg[16]
-constexpr int LIST_CHUNK = 512;      // uncovered-list entries per evaluator unit
+constexpr int LIST_CHUNK = 64;       // uncovered-list entries per evaluator unit: one round of a wave, two
+                                     // dependent loads (entry, then its plane bit), so no unit is a long chain
 constexpr int64_t DSEL_RANDOM = 1LL << 60;
 constexpr int TRW = 16;
 constexpr int DIRTY_CAP = 4096;      // logged codeword changes before a full best copy
@@ -103,6 +104,12 @@ struct alignas(16) Ctl {
     // uncovered list is valid and short (wnow, decided per step); per =
     // candidates per candidate codeword (n (q-1) when wnow, else dtar)
     int32_t wide, wnow, per, pad_w;
+    // the evaluator's ring refill, decided by begin (the last writer before
+    // the evaluator) so that every filling workgroup sees one decision:
+    // fill_on, the first position and the generator state there
+    int64_t fill_g0;
+    uint64_t fill_s[4];
+    int32_t fill_on, pad_f;
     int64_t* trace;         // per-run trace records (tcap of them), or null
     int64_t tcap;
 };
@@ -441,14 +448,17 @@ __device__ inline int64_t wave_ring_fill(const Cell& C, Ctl* ctl, int64_t upto, 
     return g - g0;
 }
 
-// Four waves at once.  The generator's state transition is linear over
+// Many waves at once.  The generator's state transition is linear over
 // GF(2), so the state RNG_PART * w draws ahead is J_w s for a fixed 256 x 256
 // bit matrix J_w = T^(RNG_PART w).  A matrix is 256 rows of 4 words: bit j of
 // row b is set when bit b of the result depends on bit j of the state, bit b
-// of a state being bit b % 64 of word b / 64.
-constexpr int RNG_SPLIT = 4;                    // waves of the filling workgroup
-constexpr int RNG_PART = 256;                   // draws per wave
+// of a state being bit b % 64 of word b / 64.  A part's draws are a serial
+// chain (about 110 cycles each on the scalar unit), so the chunk is cut in
+// parts of one group of 64.
+constexpr int RNG_SPLIT = 16;                   // parts of a fill
+constexpr int RNG_PART = 64;                    // draws per part
 constexpr int RNG_CHUNK = RNG_SPLIT * RNG_PART; // draws per fill
+constexpr int RNG_WGS = RNG_SPLIT / 4;          // filling workgroups (of 4 waves) when the grid allows
 constexpr int RNG_JUMP_WORDS = (RNG_SPLIT - 1) * 256 * 4;
 
 // J_1 .. J_{RNG_SPLIT-1}, on the host (out: RNG_JUMP_WORDS words)
@@ -499,38 +509,46 @@ inline void rng_jump_tables(uint64_t* out)
     }
 }
 
-// RNG_CHUNK draws from position g0 (state s0) by the RNG_SPLIT waves of one
-// workgroup, wave w the RNG_PART draws from g0 + RNG_PART w, its state J_w s0
-// (lane l computes bits l, l+64, l+128, l+192 of it, one ballot per word).
-// The caller has checked g0 + RNG_CHUNK <= rlow + RB.  All threads.
-__device__ inline void wg_ring_fill4(const Cell& C, Ctl* ctl, const uint64_t* s0, int64_t g0)
+// The fill decided by begin (ctl->fill_on): RNG_CHUNK draws from fill_g0
+// (state fill_s), part w the RNG_PART draws from fill_g0 + RNG_PART w, its
+// state J_w fill_s (lane l computes bits l, l+64, l+128, l+192 of it, one
+// ballot per word).  Waves wi = 0 .. nwaves-1 of the filling workgroups take
+// parts wi, wi + nwaves, ...; the wave with the last part stores the state
+// after the chunk.  Nothing else reads or writes rgen or rst while the
+// evaluator runs.  All threads of a filling workgroup.
+__device__ inline void ring_fill_parts(const Cell& C, Ctl* ctl, int wi, int nwaves)
 {
-    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    uint64_t s[4] = {uniform64(s0[0]), uniform64(s0[1]), uniform64(s0[2]), uniform64(s0[3])};
-    if (w > 0) {
-        const uint64_t* M = C.jump + (size_t)(w - 1) * 256 * 4;
-        uint64_t o[4];
+    const int lane = threadIdx.x & 63;
+    const int64_t g0 = (int64_t)uniform64((uint64_t)ctl->fill_g0);
+    const uint64_t s0[4] = {uniform64(ctl->fill_s[0]), uniform64(ctl->fill_s[1]), uniform64(ctl->fill_s[2]),
+                            uniform64(ctl->fill_s[3])};
+    for (int w = wi; w < RNG_SPLIT; w += nwaves) {
+        uint64_t s[4] = {s0[0], s0[1], s0[2], s0[3]};
+        if (w > 0) {
+            const uint64_t* M = C.jump + (size_t)(w - 1) * 256 * 4;
+            uint64_t o[4];
 #pragma unroll
-        for (int k = 0; k < 4; k++) {
-            const uint64_t* row = M + (size_t)(64 * k + lane) * 4;
-            const uint64_t x = (row[0] & s[0]) ^ (row[1] & s[1]) ^ (row[2] & s[2]) ^ (row[3] & s[3]);
-            o[k] = __ballot(__popcll(x) & 1);
-        }
+            for (int k = 0; k < 4; k++) {
+                const uint64_t* row = M + (size_t)(64 * k + lane) * 4;
+                const uint64_t x = (row[0] & s0[0]) ^ (row[1] & s0[1]) ^ (row[2] & s0[2]) ^ (row[3] & s0[3]);
+                o[k] = __ballot(__popcll(x) & 1);
+            }
 #pragma unroll
-        for (int k = 0; k < 4; k++) s[k] = uniform64(o[k]);
-    }
-    const int64_t gb = g0 + (int64_t)RNG_PART * w;
-    for (int grp = 0; grp < RNG_PART / 64; grp++) {
-        uint64_t mine = 0;
-        for (int k = 0; k < 64; k++) {
-            const uint64_t r = xo_next(s);
-            if (lane == k) mine = r;
+            for (int k = 0; k < 4; k++) s[k] = uniform64(o[k]);
         }
-        C.rbuf[(gb + 64 * grp + lane) & (RB - 1)] = mine;
-    }
-    if (w == RNG_SPLIT - 1 && lane == 0) {
-        ctl->rst[0] = s[0]; ctl->rst[1] = s[1]; ctl->rst[2] = s[2]; ctl->rst[3] = s[3];
-        ctl->rgen = g0 + RNG_CHUNK;
+        const int64_t gb = g0 + (int64_t)RNG_PART * w;
+        for (int grp = 0; grp < RNG_PART / 64; grp++) {
+            uint64_t mine = 0;
+            for (int k = 0; k < 64; k++) {
+                const uint64_t r = xo_next(s);
+                if (lane == k) mine = r;
+            }
+            C.rbuf[(gb + 64 * grp + lane) & (RB - 1)] = mine;
+        }
+        if (w == RNG_SPLIT - 1 && lane == 0) {
+            ctl->rst[0] = s[0]; ctl->rst[1] = s[1]; ctl->rst[2] = s[2]; ctl->rst[3] = s[3];
+            ctl->rgen = g0 + RNG_CHUNK;
+        }
     }
 }
 

@@ -560,6 +560,12 @@ __device__ __forceinline__ void wg_begin(const Cell& C, Ctl* ctl, NarrowSh& sh)
         ctl->move_steps += 1;
         ctl->selected = 0;
         ctl->begun = (ncand > 0 && take <= RB - 4096) ? 1 : 0;
+        // the evaluator's ring refill: a chunk when fewer than 4096 draws are
+        // ahead and the chunk overwrites no draw still to be read
+        const int64_t g0 = ctl->rgen;
+        ctl->fill_on = g0 < ctl->ruse + 4096 && ctl->budget >= RNG_CHUNK && g0 + RNG_CHUNK <= ctl->rlow + RB;
+        ctl->fill_g0 = g0;
+        for (int k = 0; k < 4; k++) ctl->fill_s[k] = ctl->rst[k];
         ctl->tph[2] += clk() - tb0;
     }
     __syncthreads();
@@ -568,25 +574,32 @@ __device__ __forceinline__ void wg_begin(const Cell& C, Ctl* ctl, NarrowSh& sh)
 // ------------------------------------------------------- evaluator units
 // Set bits of `plane` over the P words of the sphere of (base, pk) on subset
 // sub, summed over the wave.  K loads per lane are issued before the first is
-// used, so a sphere of up to 64 K words costs one memory round trip.
+// used, so a sphere of up to 64 K words costs one memory round trip: the
+// loads are unconditional (a pattern number past P is clamped to P - 1, an
+// in-bounds word whose bit is then masked), since a load guarded by its own
+// branch is waited for inside that branch, one round trip per load.
 template <int K>
 __device__ __forceinline__ int64_t sphere_count_k(const Cell& C, const uint32_t* __restrict__ plane, int64_t base,
                                                   uint64_t pk, uint32_t sub, const WalkTab& tb, bool tab)
 {
     const int lane = threadIdx.x & 63;
+    const uint32_t last = (uint32_t)(C.P - 1);
     int64_t c = 0;
     for (int64_t t0 = 0; t0 < C.P; t0 += 64 * K) {
-        uint32_t b[K];
+        uint32_t v[K], sh[K];
 #pragma unroll
         for (int k = 0; k < K; k++) {
             const int64_t t = t0 + k * 64 + lane;
-            b[k] = 0;
-            if (t < C.P)
-                b[k] = bit_of(plane, tab ? walk_word(C, tb, base, (uint32_t)t)
-                                         : pattern_word(C, base, pk, sub, C.R, (uint32_t)t, nullptr));
+            const uint32_t tc = t < C.P ? (uint32_t)t : last;
+            const int64_t w = tab ? walk_word(C, tb, base, tc) : pattern_word(C, base, pk, sub, C.R, tc, nullptr);
+            v[k] = plane[w >> 5];
+            sh[k] = (uint32_t)(w & 31);
         }
 #pragma unroll
-        for (int k = 0; k < K; k++) c += __popcll(__ballot(b[k]));
+        for (int k = 0; k < K; k++) {
+            const bool in = t0 + k * 64 + lane < C.P;
+            c += __popcll(__ballot(in && ((v[k] >> sh[k]) & 1u)));
+        }
     }
     return c;
 }
@@ -1365,42 +1378,35 @@ __device__ __forceinline__ void eval_body(const Cell& C, int bid, int nblk)
     const int64_t th0 = clk();
     if (bid == 0 && threadIdx.x == 0 && ctl->t_cme) ctl->tph[10] += th0 - ctl->t_cme;
     // workgroup 0: the shuffle.  The rng refill, keeping ~4096 draws ahead:
-    // on three or more groups all of workgroup 1, RNG_CHUNK draws at once by
-    // its four waves; else wave 0 of workgroup 1 (or of 0 on a one-group grid,
-    // after the shuffle), one draw after another.
-    static_assert(RNG_SPLIT == WAVES, "one wave per rng part");
+    // on three or more groups a chunk of RNG_CHUNK draws in parts by the
+    // waves of workgroups 1 .. nfw (when begin decided on one), else wave 0
+    // of workgroup 1 (or of 0 on a one-group grid, after the shuffle), one
+    // draw after another.  The other workgroups run the units.
+    const int nfw = nblk >= 3 ? (nblk - 2 < RNG_WGS ? nblk - 2 : RNG_WGS) : 0;
     const int rng_wg = nblk > 1 ? 1 : 0;
-    const bool rng4 = nblk >= 3;
     if (bid == 0) {
         wg_shuffle(C, ctl, shs);
         wg_stage(C, ctl);
         if (threadIdx.x == 0) ctl->tph[5] += clk() - th0;
     }
-    if (bid == rng_wg && rng4) {
-        __shared__ int64_t rsh[6];
-        if (threadIdx.x == 0) {
-            const int64_t g0 = ctl->rgen;
-            rsh[0] = g0 < ctl->ruse + 4096 && ctl->budget >= RNG_CHUNK && g0 + RNG_CHUNK <= ctl->rlow + RB;
-            rsh[1] = g0;
-            for (int k = 0; k < 4; k++) rsh[2 + k] = (int64_t)ctl->rst[k];
-        }
-        __syncthreads();
-        if (rsh[0]) {
-            wg_ring_fill4(C, ctl, reinterpret_cast<const uint64_t*>(&rsh[2]), rsh[1]);
-        } else if (wv == 0) {
+    if (nfw > 0 && bid >= 1 && bid <= nfw) {
+        if (ctl->fill_on) {
+            ring_fill_parts(C, ctl, (bid - 1) * WAVES + wv, nfw * WAVES);
+        } else if (bid == 1 && wv == 0) {
             if (ctl->rgen < ctl->ruse + 4096) wave_ring_fill(C, ctl, ctl->ruse + 4096, ctl->budget);
         }
-        if (threadIdx.x == 0) ctl->tph[6] += clk() - th0;
-    } else if (bid == rng_wg && wv == 0) {
+        if (bid == 1 && threadIdx.x == 0) ctl->tph[6] += clk() - th0;
+    } else if (nfw == 0 && bid == rng_wg && wv == 0) {
         if (ctl->rgen < ctl->ruse + 4096) wave_ring_fill(C, ctl, ctl->ruse + 4096, ctl->budget);
         if (threadIdx.x == 0) ctl->tph[6] += clk() - th0;
     }
-    if (rng4) {
-        // units over the waves of workgroups >= 2
-        if (bid >= 2) eval_units(C, ctl, (int64_t)(bid - 2) * WAVES + wv, (int64_t)(nblk - 2) * WAVES, wtab[wv]);
+    if (nfw > 0) {
+        // units over the waves of workgroups past the filling ones
+        if (bid > nfw)
+            eval_units(C, ctl, (int64_t)(bid - 1 - nfw) * WAVES + wv, (int64_t)(nblk - 1 - nfw) * WAVES, wtab[wv]);
     } else if (bid != 0) {
-        // units over all waves of workgroups >= 1, except the rng wave
-        const int64_t wid = (int64_t)(bid - 1) * WAVES + wv - (rng_wg == 1 ? 1 : 0);
+        // units over all waves of workgroup 1, except the rng wave
+        const int64_t wid = (int64_t)(bid - 1) * WAVES + wv - 1;
         const int64_t nw = (int64_t)(nblk - 1) * WAVES - 1;
         if (!(bid == 1 && wv == 0)) eval_units(C, ctl, wid, nw, wtab[wv]);
     } else if (nblk == 1) {
@@ -1728,13 +1734,16 @@ __device__ __forceinline__ int64_t commit_units(const Cell& C, const Ctl* ctl, i
             const uint32_t sub = C.psub[(int64_t)bp * C.E + e];
             a = commit_word(C, T, base, sub, t);
             b = a + dv;
+            // the counts and (when kept) the coverer XORs read together,
+            // before any store: the arrays may alias as far as the compiler
+            // knows, so a read after a store would wait for its own round trip
             const uint16_t ca = cnt_get(C, a), cb = cnt_get(C, b);
+            const uint32_t xa = th ? xor_get(C, a) : 0u, xb = th ? xor_get(C, b) : 0u;
             cnt_put(C, a, (uint16_t)(ca - 1));
             cnt_put(C, b, (uint16_t)(cb + 1));
             if (th) {
                 // bi leaves a's coverers and joins b's; a word left with one
                 // coverer counts for it, b's former only coverer loses b
-                const uint32_t xa = xor_get(C, a), xb = xor_get(C, b);
                 xor_put(C, a, xa ^ (uint32_t)bi);
                 xor_put(C, b, xb ^ (uint32_t)bi);
                 if (ca == 2 || cb == 1) {
