@@ -157,6 +157,7 @@ struct Cell {
     int64_t ul_use;         // list length below which the evaluator reads the list, (R+1) E P / n
     int64_t pw[MAXN];
     uint32_t magic;         // ceil(2^32 / (q-1)): t / (q-1) = umulhi(t, magic)
+    uint64_t nmag, mmag;    // floor(2^64 / N), floor(2^64 / M) (0 for M = 1): see mod_magic
     // sphere walks by tables: pattern t of an R-subset = hi[t / wnl] + lo[t % wnl],
     // lo over the wa innermost positions of the subset (wnl = (q-1)^wa words),
     // hi over the other R - wa (wnh words); wok = 0 when they do not fit
@@ -245,6 +246,32 @@ __device__ __forceinline__ void bit_clr(uint32_t* plane, int64_t w)
 __host__ __device__ __forceinline__ uint32_t mulhi32(uint32_t a, uint32_t b)
 {
     return (uint32_t)(((uint64_t)a * (uint64_t)b) >> 32);
+}
+
+// high half of a 64 x 64-bit product
+__host__ __device__ __forceinline__ uint64_t mulhi64(uint64_t a, uint64_t b)
+{
+#ifdef __HIP_DEVICE_COMPILE__
+    return __umul64hi(a, b);
+#else
+    return (uint64_t)(((unsigned __int128)a * b) >> 64);
+#endif
+}
+
+// x mod d for a fixed d >= 2 with mag = floor(2^64 / d): the estimate
+// mulhi(x, mag) is floor(x / d) or one less (mag > 2^64/d - 1, so
+// x mag / 2^64 > x/d - 1), hence one conditional subtraction.  Exact for
+// every 64-bit x; a few multiplies instead of the 64-bit division routine.
+__host__ __device__ __forceinline__ uint64_t mod_magic(uint64_t x, uint64_t d, uint64_t mag)
+{
+    if (d == 1) return 0;
+    uint64_t r = x - mulhi64(x, mag) * d;
+    return r >= d ? r - d : r;
+}
+// floor(2^64 / d) for d >= 2, on the host (0 for d = 1)
+inline uint64_t magic64(uint64_t d)
+{
+    return d <= 1 ? 0 : (uint64_t)(((unsigned __int128)1 << 64) / d);
 }
 
 // nibble-wise "differs" mask: bit 4j set iff digit j of a and b differ

@@ -99,6 +99,8 @@ inline void plan_cell(const Shape& s, Cell& C, SubsetTables& t)
     }
     // t / (q-1) = umulhi(t, ceil(2^32 / (q-1))) for the pattern numbers t < (q-1)^R
     C.magic = (q - 1 == 1) ? 0u : (uint32_t)((((uint64_t)1 << 32) + (uint64_t)(q - 2)) / (uint64_t)(q - 1));
+    C.nmag = magic64((uint64_t)s.N);
+    C.mmag = magic64((uint64_t)s.M);
     {
         // sphere-walk tables: the innermost ceil(R/2) positions in lo
         const int a = (R + 1) / 2;

@@ -157,18 +157,19 @@ __device__ __forceinline__ int64_t wg_pick(const Cell& C, Ctl* ctl, NarrowSh& sh
     wg_ensure(C, ctl, 65);
     const int tid = threadIdx.x, wv = tid >> 6;
     const int64_t pos = ctl->ruse;
-    // stage 1: the draws
-    const uint64_t rv = tid < 65 ? ring_at(C, pos + tid) : 0;
-    if (tid == 64) sh.v64[1] = (int64_t)(rv % (uint64_t)C.N);
-    __syncthreads();
-    const int64_t start = sh.v64[1];
+    // stage 1: the draws, each reduced mod N where it is read: probe k by
+    // lane k of wave 0, the scan's start (draw 64) by every other thread
+    const uint64_t rv = ring_at(C, pos + (tid < 64 ? tid : 64));
+    const int64_t wdraw = (int64_t)mod_magic(rv, (uint64_t)C.N, C.nmag);
+    const int64_t start = tid < 64 ? 0 : wdraw;     // wave 0 reads it after the barrier below
+    if (tid == 64) sh.v64[1] = wdraw;
     const int64_t sw = start >> 5, sb = start >> BLK_SHIFT;
     const int64_t bw0 = sb << (BLK_SHIFT - 5);
     const int64_t bw1 = (bw0 + (BLK_WORDS / 32) < C.nzw) ? bw0 + (BLK_WORDS / 32) : C.nzw;
     // stage 2
     int64_t cand_s = INT64_MAX, cand_b = INT64_MAX;
     if (tid < 64) {
-        const int64_t w = (int64_t)(rv % (uint64_t)C.N);
+        const int64_t w = wdraw;
         const bool hit = zbit_acq(C, w) != 0;
         const uint64_t mask = __ballot(hit);
         const int fh = mask ? (__ffsll((unsigned long long)mask) - 1) : -1;
@@ -197,6 +198,7 @@ __device__ __forceinline__ int64_t wg_pick(const Cell& C, Ctl* ctl, NarrowSh& sh
     }
     const int fh = sh.v32[0];
     const int64_t wp = sh.v64[0];
+    const int64_t start_all = sh.v64[1];
     __syncthreads();
     if (fh >= 0) {
         if (tid == 0) ctl->ruse = pos + fh + 1;
@@ -212,7 +214,7 @@ __device__ __forceinline__ int64_t wg_pick(const Cell& C, Ctl* ctl, NarrowSh& sh
         const int64_t x1 = (x0 + (BLK_WORDS / 32) < C.nzw) ? x0 + (BLK_WORDS / 32) : C.nzw;
         return wg_first_bits(C, x0, x1, ~0u, ~0u, sh);
     }
-    return wg_first_from(C, start, sh);
+    return wg_first_from(C, start_all, sh);
 }
 
 // ------------------------------------------------------ uncovered list
@@ -323,6 +325,25 @@ __device__ __forceinline__ void wg_begin(const Cell& C, Ctl* ctl, NarrowSh& sh)
     }
     if (threadIdx.x == 0) ctl->it = it + 1;
 
+    // The scan below reads every codeword's digits; up to SCAN_ILP * WG
+    // codewords they are loaded now, SCAN_ILP per thread in flight (index
+    // clamped to the last codeword, so no load is guarded by a branch), and
+    // arrive while the list upkeep and the pick run.  The thread keeps them
+    // in registers from the scan to the slot emission.
+    constexpr int SCAN_ILP = 16;
+    const bool one_round = C.M <= SCAN_ILP * WG;
+    uint64_t c[SCAN_ILP] = {};
+    // (rounds past the last codeword are skipped: the bound is uniform)
+    auto load = [&](int i0) {
+#pragma unroll
+        for (int k = 0; k < SCAN_ILP; k++) {
+            if (i0 + k * WG >= C.M) break;
+            const int i = i0 + k * WG + threadIdx.x;
+            c[k] = C.cpk[i < C.M ? i : C.M - 1];
+        }
+    };
+    if (one_round) load(0);
+
     // uncovered-list upkeep (speed only: the list never changes a value); an
     // overflow since the last step (inserts beyond capacity) invalidates it
     const int64_t tl0 = clk();
@@ -383,23 +404,13 @@ __device__ __forceinline__ void wg_begin(const Cell& C, Ctl* ctl, NarrowSh& sh)
     // each (a ballot per wave, in LDS).  R+1 is the target distance unless no
     // codeword is that close; then a second pass marks those at the minimum
     // (:1342-1351).
-    constexpr int SCAN_ILP = 8;
     const int nwm = (C.M + 63) >> 6;
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     int64_t dmin = INT64_MAX;
-    // up to SCAN_ILP * WG codewords the thread keeps its words' digits in
-    // registers from the scan to the slot emission
-    uint64_t c[SCAN_ILP];
-    auto load = [&](int i0) {
-#pragma unroll
-        for (int k = 0; k < SCAN_ILP; k++) {
-            const int i = i0 + k * WG + threadIdx.x;
-            c[k] = i < C.M ? C.cpk[i] : 0;
-        }
-    };
     auto mark = [&](int i0, int target, bool with_min) {
 #pragma unroll
         for (int k = 0; k < SCAN_ILP; k++) {
+            if (i0 + k * WG >= C.M) break;
             const int i = i0 + k * WG + threadIdx.x;
             const int64_t d = nib_dist(c[k], upk);
             if (with_min && i < C.M && d < dmin) dmin = d;
@@ -408,7 +419,6 @@ __device__ __forceinline__ void wg_begin(const Cell& C, Ctl* ctl, NarrowSh& sh)
             if (lane == 0 && wi < nwm) sh.qm[wi] = m;
         }
     };
-    const bool one_round = C.M <= SCAN_ILP * WG;
     // codes above SCAN_DEEP codewords: SCAN_BIG loads in flight per thread
     // per round (the scan is a chain of load round trips on one workgroup);
     // below, rounds of SCAN_ILP cost fewer instructions
@@ -443,7 +453,6 @@ __device__ __forceinline__ void wg_begin(const Cell& C, Ctl* ctl, NarrowSh& sh)
             }
         }
     };
-    if (one_round) load(0);
     mark_pass(C.R + 1, true);
     __syncthreads();
     // a scan of the mask words' counts gives each word its first slot
@@ -481,20 +490,53 @@ __device__ __forceinline__ void wg_begin(const Cell& C, Ctl* ctl, NarrowSh& sh)
         }
     }
     // codeword i takes slot first(i / 64) + its rank in its mask word: index order
+    // (the slot's differing positions, increasing, packed in two words in
+    // registers and stored at once: byte t of the slot's MAXN bytes is the
+    // t-th position where the codeword differs from u; the bytes past the
+    // count are never read)
     auto emit = [&](int i, uint64_t mine) {
         const int64_t slot = sh.qfirst[i >> 6] + __popcll(sh.qm[i >> 6] & ((1ull << (i & 63)) - 1));
         const uint64_t ne = nib_ne(mine, upk);
         C.slot_cw[slot] = i;
+        uint64_t lo = 0, hi = 0;
         int t = 0;
-        for (int p = 0; p < C.n; p++)
-            if ((ne >> (4 * p)) & 1) C.slot_pos[slot * MAXN + t++] = (uint8_t)p;
+#pragma unroll
+        for (int p = 0; p < MAXN; p++) {
+            const bool d = (ne >> (4 * p)) & 1;
+            const uint64_t b = (uint64_t)p << (8 * (t & 7));
+            if (d && t < 8) lo |= b;
+            if (d && t >= 8) hi |= b;
+            t += d;
+        }
+        uint64_t* dst = reinterpret_cast<uint64_t*>(C.slot_pos + slot * MAXN);
+        dst[0] = lo;
+        dst[1] = hi;
     };
     if (one_round) {
-        // the thread that marked a codeword emits it, its digits still in registers
+        // each wave, round by round, emits the marked codewords of its mask
+        // word one after another (the loop is uniform): the digits come from
+        // the lane that holds them, and lane p < n writes position p at its
+        // rank among the differing positions
 #pragma unroll
         for (int k = 0; k < SCAN_ILP; k++) {
-            const int i = k * WG + threadIdx.x;
-            if (i < C.M && ((sh.qm[i >> 6] >> (i & 63)) & 1)) emit(i, c[k]);
+            if (k * WG >= C.M) break;
+            const int wi = k * WAVES + wv;          // codewords k WG + 64 wv + lane
+            if (wi < nwm) {
+                uint64_t m = sh.qm[wi];
+                int64_t slot = sh.qfirst[wi];
+                while (m) {
+                    const int l = __ffsll((unsigned long long)m) - 1;
+                    m &= m - 1;
+                    const uint64_t ci = __shfl((unsigned long long)c[k], l, 64);
+                    const uint64_t ne = nib_ne(ci, upk);
+                    if (lane < MAXN && ((ne >> (4 * lane)) & 1)) {
+                        const int rank = __popcll(ne & ((1ull << (4 * lane)) - 1));
+                        C.slot_pos[slot * MAXN + rank] = (uint8_t)lane;
+                    }
+                    if (lane == 0) C.slot_cw[slot] = k * WG + 64 * wv + l;
+                    slot++;
+                }
+            }
         }
     } else {
         // each thread lists the set bits of the QPT mask words it counted at

@@ -119,6 +119,24 @@ int main(int argc, char** argv)
     }
     if (C.wok && (int64_t)C.wnl * C.wnh != C.P) FAIL("walk tables %d x %d for P = %lld", C.wnl, C.wnh, (long long)C.P);
 
+    // the draws' reduction mod N and mod M by multiply-high: edge operands
+    // and random ones against the division, for this cell's N and a spread
+    // of code sizes
+    {
+        const uint64_t ds[] = {(uint64_t)C.N, 2, 3, 7, 65535, 65536, 0xFFFFFFFFull, 0x100000001ull,
+                               (uint64_t)4e10, (uint64_t)(sm64(rs) % 65535) + 1, (uint64_t)(sm64(rs) % (uint64_t)C.N) + 1};
+        for (const uint64_t d : ds) {
+            const uint64_t mag = magic64(d);
+            const uint64_t xs[] = {0, 1, d - 1, d, d + 1, 2 * d - 1, ~0ull, ~0ull - 1, ~0ull / d * d, ~0ull / d * d - 1};
+            for (const uint64_t x : xs)
+                if (mod_magic(x, d, mag) != x % d) FAIL("mod_magic(%llu, %llu)", (unsigned long long)x, (unsigned long long)d);
+            for (int i = 0; i < 200000; i++) {
+                const uint64_t x = sm64(rs);
+                if (mod_magic(x, d, mag) != x % d) FAIL("mod_magic(%llu, %llu)", (unsigned long long)x, (unsigned long long)d);
+            }
+        }
+    }
+
     // pack_index on the extremes and random words
     for (int i = 0; i < 1000; i++) {
         const int64_t w = i == 0 ? 0 : i == 1 ? C.N - 1 : (int64_t)(sm64(rs) % (uint64_t)C.N);
