@@ -53,11 +53,11 @@ def build(verbose=False):
     if not usage:
         raise RuntimeError('hipcc reported no kernel resource usage')
     with open(RESOURCES, 'w') as fh:
-        fh.write('%-56s %6s %6s %8s %8s %6s\n' % ('kernel', 'VGPRs', 'SGPRs', 'scratch', 'LDS', 'occ'))
+        fh.write('%-56s %6s %6s %6s %8s %8s %6s\n' % ('kernel', 'VGPRs', 'SGPRs', 'sspill', 'scratch', 'LDS', 'occ'))
         for name, u in sorted(usage.items()):
-            fh.write('%-56s %6d %6d %8d %8d %6d\n' % (name[:56], u.get('VGPRs', 0), u.get('TotalSGPRs', 0),
-                                                     u.get('ScratchSize', 0), u.get('LDS', 0),
-                                                     u.get('Occupancy', 0)))
+            fh.write('%-56s %6d %6d %6d %8d %8d %6d\n' % (name[:56], u.get('VGPRs', 0), u.get('TotalSGPRs', 0),
+                                                          u.get('SGPRsSpill', 0), u.get('ScratchSize', 0),
+                                                          u.get('LDS', 0), u.get('Occupancy', 0)))
     private = sorted(k for k, u in usage.items() if u.get('ScratchSize', 0) or u.get('VGPRsSpill', 0))
     if private:
         os.unlink(tmp)
@@ -67,7 +67,7 @@ def build(verbose=False):
 
 
 _FIELDS = {'TotalSGPRs': 'TotalSGPRs', 'VGPRs': 'VGPRs', 'ScratchSize [bytes/lane]': 'ScratchSize',
-           'Occupancy [waves/SIMD]': 'Occupancy', 'VGPRs Spill': 'VGPRsSpill',
+           'Occupancy [waves/SIMD]': 'Occupancy', 'VGPRs Spill': 'VGPRsSpill', 'SGPRs Spill': 'SGPRsSpill',
            'LDS Size [bytes/block]': 'LDS'}
 
 

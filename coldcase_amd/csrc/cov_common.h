@@ -82,6 +82,8 @@ struct alignas(16) Ctl {
     int32_t nstale;         // codewords whose th rows the next evaluator recounts
     int32_t stale[STALE_MAX];
     int32_t ndirty;         // codewords logged in dirty since best was last made equal to code
+    int64_t stale_idx[STALE_MAX];   // their word indices and packed digits after the step,
+    uint64_t stale_pk[STALE_MAX];   // so the evaluator's recount reads them with the block
     // wall_clock64 ticks spent per phase (diagnostics): 0 select, 1 post,
     // 2 begin, 3 pick, 4 list upkeep, 5 shuffle helper, 6 rng helper, 7 kick,
     // 8 previous begin's end -> selection start (the evaluator's span),
@@ -268,6 +270,21 @@ __host__ __device__ __forceinline__ uint64_t mod_magic(uint64_t x, uint64_t d, u
     uint64_t r = x - mulhi64(x, mag) * d;
     return r >= d ? r - d : r;
 }
+// x mod d for 1 <= d <= 65536 in 32-bit steps: with x = hi 2^32 + lo,
+// x mod d = ((hi mod d)(2^32 mod d) + lo mod d) mod d, every term below
+// d (d - 1) < 2^32, and the first three remainders independent of each other
+__host__ __device__ __forceinline__ uint32_t mod_u16(uint64_t x, uint32_t d)
+{
+    const uint32_t hi = (uint32_t)(x >> 32), lo = (uint32_t)x;
+    const uint32_t p32 = (0xFFFFFFFFu % d + 1u) % d;
+    return ((hi % d) * p32 + lo % d) % d;
+}
+// x mod d for any d >= 1 (the short form when d allows)
+__host__ __device__ __forceinline__ uint64_t mod_any(uint64_t x, uint64_t d)
+{
+    return d <= 65536 ? (uint64_t)mod_u16(x, (uint32_t)d) : x % d;
+}
+
 // floor(2^64 / d) for d >= 2, on the host (0 for d = 1)
 inline uint64_t magic64(uint64_t d)
 {
@@ -365,6 +382,22 @@ __host__ __device__ __forceinline__ int64_t pattern_word(const Cell& C, int64_t 
     }
     if (wpk) *wpk = p2;
     return a;
+}
+
+// the packed digits alone of pattern t (pattern_word's second result, with
+// no access to the powers)
+__host__ __device__ __forceinline__ uint64_t pattern_digits(const Cell& C, uint64_t pk, uint32_t sub, int r, uint32_t t)
+{
+    uint64_t p2 = pk;
+    for (int i = r - 1; i >= 0; i--) {
+        const uint32_t tq = div_q1(C, t);
+        const int k = (int)(t - tq * (uint32_t)C.q1);
+        t = tq;
+        const int j = (sub >> (4 * i)) & 15;
+        const int c = nib_get(pk, j);
+        p2 = nib_put(p2, j, k + (k >= c));
+    }
+    return p2;
 }
 
 // Offsets of the patterns of the R-subset `sub` around the word with digits
@@ -536,19 +569,20 @@ inline void rng_jump_tables(uint64_t* out)
     }
 }
 
-// The fill decided by begin (ctl->fill_on): RNG_CHUNK draws from fill_g0
+// The fill decided by begin (fill_on): RNG_CHUNK draws from fill_g0
 // (state fill_s), part w the RNG_PART draws from fill_g0 + RNG_PART w, its
 // state J_w fill_s (lane l computes bits l, l+64, l+128, l+192 of it, one
 // ballot per word).  Waves wi = 0 .. nwaves-1 of the filling workgroups take
 // parts wi, wi + nwaves, ...; the wave with the last part stores the state
 // after the chunk.  Nothing else reads or writes rgen or rst while the
-// evaluator runs.  All threads of a filling workgroup.
-__device__ inline void ring_fill_parts(const Cell& C, Ctl* ctl, int wi, int nwaves)
+// evaluator runs.  The decision is read from rd, the state stored to ctl.
+// All threads of a filling workgroup.
+__device__ inline void ring_fill_parts(const Cell& C, const Ctl* rd, Ctl* ctl, int wi, int nwaves)
 {
     const int lane = threadIdx.x & 63;
-    const int64_t g0 = (int64_t)uniform64((uint64_t)ctl->fill_g0);
-    const uint64_t s0[4] = {uniform64(ctl->fill_s[0]), uniform64(ctl->fill_s[1]), uniform64(ctl->fill_s[2]),
-                            uniform64(ctl->fill_s[3])};
+    const int64_t g0 = (int64_t)uniform64((uint64_t)rd->fill_g0);
+    const uint64_t s0[4] = {uniform64(rd->fill_s[0]), uniform64(rd->fill_s[1]), uniform64(rd->fill_s[2]),
+                            uniform64(rd->fill_s[3])};
     for (int w = wi; w < RNG_SPLIT; w += nwaves) {
         uint64_t s[4] = {s0[0], s0[1], s0[2], s0[3]};
         if (w > 0) {
@@ -580,6 +614,13 @@ __device__ inline void ring_fill_parts(const Cell& C, Ctl* ctl, int wi, int nwav
 }
 
 __device__ __forceinline__ int64_t clk() { return (int64_t)wall_clock64(); }
+
+// a phase timer of the control block in global memory, added without a read
+// (the wide kernels; the one-workgroup phases add to their LDS copy)
+__device__ __forceinline__ void tph_add(Ctl* ctl, int k, int64_t ticks)
+{
+    atomicAdd(reinterpret_cast<unsigned long long*>(&ctl->tph[k]), (unsigned long long)ticks);
+}
 
 __device__ __forceinline__ uint64_t ring_at(const Cell& C, int64_t pos)
 {

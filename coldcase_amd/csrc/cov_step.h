@@ -648,6 +648,12 @@ __device__ __forceinline__ int64_t sphere_count_k(const Cell& C, const uint32_t*
 __device__ __forceinline__ int64_t sphere_count(const Cell& C, const uint32_t* __restrict__ plane, int64_t base,
                                                 uint64_t pk, uint32_t sub, WalkTab& tb)
 {
+    // the unit's subset, centre digits and word are the wave's: made uniform,
+    // the table build reads the cell's powers q^(n-1-j) with scalar loads
+    // (a per-lane index into the kernel argument is a vector memory load)
+    sub = __builtin_amdgcn_readfirstlane(sub);
+    pk = uniform64(pk);
+    base = (int64_t)uniform64((uint64_t)base);
     const bool tab = C.wok != 0;
     if (tab) walk_tables(C, pk, sub, tb);
     const int64_t c = C.P > 256 ? sphere_count_k<16>(C, plane, base, pk, sub, tb, tab)
@@ -673,12 +679,13 @@ __device__ __forceinline__ void unit_leave(const Cell& C, int slot, int f, WalkT
     }
 }
 
-// the leaving counts of moved codeword i, subset f (its th row starts at zero)
-__device__ __forceinline__ void unit_stale(const Cell& C, int i, int f, WalkTab& tb)
+// the leaving counts of moved codeword i (now at word idx, digits pk),
+// subset f (its th row starts at zero)
+__device__ __forceinline__ void unit_stale(const Cell& C, int i, int64_t idx, uint64_t pk, int f, WalkTab& tb)
 {
     const int lane = threadIdx.x & 63;
     const uint32_t sub = C.fsub[f];
-    const int64_t c1 = sphere_count(C, C.ob, C.cidx[i], C.cpk[i], sub, tb);
+    const int64_t c1 = sphere_count(C, C.ob, idx, pk, sub, tb);
     if (lane == 0 && c1) {
         int32_t* t = C.th + (int64_t)i * THW;
         atomicAdd(&t[0], (int)c1);
@@ -780,6 +787,20 @@ __device__ __forceinline__ void unit_list(const Cell& C, int slot, int64_t e0, i
     }
 }
 
+// quotient and remainder, in 32-bit arithmetic when both fit (the unit
+// numbers of every workload do; the 64-bit division is a long routine)
+__device__ __forceinline__ int64_t div_small(int64_t a, int64_t b, int64_t* r)
+{
+    if (((uint64_t)a | (uint64_t)b) >> 32 == 0) {
+        const uint32_t q = (uint32_t)a / (uint32_t)b;
+        *r = (int64_t)((uint32_t)a - q * (uint32_t)b);
+        return q;
+    }
+    const int64_t q = a / b;
+    *r = a - q * b;
+    return q;
+}
+
 // run this wave's share of the step's evaluator units
 __device__ __forceinline__ void eval_units(const Cell& C, const Ctl* ctl, int64_t wid, int64_t nw, WalkTab& tb)
 {
@@ -791,23 +812,23 @@ __device__ __forceinline__ void eval_units(const Cell& C, const Ctl* ctl, int64_
     const int th_on = ctl->th_on;
     for (int64_t g = wid; g < nl + ne; g += nw) {
         if (g < nl) {
-            const int slot = (int)(g / C.F);
-            if (th_on) unit_stale(C, ctl->stale[slot], (int)(g - (int64_t)slot * C.F), tb);
-            else unit_leave(C, slot, (int)(g - (int64_t)slot * C.F), tb);
+            int64_t f;
+            const int slot = (int)div_small(g, C.F, &f);
+            if (th_on)
+                unit_stale(C, ctl->stale[slot], ctl->stale_idx[slot], ctl->stale_pk[slot], (int)f, tb);
+            else unit_leave(C, slot, (int)f, tb);
         } else if (use_list) {
-            const int64_t h = g - nl;
-            const int slot = (int)(h / nchunk);
-            const int64_t ch = h - (int64_t)slot * nchunk;
+            int64_t ch;
+            const int slot = (int)div_small(g - nl, nchunk, &ch);
             const int64_t e0 = ch * LIST_CHUNK;
             const int64_t e1 = (e0 + LIST_CHUNK < ul_n) ? e0 + LIST_CHUNK : ul_n;
             if (wnow) unit_list_wide(C, slot, e0, e1, ul_n);
             else unit_list(C, slot, e0, e1, upk, ul_n);
         } else {
-            const int64_t h = g - nl;
-            const int slot = (int)(h / per_slot_walk);
-            const int64_t r = h - (int64_t)slot * per_slot_walk;
-            const int t = (int)(r / C.E);
-            const int e = (int)(r - (int64_t)t * C.E);
+            int64_t r, e64;
+            const int slot = (int)div_small(g - nl, per_slot_walk, &r);
+            const int t = (int)div_small(r, C.E, &e64);
+            const int e = (int)e64;
             const int p = C.slot_pos[(int64_t)slot * MAXN + t];
             unit_walk(C, slot, p, nib_get(upk, p), e, tb);
         }
@@ -840,15 +861,18 @@ struct ShufSh {
 // partner value (a count, a prefix sum and a scatter in LDS), and each draw
 // looks through its group, a few entries; with more candidates the (draw,
 // chunk of 32 earlier draws) pairs are scanned with LDS maxima.
-__device__ __forceinline__ void wg_shuffle(const Cell& C, const Ctl* ctl, ShufSh& sh)
+// Returns the calling thread's own entry C.shuf[tid] when the parallel
+// forms ran (tid < take), else ~0u.
+__device__ __forceinline__ uint32_t wg_shuffle(const Cell& C, const Ctl* ctl, ShufSh& sh)
 {
+    uint32_t mine = ~0u;
     const int tid = threadIdx.x;
     const int64_t ncand = ctl->ncand, take = ctl->take, pos0 = ctl->sh_pos;
     if (ncand <= SHUF_PAR) {
         const int nc = (int)ncand, tk = (int)take;
         for (int v = tid; v <= nc; v += WG) sh.off[v] = 0;
         for (int s = tid; s < tk; s += WG)
-            sh.j[s] = (uint32_t)(s + (int)(ring_at(C, pos0 + s) % (uint64_t)(nc - s)));
+            sh.j[s] = (uint32_t)(s + (int)mod_u16(ring_at(C, pos0 + s), (uint32_t)(nc - s)));
         __syncthreads();
         for (int s = tid; s < tk; s += WG) atomicAdd(&sh.off[sh.j[s]], 1);
         __syncthreads();
@@ -925,12 +949,13 @@ __device__ __forceinline__ void wg_shuffle(const Cell& C, const Ctl* ctl, ShufSh
                 out = (uint32_t)t;
             }
             C.shuf[s] = out;
+            if (s == tid) mine = out;
         }
-        return;
+        return mine;
     }
     if (take <= SHUF_PAR) {
         for (int64_t s = tid; s < take; s += WG) {
-            sh.j[s] = (uint32_t)(s + (int64_t)(ring_at(C, pos0 + s) % (uint64_t)(ncand - s)));
+            sh.j[s] = (uint32_t)(s + (int64_t)mod_any(ring_at(C, pos0 + s), (uint64_t)(ncand - s)));
             sh.g[s] = -1;
             sh.f[s] = -1;
         }
@@ -971,8 +996,9 @@ __device__ __forceinline__ void wg_shuffle(const Cell& C, const Ctl* ctl, ShufSh
                 out = (uint32_t)t;
             }
             C.shuf[s] = out;
+            if (s == tid) mine = out;
         }
-        return;
+        return mine;
     }
     // serial path for large samples (thread 0, the candidate array in HBM)
     uint32_t* a = C.shuf;
@@ -994,6 +1020,7 @@ __device__ __forceinline__ void wg_shuffle(const Cell& C, const Ctl* ctl, ShufSh
         }
         __syncthreads();
     }
+    return mine;
 }
 
 // Candidate k of the unshuffled order: its slot (candidate codeword),
@@ -1031,14 +1058,16 @@ __device__ __forceinline__ int32_t cand_enter(const Cell& C, const Ctl* ctl, int
 // of the first take <= SEL_WAVE candidates in shuffled order, its codeword,
 // slot, position, new and old digit, tabu stamp and the codeword's word
 // index.  Nothing the evaluator runs changes them.  All threads of the group.
-__device__ __forceinline__ void wg_stage(const Cell& C, const Ctl* ctl)
+__device__ __forceinline__ void wg_stage(const Cell& C, const Ctl* ctl, uint32_t mine)
 {
     const int take = ctl->take;
     if (take > SEL_WAVE) return;
-    __syncthreads();    // C.shuf was written by this workgroup
+    // take <= SEL_WAVE = WG <= SHUF_PAR: the shuffle ran a parallel form, in
+    // which this thread made entry s = tid itself and returned it
+    static_assert(SEL_WAVE <= WG && SEL_WAVE <= SHUF_PAR, "the staged entries come from the shuffle's own threads");
     const int s = threadIdx.x;
     if (s < take) {
-        const uint32_t k = C.shuf[s];
+        const uint32_t k = mine;
         int slot, p, v;
         cand_move(C, ctl, k, &slot, &p, &v);
         const int i = C.slot_cw[slot];
@@ -1216,14 +1245,14 @@ __device__ __forceinline__ void wave_select(const Cell& C, Ctl* ctl, SelSh& sh)
             if (nb[k]) tb = Tk[k];
             const int64_t Tb = tb < 0 ? 0 : tb;
             bool ev = nb[k];
-            if (tie[k]) ev = (sh.rv[Tk[k] - 1] % (uint64_t)(1 + Tk[k] - Tb)) == 0;
+            if (tie[k]) ev = mod_any(sh.rv[Tk[k] - 1], (uint64_t)(1 + Tk[k] - Tb)) == 0;
             if (ev) wl = (int64_t)SEL_PER * lane + k;
         }
         const int64_t winner = wave_max64(wl);
         const int64_t Tall = __shfl(tincl, 63, 64);
         int64_t Tball = wave_max64(mk);
         if (Tball < 0) Tball = 0;
-        const int64_t ssel = winner >= 0 ? winner : (int64_t)(sh.rv[0] % (uint64_t)take);
+        const int64_t ssel = winner >= 0 ? winner : (int64_t)mod_any(sh.rv[0], (uint64_t)take);
 #pragma unroll
         for (int k = 0; k < SEL_PER; k++) {
             if ((int64_t)SEL_PER * lane + k == ssel) {
@@ -1302,7 +1331,7 @@ __device__ __forceinline__ void wg_select(const Cell& C, Ctl* ctl, SelSh& sh)
         bool ev = nb;
         if (tie) {
             const uint64_t rv = ring_at(C, r0 + T - 1);
-            ev = (rv % (uint64_t)(1 + T - Tb)) == 0;
+            ev = mod_any(rv, (uint64_t)(1 + T - Tb)) == 0;
         }
         const int64_t wch = wg_max(ev ? s : -1, sh.red);
         if (wch > winner) winner = wch;
@@ -1319,7 +1348,7 @@ __device__ __forceinline__ void wg_select(const Cell& C, Ctl* ctl, SelSh& sh)
         int nties;
         int64_t s;
         if (winner < 0) {
-            s = (int64_t)(ring_at(C, r0) % (uint64_t)take);
+            s = (int64_t)mod_any(ring_at(C, r0), (uint64_t)take);
             draws = 1;
             dsel = DSEL_RANDOM;
             nties = 0;
@@ -1412,13 +1441,18 @@ __device__ __forceinline__ bool wg_arrive_last(int32_t* sub, int32_t* top, int* 
 // The evaluator of one cell on nblk workgroups; this one is number bid.
 __device__ __forceinline__ void eval_body(const Cell& C, int bid, int nblk)
 {
+    // the control block is read from an LDS copy, made with one round trip
+    // for all its fields (the units' and the shuffle's inputs alike), and
+    // written in global memory; the phase timers are added without reading
     Ctl* ctl = C.ctl;
+    __shared__ Ctl lc;
     __shared__ ShufSh shs;
     __shared__ WalkTab wtab[WAVES];
-    if (!ctl->begun || ctl->done || ctl->selected) return;
-    const int wv = threadIdx.x >> 6;
     const int64_t th0 = clk();
-    if (bid == 0 && threadIdx.x == 0 && ctl->t_cme) ctl->tph[10] += th0 - ctl->t_cme;
+    ctl_to_lds(&lc, ctl);
+    if (!lc.begun || lc.done || lc.selected) return;
+    const int wv = threadIdx.x >> 6;
+    if (bid == 0 && threadIdx.x == 0 && lc.t_cme) tph_add(ctl, 10, th0 - lc.t_cme);
     // workgroup 0: the shuffle.  The rng refill, keeping ~4096 draws ahead:
     // on three or more groups a chunk of RNG_CHUNK draws in parts by the
     // waves of workgroups 1 .. nfw (when begin decided on one), else wave 0
@@ -1427,32 +1461,36 @@ __device__ __forceinline__ void eval_body(const Cell& C, int bid, int nblk)
     const int nfw = nblk >= 3 ? (nblk - 2 < RNG_WGS ? nblk - 2 : RNG_WGS) : 0;
     const int rng_wg = nblk > 1 ? 1 : 0;
     if (bid == 0) {
-        wg_shuffle(C, ctl, shs);
-        wg_stage(C, ctl);
-        if (threadIdx.x == 0) ctl->tph[5] += clk() - th0;
+        const uint32_t mine = wg_shuffle(C, &lc, shs);
+        wg_stage(C, &lc, mine);
+        if (threadIdx.x == 0) tph_add(ctl, 5, clk() - th0);
     }
     if (nfw > 0 && bid >= 1 && bid <= nfw) {
-        if (ctl->fill_on) {
-            ring_fill_parts(C, ctl, (bid - 1) * WAVES + wv, nfw * WAVES);
+        if (lc.fill_on) {
+            ring_fill_parts(C, &lc, ctl, (bid - 1) * WAVES + wv, nfw * WAVES);
         } else if (bid == 1 && wv == 0) {
-            if (ctl->rgen < ctl->ruse + 4096) wave_ring_fill(C, ctl, ctl->ruse + 4096, ctl->budget);
+            if (lc.rgen < lc.ruse + 4096) wave_ring_fill(C, ctl, lc.ruse + 4096, lc.budget);
         }
-        if (bid == 1 && threadIdx.x == 0) ctl->tph[6] += clk() - th0;
+        if (bid == 1 && threadIdx.x == 0) tph_add(ctl, 6, clk() - th0);
     } else if (nfw == 0 && bid == rng_wg && wv == 0) {
-        if (ctl->rgen < ctl->ruse + 4096) wave_ring_fill(C, ctl, ctl->ruse + 4096, ctl->budget);
-        if (threadIdx.x == 0) ctl->tph[6] += clk() - th0;
+        if (lc.rgen < lc.ruse + 4096) wave_ring_fill(C, ctl, lc.ruse + 4096, lc.budget);
+        if (threadIdx.x == 0) tph_add(ctl, 6, clk() - th0);
     }
     if (nfw > 0) {
         // units over the waves of workgroups past the filling ones
+        // (wave-major numbering: the units go to wave 0 of every workgroup
+        // first, spreading them over the CUs: a unit's scattered loads are
+        // bound by its CU's rate of cache-line requests)
         if (bid > nfw)
-            eval_units(C, ctl, (int64_t)(bid - 1 - nfw) * WAVES + wv, (int64_t)(nblk - 1 - nfw) * WAVES, wtab[wv]);
+            eval_units(C, &lc, (int64_t)(bid - 1 - nfw) + (int64_t)wv * (nblk - 1 - nfw),
+                       (int64_t)(nblk - 1 - nfw) * WAVES, wtab[wv]);
     } else if (bid != 0) {
         // units over all waves of workgroup 1, except the rng wave
         const int64_t wid = (int64_t)(bid - 1) * WAVES + wv - 1;
         const int64_t nw = (int64_t)(nblk - 1) * WAVES - 1;
-        if (!(bid == 1 && wv == 0)) eval_units(C, ctl, wid, nw, wtab[wv]);
+        if (!(bid == 1 && wv == 0)) eval_units(C, &lc, wid, nw, wtab[wv]);
     } else if (nblk == 1) {
-        eval_units(C, ctl, wv, WAVES, wtab[wv]);
+        eval_units(C, &lc, wv, WAVES, wtab[wv]);
     }
 }
 
@@ -1463,10 +1501,10 @@ __device__ __forceinline__ void select_body(const Cell& C)
 {
     Ctl* ctl = C.ctl;
     __shared__ SelSh sel;
-    if (!ctl->begun || ctl->done || ctl->selected) return;
-    const int64_t ts0 = clk();
     __shared__ Ctl lc;
+    const int64_t ts0 = clk();
     ctl_to_lds(&lc, ctl);
+    if (!lc.begun || lc.done || lc.selected) return;
     if (threadIdx.x == 0 && lc.t_cme) lc.tph[8] += ts0 - lc.t_cme;
     wg_select(C, &lc, sel);
     if (threadIdx.x == 0) {
@@ -1506,13 +1544,20 @@ __device__ __forceinline__ void wg_ball(const Cell& C, Ctl* ctl, int i, int delt
     __syncthreads();
 }
 
-// codeword i moved: its leaving counts are recounted by the next evaluator
-// (thread 0; the list holds the move and the kicks, STALE_MAX at most)
-__device__ __forceinline__ void stale_add(Ctl* ctl, int i)
+// codeword i moved to word idx, digits pk: its leaving counts are recounted
+// by the next evaluator (thread 0; the list holds the move and the kicks,
+// STALE_MAX at most; a codeword moved twice keeps its last place)
+__device__ __forceinline__ void stale_add(Ctl* ctl, int i, int64_t idx, uint64_t pk)
 {
-    for (int k = 0; k < ctl->nstale; k++)
-        if (ctl->stale[k] == i) return;
-    if (ctl->nstale < STALE_MAX) ctl->stale[ctl->nstale++] = i;
+    int k = 0;
+    while (k < ctl->nstale && ctl->stale[k] != i) k++;
+    if (k == ctl->nstale) {
+        if (k == STALE_MAX) return;
+        ctl->nstale = k + 1;
+    }
+    ctl->stale[k] = i;
+    ctl->stale_idx[k] = idx;
+    ctl->stale_pk[k] = pk;
 }
 
 // codeword i changed: log it for the next best copy (thread 0)
@@ -1564,7 +1609,7 @@ __device__ __forceinline__ void wg_kick(const Cell& C, Ctl* ctl, NarrowSh& sh)
             C.cpk[i] = pk;
             C.cidx[i] = idx;
             log_dirty(C, ctl, i);
-            if (ctl->th_on) stale_add(ctl, i);
+            if (ctl->th_on) stale_add(ctl, i, idx, pk);
         }
         __syncthreads();
         wg_ball(C, ctl, i, +1, sh);
@@ -1588,13 +1633,16 @@ __device__ __forceinline__ void wg_post(const Cell& C, Ctl* ctl, NarrowSh& sh)
         ctl->dunc = 0;
         ctl->unc += dunc;
         const int bi = ctl->bi, bp = ctl->bp, bv = ctl->bv, oldv = ctl->oldv;
+        // (bpk and base are codeword bi's digits and word index before the move)
+        const uint64_t npk = nib_put(ctl->bpk, bp, bv);
+        const int64_t nidx = ctl->base + ctl->dv;
         C.code[(int64_t)bi * n + bp] = (uint8_t)bv;
-        C.cpk[bi] = nib_put(C.cpk[bi], bp, bv);
-        C.cidx[bi] += ctl->dv;
+        C.cpk[bi] = npk;
+        C.cidx[bi] = nidx;
         C.tabu[((int64_t)bi * n + bp) * q + oldv] = ctl->stamp;
         log_dirty(C, ctl, bi);
         ctl->nstale = 0;
-        if (ctl->th_on) stale_add(ctl, bi);
+        if (ctl->th_on) stale_add(ctl, bi, nidx, npk);
         if (ctl->ul_ovf) { ctl->ul_valid = 0; ctl->ul_ovf = 0; ctl->ul_n = 0; }
         ctl->commit_bytes += (ctl->th_on ? 16 : 8) * (int64_t)C.E * C.P;
         sh.v64[3] = dunc;
@@ -1771,8 +1819,9 @@ __device__ __forceinline__ int64_t commit_units(const Cell& C, const Ctl* ctl, i
         uint32_t za = 0, oa_set = 0, oa_clr = 0, zb_clr = 0, ob_set = 0, ob_clr = 0;
         int da = 0, db = 0;
         if (act) {
-            const int64_t e = g / C.P;
-            const uint32_t t = (uint32_t)(g - e * C.P);
+            int64_t t64;
+            const int64_t e = div_small(g, C.P, &t64);
+            const uint32_t t = (uint32_t)t64;
             const uint32_t sub = C.psub[(int64_t)bp * C.E + e];
             a = commit_word(C, T, base, sub, t);
             b = a + dv;
@@ -1789,8 +1838,7 @@ __device__ __forceinline__ int64_t commit_units(const Cell& C, const Ctl* ctl, i
                 xor_put(C, a, xa ^ (uint32_t)bi);
                 xor_put(C, b, xb ^ (uint32_t)bi);
                 if (ca == 2 || cb == 1) {
-                    uint64_t apk;
-                    (void)pattern_word(C, base, pk, sub, C.R, t, &apk);
+                    const uint64_t apk = pattern_digits(C, pk, sub, C.R, t);
                     if (ca == 2) th_event(C, (int)(xa ^ (uint32_t)bi), apk, +1);
                     if (cb == 1) th_event(C, (int)xb, nib_put(apk, bp, bv), -1);
                 }
@@ -1798,11 +1846,7 @@ __device__ __forceinline__ int64_t commit_units(const Cell& C, const Ctl* ctl, i
             const uint32_t ma = 1u << (a & 31), mb = 1u << (b & 31);
             if (ca == 1) {
                 za = ma; oa_clr = ma; da = 1;
-                if (list_on) {
-                    uint64_t apk;
-                    (void)pattern_word(C, base, pk, sub, C.R, t, &apk);
-                    list_insert(C, C.ctl, a, apk);
-                }
+                if (list_on) list_insert(C, C.ctl, a, pattern_digits(C, pk, sub, C.R, t));
                 d++;
             } else if (ca == 2) {
                 oa_set = ma;
@@ -1864,14 +1908,19 @@ __device__ __forceinline__ int64_t commit_units(const Cell& C, const Ctl* ctl, i
 // it, so these workgroups end without a device-scope release of their own.
 __device__ __forceinline__ void commit_body(const Cell& C, int bid, int nblk)
 {
+    // the control block read from an LDS copy (one round trip), written in
+    // global memory (the uncovered delta, the list's inserts, the timers)
     Ctl* ctl = C.ctl;
+    __shared__ Ctl lc;
     __shared__ WgRed red;
-    if (!ctl->begun || ctl->done || !ctl->selected) return;
-    if (bid == 0 && threadIdx.x == 0 && ctl->t_eve) ctl->tph[11] += clk() - ctl->t_eve;
+    const int64_t tc0 = clk();
+    ctl_to_lds(&lc, ctl);
+    if (!lc.begun || lc.done || !lc.selected) return;
+    if (bid == 0 && threadIdx.x == 0 && lc.t_eve) tph_add(ctl, 11, tc0 - lc.t_eve);
     const int64_t gt = (int64_t)bid * WG + threadIdx.x;
     __shared__ int64_t ctab[MAXN * 16];
-    commit_table(C, ctl->bpk, ctab);
-    const int64_t d = commit_units(C, ctl, gt, (int64_t)nblk * WG, ctab, ctl->th_on != 0);
+    commit_table(C, lc.bpk, ctab);
+    const int64_t d = commit_units(C, &lc, gt, (int64_t)nblk * WG, ctab, lc.th_on != 0);
     const int64_t ds = wg_sum(d, red);
     if (threadIdx.x == 0 && ds) atomicAdd(reinterpret_cast<unsigned long long*>(&ctl->dunc), (unsigned long long)ds);
 }

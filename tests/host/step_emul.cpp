@@ -137,6 +137,21 @@ int main(int argc, char** argv)
         }
     }
 
+    // the short remainders: every divisor 1 .. 65536 on edge and random
+    // operands, and the fallback just past it
+    for (uint32_t d = 1; d <= 65536; d += (d < 1100 ? 1 : 97)) {
+        const uint64_t xs[] = {0, 1, d - 1u, d, ~0ull, ~0ull - 1, (uint64_t)d << 32, ((uint64_t)d << 32) - 1, sm64(rs), sm64(rs)};
+        for (const uint64_t x : xs) {
+            if (mod_u16(x, d) != x % d) FAIL("mod_u16(%llu, %u)", (unsigned long long)x, d);
+            if (mod_any(x, d) != x % d) FAIL("mod_any(%llu, %u)", (unsigned long long)x, d);
+        }
+    }
+    for (const uint64_t d : {65536ull, 65537ull, 1ull << 40, 999999999999ull})
+        for (int i = 0; i < 1000; i++) {
+            const uint64_t x = sm64(rs);
+            if (mod_any(x, d) != x % d) FAIL("mod_any(%llu, %llu)", (unsigned long long)x, (unsigned long long)d);
+        }
+
     // pack_index on the extremes and random words
     for (int i = 0; i < 1000; i++) {
         const int64_t w = i == 0 ? 0 : i == 1 ? C.N - 1 : (int64_t)(sm64(rs) % (uint64_t)C.N);
@@ -186,6 +201,7 @@ int main(int argc, char** argv)
             if (w < 0 || w >= C.N) FAIL("%s: pattern_word %lld out of [0, N)", what, (long long)w);
             const std::vector<int> want = brute_pattern(cd, sub, R, t);
             if (w != B.index(want) || wpk != B.packed(want)) FAIL("%s: pattern %u of subset %x", what, t, sub);
+            if (pattern_digits(C, pk, sub, R, t) != wpk) FAIL("%s: pattern_digits(%u) of subset %x", what, t, sub);
             if (nib_dist(wpk, pk) != R) FAIL("%s: distance %d", what, nib_dist(wpk, pk));
             if (C.wok && walk_word(C, lo, hi, base, t) != w) FAIL("%s: walk_word(%u) differs", what, t);
             if (all) seen.push_back(w);
