@@ -305,7 +305,8 @@ def bench_transform(a, reps=20, warmup=2):
     alg_t = 2 * n * N
     alg_r = 2 * N + 4 * M * V
     ach = alg_t / (dt_us * 1e-6) / 1e9
-    traffic = measured_traffic('pmc_k_dt_seed_k_dt_col_K7_9_4.md')
+    traffic = measured_traffic('transforms_K7_9_4.md')
+    ct_traffic = measured_traffic('transforms_K7_9_4.md', 'k_ct')
     out = {'workload': 'K7(9,4) M=1480 random code (numpy seed 3): distance transform + recount',
            'words': N, 'passes_per_transform': npass, 'avg_transform_us': round(dt_us, 2),
            'with_histogram_us': round(dt_hist_us, 2), 'radius': rad, 'covered': int(within),
@@ -328,7 +329,10 @@ def bench_transform(a, reps=20, warmup=2):
                               'achieved_GBs': round(alg_c / (ct_us * 1e-6) / 1e9, 1),
                               'pass_bytes': own_c,
                               'pass_bytes_GBs': round(own_c / (ct_us * 1e-6) / 1e9, 1),
-                              'pass_bytes_frac': round(own_c / (ct_us * 1e-6) / 1e9 / HBM_PEAK_GBS, 4)}
+                              'pass_bytes_frac': round(own_c / (ct_us * 1e-6) / 1e9 / HBM_PEAK_GBS, 4),
+                              'traffic': ct_traffic}
+    if ct_traffic:
+        out['count_transform']['traffic_frac'] = round(ct_traffic / (ct_us * 1e-6) / 1e9 / HBM_PEAK_GBS, 4)
     if traffic:
         tr = traffic / (dt_us * 1e-6) / 1e9
         out['roofline']['traffic_achieved'] = round(tr, 1)
