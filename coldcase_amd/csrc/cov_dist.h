@@ -27,6 +27,7 @@
 // The per-thread phases are __host__ __device__ functions so that a host build
 // can run them thread by thread over plain arrays.
 #pragma once
+#include <algorithm>
 #include "cov_common.h"
 
 namespace cov {
@@ -80,6 +81,23 @@ struct DtPass {
     int32_t P;           // LDS bytes per group row: G rounded up to 4
     uint32_t mG;         // magic of G
     uint64_t sbase[DT_SMAX]; // digit j < ns of a group: the words whose digit j is 0
+    // the seed pass's closed form for tiles with few codewords
+    int32_t sk;          // most codewords a tile may hold for it (0: never)
+    int32_t Ls;          // LDS bytes per codeword's column table
+    int32_t hoff;        // LDS offset of the row tables
+    int32_t Hs;          // LDS bytes per codeword's row table, nrow + 1 rounded up to 4
+    int32_t Pd, Rp;      // the store's dword period G / gcd(G, 4) and the rows it spans, 4 / gcd(G, 4)
+    uint32_t rb;         // group rows per block of q^a words, q^nax
+    uint32_t mrb;        // magic of rb
+    // two-stage column passes (rows through LDS): nrow = q^(b1 + b2) rows,
+    // tiles of Wd dword columns
+    int32_t tcol;        // 1: a two-stage column pass of nax = b1 + b2 axes
+    int32_t b1;          // the stage-A axes (the row index's low digits)
+    int32_t Wd;          // dword columns per tile
+    int32_t nchunk;      // tiles per row block, ceil(S_row / (4 Wd))
+    int32_t nt;          // threads per workgroup
+    int32_t grid;        // workgroups
+    uint32_t mR;         // magic of nrow
 };
 
 // exact floor(f / d) for f * d < 2^32, with m = floor(2^32 / d) + 1 (d > 1)
@@ -455,17 +473,41 @@ __host__ __device__ inline void dt_hist_add4(DtHist& h, uint32_t v, uint32_t nva
 }
 
 // the last pass without the full histogram: #{values <= rcut} and the
-// maximum, per thread; per-byte max(a, b) = a + b - min(a, b) (bytes <= 127)
+// maximum, per thread.  A byte x <= 127 is <= rcut exactly when the top bit
+// of (0x80 | rcut) - x is set (one subtract, mask and bit count for four
+// bytes).  The maximum is kept as packed 16-bit maxima: mo over the dwords
+// themselves (lane k's high byte is byte 2k + 1) and me over the dwords
+// shifted up a byte (byte 2k); dt_stat_max reads the largest high byte.
 struct DtStat {
-    uint32_t le, mx;
+    uint32_t le, mo, me;
 };
+__host__ __device__ inline uint32_t dt_max16(uint32_t a, uint32_t b)
+{
+#ifdef __HIP_DEVICE_COMPILE__
+    typedef uint16_t u16x2 __attribute__((ext_vector_type(2)));
+    return __builtin_bit_cast(uint32_t, __builtin_elementwise_max(__builtin_bit_cast(u16x2, a),
+                                                                  __builtin_bit_cast(u16x2, b)));
+#else
+    const uint32_t l = (a & 0xFFFFu) > (b & 0xFFFFu) ? (a & 0xFFFFu) : (b & 0xFFFFu);
+    const uint32_t h = (a >> 16) > (b >> 16) ? (a >> 16) : (b >> 16);
+    return l | (h << 16);
+#endif
+}
 __host__ __device__ inline void dt_stat_add4(DtStat& st, uint32_t v, uint32_t nvalid, int rcut)
 {
-    const uint32_t keep = nvalid >= 4 ? 0xFFFFFFFFu : ((1u << (8 * nvalid)) - 1u);
-    v &= keep;
-    st.le += (uint32_t)__builtin_popcount(((0x80808080u | ((uint32_t)rcut * 0x01010101u)) - v) & 0x80808080u) -
-             (4 - (nvalid >= 4 ? 4 : nvalid));
-    st.mx = st.mx + v - dt_pmin(st.mx, v);
+    if (nvalid < 4) {                       // bytes [nvalid, 4) count as 0 (<= rcut), discounted below
+        v &= (1u << (8 * nvalid)) - 1u;
+        st.le -= 4 - nvalid;
+    }
+    st.le += (uint32_t)__builtin_popcount(((0x80808080u | ((uint32_t)rcut * 0x01010101u)) - v) & 0x80808080u);
+    st.mo = dt_max16(st.mo, v);
+    st.me = dt_max16(st.me, v << 8);
+}
+__host__ __device__ inline uint32_t dt_stat_max(const DtStat& st)
+{
+    const uint32_t a = st.mo >> 24, b = (st.mo >> 8) & 0xFFu, c = st.me >> 24, d = (st.me >> 8) & 0xFFu;
+    const uint32_t x = a > b ? a : b, y = c > d ? c : d;
+    return x > y ? x : y;
 }
 
 // the last pass's values, read back from LDS after the store
@@ -495,11 +537,7 @@ __device__ inline void dt_stats_flush(const DtPass& p, const DtHist& h, const Dt
     __syncthreads();                            // the rare-value bins are complete
     const bool lead = (threadIdx.x & 63) == 0;
     if (!p.full) {
-        uint32_t mb = 0;
-        for (int k = 0; k < 4; k++) {
-            const uint32_t b = (st.mx >> (8 * k)) & 0xFFu;
-            mb = b > mb ? b : mb;
-        }
+        const uint32_t mb = dt_stat_max(st);
         const int64_t le = wave_sum64((int64_t)st.le);
         const int64_t mx = wave_max64((int64_t)mb);
         if (lead && le) atomicAdd(&lh[0], (unsigned int)le);
@@ -594,20 +632,12 @@ inline int dt_col_bmax(int q)
     return q == 2 ? 4 : (q == 3 ? 3 : (q <= 8 ? 2 : 1));
 }
 
-template <int Q, int B, class Mem>
-__host__ __device__ __forceinline__ void dt_col_thread(const DtPass& p, uint32_t t, const Mem& m, DtHist& h,
-                                                       DtStat& st, unsigned int* lh)
+// the B axes of Q^B rows held by one thread, four words per row: axis k has
+// fibers of Q rows u = Q^k apart
+template <int Q, int B>
+__host__ __device__ __forceinline__ void dt_col_axes(uint32_t* v)
 {
     constexpr int NR = dt_cpow(Q, B);
-    static_assert(NR <= DT_COL_MAX, "a column must fit in registers");
-    const uint32_t S = (uint32_t)p.S_row;
-    const uint32_t a = t / p.ncol;
-    const uint32_t c4 = 4 * (t - a * p.ncol);
-    const uint32_t off = a * (uint32_t)p.S_tile + c4;
-    uint32_t v[NR];
-#pragma unroll
-    for (int r = 0; r < NR; r++) v[r] = m.ld(off, (uint32_t)r * S);
-    // axis k: fibers of Q rows u = Q^k apart
     int u = 1;
 #pragma unroll
     for (int k = 0; k < B; k++) {
@@ -623,6 +653,22 @@ __host__ __device__ __forceinline__ void dt_col_thread(const DtPass& p, uint32_t
         }
         u *= Q;
     }
+}
+
+template <int Q, int B, class Mem>
+__host__ __device__ __forceinline__ void dt_col_thread(const DtPass& p, uint32_t t, const Mem& m, DtHist& h,
+                                                       DtStat& st, unsigned int* lh)
+{
+    constexpr int NR = dt_cpow(Q, B);
+    static_assert(NR <= DT_COL_MAX, "a column must fit in registers");
+    const uint32_t S = (uint32_t)p.S_row;
+    const uint32_t a = t / p.ncol;
+    const uint32_t c4 = 4 * (t - a * p.ncol);
+    const uint32_t off = a * (uint32_t)p.S_tile + c4;
+    uint32_t v[NR];
+#pragma unroll
+    for (int r = 0; r < NR; r++) v[r] = m.ld(off, (uint32_t)r * S);
+    dt_col_axes<Q, B>(v);
     const uint32_t nb = S - c4;                 // the row's words in this column (>= 1)
     if (nb >= 4) {
 #pragma unroll
@@ -702,6 +748,184 @@ inline bool dt_col_dispatch(int q, int b, F&& f)
     default: break;
     }
 #undef DT_QB
+    return false;
+}
+
+// --------------------------------------------------- two-stage column pass
+// A group of b = b1 + b2 axes with q^b1, q^b2 <= 64: a tile is the R = q^b
+// rows of a row block, Wd dword columns (4 Wd words) of each, in LDS column
+// by column (dword r of column c at c R + r, so that both stages address
+// their rows at compile-time offsets).
+// Stage A: thread (hh, c), hh < R2 = q^b2, c < Wd, loads column c of the R1
+// = q^b1 rows hh R1 + i, applies the b1 axes of the row index's low digits in
+// registers and writes the rows to LDS.  Stage B: thread (l, c), l < R1, reads
+// column c of rows i R1 + l, i < R2, applies the b2 high-digit axes in
+// registers and stores them.  One read and one write of the words for b
+// axes, where register column passes take ceil(b / b1) of each.  Columns at
+// or past the row's end are skipped (a fiber never mixes columns); a row's
+// last column with 1-3 of its words is stored byte by byte.  Offsets are
+// 32-bit: N + 64 < 2^32.
+constexpr int DT_TC_LDS = 163840 - 512;      // LDS of a tile: 160 KiB less the static bins
+
+// x, opaque to the optimiser: the per-row offsets i x of a stage are then
+// computed where they are used, not hoisted out of the tile loop (the two
+// stages' offsets together would hold 2 R1 scalar registers for the whole
+// kernel and spill)
+__host__ __device__ inline uint32_t dt_opaque(uint32_t x)
+{
+#ifdef __HIP_DEVICE_COMPILE__
+    asm volatile("" : "+s"(x));
+#endif
+    return x;
+}
+
+template <int Q, int B1, int B2, class Mem>
+__host__ __device__ __forceinline__ void dt_tc_stage_a(const DtPass& p, uint32_t base, uint32_t x0, uint32_t tid,
+                                                       const Mem& m, uint32_t* lds)
+{
+    constexpr int R1 = dt_cpow(Q, B1), R2 = dt_cpow(Q, B2);
+    const uint32_t Wd = (uint32_t)p.Wd, S = (uint32_t)p.S_row;
+    const uint32_t hh = tid / Wd, c = tid - hh * Wd;
+    if (hh >= (uint32_t)R2) return;
+    const uint32_t col = x0 + 4 * c;
+    if (col >= S) return;
+    const uint32_t off = base + col + hh * (uint32_t)R1 * S;
+    const uint32_t Su = dt_opaque(S);
+    uint32_t v[R1];
+#pragma unroll
+    for (int i = 0; i < R1; i++) v[i] = m.ld(off, (uint32_t)i * Su);
+    dt_col_axes<Q, B1>(v);
+    uint32_t* col_l = lds + c * (uint32_t)(R1 * R2) + hh * (uint32_t)R1;
+#pragma unroll
+    for (int i = 0; i < R1; i++) col_l[i] = v[i];
+}
+
+// stage B; on the last pass the values also go back to LDS for dt_tc_stats
+template <int Q, int B1, int B2, class Mem>
+__host__ __device__ __forceinline__ void dt_tc_stage_b(const DtPass& p, uint32_t base, uint32_t x0, uint32_t tid,
+                                                       const Mem& m, uint32_t* lds)
+{
+    constexpr int R1 = dt_cpow(Q, B1), R2 = dt_cpow(Q, B2);
+    const uint32_t Wd = (uint32_t)p.Wd, S = (uint32_t)p.S_row;
+    const uint32_t l = tid / Wd, c = tid - l * Wd;
+    if (l >= (uint32_t)R1) return;
+    const uint32_t col = x0 + 4 * c;
+    if (col >= S) return;
+    uint32_t u[R2];
+    uint32_t* col_l = lds + c * (uint32_t)(R1 * R2) + l;
+#pragma unroll
+    for (int i = 0; i < R2; i++) u[i] = col_l[i * R1];
+    dt_col_axes<Q, B2>(u);
+    const uint32_t off = base + col + l * S;
+    const uint32_t nb = S - col;                   // the row's words in this column (>= 1)
+    const uint32_t step = dt_opaque((uint32_t)R1 * S);
+    if (nb >= 4) {
+#pragma unroll
+        for (int i = 0; i < R2; i++) m.st(off, (uint32_t)i * step, u[i]);
+    } else {
+#pragma unroll
+        for (int i = 0; i < R2; i++)
+            for (uint32_t k = 0; k < nb; k++) m.st8(off + k, (uint32_t)i * step, u[i] >> (8 * k));
+    }
+    if (p.last) {
+#pragma unroll
+        for (int i = 0; i < R2; i++) col_l[i * R1] = u[i];
+    }
+}
+
+// the last pass's statistics of a tile from LDS (after a barrier), one
+// dword at a time: dword e = column e / R, row e % R
+__host__ __device__ inline void dt_tc_stats(const DtPass& p, uint32_t x0, int tid, int nth, const uint32_t* lds,
+                                            DtHist& h, DtStat& st, unsigned int* lh)
+{
+    const uint32_t R = (uint32_t)p.nrow, S = (uint32_t)p.S_row, ne = R * (uint32_t)p.Wd;
+    for (uint32_t e = tid; e < ne; e += nth) {
+        const uint32_t c = dt_div(e, R, p.mR);
+        const uint32_t col = x0 + 4 * c;
+        if (col >= S) break;                       // e only grows: every later column is past the end too
+        const uint32_t nb = S - col < 4 ? S - col : 4u;
+        if (p.full) dt_hist_add4(h, lds[e], nb, p.hn, lh);
+        else dt_stat_add4(st, lds[e], nb, p.rcut);
+    }
+}
+
+// the tile of workgroup slot k (k = blockIdx + i gridDim): workgroups are
+// dealt to the 8 XCDs in turn (slot k on XCD k % 8), and each XCD takes a
+// contiguous run of tiles, so that neighbouring tiles, which share the cache
+// lines at their edges, meet in one L2
+__host__ __device__ inline uint32_t dt_tc_slot_tile(uint32_t k, uint32_t ntiles)
+{
+    const uint32_t per = (ntiles + 7) / 8;
+    const uint32_t x = k & 7u, i = k >> 3;
+    const uint32_t full = ntiles - 8 * (per - 1);       // XCDs 0 .. full-1 take per tiles, the rest per - 1
+    return x < full ? x * per + i : full * per + (x - full) * (per - 1) + i;
+}
+
+// tile t of a two-stage pass: its row block's first word and its first column
+__host__ __device__ inline void dt_tc_tile(const DtPass& p, uint32_t t, uint32_t& base, uint32_t& x0)
+{
+    const uint32_t blk = t / (uint32_t)p.nchunk, ch = t - blk * (uint32_t)p.nchunk;
+    base = blk * (uint32_t)p.S_tile;
+    x0 = ch * 4u * (uint32_t)p.Wd;
+}
+
+// workgroup threads at most: 1024 (four waves per SIMD, 128 registers each)
+// unless a thread holds 64 rows, which needs more registers than that
+constexpr int dt_tc_ntmax(int q, int b1, int b2)
+{
+    return (dt_cpow(q, b1) >= 64 || dt_cpow(q, b2) >= 64) ? 512 : 1024;
+}
+
+template <int Q, int B1, int B2>
+__global__ void __launch_bounds__(dt_tc_ntmax(Q, B1, B2)) k_dt_tcol(const DtPass p, uint8_t* D,
+                                                                    unsigned long long* hist)
+{
+    extern __shared__ __align__(16) uint8_t sm[];
+    __shared__ unsigned int lh[DT_HB];
+    if (threadIdx.x < DT_HB) lh[threadIdx.x] = 0;
+    DtHist h{};
+    DtStat st{};
+#ifdef __HIP_DEVICE_COMPILE__
+    DtBufMem m;
+    m.rs = __builtin_amdgcn_make_buffer_rsrc(D, (short)0, (int)p.nbytes, 0x00020000);
+    uint32_t* lds = reinterpret_cast<uint32_t*>(sm);
+    for (uint32_t k = blockIdx.x; k < (uint32_t)p.ntiles; k += gridDim.x) {
+        uint32_t base, x0;
+        dt_tc_tile(p, dt_tc_slot_tile(k, (uint32_t)p.ntiles), base, x0);
+        __syncthreads();                            // the previous tile's stage B is done with LDS
+        dt_tc_stage_a<Q, B1, B2>(p, base, x0, threadIdx.x, m, lds);
+        __syncthreads();
+        dt_tc_stage_b<Q, B1, B2>(p, base, x0, threadIdx.x, m, lds);
+        if (p.last) {
+            __syncthreads();
+            dt_tc_stats(p, x0, threadIdx.x, blockDim.x, lds, h, st, lh);
+        }
+    }
+#endif
+    if (p.last) dt_stats_flush(p, h, st, lh, hist);
+}
+
+// the (q, b1, b2) of two-stage passes, each its own kernel: b1 = ceil(b / 2)
+// of b in (bmax, 2 bmax] axes, bmax = dt_col_bmax(q)
+template <int Q, int B1, int B2>
+struct DtQBB {
+    static constexpr int q = Q, b1 = B1, b2 = B2;
+};
+template <class F>
+inline bool dt_tcol_dispatch(int q, int b1, int b2, F&& f)
+{
+#define DT_QBB(Q, B1, B2) \
+    case (Q * 8 + B1) * 8 + B2: f(DtQBB<Q, B1, B2>{}); return true;
+    switch ((q * 8 + b1) * 8 + b2) {
+        DT_QBB(2, 3, 2) DT_QBB(2, 3, 3) DT_QBB(2, 4, 3) DT_QBB(2, 4, 4)
+        DT_QBB(3, 2, 2) DT_QBB(3, 3, 2) DT_QBB(3, 3, 3)
+        DT_QBB(4, 2, 1) DT_QBB(4, 2, 2) DT_QBB(5, 2, 1) DT_QBB(5, 2, 2) DT_QBB(6, 2, 1) DT_QBB(6, 2, 2)
+        DT_QBB(7, 2, 1) DT_QBB(7, 2, 2) DT_QBB(8, 2, 1) DT_QBB(8, 2, 2)
+        DT_QBB(9, 1, 1) DT_QBB(10, 1, 1) DT_QBB(11, 1, 1) DT_QBB(12, 1, 1) DT_QBB(13, 1, 1) DT_QBB(14, 1, 1)
+        DT_QBB(15, 1, 1) DT_QBB(16, 1, 1)
+    default: break;
+    }
+#undef DT_QBB
     return false;
 }
 
@@ -893,12 +1117,238 @@ __host__ __device__ inline void dt_seed_store(const DtPass& p, const DtTile& T, 
     }
 }
 
+// ------------------------------------------------------- sparse seed tiles
+// After the a inner axes a word x holds the minimum, over the codewords c of
+// its block (the q^a words that share x's n - a outer digits), of the Hamming
+// distance on the a inner digits.  A tile with at most sk codewords takes
+// that closed form instead of the level sets and row axes.  With x = hi G +
+// lo (group row hi, column lo), d(x, c) = H_c(hi) + L_c(lo): the distance on
+// the nax row digits plus the one on the ns group digits.  LDS holds, per
+// codeword, L_c over Ls >= G + 8 columns (wrapping into the next row) and
+// H_c over the rows (127 for a row of another block, and for row nrow).
+//
+// The store writes aligned dwords.  Tile byte b = head + 4 j of dword j; its
+// four bytes lie in at most two rows (G >= 4).  The dwords repeat their
+// column offsets with period Pd = G / gcd(G, 4), which spans Rp = 4 / gcd(G,
+// 4) rows: dword j + Pd is dword j moved Rp rows down.  So thread (a, p), p <
+// Pd, keeps the column part of its dwords p + Pd Q (Q = a, a + A, ...) per
+// codeword in registers and per dword only reads H_c of two rows: a
+// multiply, a bit-field insert, an add and a per-byte minimum per codeword.
+// The tile's 0-3 head and tail bytes are byte stores.  LDS: positions (DT_SK
+// int32) at 0, column tables of Ls bytes at DT_SPL, row tables of Hs bytes at
+// hoff.
+constexpr int DT_SK = 4;
+constexpr int DT_SPL = 4 * DT_SK;    // LDS offset of the column tables
+
+// bitmap dword d restricted to the words [lo, hi)
+__host__ __device__ inline uint32_t dt_sp_bits(const uint32_t* cwb, uint32_t d, uint32_t lo, uint32_t hi)
+{
+    uint32_t v = cwb[d];
+    const uint32_t w0 = d * 32;
+    if (w0 < lo) v &= ~0u << (lo - w0);
+    if (w0 + 32 > hi) v &= (hi - w0 >= 32) ? ~0u : ((1u << (hi - w0)) - 1u);
+    return v;
+}
+
+// the tile's codewords: each thread counts the bits of its bitmap dwords,
+// reserves that many slots with one add to *cnt, and writes the tile offsets
+// of those slots below sk.  After a barrier *cnt is the tile's count.
+__host__ __device__ inline void dt_sp_collect(const DtPass& p, const DtTile& T, int tid, int nth, const uint32_t* cwb,
+                                              int32_t* pos, int* cnt)
+{
+    const uint32_t g0 = (uint32_t)T.gs0, g1 = g0 + (uint32_t)p.S_tile;
+    const uint32_t d0 = g0 >> 5, nd = ((g1 - 1) >> 5) - d0 + 1;
+    uint32_t c = 0;
+    for (uint32_t j = tid; j < nd; j += nth) c += (uint32_t)__builtin_popcount(dt_sp_bits(cwb, d0 + j, g0, g1));
+    if (!c) return;
+#ifdef __HIP_DEVICE_COMPILE__
+    int slot = atomicAdd(cnt, (int)c);
+#else
+    int slot = *cnt;
+    *cnt += (int)c;
+#endif
+    for (uint32_t j = tid; j < nd && slot < p.sk; j += nth) {
+        uint32_t b = dt_sp_bits(cwb, d0 + j, g0, g1);
+        while (b && slot < p.sk) {
+            pos[slot++] = (int32_t)((d0 + j) * 32 + (uint32_t)__builtin_ctz(b) - g0);
+            b &= b - 1;
+        }
+    }
+}
+
+// Hamming distance of the nd low base-q digits of a and c
+__host__ __device__ inline uint32_t dt_sp_ham(const DtPass& p, uint32_t a, uint32_t c, int nd)
+{
+    uint32_t d = 0;
+    for (int j = 0; j < nd; j++) {
+        const uint32_t a1 = dt_div(a, (uint32_t)p.q, p.mq), c1 = dt_div(c, (uint32_t)p.q, p.mq);
+        d += (a - a1 * (uint32_t)p.q) != (c - c1 * (uint32_t)p.q);
+        a = a1;
+        c = c1;
+    }
+    return d;
+}
+
+// the column and row tables of the k <= sk codewords at pos[]
+__host__ __device__ inline void dt_sp_tables(const DtPass& p, int k, const int32_t* pos, int tid, int nth, uint8_t* sm)
+{
+    const uint32_t G = (uint32_t)p.G, Ls = (uint32_t)p.Ls, Hs = (uint32_t)p.Hs, nrow = (uint32_t)p.nrow;
+    for (int i = 0; i < k; i++) {
+        const uint32_t o = (uint32_t)pos[i];
+        const uint32_t cr = dt_div(o, G, p.mG), cl = o - cr * G;
+        const uint32_t cb = dt_div(cr, p.rb, p.mrb), crl = cr - cb * p.rb;
+        for (uint32_t x = tid; x < Ls; x += nth) {
+            const uint32_t r = dt_div(x, G, p.mG);
+            sm[DT_SPL + i * Ls + x] = (uint8_t)dt_sp_ham(p, x - r * G, cl, p.ns);
+        }
+        for (uint32_t hi = tid; hi < Hs; hi += nth) {
+            const uint32_t hb = dt_div(hi, p.rb, p.mrb);
+            sm[p.hoff + i * Hs + hi] = (uint8_t)(hi >= nrow || hb != cb ? DT_INF
+                                                                         : dt_sp_ham(p, hi - hb * p.rb, crl, p.nax));
+        }
+    }
+}
+
+// bytes over 127 (127 + a column distance, rows of another block) to 127
+__host__ __device__ inline uint32_t dt_sp_clamp(uint32_t d)
+{
+    const uint32_t t = d & 0x80808080u;
+    return (d | (t - (t >> 7))) & ~t;
+}
+
+// bytes of {s0, s1} by selector (v_perm_b32): selector byte i picks byte
+// sel_i of s1 (0-3) or byte sel_i - 4 of s0 (4-7)
+__host__ __device__ inline uint32_t dt_perm(uint32_t s0, uint32_t s1, uint32_t sel)
+{
+#ifdef __HIP_DEVICE_COMPILE__
+    return __builtin_amdgcn_perm(s0, s1, sel);
+#else
+    const uint64_t x = ((uint64_t)s0 << 32) | s1;
+    uint32_t r = 0;
+    for (int i = 0; i < 4; i++) r |= (uint32_t)((x >> (8 * ((sel >> (8 * i)) & 7u))) & 0xFFu) << (8 * i);
+    return r;
+#endif
+}
+
+// tile byte b from the tables of its k codewords
+__host__ __device__ inline uint32_t dt_sp_byte(const DtPass& p, const uint8_t* sm, int k, uint32_t b)
+{
+    const uint32_t G = (uint32_t)p.G;
+    const uint32_t hi = dt_div(b, G, p.mG), lo = b - hi * G;
+    uint32_t v = DT_INF;
+    for (int i = 0; i < k; i++) {
+        const uint32_t d = sm[DT_SPL + i * (uint32_t)p.Ls + lo] + (uint32_t)sm[p.hoff + i * (uint32_t)p.Hs + hi];
+        v = d < v ? d : v;
+    }
+    return v;
+}
+
+// the closed-form dwords of thread (aa, pp): dwords j = Q Pd + pp for Q = aa,
+// aa + A, ...; K >= k codewords (the ones past k skipped), CLAMP when rows
+// of another block (127 + a column distance) can occur
+template <int K, bool CLAMP, class Mem>
+__host__ __device__ inline void dt_sp_loop(const DtPass& p, const uint8_t* sm, int k, uint32_t aa, uint32_t pp,
+                                           uint32_t A, uint32_t head, uint32_t nk, uint32_t g0, const Mem& m,
+                                           DtHist& h, DtStat& st, unsigned int* lh)
+{
+    const uint32_t G = (uint32_t)p.G, Pd = (uint32_t)p.Pd, Rp = (uint32_t)p.Rp;
+    const uint32_t Ls = (uint32_t)p.Ls, Hs = (uint32_t)p.Hs;
+    const uint32_t b0 = head + 4 * pp;
+    const uint32_t r0 = dt_div(b0, G, p.mG), lo0 = b0 - r0 * G;
+    const uint32_t left = G - lo0;                       // bytes of row r0 in the dword
+    // byte i from row r (selector 0: byte 0 of s1) or r + 1 (selector 4: byte 0 of s0)
+    const uint32_t sel = left >= 4 ? 0u : (0x04040404u & ~((1u << (8 * left)) - 1u));
+    const uint32_t* L = reinterpret_cast<const uint32_t*>(sm);
+    const uint8_t* H = sm + p.hoff;
+    uint32_t lq[K];
+#pragma unroll
+    for (int i = 0; i < K; i++) {
+        const uint32_t at = DT_SPL + (uint32_t)i * Ls + lo0;
+        lq[i] = (K == 1 || i < k) ? dt_align(L[(at >> 2) + 1], L[at >> 2], at & 3u) : 0u;
+    }
+    for (uint32_t Q = aa;; Q += A) {
+        const uint32_t j = Q * Pd + pp;
+        if (j >= nk) break;
+        const uint32_t r = r0 + Rp * Q;
+        uint32_t h0[K], h1[K];
+#pragma unroll
+        for (int i = 0; i < K; i++) {
+            if (K <= 2 || i < k) {
+                h0[i] = H[(uint32_t)i * Hs + r];
+                h1[i] = H[(uint32_t)i * Hs + r + 1];
+            }
+        }
+        uint32_t v = DT_INF4;
+#pragma unroll
+        for (int i = 0; i < K; i++) {
+            if (K <= 2 || i < k) {
+                uint32_t d = lq[i] + dt_perm(h1[i], h0[i], sel);
+                if (CLAMP) d = dt_sp_clamp(d);
+                v = i == 0 ? d : dt_pmin(v, d);
+            }
+        }
+        m.st(head + 4 * j, g0, v);
+        if (p.last) {
+            if (p.full) dt_hist_add4(h, v, 4, p.hn, lh);
+            else dt_stat_add4(st, v, 4, p.rcut);
+        }
+    }
+}
+
+// the store of a tile in the closed form (k = 0: all unreached) and the last
+// pass's statistics
+template <class Mem>
+__host__ __device__ inline void dt_sp_store(const DtPass& p, const DtTile& T, int tid, int nth, const uint8_t* sm,
+                                            int k, const Mem& m, DtHist& h, DtStat& st, unsigned int* lh)
+{
+    const uint32_t tot = (uint32_t)p.S_tile, g0 = (uint32_t)T.gs0, G = (uint32_t)p.G;
+    const uint32_t head = (4u - (g0 & 3u)) & 3u;
+    const uint32_t nk = (tot - head) >> 2;
+    if (k == 0) {
+        for (uint32_t j = tid; j < nk; j += nth) {
+            m.st(head + 4 * j, g0, DT_INF4);
+            if (p.last) {
+                if (p.full) dt_hist_add4(h, DT_INF4, 4, p.hn, lh);
+                else dt_stat_add4(st, DT_INF4, 4, p.rcut);
+            }
+        }
+    } else {
+        const uint32_t Pd = (uint32_t)p.Pd, A = (uint32_t)nth / Pd;
+        const uint32_t aa = (uint32_t)tid / Pd, pp = (uint32_t)tid - aa * Pd;
+        if (aa < A) {
+            const bool blocks = (uint32_t)p.nrow > p.rb;
+            if (k == 1) {
+                if (blocks) dt_sp_loop<1, true>(p, sm, k, aa, pp, A, head, nk, g0, m, h, st, lh);
+                else dt_sp_loop<1, false>(p, sm, k, aa, pp, A, head, nk, g0, m, h, st, lh);
+            } else if (k == 2) {
+                if (blocks) dt_sp_loop<2, true>(p, sm, k, aa, pp, A, head, nk, g0, m, h, st, lh);
+                else dt_sp_loop<2, false>(p, sm, k, aa, pp, A, head, nk, g0, m, h, st, lh);
+            } else {
+                dt_sp_loop<DT_SK, true>(p, sm, k, aa, pp, A, head, nk, g0, m, h, st, lh);
+            }
+        }
+    }
+    if (tid == nth - 1) {
+        const uint32_t t0 = head + 4 * nk, ne = head + (tot - t0);
+        for (uint32_t e = 0; e < ne; e++) {
+            const uint32_t b = e < head ? e : t0 + (e - head);
+            const uint32_t x = k ? dt_sp_byte(p, sm, k, b) : DT_INF;
+            m.st8(b, g0, x);
+            if (p.last) {
+                if (p.full) dt_hist_add4(h, x, 1, p.hn, lh);
+                else dt_stat_add4(st, x, 1, p.rcut);
+            }
+        }
+    }
+}
+
 template <int Q>
 __global__ void __launch_bounds__(DT_COL_NT) k_dt_seed(const DtPass p, uint8_t* D, const uint32_t* cwb,
                                                        unsigned long long* hist)
 {
     extern __shared__ __align__(16) uint8_t sm[];
     __shared__ unsigned int lh[DT_HB];
+    __shared__ int ncw;
     if (threadIdx.x < DT_HB) lh[threadIdx.x] = 0;
     DtHist h{};
     DtStat st{};
@@ -907,6 +1357,23 @@ __global__ void __launch_bounds__(DT_COL_NT) k_dt_seed(const DtPass p, uint8_t* 
     m.rs = __builtin_amdgcn_make_buffer_rsrc(D, (short)0, (int)p.nbytes, 0x00020000);
     for (int64_t t = blockIdx.x; t < p.ntiles; t += gridDim.x) {
         const DtTile T = dt_tile(p, t);
+        int k = p.sk + 1;
+        if (p.sk) {
+            __syncthreads();                        // the previous tile is done with LDS and ncw
+            if (threadIdx.x == 0) ncw = 0;
+            __syncthreads();
+            dt_sp_collect(p, T, threadIdx.x, DT_COL_NT, cwb, reinterpret_cast<int32_t*>(sm), &ncw);
+            __syncthreads();
+            k = ncw;
+        }
+        if (k <= p.sk) {
+            if (k) {
+                dt_sp_tables(p, k, reinterpret_cast<const int32_t*>(sm), threadIdx.x, DT_COL_NT, sm);
+                __syncthreads();
+            }
+            dt_sp_store(p, T, threadIdx.x, DT_COL_NT, sm, k, m, h, st, lh);
+            continue;
+        }
         __syncthreads();
         int any = 0;
         for (uint32_t gi = threadIdx.x; gi < (uint32_t)p.nrow; gi += DT_COL_NT)
@@ -986,12 +1453,49 @@ inline int dt_seed_smax(int q)
     return q == 2 ? 6 : (q <= 4 ? 3 : (q <= 8 ? 2 : 1));
 }
 
-// column passes for the outer axes [0, m): dt_col_bmax(q) axes each at
-// most, spread evenly; appends them to out[np..]; returns the new count
-inline int dt_plan_columns(int q, int n, int64_t N, int m, DtPass* out, int np)
+// the columns and threads of a two-stage pass (nrow, S_row, S_tile set): the
+// tile width that finishes its tiles in the fewest resident rounds of ncu
+// CUs, each round costing the bytes a CU holds in it (ties: the wider tile)
+inline bool dt_plan_tcol(DtPass& p, int q, int64_t nblk, int ncu)
+{
+    const int64_t R = p.nrow, R1 = dt_ipow(q, p.b1), R2 = R / R1;
+    const int64_t rmax = R1 > R2 ? R1 : R2;
+    const int64_t ntmax = dt_tc_ntmax(q, p.b1, p.nax - p.b1);
+    int64_t best = -1, wbest = 0;
+    for (int64_t Wd = std::min<int64_t>(ntmax / rmax, DT_TC_LDS / (4 * R)); Wd >= 4; Wd--) {
+        const int64_t nt = (rmax * Wd + 63) / 64 * 64;
+        int64_t per = DT_TC_LDS / (4 * R * Wd);
+        if (per > 2048 / nt) per = 2048 / nt;
+        if (per < 1) continue;
+        const int64_t ntiles = nblk * ((p.S_row + 4 * Wd - 1) / (4 * Wd));
+        const int64_t rounds = (ntiles + ncu * per - 1) / (ncu * per);
+        const int64_t cost = rounds * per * R * Wd;
+        if (best < 0 || cost < best) {
+            best = cost;
+            wbest = Wd;
+        }
+    }
+    if (best < 0) return false;
+    p.Wd = (int32_t)wbest;
+    p.nchunk = (int32_t)((p.S_row + 4 * wbest - 1) / (4 * wbest));
+    p.ntiles = nblk * p.nchunk;
+    p.nt = (int32_t)((rmax * wbest + 63) / 64 * 64);
+    p.lds = 4 * R * wbest;
+    p.mR = dt_magic((uint64_t)R);
+    int64_t per = DT_TC_LDS / p.lds;
+    if (per > 2048 / p.nt) per = 2048 / p.nt;
+    p.grid = (int32_t)std::min<int64_t>(p.ntiles, per * ncu);
+    return true;
+}
+
+// column passes for the outer axes [0, m), appended to out[np..]; returns the
+// new count.  Groups of at most bmax = dt_col_bmax(q) axes are register
+// passes; with tcol, groups of up to 2 bmax axes are two-stage passes.
+inline int dt_plan_columns(int q, int n, int64_t N, int m, DtPass* out, int np, bool tcol = true, int ncu = 256)
 {
     const int bmax = dt_col_bmax(q);
-    const int ngroups = (m + bmax - 1) / bmax;
+    const int gmax = tcol ? 2 * bmax : bmax;
+    const int ngroups = (m + gmax - 1) / gmax;
     int j1 = m;
     for (int g = 0; g < ngroups; g++) {
         const int left = ngroups - g;
@@ -1007,6 +1511,12 @@ inline int dt_plan_columns(int q, int n, int64_t N, int m, DtPass* out, int np)
         p.ntiles = dt_ipow(q, j0) * (int64_t)p.ncol;
         p.nthr = (uint32_t)p.ntiles;
         p.nbytes = (uint32_t)(N + 64);
+        if (b > bmax) {
+            p.col = 0;
+            p.tcol = 1;
+            p.b1 = (b + 1) / 2;
+            if (!dt_plan_tcol(p, q, dt_ipow(q, j0), ncu)) return 0;
+        }
         j1 = j0;
     }
     return np;
@@ -1014,7 +1524,7 @@ inline int dt_plan_columns(int q, int n, int64_t N, int m, DtPass* out, int np)
 
 // the seed pass over the a innermost axes (tiles of `per` blocks of q^a
 // words, >= 512 tiles on large spaces), then column passes
-inline int dt_plan_seed(int q, int n, int64_t N, int budget, DtPass* out)
+inline int dt_plan_seed(int q, int n, int64_t N, int budget, DtPass* out, bool tcol, int ncu)
 {
     const int smax = dt_seed_smax(q);
     auto lds_of = [&](int a) {
@@ -1056,7 +1566,33 @@ inline int dt_plan_seed(int q, int n, int64_t N, int budget, DtPass* out)
         p.sbase[j] = b;
     }
     if (N % p.S_tile || p.lds > budget) return 0;
-    const int np = dt_plan_columns(q, n, N, n - a, out, 1);
+    // the closed form for tiles with few codewords: as many codewords as the
+    // LDS budget holds tables for, up to DT_SK; groups of 4 words or more
+    p.mq = dt_magic((uint64_t)q);
+    p.rb = (uint32_t)dt_ipow(q, a - s);
+    p.mrb = dt_magic(p.rb);
+    p.Ls = (p.G + 8 + 3) & ~3;
+    p.Hs = (int32_t)((p.nrow + 1 + 3) & ~(int64_t)3);
+    {
+        const int g4 = (p.G % 4 == 0) ? 4 : (p.G % 2 == 0 ? 2 : 1);
+        p.Pd = p.G / g4;
+        p.Rp = 4 / g4;
+    }
+    p.sk = 0;
+    if (p.G >= 4) {
+        for (int sk = DT_SK; sk >= 1; sk--) {
+            const int64_t hoff = DT_SPL + (int64_t)sk * p.Ls;
+            const int64_t need = hoff + (int64_t)sk * p.Hs;
+            if (need <= budget) {
+                p.sk = sk;
+                p.hoff = (int32_t)hoff;
+                if (need > p.lds) p.lds = need;
+                break;
+            }
+        }
+    }
+    const int np = dt_plan_columns(q, n, N, n - a, out, 1, tcol, ncu);
+    if (np <= 0) return 0;
     out[np - 1].last = 1;
     return np;
 }
@@ -1064,10 +1600,11 @@ inline int dt_plan_seed(int q, int n, int64_t N, int budget, DtPass* out)
 // the passes for a cell; returns their number (0 on a bad budget): a seed
 // pass and column passes, unless `columns` is false or N is too large for
 // 32-bit offsets; then an inner pass and outer passes in LDS tiles.
-inline int dt_plan(int q, int n, int64_t N, int budget, DtPass* out, bool columns = true)
+inline int dt_plan(int q, int n, int64_t N, int budget, DtPass* out, bool columns = true, bool tcol = true,
+                   int ncu = 256)
 {
-    if (budget < 4096 || budget > DT_LDS_MAX || q < 2 || q > 16 || n < 1 || n > MAXN) return 0;
-    if (columns && N + 64 <= (int64_t)0xFFFFFFFFu) return dt_plan_seed(q, n, N, budget, out);
+    if (budget < 4096 || budget > DT_LDS_MAX || q < 2 || q > 16 || n < 1 || n > MAXN || ncu < 1) return 0;
+    if (columns && N + 64 <= (int64_t)0xFFFFFFFFu) return dt_plan_seed(q, n, N, budget, out, tcol, ncu);
     int np = 0;
     // inner pass: rows of one innermost fiber, padded to W = 4, 8 or 16 bytes;
     // a axes (the innermost + a-1 across rows) in blocks of q^a words,

@@ -64,6 +64,7 @@ struct cov_cell {
     int leave_mode = COV_LEAVE_AUTO;      // how the steps find the leaving side (cov_set_leave_mode)
     int th_on = 0;                        // the control block's th_on as last set from here
     double clk_hz = 1e8;                  // device wall clock rate (hipDeviceAttributeWallClockRate)
+    int ncu = 256;                        // compute units (hipDeviceAttributeMultiprocessorCount)
     // count transform: (R+1) int32 layers of N words (allocated on first
     // use); layer 0 holds the map after a sum transform
     int32_t* ct = nullptr;
@@ -581,6 +582,9 @@ int cov_create(int q, int n, int R, int M, cov_cell_t** out)
         if (hipGetDevice(&dev) == hipSuccess &&
             hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, dev) == hipSuccess && khz > 0)
             h->clk_hz = (double)khz * 1e3;
+        int ncu = 0;
+        if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess && ncu > 0)
+            h->ncu = ncu;
     }
     Cell& C = h->h;
     SubsetTables tabs;
@@ -988,7 +992,7 @@ static int run_transform(cov_cell_t* h, int full, int rcut, int64_t hh[DT_HB])
         HCK(h, hipMemsetAsync(h->cwb, 0, wb, h->stream));
     }
     DtPass pass[MAXN + 1];
-    const int np = dt_plan(C.q, C.n, C.N, h->dt_lds ? h->dt_lds : DT_LDS_DEFAULT, pass);
+    const int np = dt_plan(C.q, C.n, C.N, h->dt_lds ? h->dt_lds : DT_LDS_DEFAULT, pass, true, true, h->ncu);
     if (np <= 0) {
         h->err = "distance transform: no tile plan for this shape and LDS budget";
         return COV_ELIMIT;
@@ -1015,6 +1019,22 @@ static int run_transform(cov_cell_t* h, int full, int rcut, int64_t hh[DT_HB])
             });
             if (!ok) {
                 h->err = "distance transform: no seed kernel for this q";
+                return COV_ELIMIT;
+            }
+        } else if (p.tcol) {
+            const bool ok = dt_tcol_dispatch(p.q, p.b1, p.nax - p.b1, [&](auto qb) {
+                using QB = decltype(qb);
+                const hipError_t e = hipFuncSetAttribute((const void*)k_dt_tcol<QB::q, QB::b1, QB::b2>,
+                                                         hipFuncAttributeMaxDynamicSharedMemorySize, (int)p.lds);
+                if (e != hipSuccess) {
+                    lrc = -(100 + (int)e);
+                    return;
+                }
+                lrc = launch_t(h, 2, (k_dt_tcol<QB::q, QB::b1, QB::b2>), dim3(p.grid), dim3(p.nt), (uint32_t)p.lds, p,
+                               C.dist, dh);
+            });
+            if (!ok) {
+                h->err = "distance transform: no two-stage column kernel for this (q, axes)";
                 return COV_ELIMIT;
             }
         } else if (p.col) {
@@ -1481,7 +1501,7 @@ int cov_set_transform_tile(cov_cell_t* h, int lds_bytes, int threads)
     h->dt_threads = threads;
     DtPass pass[MAXN + 1];
     const Cell& C = h->h;
-    if (lds_bytes && dt_plan(C.q, C.n, C.N, lds_bytes, pass) <= 0) return COV_EPARAM;
+    if (lds_bytes && dt_plan(C.q, C.n, C.N, lds_bytes, pass, true, true, h->ncu) <= 0) return COV_EPARAM;
     h->dt_lds = lds_bytes;
     return COV_OK;
 }

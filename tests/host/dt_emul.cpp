@@ -76,9 +76,12 @@ int main(int argc, char** argv)
     }
     DtPass pass[MAXN + 1];
     const bool columns = argc > 8 ? atoi(argv[8]) != 0 : true;
-    const int np = dt_plan(q, n, N, budget, pass, columns);
+    const bool tcol = argc > 9 ? atoi(argv[9]) != 0 : true;
+    const int ncu = argc > 10 ? atoi(argv[10]) : 256;
+    const int np = dt_plan(q, n, N, budget, pass, columns, tcol, ncu);
     if (np <= 0) { printf("FAIL plan\n"); return 1; }
 
+    int ntc = 0;                        // two-stage column passes
     int axes = 0;                       // the inner pass's rows hold the innermost axis, a seed pass's groups ns axes
     for (int i = 0; i < np; i++) axes += pass[i].nax + (pass[i].seed ? pass[i].ns : (pass[i].inner ? 1 : 0));
     if (axes != n) { printf("FAIL axes %d\n", axes); return 1; }
@@ -96,10 +99,11 @@ int main(int argc, char** argv)
     pass[np - 1].full = light < 0;
     pass[np - 1].rcut = light < 0 ? 0 : light;
     uint64_t lt_le = 0, lt_max = 0;
+    int64_t sparse_tiles = 0, empty_tiles = 0, dense_tiles = 0;     // seed tiles by the path they took
     auto flush = [&](const DtPass& p, const DtHist& h, const DtStat& st, const unsigned int* lh) -> bool {
         if (!p.full) {
             lt_le += st.le;
-            for (int k = 0; k < 4; k++) lt_max = std::max<uint64_t>(lt_max, (st.mx >> (8 * k)) & 0xFF);
+            lt_max = std::max<uint64_t>(lt_max, dt_stat_max(st));
             return true;
         }
         uint32_t prev = 0;
@@ -113,6 +117,60 @@ int main(int argc, char** argv)
     };
     for (int pi = 0; pi < np; pi++) {
         const DtPass& p = pass[pi];
+        if (p.tcol) {
+            // the two-stage pass: per tile, stage A for every thread, then stage B
+            const int64_t R1 = dt_ipow(p.q, p.b1), R2 = p.nrow / R1;
+            if (pi == 0 || p.col || R1 > 64 || R2 > 64 || R1 * R2 != p.nrow || p.nbytes != (uint64_t)N + 64 ||
+                p.lds != 4 * p.nrow * p.Wd || p.lds > DT_TC_LDS || p.nt > dt_tc_ntmax(p.q, p.b1, p.nax - p.b1) || p.nt % 64 ||
+                p.nt < std::max(R1, R2) * p.Wd || p.Wd < 4 || (int64_t)p.nchunk * 4 * p.Wd < p.S_row ||
+                (int64_t)(p.nchunk - 1) * 4 * p.Wd >= p.S_row || p.S_tile != p.nrow * p.S_row ||
+                p.ntiles * p.S_tile != N * p.nchunk || p.grid < 1 || p.grid > p.ntiles) {
+                printf("FAIL two-stage plan\n");
+                return 1;
+            }
+            ntc++;
+            const HostMem m{D, (uint64_t)p.nbytes};
+            std::vector<uint32_t> lds((size_t)p.lds / 4, 0xDEADBEEFu);
+            bool ok = true;
+            std::vector<uint8_t> seen(p.ntiles, 0);     // every tile exactly once
+            const bool known = dt_tcol_dispatch(p.q, p.b1, p.nax - p.b1, [&](auto qb) {
+                using QB = decltype(qb);
+                // workgroup g takes tiles g, g + grid, ...: emulated in that order
+                for (int g = 0; g < p.grid && ok; g++) {
+                    DtHist h{};
+                    DtStat st{};
+                    unsigned int lh[DT_HB] = {0};
+                    std::vector<DtHist> hs(p.nt);
+                    std::vector<DtStat> sts(p.nt);
+                    for (auto& x : hs) memset(&x, 0, sizeof x);
+                    for (auto& x : sts) memset(&x, 0, sizeof x);
+                    for (int64_t k = g; k < p.ntiles; k += p.grid) {
+                        const uint32_t t = dt_tc_slot_tile((uint32_t)k, (uint32_t)p.ntiles);
+                        if (t >= (uint32_t)p.ntiles || seen[t]++) { printf("FAIL slot %lld -> tile %u\n", (long long)k, t); ok = false; break; }
+                        uint32_t base, x0;
+                        dt_tc_tile(p, t, base, x0);
+                        for (int tid = 0; tid < p.nt; tid++)
+                            dt_tc_stage_a<QB::q, QB::b1, QB::b2>(p, base, x0, tid, m, lds.data());
+                        for (int tid = 0; tid < p.nt; tid++)
+                            dt_tc_stage_b<QB::q, QB::b1, QB::b2>(p, base, x0, tid, m, lds.data());
+                        if (p.last)
+                            for (int tid = 0; tid < p.nt; tid++) dt_tc_stats(p, x0, tid, p.nt, lds.data(), hs[tid], sts[tid], lh);
+                    }
+                    if (p.last)
+                        for (int tid = 0; tid < p.nt && ok; tid++) {
+                            unsigned int none[DT_HB] = {0};
+                            ok = flush(p, hs[tid], sts[tid], tid == 0 ? lh : none);
+                        }
+                    (void)h;
+                    (void)st;
+                }
+            });
+            if (!known) { printf("FAIL no two-stage kernel q=%d b1=%d b2=%d\n", p.q, p.b1, p.nax - p.b1); return 1; }
+            if (!ok) return 1;
+            for (int64_t t = 0; t < p.ntiles; t++)
+                if (seen[t] != 1) { printf("FAIL tile %lld taken %d times\n", (long long)t, seen[t]); return 1; }
+            continue;
+        }
         if (p.col) {
             if (pi == 0 || p.nthr != (uint32_t)p.ntiles || p.nrow > DT_COL_MAX || p.nbytes != (uint64_t)N + 64 ||
                 (int64_t)p.ncol * 4 < p.S_row || p.S_tile != p.nrow * p.S_row) {
@@ -139,7 +197,11 @@ int main(int argc, char** argv)
         if (lds + 4 * DT_HB > 65536 || lds > budget) { printf("FAIL lds %lld\n", (long long)lds); return 1; }
         if ((p.inner != 0 || p.seed != 0) != (pi == 0) || (p.first != 0) != (pi == 0)) { printf("FAIL flags\n"); return 1; }
         if (p.seed) {
-            if (p.G > 64 || p.P % 4 || p.P < p.G || p.nbytes != (uint64_t)N + 64 || p.nrow * p.P != p.lds) {
+            if (p.G > 64 || p.P % 4 || p.P < p.G || p.nbytes != (uint64_t)N + 64 || p.nrow * p.P > p.lds ||
+                p.sk < 0 || p.sk > DT_SK || (p.sk && (p.G < 4 || p.Ls < p.G + 8 || p.Ls % 4 || p.hoff % 4 ||
+                                                      p.hoff < DT_SPL + p.sk * p.Ls || p.Hs < p.nrow + 1 || p.hoff + p.sk * p.Hs > p.lds ||
+                                                      p.Pd * 4 != p.Rp * p.G || p.Pd > 64 ||
+                                                      p.rb * (int64_t)p.G * (p.S_tile / (p.rb * (int64_t)p.G)) != p.S_tile))) {
                 printf("FAIL seed plan\n");
                 return 1;
             }
@@ -151,6 +213,29 @@ int main(int argc, char** argv)
                 using QB = decltype(qb);
                 for (int64_t t = 0; t < p.ntiles && ok; t++) {
                     const DtTile T = dt_tile(p, t);
+                    // the closed form for tiles with at most sk codewords, as k_dt_seed
+                    int k = p.sk + 1;
+                    if (p.sk) {
+                        int ncw = 0;
+                        for (int tid = 0; tid < nth; tid++)
+                            dt_sp_collect(p, T, tid, nth, cwb, reinterpret_cast<int32_t*>(sm), &ncw);
+                        k = ncw;
+                    }
+                    if (k <= p.sk) {
+                        (k ? sparse_tiles : empty_tiles)++;
+                        if (k)
+                            for (int tid = 0; tid < nth; tid++)
+                                dt_sp_tables(p, k, reinterpret_cast<const int32_t*>(sm), tid, nth, sm);
+                        for (int tid = 0; tid < nth && ok; tid++) {
+                            DtHist h{};
+                            DtStat st{};
+                            unsigned int lh[DT_HB] = {0};
+                            dt_sp_store(p, T, tid, nth, sm, k, m, h, st, lh);
+                            if (p.last) ok = flush(p, h, st, lh);
+                        }
+                        continue;
+                    }
+                    dense_tiles++;
                     bool any = false;
                     for (uint32_t gi = 0; gi < (uint32_t)p.nrow; gi++) any |= dt_seed_group<QB::q>(p, T, gi, sm, cwb);
                     if (any)
@@ -248,6 +333,8 @@ int main(int argc, char** argv)
     if (bad) { printf("FAIL %lld mismatches\n", (long long)bad); return 1; }
     printf("ok passes=%d tiles=", np);
     for (int i = 0; i < np; i++) printf("%lld%s", (long long)pass[i].ntiles, i + 1 < np ? "," : "");
+    printf(" seed=%lld/%lld/%lld tcol=%d", (long long)empty_tiles, (long long)sparse_tiles, (long long)dense_tiles, ntc);
+    if (pass[0].seed && pass[0].rb) printf(" blocks/tile=%lld", (long long)(pass[0].nrow / pass[0].rb));
     printf(" hist=");
     for (int b = 0; b <= n; b++) printf("%llu ", (unsigned long long)hist[b]);
     printf("\n");

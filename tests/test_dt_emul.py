@@ -3,7 +3,9 @@ and every per-thread phase of coldcase_amd/csrc/cov_dist.h run thread by
 thread over exact-size heap buffers under AddressSanitizer, against a
 brute-force minimum distance and its histogram, for shapes that exercise
 one-pass, two-pass and many-pass plans, column passes and LDS-tile outer
-passes, ragged row chunks, odd strides and small LDS budgets."""
+passes, ragged row chunks, odd strides and small LDS budgets; seed tiles
+in the closed form (few codewords) and by the row axes (many); two-stage
+column passes (rows through LDS) and register column passes."""
 import os
 import shutil
 import subprocess
@@ -28,6 +30,13 @@ SHAPES = [  # q n M seed lds
     # and a seed pass that is also the last pass (light statistics)
     (4, 9, 50, 51, 32768), (3, 12, 100, 52, 16384), (9, 5, 400, 53, 32768, 512), (11, 4, 300, 54, 8192),
     (8, 6, 2000, 55, 32768), (5, 9, 3000, 56, 49152, 256, 3), (16, 3, 10, 57, 32768, 256, 1),
+    # seed tiles in the closed form and by the row axes in one pass, with the
+    # full histogram and the light statistics; closed-form tiles of two blocks
+    (5, 8, 1500, 67, 8192), (5, 8, 1400, 68, 8192, 512, 3), (7, 7, 1900, 69, 16384), (4, 10, 300, 70, 4096),
+    # register column passes only (no two-stage pass); two-stage passes whose
+    # workgroups loop over tiles (3 and 5 compute units)
+    (7, 9, 1480, 23, 32768, 256, -1, 1, 0), (7, 9, 1480, 26, 49152, 512, 3, 1, 1, 3),
+    (3, 12, 100, 52, 16384, 256, -1, 1, 1, 5),
     # outer passes in LDS tiles (the plan for spaces of 2^32 words and more)
     (7, 9, 1480, 41, 49152, 512, -1, 0), (6, 10, 2784, 42, 49152, 512, 4, 0), (2, 16, 3, 43, 8192, 256, -1, 0),
     (3, 11, 25, 44, 4096, 1024, 2, 0), (13, 4, 9, 45, 4096, 256, -1, 0),
@@ -52,7 +61,9 @@ def emul(tmp_path_factory):
 def _id(s):
     light = '_r%d' % s[6] if len(s) > 6 and s[6] >= 0 else ''
     lds = '_ldsouter' if len(s) > 7 and not s[7] else ''
-    return 'K%d(%d)M%d_lds%d_t%d%s%s' % (s[0], s[1], s[2], s[4], s[5] if len(s) > 5 else 256, light, lds)
+    tc = '_notcol' if len(s) > 8 and not s[8] else ''
+    cu = '_cu%d' % s[9] if len(s) > 9 else ''
+    return 'K%d(%d)M%d_lds%d_t%d%s%s%s%s' % (s[0], s[1], s[2], s[4], s[5] if len(s) > 5 else 256, light, lds, tc, cu)
 
 
 @pytest.mark.parametrize('shape', SHAPES, ids=_id)

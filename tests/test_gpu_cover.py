@@ -133,6 +133,51 @@ def test_distance_transform_tile_plans(q, n, R, M, lds):
             assert (hist == np.bincount(want, minlength=n + 1)[:n + 1]).all()
 
 
+def axis_dp_dist(code, n, q):
+    """dist[w] by the textbook axis DP over the whole array, one numpy pass
+    per axis (min over the q words of a fiber, plus one); independent of the
+    kernels' tiles, seed closed form and pass grouping."""
+    N = q ** n
+    d = np.full(N, 127, np.uint8)
+    w = np.zeros(len(code), np.int64)
+    for p in range(n):
+        w = w * q + np.asarray(code)[:, p].astype(np.int64)
+    d[w] = 0
+    for j in range(n):
+        v = d.reshape(q ** j, q, q ** (n - 1 - j))
+        m = v.min(axis=1, keepdims=True)
+        np.minimum(v, np.minimum(m, 126) + 1, out=v)
+    return d
+
+
+def test_axis_dp_reference_is_brute_force():
+    rng = np.random.default_rng(5)
+    for q, n, M in [(3, 6, 5), (4, 5, 9), (7, 4, 3), (2, 9, 4)]:
+        code = rng.integers(0, q, size=(M, n)).astype(np.uint8)
+        assert (axis_dp_dist(code, n, q) == mindist(code, n, q)).all()
+
+
+@pytest.mark.parametrize('q,n,M,seed', [(7, 9, 1480, 1), (7, 9, 40, 2), (7, 9, 9000, 3), (6, 10, 2784, 4),
+                                        (6, 9, 660, 5), (8, 8, 300, 6), (8, 9, 5000, 7), (5, 9, 3000, 8),
+                                        (5, 10, 1000, 9), (3, 15, 2000, 10)])
+def test_distance_transform_full_size(q, n, M, seed):
+    """Whole distance arrays at the workload sizes (seed tiles in the closed
+    form and by the row axes, two-stage and register column passes) against
+    the numpy axis DP, with the histogram and the radius."""
+    print('case transform full K%d(%d) M=%d' % (q, n, M), flush=True)
+    rng = np.random.default_rng(seed)
+    code = rng.integers(0, q, size=(M, n)).astype(np.uint8)
+    want = axis_dp_dist(code, n, q)
+    with cell(q, n, 1, M) as c:
+        c.set_code(code, recount=False)
+        dist, hist, rad = c.distance_transform(want_array=True)
+        assert (dist == want).all()
+        assert rad == int(want.max())
+        assert (hist == np.bincount(want, minlength=n + 1)[:n + 1]).all()
+        within, rad2 = c.distance_stats(3)
+        assert within == int((want <= 3).sum()) and rad2 == rad
+
+
 @pytest.mark.parametrize('fname,q,n,R', [('K6_9_4_M660.txt', 6, 9, 4), ('K7_9_4_M1480.txt', 7, 9, 4),
                                          ('K6_10_4_M2784.txt', 6, 10, 4), ('K6_8_4_M169.txt', 6, 8, 4)])
 def test_record_codes_cover(fname, q, n, R):
