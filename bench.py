@@ -60,7 +60,7 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
 SEED = 1
-KERNELS = {0: 'k_eval', 4: 'k_select', 1: 'k_commit', 5: 'k_post'}
+KERNELS = {0: 'k_eval', 1: 'k_commit', 5: 'k_post'}   # k_commit selects the move first
 
 
 def measured_traffic(name, kernel=None):

@@ -1,7 +1,7 @@
 // cov_batch.h -- many cells advanced by the same launches.
 //
-// A batched step is the four launches of the single-cell step (evaluator,
-// selection, commit, post) over the workgroups of every cell at once: the grid is cut into
+// A batched step is the three launches of the single-cell step (evaluator,
+// selection and commit, post) over the workgroups of every cell at once: the grid is cut into
 // consecutive ranges, cell i owning workgroups [off[i], off[i+1]), and each
 // workgroup runs the single-cell body for its cell with its index inside the
 // range.  Every cell keeps its own control block, counters, rng ring and tabu
@@ -29,12 +29,6 @@ __global__ void __launch_bounds__(WG, 4) k_eval_b(const Cell* __restrict__ cells
 {
     const int i = batch_cell(off, ncell, blockIdx.x);
     eval_body(cells[i], blockIdx.x - off[i], off[i + 1] - off[i]);
-}
-
-// one workgroup per cell
-__global__ void __launch_bounds__(WG) k_select_b(const Cell* __restrict__ cells)
-{
-    select_body(cells[blockIdx.x]);
 }
 
 __global__ void __launch_bounds__(WG) k_commit_b(const Cell* __restrict__ cells, const int32_t* __restrict__ off,

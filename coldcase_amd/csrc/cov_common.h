@@ -114,6 +114,14 @@ struct alignas(16) Ctl {
     int32_t fill_on, pad_f;
     int64_t* trace;         // per-run trace records (tcap of them), or null
     int64_t tcap;
+    // the step's selection as commit workgroup 0 publishes it, applied by the
+    // post kernel: kept apart from the fields the selection reads (cursors,
+    // generator state, `selected`), which commit workgroups that start later
+    // still read to make the same selection
+    uint64_t p_rst[4];
+    int64_t p_rgen, p_ruse, p_rlow, p_dsel, p_base, p_dv, p_t_eve;
+    uint64_t p_bpk;
+    int32_t p_bi, p_bp, p_bv, p_oldv, p_nties, p_stamp, p_ncw, p_ok;
 };
 static_assert(sizeof(Ctl) % 16 == 0, "Ctl is copied in 8-byte words");
 

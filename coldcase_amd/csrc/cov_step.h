@@ -1,6 +1,6 @@
 // cov_step.h -- the device-resident tabu step (kernel a) and its narrow phases.
 //
-// One search step of cov/engine/covsearch2e.c:1304-1580 is four launches:
+// One search step of cov/engine/covsearch2e.c:1304-1580 is three launches:
 //
 //   k_eval    (wide)  units of work over the candidate codewords:
 //                       leaving side: per (codeword, R-subset of all n
@@ -16,13 +16,13 @@
 //                     candidates meanwhile (the set of codewords to evaluate
 //                     does not depend on the order), workgroup 1 refills the
 //                     rng ring
-//   k_select  (one workgroup)  selects the move (tabu, aspiration, reservoir
-//                     ties) with a parallel prefix formulation of the
-//                     sequential scan (:1503-1518)
-//   k_commit  (wide)  --cnt[a], ++cnt[b=a+dv] over the C(n-1,R)(q-1)^R
-//                     patterns (every word once, no counter atomics, :562-590);
-//                     plane bits, block counts and the uncovered list follow
-//                     the transitions
+//   k_commit  (wide)  every workgroup first selects the move (tabu,
+//                     aspiration, reservoir ties) with a parallel prefix
+//                     formulation of the sequential scan (:1503-1518), the
+//                     same in each; then --cnt[a], ++cnt[b=a+dv] over the
+//                     C(n-1,R)(q-1)^R patterns (every word once, no counter
+//                     atomics, :562-590); plane bits, block counts and the
+//                     uncovered list follow the transitions
 //   k_post    (one workgroup)  applies the move, stamps tabu, tracks the best
 //                     code, kicks on stagnation, writes the trace record and
 //                     begins the next step: pick (:973-1016), distances,
@@ -1378,12 +1378,20 @@ __device__ __forceinline__ void wg_select(const Cell& C, Ctl* ctl, SelSh& sh)
     }
     }
     __syncthreads();
-    // the accumulators start the next step at zero
+}
+
+// After the step's commit (one workgroup, before the trace): the evaluator's
+// accumulators start the next step at zero, and the distinct codewords among
+// the first `take` candidates (fewer than nq only when --cand cuts the
+// sample).  Here rather than in the selection, which every commit workgroup
+// computes from the accumulators.  All threads.
+__device__ __forceinline__ void wg_sel_after(const Cell& C, Ctl* ctl, WgRed& red)
+{
+    const int tid = threadIdx.x;
+    const int64_t take = ctl->take;
     for (int64_t x = tid; x < (int64_t)ctl->nq * ACCW; x += WG) C.acc[x] = 0;
     if (ctl->wnow)
         for (int64_t x = tid; x < (int64_t)ctl->nq * WIDE_W; x += WG) C.accw[x] = 0;
-    // distinct codewords among the first `take` (differs from nq only when
-    // --cand cuts the sample)
     if (take < ctl->ncand) {
         int64_t cntd = 0;
         for (int64_t s0 = 0; s0 < take; s0 += WG) {
@@ -1397,10 +1405,10 @@ __device__ __forceinline__ void wg_select(const Cell& C, Ctl* ctl, SelSh& sh)
             }
             cntd += first;
         }
-        cntd = wg_sum(cntd, sh.red);
+        cntd = wg_sum(cntd, red);
         if (tid == 0) ctl->ncw = (int32_t)cntd;
-        __syncthreads();
     }
+    __syncthreads();
 }
 
 // Arrival of the nblk workgroups of one kernel, in two levels, so that the
@@ -1494,36 +1502,9 @@ __device__ __forceinline__ void eval_body(const Cell& C, int bid, int nblk)
     }
 }
 
-// The selection of one cell, one workgroup, launched after the evaluator: the
-// kernel boundary orders every evaluator write before it, so the evaluator's
-// workgroups end without a device-scope release of their own.
-__device__ __forceinline__ void select_body(const Cell& C)
-{
-    Ctl* ctl = C.ctl;
-    __shared__ SelSh sel;
-    __shared__ Ctl lc;
-    const int64_t ts0 = clk();
-    ctl_to_lds(&lc, ctl);
-    if (!lc.begun || lc.done || lc.selected) return;
-    if (threadIdx.x == 0 && lc.t_cme) lc.tph[8] += ts0 - lc.t_cme;
-    wg_select(C, &lc, sel);
-    if (threadIdx.x == 0) {
-        const int64_t te = clk();
-        lc.tph[0] += te - ts0;
-        lc.t_eve = te;
-        lc.nstale = 0;      // the evaluator has recounted them
-    }
-    ctl_from_lds(ctl, &lc);
-}
-
 __global__ void __launch_bounds__(WG, 4) k_eval(const Cell C)
 {
     eval_body(C, blockIdx.x, gridDim.x);
-}
-
-__global__ void __launch_bounds__(WG) k_select(const Cell C)
-{
-    select_body(C);
 }
 
 // kick: teleport `kick` codewords near uncovered words (:1553-1578).  All threads.
@@ -1902,10 +1883,57 @@ __device__ __forceinline__ int64_t commit_units(const Cell& C, const Ctl* ctl, i
     return d;
 }
 
-// The commit of one cell on nblk workgroups; this one is number bid.
-// The commit of a search step, on nblk workgroups: the pattern updates only.
-// The step's bookkeeping and the next begin run in post_body, launched after
-// it, so these workgroups end without a device-scope release of their own.
+// The selection's results, published by commit workgroup 0 into the p_
+// fields (the other workgroups may not have read the control block yet, and
+// read the cursors, generator state and `selected` to make the same
+// selection).  Thread 0.
+__device__ __forceinline__ void sel_publish(Ctl* g, const Ctl* l)
+{
+    for (int k = 0; k < 4; k++) g->p_rst[k] = l->rst[k];
+    g->p_rgen = l->rgen;
+    g->p_ruse = l->ruse;
+    g->p_rlow = l->rlow;
+    g->p_bi = l->bi; g->p_bp = l->bp; g->p_bv = l->bv; g->p_oldv = l->oldv;
+    g->p_dsel = l->dsel;
+    g->p_nties = l->nties;
+    g->p_stamp = l->stamp;
+    g->p_base = l->base;
+    g->p_bpk = l->bpk;
+    g->p_dv = l->dv;
+    g->p_ncw = l->ncw;
+    g->p_t_eve = l->t_eve;
+    g->p_ok = 1;
+}
+
+// The post kernel's start: the published selection into the control block
+// (its LDS copy).  Thread 0.
+__device__ __forceinline__ void sel_apply(Ctl* l)
+{
+    for (int k = 0; k < 4; k++) l->rst[k] = l->p_rst[k];
+    l->rgen = l->p_rgen;
+    l->ruse = l->p_ruse;
+    l->rlow = l->p_rlow;
+    l->bi = l->p_bi; l->bp = l->p_bp; l->bv = l->p_bv; l->oldv = l->p_oldv;
+    l->dsel = l->p_dsel;
+    l->nties = l->p_nties;
+    l->stamp = l->p_stamp;
+    l->base = l->p_base;
+    l->bpk = l->p_bpk;
+    l->dv = l->p_dv;
+    l->ncw = l->p_ncw;
+    l->t_eve = l->p_t_eve;
+    l->nstale = 0;          // the evaluator has recounted them
+    l->selected = 1;
+    l->p_ok = 0;
+}
+
+// The selection and commit of a search step, on nblk workgroups; this one is
+// number bid.  Every workgroup makes the selection itself, from the same
+// evaluator results and ring draws, so it needs no kernel of its own and no
+// workgroup waits for another; workgroup 0 publishes it.  Then the pattern
+// updates.  The step's bookkeeping and the next begin run in post_body,
+// launched after it, so these workgroups end without a device-scope release
+// of their own.
 __device__ __forceinline__ void commit_body(const Cell& C, int bid, int nblk)
 {
     // the control block read from an LDS copy (one round trip), written in
@@ -1913,10 +1941,19 @@ __device__ __forceinline__ void commit_body(const Cell& C, int bid, int nblk)
     Ctl* ctl = C.ctl;
     __shared__ Ctl lc;
     __shared__ WgRed red;
+    __shared__ SelSh sel;
     const int64_t tc0 = clk();
     ctl_to_lds(&lc, ctl);
-    if (!lc.begun || lc.done || !lc.selected) return;
-    if (bid == 0 && threadIdx.x == 0 && lc.t_eve) tph_add(ctl, 11, tc0 - lc.t_eve);
+    if (!lc.begun || lc.done || lc.selected) return;
+    wg_select(C, &lc, sel);
+    __syncthreads();
+    if (bid == 0 && threadIdx.x == 0) {
+        const int64_t te = clk();
+        if (lc.t_cme) tph_add(ctl, 8, tc0 - lc.t_cme);
+        tph_add(ctl, 0, te - tc0);
+        lc.t_eve = te;
+        sel_publish(ctl, &lc);
+    }
     const int64_t gt = (int64_t)bid * WG + threadIdx.x;
     __shared__ int64_t ctab[MAXN * 16];
     commit_table(C, lc.bpk, ctab);
@@ -1934,8 +1971,13 @@ __device__ __forceinline__ void post_body(const Cell& C)
     const int64_t tp0 = clk();
     ctl_to_lds(&lc, C.ctl);
     Ctl* L = &lc;
-    if (!L->begun || L->done || !L->selected) return;
-    if (threadIdx.x == 0 && L->t_eve) L->tph[9] += tp0 - L->t_eve;
+    if (!L->begun || L->done || !L->p_ok) return;
+    if (threadIdx.x == 0) {
+        sel_apply(L);
+        if (L->t_eve) L->tph[9] += tp0 - L->t_eve;
+    }
+    __syncthreads();
+    wg_sel_after(C, L, sh.red);
     wg_post(C, L, sh);
     wg_begin(C, L, sh);
     if (threadIdx.x == 0) L->t_cme = clk();

@@ -335,7 +335,6 @@ int list_rebuild(cov_cell* h)
 int launch_step(cov_cell* h)
 {
     RC(launch_t(h, 0, k_eval, dim3(h->eval_groups), dim3(WG), 0, h->h));
-    RC(launch_t(h, 4, k_select, dim3(1), dim3(WG), 0, h->h));
     RC(launch_t(h, 1, k_commit, dim3(h->commit_groups), dim3(WG), 0, h->h));
     RC(launch_t(h, 5, k_post, dim3(1), dim3(WG), 0, h->h));
     return COV_OK;
@@ -349,7 +348,6 @@ int ensure_graph(cov_cell* h)
     HCK(h, hipStreamBeginCapture(h->stream, hipStreamCaptureModeRelaxed));
     for (int s = 0; s < GRAPH_STEPS; s++) {
         hipLaunchKernelGGL(k_eval, dim3(h->eval_groups), dim3(WG), 0, h->stream, h->h);
-        hipLaunchKernelGGL(k_select, dim3(1), dim3(WG), 0, h->stream, h->h);
         hipLaunchKernelGGL(k_commit, dim3(h->commit_groups), dim3(WG), 0, h->stream, h->h);
         hipLaunchKernelGGL(k_post, dim3(1), dim3(WG), 0, h->stream, h->h);
     }
@@ -1694,7 +1692,6 @@ int cov_batch_launch(cov_batch_t* b, int64_t nsteps, int mode)
             HCK(b, hipStreamBeginCapture(b->stream, hipStreamCaptureModeRelaxed));
             for (int st = 0; st < GRAPH_STEPS; st++) {
                 hipLaunchKernelGGL(k_eval_b, dim3(b->eval_total), dim3(WG), 0, b->stream, b->d_cells, b->d_eoff, k);
-                hipLaunchKernelGGL(k_select_b, dim3(k), dim3(WG), 0, b->stream, b->d_cells);
                 hipLaunchKernelGGL(k_commit_b, dim3(b->commit_total), dim3(WG), 0, b->stream, b->d_cells, b->d_coff,
                                    k);
                 hipLaunchKernelGGL(k_post_b, dim3(k), dim3(WG), 0, b->stream, b->d_cells);
@@ -1711,7 +1708,6 @@ int cov_batch_launch(cov_batch_t* b, int64_t nsteps, int mode)
     }
     for (; left > 0; left--) {
         hipLaunchKernelGGL(k_eval_b, dim3(b->eval_total), dim3(WG), 0, b->stream, b->d_cells, b->d_eoff, k);
-        hipLaunchKernelGGL(k_select_b, dim3(k), dim3(WG), 0, b->stream, b->d_cells);
         hipLaunchKernelGGL(k_commit_b, dim3(b->commit_total), dim3(WG), 0, b->stream, b->d_cells, b->d_coff, k);
         hipLaunchKernelGGL(k_post_b, dim3(k), dim3(WG), 0, b->stream, b->d_cells);
         HCK(b, hipGetLastError());
