@@ -204,6 +204,8 @@ struct Cell {
     GPTR(int32_t) cst;           // staged selection reads: codeword, slot, digits, tabu stamp (4 each)
     GPTR(int64_t) cbs;           // staged selection reads: word index of the candidate's codeword
     GPTR(uint64_t) cps;          // staged selection reads: packed digits of the candidate's codeword
+    GPTR(uint64_t) csr;          // staged selection reads: the ring draws from the selection's cursor (SEL_WAVE + 1)
+    GPTR(int32_t) dth;           // th changes made since the last evaluator (THW per codeword), which folds them in
     GPTR(uint8_t) dist;          // distance transform output (N bytes), lazily
     GPTR(int32_t) accw;          // --wide: entering counts per slot, position and value (WIDE_W each), lazily
     GPTR(Ctl) ctl;
@@ -311,13 +313,15 @@ __host__ __device__ __forceinline__ int nib_dist(uint64_t a, uint64_t b)
 }
 
 // word w (digits wpk) became, or stopped being, covered by codeword o alone
-// (sign +1 / -1): o's leaving counts change when d(w, o) = R
+// (sign +1 / -1): o's leaving counts change when d(w, o) = R.  The change goes
+// to dth, not th: th stays as the evaluator left it while the commit's
+// workgroups select from it; the next evaluator folds dth into th.
 __device__ __forceinline__ void th_event(const Cell& C, int o, uint64_t wpk, int sign)
 {
     if ((unsigned)o >= (unsigned)C.M) return;
     const uint64_t ne = nib_ne(wpk, C.cpk[o]);
     if (__popcll(ne) != C.R) return;
-    int32_t* t = C.th + (int64_t)o * THW;
+    int32_t* t = C.dth + (int64_t)o * THW;
     atomicAdd(&t[0], sign);
     uint64_t m = ne;
     while (m) {

@@ -574,7 +574,9 @@ __device__ __forceinline__ void wg_begin(const Cell& C, Ctl* ctl, NarrowSh& sh)
     const int64_t nchunk = (ctl->ul_n + LIST_CHUNK - 1) / LIST_CHUNK;
     // the shuffle's draws (:1384-1389) are reserved now; the evaluator's
     // helper wave computes the permutation while the units run
-    if (take <= RB - 4096) wg_ensure(C, ctl, take);
+    // (and, for the one-wave selection, the take + 1 draws after them, which
+    // the evaluator stages for it)
+    if (take <= RB - 4096) wg_ensure(C, ctl, take <= SEL_WAVE ? 2 * take + 2 : take);
     if (threadIdx.x == 0) {
         if (ncand == 0) { ctl->fault = 1; ctl->done = 1; }
         if (take > RB - 4096) { ctl->fault = 2; ctl->done = 1; }
@@ -1057,7 +1059,9 @@ __device__ __forceinline__ int32_t cand_enter(const Cell& C, const Ctl* ctl, int
 // counts, made by the shuffle's workgroup once the order is known: for each
 // of the first take <= SEL_WAVE candidates in shuffled order, its codeword,
 // slot, position, new and old digit, tabu stamp and the codeword's word
-// index.  Nothing the evaluator runs changes them.  All threads of the group.
+// index; and the take + 1 ring draws from the selection's cursor (begin made
+// sure they exist).  Nothing the evaluator runs changes them.  All threads of
+// the group.
 __device__ __forceinline__ void wg_stage(const Cell& C, const Ctl* ctl, uint32_t mine)
 {
     const int take = ctl->take;
@@ -1076,6 +1080,52 @@ __device__ __forceinline__ void wg_stage(const Cell& C, const Ctl* ctl, uint32_t
         reinterpret_cast<int4*>(C.cst)[s] = make_int4(i, slot, p | (v << 8) | (cold << 16), stamp);
         C.cbs[s] = C.cidx[i];
         C.cps[s] = C.cpk[i];
+    }
+    if (s <= take) C.csr[s] = ring_at(C, ctl->ruse + s);
+    if (s == 0 && take == SEL_WAVE) C.csr[SEL_WAVE] = ring_at(C, ctl->ruse + SEL_WAVE);
+}
+
+// The one-wave selection's inputs for lane `lane` of the commit's wave 0,
+// loaded before the control block is known: the staged entries and draws of
+// candidates SEL_PER lane .. + SEL_PER - 1 (entries past `take` are stale and
+// unused; their indices are clamped so every load stays in bounds), then, from
+// those, both forms of each count (the accumulators and the kept counts, the
+// narrow and the --wide entering counts): the control block says which to
+// use.  The kept counts (th) do not change during the commit: its events go
+// to dth.
+struct SelPre {
+    int4 st[SEL_PER];
+    int64_t cb[SEL_PER];
+    uint64_t cp[SEL_PER];
+    uint64_t rv[SEL_PER];
+    uint64_t rv_last;
+    int32_t a0a[SEL_PER], apa[SEL_PER], ena[SEL_PER], a0t[SEL_PER], apt[SEL_PER], enw[SEL_PER];
+};
+__device__ __forceinline__ void sel_preload(const Cell& C, int lane, SelPre& pr)
+{
+#pragma unroll
+    for (int k = 0; k < SEL_PER; k++) {
+        const int s = SEL_PER * lane + k;
+        pr.st[k] = reinterpret_cast<const int4*>(C.cst)[s];
+        pr.cb[k] = C.cbs[s];
+        pr.cp[k] = C.cps[s];
+        pr.rv[k] = C.csr[s];
+    }
+    pr.rv_last = C.csr[SEL_WAVE];
+#pragma unroll
+    for (int k = 0; k < SEL_PER; k++) {
+        const uint32_t ci = (uint32_t)pr.st[k].x < (uint32_t)C.M ? (uint32_t)pr.st[k].x : 0u;
+        const uint32_t slot = (uint32_t)pr.st[k].y < (uint32_t)C.M ? (uint32_t)pr.st[k].y : 0u;
+        const uint32_t p0 = (uint32_t)pr.st[k].z & 255u, v0 = ((uint32_t)pr.st[k].z >> 8) & 255u;
+        const uint32_t p = p0 < (uint32_t)C.n ? p0 : 0u, v = v0 < (uint32_t)C.q ? v0 : 0u;
+        const int32_t* a = C.acc + (int64_t)slot * ACCW;
+        const int32_t* t = C.th + (int64_t)ci * THW;
+        pr.a0a[k] = a[0];
+        pr.apa[k] = a[1 + p];
+        pr.ena[k] = a[1 + MAXN + p];
+        pr.a0t[k] = t[0];
+        pr.apt[k] = t[1 + p];
+        pr.enw[k] = C.accw ? C.accw[(int64_t)slot * WIDE_W + p * MAXQ + v] : 0;
     }
 }
 
@@ -1143,7 +1193,7 @@ __device__ __forceinline__ int64_t cand_delta(const Cell& C, const Ctl* ctl, uin
 // wave scan, with no workgroup barrier; the ring draws r0 .. r0 + take sit in
 // LDS, and the lane holding the selected candidate publishes it.  Same
 // results as the workgroup form.  Called by all threads.
-__device__ __forceinline__ void wave_select(const Cell& C, Ctl* ctl, SelSh& sh)
+__device__ __forceinline__ void wave_select(const Cell& C, Ctl* ctl, SelSh& sh, const SelPre& pr)
 {
     const int tid = threadIdx.x;
     if (tid < 64) {
@@ -1154,42 +1204,22 @@ __device__ __forceinline__ void wave_select(const Cell& C, Ctl* ctl, SelSh& sh)
         int64_t cd[SEL_PER], cbase[SEL_PER];
         uint64_t cpks[SEL_PER];
         bool valid[SEL_PER];
-        uint64_t rvs[SEL_PER];
-        // the reads in two stages across the lane's candidates, each stage's
-        // loads issued together (a candidate past `take` reads candidate 0's
-        // data, in bounds, and is not used): the ring draws and what the
-        // evaluator's shuffle group staged, then the counts
-        const bool th_on = ctl->th_on != 0;
-        int slot[SEL_PER];
+        // the loads were made by sel_preload: pick the forms the control
+        // block names
+        const bool th_on = ctl->th_on != 0, wnow = ctl->wnow != 0;
         int32_t a0[SEL_PER], ap[SEL_PER], an[SEL_PER], stamp[SEL_PER];
-        int4 st[SEL_PER];
 #pragma unroll
         for (int k = 0; k < SEL_PER; k++) {
-            const int64_t s = (int64_t)SEL_PER * lane + k;
-            const int64_t s0 = s < take ? s : 0;
-            rvs[k] = ring_at(C, r0 + (s <= take ? s : 0));
-            st[k] = reinterpret_cast<const int4*>(C.cst)[s0];
-            cbase[k] = C.cbs[s0];
-            cpks[k] = C.cps[s0];
-        }
-#pragma unroll
-        for (int k = 0; k < SEL_PER; k++) {
-            ci[k] = st[k].x;
-            slot[k] = st[k].y;
-            cp[k] = st[k].z & 255;
-            cv[k] = (st[k].z >> 8) & 255;
-            cold[k] = st[k].z >> 16;
-            stamp[k] = st[k].w;
-            const int32_t* a = C.acc + (int64_t)slot[k] * ACCW;
-            if (th_on) {
-                const int32_t* t = C.th + (int64_t)ci[k] * THW;
-                a0[k] = t[0];
-                ap[k] = t[1 + cp[k]];
-            } else {
-                a0[k] = a[0];
-                ap[k] = a[1 + cp[k]];
-            }
-            an[k] = cand_enter(C, ctl, slot[k], cp[k], cv[k]);
+            ci[k] = pr.st[k].x;
+            cp[k] = pr.st[k].z & 255;
+            cv[k] = (pr.st[k].z >> 8) & 255;
+            cold[k] = pr.st[k].z >> 16;
+            stamp[k] = pr.st[k].w;
+            cbase[k] = pr.cb[k];
+            cpks[k] = pr.cp[k];
+            a0[k] = th_on ? pr.a0t[k] : pr.a0a[k];
+            ap[k] = th_on ? pr.apt[k] : pr.apa[k];
+            an[k] = wnow ? pr.enw[k] : pr.ena[k];
         }
 #pragma unroll
         for (int k = 0; k < SEL_PER; k++) {
@@ -1204,9 +1234,9 @@ __device__ __forceinline__ void wave_select(const Cell& C, Ctl* ctl, SelSh& sh)
 #pragma unroll
         for (int k = 0; k < SEL_PER; k++) {
             const int64_t s = (int64_t)SEL_PER * lane + k;
-            if (s <= take) sh.rv[s] = rvs[k];
+            if (s <= take) sh.rv[s] = pr.rv[k];
         }
-        if (lane == 0 && take == SEL_WAVE) sh.rv[SEL_WAVE] = ring_at(C, r0 + SEL_WAVE);
+        if (lane == 0 && take == SEL_WAVE) sh.rv[SEL_WAVE] = pr.rv_last;
         // exclusive prefix minimum of the valid deltas
         int64_t x[SEL_PER], lmin = INT64_MAX;
 #pragma unroll
@@ -1288,7 +1318,7 @@ __device__ __forceinline__ void wave_select(const Cell& C, Ctl* ctl, SelSh& sh)
 // an exclusive prefix minimum; T(s) counts ties up to s and every tie's draw
 // is the T(s)-th of the step; nties(s) = 1 + T(s) - T(last restart <= s).
 // The winner is the last restart or successful tie.  All threads of one WG.
-__device__ __forceinline__ void wg_select(const Cell& C, Ctl* ctl, SelSh& sh)
+__device__ __forceinline__ void wg_select(const Cell& C, Ctl* ctl, SelSh& sh, const SelPre& pr)
 {
     const int tid = threadIdx.x;
     if (tid == 0) ctl->rlow = ctl->ruse;
@@ -1301,7 +1331,7 @@ __device__ __forceinline__ void wg_select(const Cell& C, Ctl* ctl, SelSh& sh)
     const int64_t r0 = ctl->ruse;
     const int n = C.n, q = C.q;
     if (take <= SEL_WAVE) {
-        wave_select(C, ctl, sh);
+        wave_select(C, ctl, sh, pr);
     } else {
     int64_t carry_pm = INT64_MAX, carry_T = 0, carry_Tb = 0, winner = -1;
     for (int64_t s0 = 0; s0 < take; s0 += WG) {
@@ -1457,8 +1487,22 @@ __device__ __forceinline__ void eval_body(const Cell& C, int bid, int nblk)
     __shared__ ShufSh shs;
     __shared__ WalkTab wtab[WAVES];
     const int64_t th0 = clk();
+    // this workgroup's slice of the kept-count changes since the last
+    // evaluator, loaded with the control block and folded into th (their
+    // rows are not being recounted: post cleared those changes)
+    const int64_t nth = (int64_t)C.M * THW;
+    const int64_t xf = (int64_t)bid * WG + threadIdx.x, sf = (int64_t)nblk * WG;
+    int32_t dv0 = 0;
+    if (xf < nth) dv0 = C.dth[xf];
     ctl_to_lds(&lc, ctl);
     if (!lc.begun || lc.done || lc.selected) return;
+    if (lc.th_on) {
+        if (dv0) { atomicAdd(&C.th[xf], dv0); C.dth[xf] = 0; }
+        for (int64_t x = xf + sf; x < nth; x += sf) {
+            const int32_t v = C.dth[x];
+            if (v) { atomicAdd(&C.th[x], v); C.dth[x] = 0; }
+        }
+    }
     const int wv = threadIdx.x >> 6;
     if (bid == 0 && threadIdx.x == 0 && lc.t_cme) tph_add(ctl, 10, th0 - lc.t_cme);
     // workgroup 0: the shuffle.  The rng refill, keeping ~4096 draws ahead:
@@ -1674,7 +1718,11 @@ __device__ __forceinline__ void wg_post(const Cell& C, Ctl* ctl, NarrowSh& sh)
     if (ctl->th_on) {
         __syncthreads();
         const int ns = ctl->nstale;
-        for (int x = threadIdx.x; x < ns * THW; x += WG) C.th[(int64_t)ctl->stale[x / THW] * THW + x % THW] = 0;
+        for (int x = threadIdx.x; x < ns * THW; x += WG) {
+            const int64_t at = (int64_t)ctl->stale[x / THW] * THW + x % THW;
+            C.th[at] = 0;
+            C.dth[at] = 0;
+        }
     }
     if (threadIdx.x == 0) {
         if (ctl->trace && ctl->tpos < ctl->tcap) {
@@ -1943,9 +1991,13 @@ __device__ __forceinline__ void commit_body(const Cell& C, int bid, int nblk)
     __shared__ WgRed red;
     __shared__ SelSh sel;
     const int64_t tc0 = clk();
+    // wave 0: the one-wave selection's loads, issued before (and arriving
+    // with) the control block's
+    SelPre pre;
+    if (threadIdx.x < 64) sel_preload(C, threadIdx.x, pre);
     ctl_to_lds(&lc, ctl);
     if (!lc.begun || lc.done || lc.selected) return;
-    wg_select(C, &lc, sel);
+    wg_select(C, &lc, sel, pre);
     __syncthreads();
     if (bid == 0 && threadIdx.x == 0) {
         const int64_t te = clk();
