@@ -313,7 +313,10 @@ def bench_transform(a, reps=20, warmup=2):
            'hist': [int(x) for x in hist],
            'roofline': {'bound': 'hbm', 'achieved': round(ach, 1), 'peak': HBM_PEAK_GBS, 'unit': 'GB/s',
                         'frac': round(ach / HBM_PEAK_GBS, 4), 'alg_bytes_per_transform': alg_t,
-                        'traffic': traffic, 'kernel': 'k_dt_seed+k_dt_col'},
+                        'traffic': traffic, 'kernel': 'k_dt_seed+k_dt_tcol',
+                        # what any transform moves at least: the distances written once and the
+                        # codeword bitmap read once
+                        'min_bytes_per_transform': N + N // 8},
            'recount': {'avg_us': round(rc_us, 2), 'alg_bytes': alg_r,
                        'achieved_GBs': round(alg_r / (rc_us * 1e-6) / 1e9, 1),
                        'atomics_per_s': round(M * V / (rc_us * 1e-6), 1)}}
@@ -337,6 +340,8 @@ def bench_transform(a, reps=20, warmup=2):
         tr = traffic / (dt_us * 1e-6) / 1e9
         out['roofline']['traffic_achieved'] = round(tr, 1)
         out['roofline']['traffic_frac'] = round(tr / HBM_PEAK_GBS, 4)
+    mb = (N + N // 8) / (dt_us * 1e-6) / 1e9
+    out['roofline']['min_bytes_frac'] = round(mb / HBM_PEAK_GBS, 4)
     return out
 
 

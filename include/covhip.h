@@ -255,17 +255,17 @@ int     cov_set_transform_tile(cov_cell_t *h, int lds_bytes, int threads);
  * stream records a start and a stop event around its own execution
  * (hipExtLaunchKernelGGL), read back in batches with no host sync between
  * launches; search steps are timed in plain mode only.  Read totals in ms and
- * launch counts for kernel kinds 0 = evaluator, 1 = commit, 2 = distance
- * transform pass, 3 = recount, 4 = selection, 5 = post, 6 = count transform
- * pass (nkinds <= 7). */
+ * launch counts for kernel kinds 0 = evaluator, 1 = selection and commit,
+ * 2 = distance transform pass, 3 = recount, 4 = (unused: the selection runs in
+ * the commit), 5 = post, 6 = count transform pass (nkinds <= 7). */
 int     cov_timing(cov_cell_t *h, int on);
 int     cov_timing_read(cov_cell_t *h, double *ms, int64_t *launches, int nkinds);
 /* Diagnostics: microseconds spent since cov_search_init, from the device wall
- * clock, in: 0 select, 1 post, 2 begin, 3 pick, 4 list upkeep, 5 shuffle,
+ * clock, in: 0 select (the commit's start to the end of its selection, in
+ * its workgroup 0), 1 post, 2 begin, 3 pick, 4 list upkeep, 5 shuffle,
  * 6 rng refill, 7 kicks, 8 the evaluator's span (previous begin's end ->
- * selection start), 9 the commit's span (selection end -> post start),
- * 10 gap previous begin -> evaluator, 11 gap selection -> commit
- * (n <= 12 entries). */
+ * commit start), 9 the commit's span (selection end -> post start),
+ * 10 gap previous begin -> evaluator, 11 (unused) (n <= 12 entries). */
 int     cov_search_phases(cov_cell_t *h, double *us, int n);
 /* Algorithmic bytes since the last cov_search_init, 2 B per counter read or
  * update (the uint16 cnt): what this engine's evaluator and commit kernels
@@ -279,9 +279,9 @@ int     cov_search_bytes(cov_cell_t *h, int64_t *eval_bytes, int64_t *commit_byt
 /* ------------------------------------------------------------ batches */
 /* Many cells, each created, configured and initialised on its own (any
  * shapes), advanced by the same launches: every batched step is one
- * evaluator, one selection, one commit and one post launch; the evaluator's
+ * evaluator, one selection-and-commit and one post launch; the evaluator's
  * and the commit's workgroups are split among the cells (each cell's share
- * is its cov_set_launch groups, fixed at creation), the other two run one
+ * is its cov_set_launch groups, fixed at creation), the post runs one
  * workgroup per cell.  A
  * cell's trajectory in a batch is the one it has alone.  The reference runs
  * one cell per process (cov/engine/covsearch2e.c:1304 is a single-cell loop);
