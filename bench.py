@@ -633,7 +633,7 @@ def main():
     ap.add_argument('--serve-moves', type=int, default=100000)
     ap.add_argument('--tick', type=int, default=240, help='serve: steps per batched tick at most')
     ap.add_argument('--budget-gb', type=float, default=64.0, help='serve: HBM budget')
-    ap.add_argument('--lane-cap', type=int, default=8, help='serve: searches a lane advances at once (0: all)')
+    ap.add_argument('--lane-cap', type=int, default=16, help='serve: searches a lane advances at once (0: all)')
     a = ap.parse_args()
     if a.gpus != 1:
         print('bench.py measures one GPU: every box here has one GPU, and the search path shards by '
