@@ -530,11 +530,10 @@ __host__ __device__ inline void dt_hist_tile(const DtPass& p, const DtTile& T, i
 // the last pass's statistics of a workgroup into its copy of the global bins.
 // Light: [0] #{values <= rcut}, [1] the maximum; full: bin d = #{values == d},
 // d <= n, and bin n + 1 the unreached.  Wave sums first, then one LDS atomic
-// per wave and bin, then one global atomic per bin.  All threads.
-__device__ inline void dt_stats_flush(const DtPass& p, const DtHist& h, const DtStat& st, unsigned int* lh,
-                                      unsigned long long* hist)
+// per wave and bin (dt_stats_lds, which may run several times), then one
+// global atomic per bin (dt_stats_global).  All threads.
+__device__ inline void dt_stats_lds(const DtPass& p, const DtHist& h, const DtStat& st, unsigned int* lh)
 {
-    __syncthreads();                            // the rare-value bins are complete
     const bool lead = (threadIdx.x & 63) == 0;
     if (!p.full) {
         const uint32_t mb = dt_stat_max(st);
@@ -553,6 +552,9 @@ __device__ inline void dt_stats_flush(const DtPass& p, const DtHist& h, const Dt
             if (lead && c) atomicAdd(&lh[r <= p.hn ? r : p.hn + 1], (unsigned int)c);
         }
     }
+}
+__device__ inline void dt_stats_global(const DtPass& p, unsigned int* lh, unsigned long long* hist)
+{
     __syncthreads();
     const int nbin = p.full ? p.hn + 2 : 2;
     unsigned long long* hc = hist + (size_t)(blockIdx.x % DT_COPIES) * DT_CSTRIDE;
@@ -560,6 +562,13 @@ __device__ inline void dt_stats_flush(const DtPass& p, const DtHist& h, const Dt
         if (p.full || threadIdx.x == 0) atomicAdd(&hc[threadIdx.x], (unsigned long long)lh[threadIdx.x]);
         else atomicMax(&hc[1], (unsigned long long)lh[1]);
     }
+}
+__device__ inline void dt_stats_flush(const DtPass& p, const DtHist& h, const DtStat& st, unsigned int* lh,
+                                      unsigned long long* hist)
+{
+    __syncthreads();                            // the rare-value bins are complete
+    dt_stats_lds(p, h, st, lh);
+    dt_stats_global(p, lh, hist);
 }
 
 // the copies folded into hh[DT_HB] (host)
@@ -883,8 +892,6 @@ __global__ void __launch_bounds__(dt_tc_ntmax(Q, B1, B2)) k_dt_tcol(const DtPass
     extern __shared__ __align__(16) uint8_t sm[];
     __shared__ unsigned int lh[DT_HB];
     if (threadIdx.x < DT_HB) lh[threadIdx.x] = 0;
-    DtHist h{};
-    DtStat st{};
 #ifdef __HIP_DEVICE_COMPILE__
     DtBufMem m;
     m.rs = __builtin_amdgcn_make_buffer_rsrc(D, (short)0, (int)p.nbytes, 0x00020000);
@@ -897,12 +904,17 @@ __global__ void __launch_bounds__(dt_tc_ntmax(Q, B1, B2)) k_dt_tcol(const DtPass
         __syncthreads();
         dt_tc_stage_b<Q, B1, B2>(p, base, x0, threadIdx.x, m, lds);
         if (p.last) {
+            // the tile's statistics into the LDS bins (the per-thread counts
+            // live only here, not across the loads of the next tile)
             __syncthreads();
+            DtHist h{};
+            DtStat st{};
             dt_tc_stats(p, x0, threadIdx.x, blockDim.x, lds, h, st, lh);
+            dt_stats_lds(p, h, st, lh);
         }
     }
 #endif
-    if (p.last) dt_stats_flush(p, h, st, lh, hist);
+    if (p.last) dt_stats_global(p, lh, hist);
 }
 
 // the (q, b1, b2) of two-stage passes, each its own kernel: b1 = ceil(b / 2)
