@@ -144,6 +144,20 @@ __device__ inline void ctl_from_lds(Ctl* g, const Ctl* lds)
     __syncthreads();
 }
 
+// One load from every 64-byte line of the cell descriptor, issued together at
+// a kernel's start: the fields read later (in branches, after the first
+// memory round trips) then hit the scalar cache instead of each missing in
+// turn.
+template <typename T>
+__device__ __forceinline__ void desc_touch(const T& d)
+{
+    const uint32_t* w = reinterpret_cast<const uint32_t*>(&d);
+    uint32_t acc = 0;
+#pragma unroll
+    for (int k = 0; k < (int)(sizeof(T) / 64); k++) acc ^= w[16 * k];
+    asm volatile("" ::"s"(acc));
+}
+
 __device__ __forceinline__ uint64_t uniform64(uint64_t v)
 {
     const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)v);

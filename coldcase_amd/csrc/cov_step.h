@@ -377,28 +377,36 @@ __device__ __forceinline__ void wg_begin(const Cell& C, Ctl* ctl, NarrowSh& sh)
 
     const int64_t tp0 = clk();
     const int64_t u = wg_pick(C, ctl, sh);
-    if (threadIdx.x == 0) { ctl->tph[4] += tp0 - tl0; ctl->tph[3] += clk() - tp0; }
+    const int64_t ta = clk();
+    if (threadIdx.x == 0) { ctl->tph[4] += tp0 - tl0; ctl->tph[3] += ta - tp0; }
     if (u < 0) {
         if (threadIdx.x == 0) ctl->done = 1;
         __syncthreads();
         return;
     }
-    // digits of u, one position per lane of wave 0, summed in the wave
-    if (threadIdx.x < 64) {
-        uint64_t part = 0;
-        if (threadIdx.x < C.n) {
-            const int j = threadIdx.x;
-            int64_t d;
-            if (C.N <= 0xFFFFFFFFLL) d = (int64_t)(((uint32_t)u / (uint32_t)C.pw[j]) % (uint32_t)C.q);
-            else d = (u / C.pw[j]) % C.q;
-            part = (uint64_t)d << (4 * j);
+    // digits of u, computed by every wave (u is the workgroup's): position j
+    // on lane j, with q^(n-1-j) selected from the uniform power table entry by
+    // entry (an index per lane into the kernel argument would be a vector
+    // memory load), then gathered lane by lane into scalar registers
+    uint64_t upk = 0;
+    {
+        const int j = threadIdx.x & 63;
+        int64_t pwj = 1;
+#pragma unroll
+        for (int k = 0; k < MAXN; k++)
+            if (j == k) pwj = C.pw[k];
+        int64_t d = 0;
+        if (j < C.n) {
+            if (C.N <= 0xFFFFFFFFLL) d = (int64_t)(((uint32_t)u / (uint32_t)pwj) % (uint32_t)C.q);
+            else d = (u / pwj) % C.q;
         }
-        part = (uint64_t)wave_sum64((int64_t)part);
-        if (threadIdx.x == 0) sh.v64[4] = (int64_t)part;
+#pragma unroll
+        for (int k = 0; k < MAXN; k++)
+            upk |= (uint64_t)__builtin_amdgcn_readlane((int)d, k) << (4 * k);
     }
-    __syncthreads();
-    const uint64_t upk = (uint64_t)sh.v64[4];
 
+    const int64_t tbb = clk();
+    if (threadIdx.x == 0) ctl->tph[7] += tbb - ta;
     // One pass over the codewords, SCAN_ILP loads in flight per thread: the
     // minimum distance to u, and the codewords at distance R+1 as one bit
     // each (a ballot per wave, in LDS).  R+1 is the target distance unless no
@@ -455,10 +463,28 @@ __device__ __forceinline__ void wg_begin(const Cell& C, Ctl* ctl, NarrowSh& sh)
     };
     mark_pass(C.R + 1, true);
     __syncthreads();
-    // a scan of the mask words' counts gives each word its first slot
+    const int64_t tcc = clk();
+    if (threadIdx.x == 0) ctl->tph[11] += tcc - tbb;
+    // a scan of the mask words' counts gives each word its first slot: for
+    // one-round codes (at most 64 words) every wave scans them itself, word
+    // j on lane j, into qf (no barrier); else a workgroup scan into LDS
     constexpr int QPT = (MAX_M_WORDS + WG - 1) / WG;   // mask words per thread
+    static_assert(SCAN_ILP * WG <= 64 * 64, "one-round codes have at most 64 mask words");
     int64_t nq;
+    int32_t qf = 0;
     auto slots = [&]() {
+        if (one_round) {
+            const int32_t cj = lane < nwm ? (int32_t)__popcll(sh.qm[lane]) : 0;
+            int32_t inc = cj;
+#pragma unroll
+            for (int o = 1; o < 64; o <<= 1) {
+                const int32_t y = __shfl_up(inc, o, 64);
+                if (lane >= o) inc += y;
+            }
+            qf = inc - cj;
+            nq = __builtin_amdgcn_readlane(inc, 63);
+            return;
+        }
         int cnt = 0;
 #pragma unroll
         for (int k = 0; k < QPT; k++) {
@@ -477,6 +503,8 @@ __device__ __forceinline__ void wg_begin(const Cell& C, Ctl* ctl, NarrowSh& sh)
         __syncthreads();
     };
     slots();
+    const int64_t tdd = clk();
+    if (threadIdx.x == 0) ctl->tph[10] += tdd - tcc;
     // u is uncovered, so no codeword is nearer than R+1: with one at R+1 the
     // target is R+1, else the minimum distance (its own pass)
     int dtar = C.R + 1;
@@ -513,28 +541,32 @@ __device__ __forceinline__ void wg_begin(const Cell& C, Ctl* ctl, NarrowSh& sh)
         dst[1] = hi;
     };
     if (one_round) {
-        // each wave, round by round, emits the marked codewords of its mask
-        // word one after another (the loop is uniform): the digits come from
-        // the lane that holds them, and lane p < n writes position p at its
-        // rank among the differing positions
+        // each wave, round by round, its mask word: a marked lane writes its
+        // codeword and its differing positions at its slot (the word's first
+        // slot plus the lane's rank among the word's marked lanes); every
+        // marked codeword differs from u in dtar positions, so the loop over
+        // them is uniform
+        const int wvu = __builtin_amdgcn_readfirstlane(wv);
 #pragma unroll
         for (int k = 0; k < SCAN_ILP; k++) {
             if (k * WG >= C.M) break;
-            const int wi = k * WAVES + wv;          // codewords k WG + 64 wv + lane
+            const int wi = k * WAVES + wvu;         // codewords k WG + 64 wv + lane
             if (wi < nwm) {
-                uint64_t m = sh.qm[wi];
-                int64_t slot = sh.qfirst[wi];
-                while (m) {
-                    const int l = __ffsll((unsigned long long)m) - 1;
-                    m &= m - 1;
-                    const uint64_t ci = __shfl((unsigned long long)c[k], l, 64);
-                    const uint64_t ne = nib_ne(ci, upk);
-                    if (lane < MAXN && ((ne >> (4 * lane)) & 1)) {
-                        const int rank = __popcll(ne & ((1ull << (4 * lane)) - 1));
-                        C.slot_pos[slot * MAXN + rank] = (uint8_t)lane;
+                const uint64_t m = sh.qm[wi];
+                const int32_t f = __builtin_amdgcn_readlane(qf, wi);
+                if ((m >> lane) & 1) {
+                    const int64_t slot = f + __popcll(m & ((1ull << lane) - 1));
+                    C.slot_cw[slot] = k * WG + 64 * wvu + lane;
+                    uint64_t ne = nib_ne(c[k], upk), lo = 0, hi = 0;
+                    for (int t = 0; ne; t++) {
+                        const uint64_t pos = (uint64_t)((__ffsll((unsigned long long)ne) - 1) >> 2);
+                        ne &= ne - 1;
+                        if (t < 8) lo |= pos << (8 * t);
+                        else hi |= pos << (8 * (t - 8));
                     }
-                    if (lane == 0) C.slot_cw[slot] = k * WG + 64 * wv + l;
-                    slot++;
+                    uint64_t* dst = reinterpret_cast<uint64_t*>(C.slot_pos + slot * MAXN);
+                    dst[0] = lo;
+                    dst[1] = hi;
                 }
             }
         }
@@ -563,6 +595,8 @@ __device__ __forceinline__ void wg_begin(const Cell& C, Ctl* ctl, NarrowSh& sh)
             emit(i, C.cpk[i]);
         }
     }
+    const int64_t tee = clk();
+    if (threadIdx.x == 0) ctl->tph[5] += tee - tdd;
     // list decision and unit counts; with --wide every move of the candidate
     // codewords is a candidate while the list is used (:1353-1376)
     const int use_list = ctl->ul_valid &&
@@ -577,6 +611,7 @@ __device__ __forceinline__ void wg_begin(const Cell& C, Ctl* ctl, NarrowSh& sh)
     // (and, for the one-wave selection, the take + 1 draws after them, which
     // the evaluator stages for it)
     if (take <= RB - 4096) wg_ensure(C, ctl, take <= SEL_WAVE ? 2 * take + 2 : take);
+    if (threadIdx.x == 0) ctl->tph[6] += clk() - tee;
     if (threadIdx.x == 0) {
         if (ncand == 0) { ctl->fault = 1; ctl->done = 1; }
         if (take > RB - 4096) { ctl->fault = 2; ctl->done = 1; }
@@ -1504,7 +1539,6 @@ __device__ __forceinline__ void eval_body(const Cell& C, int bid, int nblk)
         }
     }
     const int wv = threadIdx.x >> 6;
-    if (bid == 0 && threadIdx.x == 0 && lc.t_cme) tph_add(ctl, 10, th0 - lc.t_cme);
     // workgroup 0: the shuffle.  The rng refill, keeping ~4096 draws ahead:
     // on three or more groups a chunk of RNG_CHUNK draws in parts by the
     // waves of workgroups 1 .. nfw (when begin decided on one), else wave 0
@@ -1515,7 +1549,6 @@ __device__ __forceinline__ void eval_body(const Cell& C, int bid, int nblk)
     if (bid == 0) {
         const uint32_t mine = wg_shuffle(C, &lc, shs);
         wg_stage(C, &lc, mine);
-        if (threadIdx.x == 0) tph_add(ctl, 5, clk() - th0);
     }
     if (nfw > 0 && bid >= 1 && bid <= nfw) {
         if (lc.fill_on) {
@@ -1523,10 +1556,8 @@ __device__ __forceinline__ void eval_body(const Cell& C, int bid, int nblk)
         } else if (bid == 1 && wv == 0) {
             if (lc.rgen < lc.ruse + 4096) wave_ring_fill(C, ctl, lc.ruse + 4096, lc.budget);
         }
-        if (bid == 1 && threadIdx.x == 0) tph_add(ctl, 6, clk() - th0);
     } else if (nfw == 0 && bid == rng_wg && wv == 0) {
         if (lc.rgen < lc.ruse + 4096) wave_ring_fill(C, ctl, lc.ruse + 4096, lc.budget);
-        if (threadIdx.x == 0) tph_add(ctl, 6, clk() - th0);
     }
     if (nfw > 0) {
         // units over the waves of workgroups past the filling ones
@@ -1887,46 +1918,18 @@ __device__ __forceinline__ int64_t commit_units(const Cell& C, const Ctl* ctl, i
                 ob_clr = mb;
             }
         }
-        // the planes: one atomic per plane word and run
-        {
-            const int64_t key = (za | oa_set | oa_clr) ? (a >> 5) : -1;
-            bool head;
-            const bool tail = run_tail(key, &head);
-            uint32_t v[3] = {za, oa_set, oa_clr};
-            run_or<3>(head, v);
-            if (tail && key >= 0) {
-                if (v[0]) atomicOr(&C.zb[key], v[0]);
-                if (v[1]) atomicOr(&C.ob[key], v[1]);
-                if (v[2]) atomicAnd(&C.ob[key], ~v[2]);
-            }
-        }
-        {
-            const int64_t key = (zb_clr | ob_set | ob_clr) ? (b >> 5) : -1;
-            bool head;
-            const bool tail = run_tail(key, &head);
-            uint32_t v[3] = {zb_clr, ob_set, ob_clr};
-            run_or<3>(head, v);
-            if (tail && key >= 0) {
-                if (v[0]) atomicAnd(&C.zb[key], ~v[0]);
-                if (v[1]) atomicOr(&C.ob[key], v[1]);
-                if (v[2]) atomicAnd(&C.ob[key], ~v[2]);
-            }
-        }
-        // the uncovered-count blocks
-        {
-            const int64_t key = da ? (a >> BLK_SHIFT) : -1;
-            bool head;
-            const bool tail = run_tail(key, &head);
-            const int v = run_add(head, da);
-            if (tail && key >= 0 && v) atomicAdd(&C.blk[key], (uint32_t)v);
-        }
-        {
-            const int64_t key = db ? (b >> BLK_SHIFT) : -1;
-            bool head;
-            const bool tail = run_tail(key, &head);
-            const int v = run_add(head, -db);
-            if (tail && key >= 0 && v) atomicSub(&C.blk[key], (uint32_t)v);
-        }
+        // the planes and the uncovered-count blocks: an atomic per transition
+        // (each word is one unit's, so the bits are distinct and the order of
+        // the operations does not matter); most words cross no threshold, so
+        // most lanes issue none
+        if (za) atomicOr(&C.zb[a >> 5], za);
+        if (oa_set) atomicOr(&C.ob[a >> 5], oa_set);
+        if (oa_clr) atomicAnd(&C.ob[a >> 5], ~oa_clr);
+        if (zb_clr) atomicAnd(&C.zb[b >> 5], ~zb_clr);
+        if (ob_set) atomicOr(&C.ob[b >> 5], ob_set);
+        if (ob_clr) atomicAnd(&C.ob[b >> 5], ~ob_clr);
+        if (da) atomicAdd(&C.blk[a >> BLK_SHIFT], 1u);
+        if (db) atomicSub(&C.blk[b >> BLK_SHIFT], 1u);
     }
     return d;
 }
@@ -2001,7 +2004,6 @@ __device__ __forceinline__ void commit_body(const Cell& C, int bid, int nblk)
     __syncthreads();
     if (bid == 0 && threadIdx.x == 0) {
         const int64_t te = clk();
-        if (lc.t_cme) tph_add(ctl, 8, tc0 - lc.t_cme);
         tph_add(ctl, 0, te - tc0);
         lc.t_eve = te;
         sel_publish(ctl, &lc);
@@ -2021,6 +2023,7 @@ __device__ __forceinline__ void post_body(const Cell& C)
     __shared__ NarrowSh sh;
     __shared__ Ctl lc;
     const int64_t tp0 = clk();
+    desc_touch(C);
     ctl_to_lds(&lc, C.ctl);
     Ctl* L = &lc;
     if (!L->begun || L->done || !L->p_ok) return;
@@ -2030,6 +2033,7 @@ __device__ __forceinline__ void post_body(const Cell& C)
     }
     __syncthreads();
     wg_sel_after(C, L, sh.red);
+    if (threadIdx.x == 0) L->tph[8] += clk() - tp0;
     wg_post(C, L, sh);
     wg_begin(C, L, sh);
     if (threadIdx.x == 0) L->t_cme = clk();
