@@ -149,17 +149,21 @@ __device__ __forceinline__ void wg_ensure(const Cell& C, Ctl* ctl, int64_t need)
 // the rest of the start block and the counts of the 64 blocks after it; then,
 // only when the start block has no uncovered word at or after the start, the
 // words of the first non-empty block among those 64.  Past them, the general
-// scan takes over.  The word returned is the sequential scan's.
+// scan takes over.  The word returned is the sequential scan's.  When the
+// commit staged the draws from this cursor (spos1 = cursor + 1), the caller
+// has loaded them (sdr: draw tid for tid < 64, else draw 64) with its
+// control block, and the first stage reads no ring.
 constexpr int PICK_NB = WG - 64 - BLK_WORDS / 32;   // blocks after the start block read at once
 static_assert(PICK_NB > 0, "the pick's second round needs threads for the block counts");
-__device__ __forceinline__ int64_t wg_pick(const Cell& C, Ctl* ctl, NarrowSh& sh)
+__device__ __forceinline__ int64_t wg_pick(const Cell& C, Ctl* ctl, NarrowSh& sh, int64_t spos1 = 0,
+                                           uint64_t sdr = 0)
 {
     wg_ensure(C, ctl, 65);
     const int tid = threadIdx.x, wv = tid >> 6;
     const int64_t pos = ctl->ruse;
     // stage 1: the draws, each reduced mod N where it is read: probe k by
     // lane k of wave 0, the scan's start (draw 64) by every other thread
-    const uint64_t rv = ring_at(C, pos + (tid < 64 ? tid : 64));
+    const uint64_t rv = spos1 == pos + 1 ? sdr : ring_at(C, pos + (tid < 64 ? tid : 64));
     const int64_t wdraw = (int64_t)mod_magic(rv, (uint64_t)C.N, C.nmag);
     const int64_t start = tid < 64 ? 0 : wdraw;     // wave 0 reads it after the barrier below
     if (tid == 64) sh.v64[1] = wdraw;
@@ -295,7 +299,10 @@ __device__ __forceinline__ void wg_list_compact(const Cell& C, Ctl* ctl, NarrowS
 // Start step it+1 if the run allows it: pick u, collect the candidate
 // codewords (those at distance dtar from u, in index order), reserve the
 // shuffle draws and lay out the evaluator's units.  All threads of one WG.
-__device__ __forceinline__ void wg_begin(const Cell& C, Ctl* ctl, NarrowSh& sh)
+// (spw: the powers as pw_to_lds leaves them; spos1, sdr: the pick's staged
+// draws, as wg_pick takes them)
+__device__ __forceinline__ void wg_begin(const Cell& C, Ctl* ctl, NarrowSh& sh, const int64_t* spw,
+                                         int64_t spos1 = 0, uint64_t sdr = 0)
 {
     const int64_t tb0 = clk();
     // one reading of the clock for the whole workgroup, so every thread takes
@@ -376,25 +383,19 @@ __device__ __forceinline__ void wg_begin(const Cell& C, Ctl* ctl, NarrowSh& sh)
     }
 
     const int64_t tp0 = clk();
-    const int64_t u = wg_pick(C, ctl, sh);
-    const int64_t ta = clk();
-    if (threadIdx.x == 0) { ctl->tph[4] += tp0 - tl0; ctl->tph[3] += ta - tp0; }
+    const int64_t u = wg_pick(C, ctl, sh, spos1, sdr);
+    if (threadIdx.x == 0) { ctl->tph[4] += tp0 - tl0; ctl->tph[3] += clk() - tp0; }
     if (u < 0) {
         if (threadIdx.x == 0) ctl->done = 1;
         __syncthreads();
         return;
     }
     // digits of u, computed by every wave (u is the workgroup's): position j
-    // on lane j, with q^(n-1-j) selected from the uniform power table entry by
-    // entry (an index per lane into the kernel argument would be a vector
-    // memory load), then gathered lane by lane into scalar registers
+    // on lane j, then gathered lane by lane into scalar registers
     uint64_t upk = 0;
     {
         const int j = threadIdx.x & 63;
-        int64_t pwj = 1;
-#pragma unroll
-        for (int k = 0; k < MAXN; k++)
-            if (j == k) pwj = C.pw[k];
+        const int64_t pwj = spw[j < MAXN ? j : 0];
         int64_t d = 0;
         if (j < C.n) {
             if (C.N <= 0xFFFFFFFFLL) d = (int64_t)(((uint32_t)u / (uint32_t)pwj) % (uint32_t)C.q);
@@ -405,8 +406,6 @@ __device__ __forceinline__ void wg_begin(const Cell& C, Ctl* ctl, NarrowSh& sh)
             upk |= (uint64_t)__builtin_amdgcn_readlane((int)d, k) << (4 * k);
     }
 
-    const int64_t tbb = clk();
-    if (threadIdx.x == 0) ctl->tph[7] += tbb - ta;
     // One pass over the codewords, SCAN_ILP loads in flight per thread: the
     // minimum distance to u, and the codewords at distance R+1 as one bit
     // each (a ballot per wave, in LDS).  R+1 is the target distance unless no
@@ -463,8 +462,6 @@ __device__ __forceinline__ void wg_begin(const Cell& C, Ctl* ctl, NarrowSh& sh)
     };
     mark_pass(C.R + 1, true);
     __syncthreads();
-    const int64_t tcc = clk();
-    if (threadIdx.x == 0) ctl->tph[11] += tcc - tbb;
     // a scan of the mask words' counts gives each word its first slot: for
     // one-round codes (at most 64 words) every wave scans them itself, word
     // j on lane j, into qf (no barrier); else a workgroup scan into LDS
@@ -503,8 +500,6 @@ __device__ __forceinline__ void wg_begin(const Cell& C, Ctl* ctl, NarrowSh& sh)
         __syncthreads();
     };
     slots();
-    const int64_t tdd = clk();
-    if (threadIdx.x == 0) ctl->tph[10] += tdd - tcc;
     // u is uncovered, so no codeword is nearer than R+1: with one at R+1 the
     // target is R+1, else the minimum distance (its own pass)
     int dtar = C.R + 1;
@@ -595,8 +590,6 @@ __device__ __forceinline__ void wg_begin(const Cell& C, Ctl* ctl, NarrowSh& sh)
             emit(i, C.cpk[i]);
         }
     }
-    const int64_t tee = clk();
-    if (threadIdx.x == 0) ctl->tph[5] += tee - tdd;
     // list decision and unit counts; with --wide every move of the candidate
     // codewords is a candidate while the list is used (:1353-1376)
     const int use_list = ctl->ul_valid &&
@@ -611,7 +604,6 @@ __device__ __forceinline__ void wg_begin(const Cell& C, Ctl* ctl, NarrowSh& sh)
     // (and, for the one-wave selection, the take + 1 draws after them, which
     // the evaluator stages for it)
     if (take <= RB - 4096) wg_ensure(C, ctl, take <= SEL_WAVE ? 2 * take + 2 : take);
-    if (threadIdx.x == 0) ctl->tph[6] += clk() - tee;
     if (threadIdx.x == 0) {
         if (ncand == 0) { ctl->fault = 1; ctl->done = 1; }
         if (take > RB - 4096) { ctl->fault = 2; ctl->done = 1; }
@@ -1522,6 +1514,7 @@ __device__ __forceinline__ void eval_body(const Cell& C, int bid, int nblk)
     __shared__ ShufSh shs;
     __shared__ WalkTab wtab[WAVES];
     const int64_t th0 = clk();
+    desc_touch(C);
     // this workgroup's slice of the kept-count changes since the last
     // evaluator, loaded with the control block and folded into th (their
     // rows are not being recounted: post cleared those changes)
@@ -1539,6 +1532,7 @@ __device__ __forceinline__ void eval_body(const Cell& C, int bid, int nblk)
         }
     }
     const int wv = threadIdx.x >> 6;
+    if (bid == 0 && threadIdx.x == 0 && lc.t_cme) tph_add(ctl, 10, th0 - lc.t_cme);
     // workgroup 0: the shuffle.  The rng refill, keeping ~4096 draws ahead:
     // on three or more groups a chunk of RNG_CHUNK draws in parts by the
     // waves of workgroups 1 .. nfw (when begin decided on one), else wave 0
@@ -1549,6 +1543,7 @@ __device__ __forceinline__ void eval_body(const Cell& C, int bid, int nblk)
     if (bid == 0) {
         const uint32_t mine = wg_shuffle(C, &lc, shs);
         wg_stage(C, &lc, mine);
+        if (threadIdx.x == 0) tph_add(ctl, 5, clk() - th0);
     }
     if (nfw > 0 && bid >= 1 && bid <= nfw) {
         if (lc.fill_on) {
@@ -1556,8 +1551,10 @@ __device__ __forceinline__ void eval_body(const Cell& C, int bid, int nblk)
         } else if (bid == 1 && wv == 0) {
             if (lc.rgen < lc.ruse + 4096) wave_ring_fill(C, ctl, lc.ruse + 4096, lc.budget);
         }
+        if (bid == 1 && threadIdx.x == 0) tph_add(ctl, 6, clk() - th0);
     } else if (nfw == 0 && bid == rng_wg && wv == 0) {
         if (lc.rgen < lc.ruse + 4096) wave_ring_fill(C, ctl, lc.ruse + 4096, lc.budget);
+        if (threadIdx.x == 0) tph_add(ctl, 6, clk() - th0);
     }
     if (nfw > 0) {
         // units over the waves of workgroups past the filling ones
@@ -1790,12 +1787,21 @@ __host__ __device__ __forceinline__ int64_t commit_offset(const Cell& C, uint64_
     const int c = nib_get(pk, j);
     return (int64_t)(k + (k >= c) - c) * C.pw[j];
 }
-// (all threads)
-__device__ __forceinline__ void commit_table(const Cell& C, uint64_t pk, int64_t* T)
+// The powers q^(n-1-j) in LDS (spw), copied at a kernel's start by threads
+// j < MAXN, before a barrier: their reads by position index are then LDS
+// reads, where an index per lane into the descriptor would be a vector memory
+// load issued only when needed.
+__device__ __forceinline__ void pw_to_lds(const Cell& C, int64_t* spw)
+{
+    if (threadIdx.x < MAXN) spw[threadIdx.x] = C.pw[threadIdx.x];
+}
+// (all threads; spw as pw_to_lds leaves it, after a barrier)
+__device__ __forceinline__ void commit_table(const Cell& C, const int64_t* spw, uint64_t pk, int64_t* T)
 {
     for (int x = threadIdx.x; x < C.n * C.q1; x += WG) {
         const int j = x / C.q1, k = x - j * C.q1;
-        T[j * 16 + k] = commit_offset(C, pk, j, k);
+        const int c = nib_get(pk, j);
+        T[j * 16 + k] = (int64_t)(k + (k >= c) - c) * spw[j];
     }
     __syncthreads();
 }
@@ -1814,56 +1820,34 @@ __host__ __device__ __forceinline__ int64_t commit_word(const Cell& C, const int
     return a;
 }
 
-// Runs of contiguous lanes with equal keys (key < 0: no operation) combine
-// their values; the last lane of each run gets the run's total (segmented
-// inclusive scan).  Neighbouring patterns of a sphere walk are neighbouring
-// words, so a wave's plane-word and block updates come in long runs, and one
-// atomic per run replaces one per lane.  All 64 lanes.
-__device__ __forceinline__ bool run_tail(int64_t key, bool* head)
+// The subsets avoiding position bp, in their table order, are those avoiding
+// the last position (the subsets of 0 .. n-2) with every position >= bp moved
+// up by one: the table builds both from the same index combinations.  So a
+// unit's subset needs bp only after its load (sub_base: the subset of unit g
+// avoiding n-1, 0 past the end), and the load can precede the selection.
+__device__ __forceinline__ uint32_t sub_base(const Cell& C, int64_t g)
 {
-    const int lane = threadIdx.x & 63;
-    const int64_t kp = __shfl_up(key, 1, 64);
-    const int64_t kn = __shfl_down(key, 1, 64);
-    *head = lane == 0 || kp != key;
-    return lane == 63 || kn != key;
+    if (g >= (int64_t)C.E * C.P) return 0u;
+    int64_t t64;
+    const int64_t e = div_small(g, C.P, &t64);
+    return C.psub[(int64_t)(C.n - 1) * C.E + e];
 }
-template <int K>
-__device__ __forceinline__ void run_or(bool head, uint32_t (&v)[K])
+__host__ __device__ __forceinline__ uint32_t sub_avoiding(uint32_t base, int R, int bp)
 {
-    const int lane = threadIdx.x & 63;
-    int f = head ? 1 : 0;
-#pragma unroll
-    for (int o = 1; o < 64; o <<= 1) {
-        uint32_t y[K];
-#pragma unroll
-        for (int k = 0; k < K; k++) y[k] = __shfl_up(v[k], o, 64);
-        const int fy = __shfl_up(f, o, 64);
-        if (lane >= o && !f) {
-#pragma unroll
-            for (int k = 0; k < K; k++) v[k] |= y[k];
-        }
-        if (lane >= o) f |= fy;
+    uint32_t out = 0;
+    for (int i = 0; i < R; i++) {
+        const uint32_t v = (base >> (4 * i)) & 15u;
+        out |= (v + (v >= (uint32_t)bp ? 1u : 0u)) << (4 * i);
     }
-}
-__device__ __forceinline__ int run_add(bool head, int v)
-{
-    const int lane = threadIdx.x & 63;
-    int f = head ? 1 : 0;
-#pragma unroll
-    for (int o = 1; o < 64; o <<= 1) {
-        const int y = __shfl_up(v, o, 64);
-        const int fy = __shfl_up(f, o, 64);
-        if (lane >= o && !f) v += y;
-        if (lane >= o) f |= fy;
-    }
-    return v;
+    return out;
 }
 
 // the commit units: every pattern of the subsets avoiding bp, flattened; a
 // wave takes 64 consecutive patterns per round (all lanes take part in every
-// round, those past the end without an operation)
+// round, those past the end without an operation).  sub0: sub_base of the
+// thread's first unit (gt), loaded by the caller.
 __device__ __forceinline__ int64_t commit_units(const Cell& C, const Ctl* ctl, int64_t gt, int64_t nt,
-                                                const int64_t* T, bool th)
+                                                const int64_t* T, bool th, uint32_t sub0)
 {
     const int lane = threadIdx.x & 63;
     const int bi = ctl->bi, bp = ctl->bp, bv = ctl->bv;
@@ -1882,7 +1866,8 @@ __device__ __forceinline__ int64_t commit_units(const Cell& C, const Ctl* ctl, i
             int64_t t64;
             const int64_t e = div_small(g, C.P, &t64);
             const uint32_t t = (uint32_t)t64;
-            const uint32_t sub = C.psub[(int64_t)bp * C.E + e];
+            const uint32_t sb = g == gt ? sub0 : C.psub[(int64_t)(C.n - 1) * C.E + e];
+            const uint32_t sub = sub_avoiding(sb, C.R, bp);
             a = commit_word(C, T, base, sub, t);
             b = a + dv;
             // the counts and (when kept) the coverer XORs read together,
@@ -1953,6 +1938,7 @@ __device__ __forceinline__ void sel_publish(Ctl* g, const Ctl* l)
     g->p_dv = l->dv;
     g->p_ncw = l->ncw;
     g->p_t_eve = l->t_eve;
+    g->p_dpos = l->rgen >= l->ruse + 65 ? l->ruse + 1 : 0;
     g->p_ok = 1;
 }
 
@@ -1994,24 +1980,37 @@ __device__ __forceinline__ void commit_body(const Cell& C, int bid, int nblk)
     __shared__ WgRed red;
     __shared__ SelSh sel;
     const int64_t tc0 = clk();
+    desc_touch(C);
     // wave 0: the one-wave selection's loads, issued before (and arriving
     // with) the control block's
     SelPre pre;
     if (threadIdx.x < 64) sel_preload(C, threadIdx.x, pre);
+    const int64_t gt = (int64_t)bid * WG + threadIdx.x;
+    const uint32_t sub0 = sub_base(C, gt);
+    __shared__ int64_t spw[MAXN];
+    pw_to_lds(C, spw);
     ctl_to_lds(&lc, ctl);
     if (!lc.begun || lc.done || lc.selected) return;
     wg_select(C, &lc, sel, pre);
     __syncthreads();
     if (bid == 0 && threadIdx.x == 0) {
         const int64_t te = clk();
+        if (lc.t_cme) tph_add(ctl, 8, tc0 - lc.t_cme);
         tph_add(ctl, 0, te - tc0);
         lc.t_eve = te;
         sel_publish(ctl, &lc);
     }
-    const int64_t gt = (int64_t)bid * WG + threadIdx.x;
+    // workgroup 0 also stages the next pick's 65 draws (from the cursor after
+    // the selection, when they are generated: nothing refills the ring before
+    // the pick): loaded now, their round trip overlapping the units', and
+    // stored after them
+    const bool stage = bid == 0 && threadIdx.x < 65 && lc.rgen >= lc.ruse + 65;
+    uint64_t sdr = 0;
+    if (stage) sdr = ring_at(C, lc.ruse + threadIdx.x);
     __shared__ int64_t ctab[MAXN * 16];
-    commit_table(C, lc.bpk, ctab);
-    const int64_t d = commit_units(C, &lc, gt, (int64_t)nblk * WG, ctab, lc.th_on != 0);
+    commit_table(C, spw, lc.bpk, ctab);
+    const int64_t d = commit_units(C, &lc, gt, (int64_t)nblk * WG, ctab, lc.th_on != 0, sub0);
+    if (stage) C.pdr[threadIdx.x] = sdr;
     const int64_t ds = wg_sum(d, red);
     if (threadIdx.x == 0 && ds) atomicAdd(reinterpret_cast<unsigned long long*>(&ctl->dunc), (unsigned long long)ds);
 }
@@ -2024,19 +2023,27 @@ __device__ __forceinline__ void post_body(const Cell& C)
     __shared__ Ctl lc;
     const int64_t tp0 = clk();
     desc_touch(C);
+    // the pick's draws as the commit staged them, loaded with the control
+    // block (used only if staged from the cursor the pick starts at)
+    const uint64_t sdr = C.pdr[threadIdx.x < 64 ? threadIdx.x : 64];
+    __shared__ int64_t spw[MAXN];
+    pw_to_lds(C, spw);
     ctl_to_lds(&lc, C.ctl);
     Ctl* L = &lc;
     if (!L->begun || L->done || !L->p_ok) return;
+    const int64_t spos1 = L->p_dpos;
     if (threadIdx.x == 0) {
         sel_apply(L);
         if (L->t_eve) L->tph[9] += tp0 - L->t_eve;
     }
     __syncthreads();
     wg_sel_after(C, L, sh.red);
-    if (threadIdx.x == 0) L->tph[8] += clk() - tp0;
     wg_post(C, L, sh);
-    wg_begin(C, L, sh);
-    if (threadIdx.x == 0) L->t_cme = clk();
+    wg_begin(C, L, sh, spw, spos1, sdr);
+    if (threadIdx.x == 0) {
+        L->p_dpos = 0;
+        L->t_cme = clk();
+    }
     ctl_from_lds(C.ctl, &lc);
 }
 
@@ -2050,8 +2057,11 @@ __device__ __forceinline__ void commit_api_body(const Cell& C, int bid, int nblk
     if (!ctl->begun || ctl->done || !ctl->selected) return;
     const int64_t gt = (int64_t)bid * WG + threadIdx.x;
     __shared__ int64_t ctab[MAXN * 16];
-    commit_table(C, C.cpk[ctl->bi], ctab);
-    const int64_t d = commit_units(C, ctl, gt, (int64_t)nblk * WG, ctab, false);
+    __shared__ int64_t spw[MAXN];
+    pw_to_lds(C, spw);
+    __syncthreads();
+    commit_table(C, spw, C.cpk[ctl->bi], ctab);
+    const int64_t d = commit_units(C, ctl, gt, (int64_t)nblk * WG, ctab, false, sub_base(C, gt));
     const int64_t ds = wg_sum(d, red);
     if (threadIdx.x == 0 && ds) atomicAdd(reinterpret_cast<unsigned long long*>(&ctl->dunc), (unsigned long long)ds);
     if (!wg_arrive_last(C.arrive, &ctl->cmdone, &last, bid, nblk)) return;
@@ -2096,6 +2106,8 @@ __device__ __forceinline__ void begin_run_body(const Cell& C, int64_t nsteps, bo
 {
     __shared__ NarrowSh sh;
     __shared__ Ctl lc;
+    __shared__ int64_t spw[MAXN];
+    pw_to_lds(C, spw);
     ctl_to_lds(&lc, C.ctl);
     Ctl* ctl = &lc;
     if (threadIdx.x == 0) {
@@ -2107,7 +2119,7 @@ __device__ __forceinline__ void begin_run_body(const Cell& C, int64_t nsteps, bo
         }
     }
     __syncthreads();
-    wg_begin(C, ctl, sh);
+    wg_begin(C, ctl, sh, spw);
     if (threadIdx.x == 0) ctl->t_cme = clk();
     ctl_from_lds(C.ctl, &lc);
 }

@@ -103,6 +103,9 @@ int main(int argc, char** argv)
             check_subset(C, sub, R, "psub");
             for (int i = 0; i < R; i++)
                 if ((int)((sub >> (4 * i)) & 15) == p) FAIL("psub row %d holds position %d", p, p);
+            // the commit derives row p from the row avoiding n-1
+            const uint32_t derived = cov::sub_avoiding(psub[(int64_t)(n - 1) * C.E + e], R, p);
+            if (derived != sub) FAIL("row %d subset %d: derived %x, table %x", p, e, derived, sub);
         }
     for (int r = 0; r <= R; r++)
         for (int64_t x = C.bsub_off[r]; x < C.bsub_off[r + 1]; x++) check_subset(C, bsub[x], r, "bsub");
