@@ -213,6 +213,7 @@ struct Cell {
     GPTR(int32_t) acc;           // slot accumulators, ACCW per slot
     GPTR(int32_t) slot_cw;       // slot -> codeword
     GPTR(uint8_t) slot_pos;      // slot -> differing positions (MAXN each)
+    GPTR(uint64_t) slot_cpk;     // slot -> its codeword's packed digits (as cpk)
     GPTR(uint64_t) rbuf;
     GPTR(int64_t) scratch64;     // small reduction outputs
     GPTR(int32_t) arrive;        // workgroup arrival sub-counters of the single-move commit

@@ -517,10 +517,12 @@ __device__ __forceinline__ void wg_begin(const Cell& C, Ctl* ctl, NarrowSh& sh, 
     // registers and stored at once: byte t of the slot's MAXN bytes is the
     // t-th position where the codeword differs from u; the bytes past the
     // count are never read)
+    // (and the codeword's digits)
     auto emit = [&](int i, uint64_t mine) {
         const int64_t slot = sh.qfirst[i >> 6] + __popcll(sh.qm[i >> 6] & ((1ull << (i & 63)) - 1));
         const uint64_t ne = nib_ne(mine, upk);
         C.slot_cw[slot] = i;
+        C.slot_cpk[slot] = mine;
         uint64_t lo = 0, hi = 0;
         int t = 0;
 #pragma unroll
@@ -552,6 +554,7 @@ __device__ __forceinline__ void wg_begin(const Cell& C, Ctl* ctl, NarrowSh& sh, 
                 if ((m >> lane) & 1) {
                     const int64_t slot = f + __popcll(m & ((1ull << lane) - 1));
                     C.slot_cw[slot] = k * WG + 64 * wvu + lane;
+                    C.slot_cpk[slot] = c[k];
                     uint64_t ne = nib_ne(c[k], upk), lo = 0, hi = 0;
                     for (int t = 0; ne; t++) {
                         const uint64_t pos = (uint64_t)((__ffsll((unsigned long long)ne) - 1) >> 2);
@@ -1089,7 +1092,43 @@ __device__ __forceinline__ int32_t cand_enter(const Cell& C, const Ctl* ctl, int
 // index; and the take + 1 ring draws from the selection's cursor (begin made
 // sure they exist).  Nothing the evaluator runs changes them.  All threads of
 // the group.
-__device__ __forceinline__ void wg_stage(const Cell& C, const Ctl* ctl, uint32_t mine)
+// The first STAGE_SLOTS slots' records, copied into LDS by the shuffle's
+// workgroup at its start (loads issued with the control block's), so that the
+// staging below reads no memory for them.
+constexpr int STAGE_SLOTS = 128;
+struct StageSh {
+    int32_t cw[STAGE_SLOTS];
+    uint64_t pos[2 * STAGE_SLOTS];
+    uint64_t cpk[STAGE_SLOTS];
+};
+struct StageReg {
+    int32_t cw;
+    uint64_t p0, p1, cpk;
+};
+__device__ __forceinline__ StageReg stage_load(const Cell& C)
+{
+    StageReg r{0, 0, 0, 0};
+    if (threadIdx.x < STAGE_SLOTS) {
+        const int64_t s = threadIdx.x < C.M ? threadIdx.x : C.M - 1;
+        r.cw = C.slot_cw[s];
+        const uint64_t* ps = reinterpret_cast<const uint64_t*>(C.slot_pos + s * MAXN);
+        r.p0 = ps[0];
+        r.p1 = ps[1];
+        r.cpk = C.slot_cpk[s];
+    }
+    return r;
+}
+__device__ __forceinline__ void stage_to_lds(const StageReg& r, StageSh& st)
+{
+    const int t = threadIdx.x;
+    if (t < STAGE_SLOTS) {
+        st.cw[t] = r.cw;
+        st.pos[2 * t] = r.p0;
+        st.pos[2 * t + 1] = r.p1;
+        st.cpk[t] = r.cpk;
+    }
+}
+__device__ __forceinline__ void wg_stage(const Cell& C, const Ctl* ctl, uint32_t mine, const StageSh& st)
 {
     const int take = ctl->take;
     if (take > SEL_WAVE) return;
@@ -1098,15 +1137,34 @@ __device__ __forceinline__ void wg_stage(const Cell& C, const Ctl* ctl, uint32_t
     static_assert(SEL_WAVE <= WG && SEL_WAVE <= SHUF_PAR, "the staged entries come from the shuffle's own threads");
     const int s = threadIdx.x;
     if (s < take) {
-        const uint32_t k = mine;
-        int slot, p, v;
-        cand_move(C, ctl, k, &slot, &p, &v);
-        const int i = C.slot_cw[slot];
+        // cand_move's candidate, from the slot records
+        const int per = ctl->per;
+        const int slot = (int)(mine / (uint32_t)per);
+        const int r = (int)(mine - (uint32_t)slot * (uint32_t)per);
+        const bool in = slot < STAGE_SLOTS;
+        const int i = in ? st.cw[slot] : C.slot_cw[slot];
+        const uint64_t cpk = in ? st.cpk[slot] : C.slot_cpk[slot];
+        // the word index from the digits (Horner's rule: position 0 is the
+        // most significant digit)
+        int64_t idx = 0;
+        for (int j = 0; j < C.n; j++) idx = idx * C.q + nib_get(cpk, j);
+        int p, v;
+        if (ctl->wnow) {
+            p = r / C.q1;
+            const int t = r - p * C.q1;
+            const int cp = nib_get(cpk, p);
+            v = t + (t >= cp);
+        } else {
+            const uint64_t pw8 = in ? st.pos[2 * slot + (r >> 3)]
+                                    : reinterpret_cast<const uint64_t*>(C.slot_pos + (int64_t)slot * MAXN)[r >> 3];
+            p = (int)((pw8 >> (8 * (r & 7))) & 255);
+            v = nib_get(ctl->upk, p);
+        }
+        const int cold = nib_get(cpk, p);
         const int32_t stamp = C.tabu[((int64_t)i * C.n + p) * C.q + v];
-        const int cold = C.code[(int64_t)i * C.n + p];
         reinterpret_cast<int4*>(C.cst)[s] = make_int4(i, slot, p | (v << 8) | (cold << 16), stamp);
-        C.cbs[s] = C.cidx[i];
-        C.cps[s] = C.cpk[i];
+        C.cbs[s] = idx;
+        C.cps[s] = cpk;
     }
     if (s <= take) C.csr[s] = ring_at(C, ctl->ruse + s);
     if (s == 0 && take == SEL_WAVE) C.csr[SEL_WAVE] = ring_at(C, ctl->ruse + SEL_WAVE);
@@ -1522,8 +1580,13 @@ __device__ __forceinline__ void eval_body(const Cell& C, int bid, int nblk)
     const int64_t xf = (int64_t)bid * WG + threadIdx.x, sf = (int64_t)nblk * WG;
     int32_t dv0 = 0;
     if (xf < nth) dv0 = C.dth[xf];
+    // workgroup 0: the slot records its staging reads, with the control block
+    __shared__ StageSh sts;
+    StageReg srg{0, 0, 0, 0};
+    if (bid == 0) srg = stage_load(C);
     ctl_to_lds(&lc, ctl);
     if (!lc.begun || lc.done || lc.selected) return;
+    if (bid == 0) stage_to_lds(srg, sts);   // read after the shuffle's barriers
     if (lc.th_on) {
         if (dv0) { atomicAdd(&C.th[xf], dv0); C.dth[xf] = 0; }
         for (int64_t x = xf + sf; x < nth; x += sf) {
@@ -1542,7 +1605,7 @@ __device__ __forceinline__ void eval_body(const Cell& C, int bid, int nblk)
     const int rng_wg = nblk > 1 ? 1 : 0;
     if (bid == 0) {
         const uint32_t mine = wg_shuffle(C, &lc, shs);
-        wg_stage(C, &lc, mine);
+        wg_stage(C, &lc, mine, sts);
         if (threadIdx.x == 0) tph_add(ctl, 5, clk() - th0);
     }
     if (nfw > 0 && bid >= 1 && bid <= nfw) {
@@ -1799,7 +1862,7 @@ __device__ __forceinline__ void pw_to_lds(const Cell& C, int64_t* spw)
 __device__ __forceinline__ void commit_table(const Cell& C, const int64_t* spw, uint64_t pk, int64_t* T)
 {
     for (int x = threadIdx.x; x < C.n * C.q1; x += WG) {
-        const int j = x / C.q1, k = x - j * C.q1;
+        const int j = (int)div_q1(C, (uint32_t)x), k = x - j * C.q1;
         const int c = nib_get(pk, j);
         T[j * 16 + k] = (int64_t)(k + (k >= c) - c) * spw[j];
     }
@@ -1977,7 +2040,6 @@ __device__ __forceinline__ void commit_body(const Cell& C, int bid, int nblk)
     // global memory (the uncovered delta, the list's inserts, the timers)
     Ctl* ctl = C.ctl;
     __shared__ Ctl lc;
-    __shared__ WgRed red;
     __shared__ SelSh sel;
     const int64_t tc0 = clk();
     desc_touch(C);
@@ -1993,13 +2055,7 @@ __device__ __forceinline__ void commit_body(const Cell& C, int bid, int nblk)
     if (!lc.begun || lc.done || lc.selected) return;
     wg_select(C, &lc, sel, pre);
     __syncthreads();
-    if (bid == 0 && threadIdx.x == 0) {
-        const int64_t te = clk();
-        if (lc.t_cme) tph_add(ctl, 8, tc0 - lc.t_cme);
-        tph_add(ctl, 0, te - tc0);
-        lc.t_eve = te;
-        sel_publish(ctl, &lc);
-    }
+    const int64_t te = clk();
     // workgroup 0 also stages the next pick's 65 draws (from the cursor after
     // the selection, when they are generated: nothing refills the ring before
     // the pick): loaded now, their round trip overlapping the units', and
@@ -2011,8 +2067,21 @@ __device__ __forceinline__ void commit_body(const Cell& C, int bid, int nblk)
     commit_table(C, spw, lc.bpk, ctab);
     const int64_t d = commit_units(C, &lc, gt, (int64_t)nblk * WG, ctab, lc.th_on != 0, sub0);
     if (stage) C.pdr[threadIdx.x] = sdr;
-    const int64_t ds = wg_sum(d, red);
-    if (threadIdx.x == 0 && ds) atomicAdd(reinterpret_cast<unsigned long long*>(&ctl->dunc), (unsigned long long)ds);
+    // the selection's publication (the post kernel reads it), here rather than
+    // before the table's barrier, which would wait for its stores
+    if (bid == 0 && threadIdx.x == 0) {
+        if (lc.t_cme) tph_add(ctl, 8, tc0 - lc.t_cme);
+        tph_add(ctl, 0, te - tc0);
+        lc.t_eve = te;
+        sel_publish(ctl, &lc);
+    }
+    // the uncovered delta, summed per wave with no workgroup barrier: most
+    // waves have none
+    if (__ballot(d != 0)) {
+        const int64_t ws = wave_sum64(d);
+        if ((threadIdx.x & 63) == 0 && ws)
+            atomicAdd(reinterpret_cast<unsigned long long*>(&ctl->dunc), (unsigned long long)ws);
+    }
 }
 
 // After the commit, one workgroup: apply the move, stamp tabu, track the best
