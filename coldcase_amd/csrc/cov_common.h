@@ -122,10 +122,6 @@ struct alignas(16) Ctl {
     int64_t p_rgen, p_ruse, p_rlow, p_dsel, p_base, p_dv, p_t_eve;
     uint64_t p_bpk;
     int32_t p_bi, p_bp, p_bv, p_oldv, p_nties, p_stamp, p_ncw, p_ok;
-    // the ring cursor after that selection plus one, when commit workgroup 0
-    // has staged the 65 draws from it (the next pick's probes and scan start)
-    // in pdr; 0: nothing staged
-    int64_t p_dpos, pad_dp;
 };
 static_assert(sizeof(Ctl) % 16 == 0, "Ctl is copied in 8-byte words");
 
@@ -226,7 +222,6 @@ struct Cell {
     GPTR(uint64_t) cps;          // staged selection reads: packed digits of the candidate's codeword
     GPTR(uint64_t) csr;          // staged selection reads: the ring draws from the selection's cursor (SEL_WAVE + 1)
     GPTR(int32_t) dth;           // th changes made since the last evaluator (THW per codeword), which folds them in
-    GPTR(uint64_t) pdr;          // the next pick's 65 ring draws, staged by the commit (see Ctl::p_dpos)
     GPTR(uint8_t) dist;          // distance transform output (N bytes), lazily
     GPTR(int32_t) accw;          // --wide: entering counts per slot, position and value (WIDE_W each), lazily
     GPTR(Ctl) ctl;

@@ -149,21 +149,17 @@ __device__ __forceinline__ void wg_ensure(const Cell& C, Ctl* ctl, int64_t need)
 // the rest of the start block and the counts of the 64 blocks after it; then,
 // only when the start block has no uncovered word at or after the start, the
 // words of the first non-empty block among those 64.  Past them, the general
-// scan takes over.  The word returned is the sequential scan's.  When the
-// commit staged the draws from this cursor (spos1 = cursor + 1), the caller
-// has loaded them (sdr: draw tid for tid < 64, else draw 64) with its
-// control block, and the first stage reads no ring.
+// scan takes over.  The word returned is the sequential scan's.
 constexpr int PICK_NB = WG - 64 - BLK_WORDS / 32;   // blocks after the start block read at once
 static_assert(PICK_NB > 0, "the pick's second round needs threads for the block counts");
-__device__ __forceinline__ int64_t wg_pick(const Cell& C, Ctl* ctl, NarrowSh& sh, int64_t spos1 = 0,
-                                           uint64_t sdr = 0)
+__device__ __forceinline__ int64_t wg_pick(const Cell& C, Ctl* ctl, NarrowSh& sh)
 {
     wg_ensure(C, ctl, 65);
     const int tid = threadIdx.x, wv = tid >> 6;
     const int64_t pos = ctl->ruse;
     // stage 1: the draws, each reduced mod N where it is read: probe k by
     // lane k of wave 0, the scan's start (draw 64) by every other thread
-    const uint64_t rv = spos1 == pos + 1 ? sdr : ring_at(C, pos + (tid < 64 ? tid : 64));
+    const uint64_t rv = ring_at(C, pos + (tid < 64 ? tid : 64));
     const int64_t wdraw = (int64_t)mod_magic(rv, (uint64_t)C.N, C.nmag);
     const int64_t start = tid < 64 ? 0 : wdraw;     // wave 0 reads it after the barrier below
     if (tid == 64) sh.v64[1] = wdraw;
@@ -299,10 +295,8 @@ __device__ __forceinline__ void wg_list_compact(const Cell& C, Ctl* ctl, NarrowS
 // Start step it+1 if the run allows it: pick u, collect the candidate
 // codewords (those at distance dtar from u, in index order), reserve the
 // shuffle draws and lay out the evaluator's units.  All threads of one WG.
-// (spw: the powers as pw_to_lds leaves them; spos1, sdr: the pick's staged
-// draws, as wg_pick takes them)
-__device__ __forceinline__ void wg_begin(const Cell& C, Ctl* ctl, NarrowSh& sh, const int64_t* spw,
-                                         int64_t spos1 = 0, uint64_t sdr = 0)
+// (spw: the powers as pw_to_lds leaves them)
+__device__ __forceinline__ void wg_begin(const Cell& C, Ctl* ctl, NarrowSh& sh, const int64_t* spw)
 {
     const int64_t tb0 = clk();
     // one reading of the clock for the whole workgroup, so every thread takes
@@ -383,7 +377,7 @@ __device__ __forceinline__ void wg_begin(const Cell& C, Ctl* ctl, NarrowSh& sh, 
     }
 
     const int64_t tp0 = clk();
-    const int64_t u = wg_pick(C, ctl, sh, spos1, sdr);
+    const int64_t u = wg_pick(C, ctl, sh);
     if (threadIdx.x == 0) { ctl->tph[4] += tp0 - tl0; ctl->tph[3] += clk() - tp0; }
     if (u < 0) {
         if (threadIdx.x == 0) ctl->done = 1;
@@ -2001,7 +1995,6 @@ __device__ __forceinline__ void sel_publish(Ctl* g, const Ctl* l)
     g->p_dv = l->dv;
     g->p_ncw = l->ncw;
     g->p_t_eve = l->t_eve;
-    g->p_dpos = l->rgen >= l->ruse + 65 ? l->ruse + 1 : 0;
     g->p_ok = 1;
 }
 
@@ -2056,17 +2049,9 @@ __device__ __forceinline__ void commit_body(const Cell& C, int bid, int nblk)
     wg_select(C, &lc, sel, pre);
     __syncthreads();
     const int64_t te = clk();
-    // workgroup 0 also stages the next pick's 65 draws (from the cursor after
-    // the selection, when they are generated: nothing refills the ring before
-    // the pick): loaded now, their round trip overlapping the units', and
-    // stored after them
-    const bool stage = bid == 0 && threadIdx.x < 65 && lc.rgen >= lc.ruse + 65;
-    uint64_t sdr = 0;
-    if (stage) sdr = ring_at(C, lc.ruse + threadIdx.x);
     __shared__ int64_t ctab[MAXN * 16];
     commit_table(C, spw, lc.bpk, ctab);
     const int64_t d = commit_units(C, &lc, gt, (int64_t)nblk * WG, ctab, lc.th_on != 0, sub0);
-    if (stage) C.pdr[threadIdx.x] = sdr;
     // the selection's publication (the post kernel reads it), here rather than
     // before the table's barrier, which would wait for its stores
     if (bid == 0 && threadIdx.x == 0) {
@@ -2092,15 +2077,11 @@ __device__ __forceinline__ void post_body(const Cell& C)
     __shared__ Ctl lc;
     const int64_t tp0 = clk();
     desc_touch(C);
-    // the pick's draws as the commit staged them, loaded with the control
-    // block (used only if staged from the cursor the pick starts at)
-    const uint64_t sdr = C.pdr[threadIdx.x < 64 ? threadIdx.x : 64];
     __shared__ int64_t spw[MAXN];
     pw_to_lds(C, spw);
     ctl_to_lds(&lc, C.ctl);
     Ctl* L = &lc;
     if (!L->begun || L->done || !L->p_ok) return;
-    const int64_t spos1 = L->p_dpos;
     if (threadIdx.x == 0) {
         sel_apply(L);
         if (L->t_eve) L->tph[9] += tp0 - L->t_eve;
@@ -2108,11 +2089,8 @@ __device__ __forceinline__ void post_body(const Cell& C)
     __syncthreads();
     wg_sel_after(C, L, sh.red);
     wg_post(C, L, sh);
-    wg_begin(C, L, sh, spw, spos1, sdr);
-    if (threadIdx.x == 0) {
-        L->p_dpos = 0;
-        L->t_cme = clk();
-    }
+    wg_begin(C, L, sh, spw);
+    if (threadIdx.x == 0) L->t_cme = clk();
     ctl_from_lds(C.ctl, &lc);
 }
 

@@ -105,7 +105,7 @@ constexpr int GRAPH_STEPS = COV_GRAPH_STEPS;
 // device bytes of each array, in allocation order
 struct Sizes {
     int64_t cnt, plane, blk, code, cpk, cidx, tabu, fsub, psub, bsub, ssub, ul_idx, ul_pk,
-        shuf, acc, slot_cw, slot_pos, slot_cpk, rbuf, ctl, cell, small, arrive, dirty, jump, xor32, th, cst, cbs, cps, csr, dth, pdr;
+        shuf, acc, slot_cw, slot_pos, slot_cpk, rbuf, ctl, cell, small, arrive, dirty, jump, xor32, th, cst, cbs, cps, csr, dth;
 };
 
 Sizes sizes_of(const Shape& s)
@@ -143,7 +143,6 @@ Sizes sizes_of(const Shape& s)
     z.cps = SEL_WAVE * 8;
     z.csr = (SEL_WAVE + 1) * 8;
     z.dth = (int64_t)s.M * THW * 4;
-    z.pdr = 72 * 8;
     return z;
 }
 
@@ -153,7 +152,7 @@ int64_t total_bytes(const Shape& s)
     // plus the distance array and codeword bitmap the transform allocates on first use
     return z.cnt + 3 * z.plane + z.blk + 2 * z.code + z.cpk + z.cidx + z.tabu + z.fsub + z.psub +
            z.bsub + z.ssub + z.ul_idx + z.ul_pk + z.shuf + z.acc + z.slot_cw + z.slot_pos + z.slot_cpk + z.rbuf +
-           z.ctl + z.cell + z.small + z.arrive + z.dirty + z.jump + z.xor32 + z.th + z.cst + z.cbs + z.cps + z.csr + z.dth + z.pdr + (s.N + 64) +
+           z.ctl + z.cell + z.small + z.arrive + z.dirty + z.jump + z.xor32 + z.th + z.cst + z.cbs + z.cps + z.csr + z.dth + (s.N + 64) +
            ((s.N >> 5) + 2) * 4;
 }
 
@@ -629,7 +628,6 @@ int cov_create(int q, int n, int R, int M, cov_cell_t** out)
     AL(C.cps, z.cps);
     AL(C.csr, z.csr);
     AL(C.dth, z.dth);
-    AL(C.pdr, z.pdr);
 #undef AL
     C.scratch64 = (decltype(C.scratch64))h->d_small;
 
