@@ -208,7 +208,6 @@ struct Cell {
     GPTR(uint32_t) shuf;         // shuffled candidate order (original indices)
     GPTR(int32_t) acc;           // slot accumulators, ACCW per slot
     GPTR(int32_t) slot_cw;       // slot -> codeword
-    GPTR(uint8_t) slot_pos;      // slot -> differing positions (MAXN each)
     GPTR(uint64_t) slot_cpk;     // slot -> its codeword's packed digits (as cpk)
     GPTR(uint64_t) rbuf;
     GPTR(int64_t) scratch64;     // small reduction outputs
@@ -326,6 +325,13 @@ __host__ __device__ __forceinline__ uint64_t nib_ne(uint64_t a, uint64_t b)
 __host__ __device__ __forceinline__ int nib_dist(uint64_t a, uint64_t b)
 {
     return __builtin_popcountll(nib_ne(a, b));
+}
+// the t-th (from 0, increasing) position where digits a and b differ
+__host__ __device__ __forceinline__ int slot_diff(uint64_t a, uint64_t b, int t)
+{
+    uint64_t ne = nib_ne(a, b);
+    for (int k = 0; k < t; k++) ne &= ne - 1;
+    return (__builtin_ctzll(ne)) >> 2;
 }
 
 // word w (digits wpk) became, or stopped being, covered by codeword o alone
@@ -681,6 +687,18 @@ __device__ __forceinline__ int64_t wave_max64(int64_t x)
         x = y > x ? y : x;
     }
     return x;
+}
+
+// the value of the wave's first lane (lowest index) with `has` set, else
+// `none`: a ballot and two lane reads (all 64 lanes)
+__device__ __forceinline__ int64_t wave_first64(bool has, int64_t v, int64_t none)
+{
+    const uint64_t m = __ballot(has);
+    if (!m) return none;
+    const int l = __ffsll((unsigned long long)m) - 1;
+    const uint32_t lo = __builtin_amdgcn_readlane((uint32_t)v, l);
+    const uint32_t hi = __builtin_amdgcn_readlane((uint32_t)((uint64_t)v >> 32), l);
+    return (int64_t)(((uint64_t)hi << 32) | lo);
 }
 
 // all threads of the workgroup must call these

@@ -186,8 +186,10 @@ __device__ __forceinline__ int64_t wg_pick(const Cell& C, Ctl* ctl, NarrowSh& sh
         const int64_t b = sb + 1 + (tid - 64 - BLK_WORDS / 32);
         if (b < C.nblk && ld_acq32(&C.blk[b])) cand_b = b;
     }
-    cand_s = wave_min64(cand_s);
-    cand_b = wave_min64(cand_b);
+    // both candidates grow with the thread index (the plane word x, then the
+    // block b, do), so a wave's least one is its first lane's that has one
+    cand_s = wave_first64(cand_s != INT64_MAX, cand_s, INT64_MAX);
+    cand_b = wave_first64(cand_b != INT64_MAX, cand_b, INT64_MAX);
     if ((tid & 63) == 0) { sh.buf[wv] = cand_s; sh.buf[WAVES + wv] = cand_b; }
     __syncthreads();
     int64_t ms = INT64_MAX, mb = INT64_MAX;
@@ -506,37 +508,18 @@ __device__ __forceinline__ void wg_begin(const Cell& C, Ctl* ctl, NarrowSh& sh, 
             slots();
         }
     }
-    // codeword i takes slot first(i / 64) + its rank in its mask word: index order
-    // (the slot's differing positions, increasing, packed in two words in
-    // registers and stored at once: byte t of the slot's MAXN bytes is the
-    // t-th position where the codeword differs from u; the bytes past the
-    // count are never read)
-    // (and the codeword's digits)
+    // codeword i takes slot first(i / 64) + its rank in its mask word: index
+    // order.  A slot holds the codeword and its packed digits; its differing
+    // positions are those where the digits differ from u's (slot_diff).
     auto emit = [&](int i, uint64_t mine) {
         const int64_t slot = sh.qfirst[i >> 6] + __popcll(sh.qm[i >> 6] & ((1ull << (i & 63)) - 1));
-        const uint64_t ne = nib_ne(mine, upk);
         C.slot_cw[slot] = i;
         C.slot_cpk[slot] = mine;
-        uint64_t lo = 0, hi = 0;
-        int t = 0;
-#pragma unroll
-        for (int p = 0; p < MAXN; p++) {
-            const bool d = (ne >> (4 * p)) & 1;
-            const uint64_t b = (uint64_t)p << (8 * (t & 7));
-            if (d && t < 8) lo |= b;
-            if (d && t >= 8) hi |= b;
-            t += d;
-        }
-        uint64_t* dst = reinterpret_cast<uint64_t*>(C.slot_pos + slot * MAXN);
-        dst[0] = lo;
-        dst[1] = hi;
     };
     if (one_round) {
         // each wave, round by round, its mask word: a marked lane writes its
-        // codeword and its differing positions at its slot (the word's first
-        // slot plus the lane's rank among the word's marked lanes); every
-        // marked codeword differs from u in dtar positions, so the loop over
-        // them is uniform
+        // codeword at its slot (the word's first slot plus the lane's rank
+        // among the word's marked lanes)
         const int wvu = __builtin_amdgcn_readfirstlane(wv);
 #pragma unroll
         for (int k = 0; k < SCAN_ILP; k++) {
@@ -549,16 +532,6 @@ __device__ __forceinline__ void wg_begin(const Cell& C, Ctl* ctl, NarrowSh& sh, 
                     const int64_t slot = f + __popcll(m & ((1ull << lane) - 1));
                     C.slot_cw[slot] = k * WG + 64 * wvu + lane;
                     C.slot_cpk[slot] = c[k];
-                    uint64_t ne = nib_ne(c[k], upk), lo = 0, hi = 0;
-                    for (int t = 0; ne; t++) {
-                        const uint64_t pos = (uint64_t)((__ffsll((unsigned long long)ne) - 1) >> 2);
-                        ne &= ne - 1;
-                        if (t < 8) lo |= pos << (8 * t);
-                        else hi |= pos << (8 * (t - 8));
-                    }
-                    uint64_t* dst = reinterpret_cast<uint64_t*>(C.slot_pos + slot * MAXN);
-                    dst[0] = lo;
-                    dst[1] = hi;
                 }
             }
         }
@@ -855,7 +828,7 @@ __device__ __forceinline__ void eval_units(const Cell& C, const Ctl* ctl, int64_
             const int slot = (int)div_small(g - nl, per_slot_walk, &r);
             const int t = (int)div_small(r, C.E, &e64);
             const int e = (int)e64;
-            const int p = C.slot_pos[(int64_t)slot * MAXN + t];
+            const int p = slot_diff(C.slot_cpk[slot], upk, t);
             unit_walk(C, slot, p, nib_get(upk, p), e, tb);
         }
     }
@@ -1065,7 +1038,7 @@ __device__ __forceinline__ void cand_move(const Cell& C, const Ctl* ctl, uint32_
         const int cp = nib_get(C.cpk[C.slot_cw[slot]], p);
         v = t + (t >= cp);
     } else {
-        p = C.slot_pos[(int64_t)slot * MAXN + r];
+        p = slot_diff(C.slot_cpk[slot], ctl->upk, r);
         v = nib_get(ctl->upk, p);
     }
     *pslot = slot;
@@ -1092,22 +1065,18 @@ __device__ __forceinline__ int32_t cand_enter(const Cell& C, const Ctl* ctl, int
 constexpr int STAGE_SLOTS = 128;
 struct StageSh {
     int32_t cw[STAGE_SLOTS];
-    uint64_t pos[2 * STAGE_SLOTS];
     uint64_t cpk[STAGE_SLOTS];
 };
 struct StageReg {
     int32_t cw;
-    uint64_t p0, p1, cpk;
+    uint64_t cpk;
 };
 __device__ __forceinline__ StageReg stage_load(const Cell& C)
 {
-    StageReg r{0, 0, 0, 0};
+    StageReg r{0, 0};
     if (threadIdx.x < STAGE_SLOTS) {
         const int64_t s = threadIdx.x < C.M ? threadIdx.x : C.M - 1;
         r.cw = C.slot_cw[s];
-        const uint64_t* ps = reinterpret_cast<const uint64_t*>(C.slot_pos + s * MAXN);
-        r.p0 = ps[0];
-        r.p1 = ps[1];
         r.cpk = C.slot_cpk[s];
     }
     return r;
@@ -1117,8 +1086,6 @@ __device__ __forceinline__ void stage_to_lds(const StageReg& r, StageSh& st)
     const int t = threadIdx.x;
     if (t < STAGE_SLOTS) {
         st.cw[t] = r.cw;
-        st.pos[2 * t] = r.p0;
-        st.pos[2 * t + 1] = r.p1;
         st.cpk[t] = r.cpk;
     }
 }
@@ -1149,9 +1116,7 @@ __device__ __forceinline__ void wg_stage(const Cell& C, const Ctl* ctl, uint32_t
             const int cp = nib_get(cpk, p);
             v = t + (t >= cp);
         } else {
-            const uint64_t pw8 = in ? st.pos[2 * slot + (r >> 3)]
-                                    : reinterpret_cast<const uint64_t*>(C.slot_pos + (int64_t)slot * MAXN)[r >> 3];
-            p = (int)((pw8 >> (8 * (r & 7))) & 255);
+            p = slot_diff(cpk, ctl->upk, r);
             v = nib_get(ctl->upk, p);
         }
         const int cold = nib_get(cpk, p);
@@ -1576,7 +1541,7 @@ __device__ __forceinline__ void eval_body(const Cell& C, int bid, int nblk)
     if (xf < nth) dv0 = C.dth[xf];
     // workgroup 0: the slot records its staging reads, with the control block
     __shared__ StageSh sts;
-    StageReg srg{0, 0, 0, 0};
+    StageReg srg{0, 0};
     if (bid == 0) srg = stage_load(C);
     ctl_to_lds(&lc, ctl);
     if (!lc.begun || lc.done || lc.selected) return;

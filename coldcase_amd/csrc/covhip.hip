@@ -105,7 +105,7 @@ constexpr int GRAPH_STEPS = COV_GRAPH_STEPS;
 // device bytes of each array, in allocation order
 struct Sizes {
     int64_t cnt, plane, blk, code, cpk, cidx, tabu, fsub, psub, bsub, ssub, ul_idx, ul_pk,
-        shuf, acc, slot_cw, slot_pos, slot_cpk, rbuf, ctl, cell, small, arrive, dirty, jump, xor32, th, cst, cbs, cps, csr, dth;
+        shuf, acc, slot_cw, slot_cpk, rbuf, ctl, cell, small, arrive, dirty, jump, xor32, th, cst, cbs, cps, csr, dth;
 };
 
 Sizes sizes_of(const Shape& s)
@@ -127,7 +127,6 @@ Sizes sizes_of(const Shape& s)
     z.shuf = (int64_t)s.M * s.n * 4 + 4;
     z.acc = (int64_t)s.M * ACCW * 4;
     z.slot_cw = (int64_t)s.M * 4;
-    z.slot_pos = (int64_t)s.M * MAXN;
     z.slot_cpk = (int64_t)s.M * 8;
     z.rbuf = (int64_t)RB * 8;
     z.ctl = sizeof(Ctl);
@@ -151,7 +150,7 @@ int64_t total_bytes(const Shape& s)
     const Sizes z = sizes_of(s);
     // plus the distance array and codeword bitmap the transform allocates on first use
     return z.cnt + 3 * z.plane + z.blk + 2 * z.code + z.cpk + z.cidx + z.tabu + z.fsub + z.psub +
-           z.bsub + z.ssub + z.ul_idx + z.ul_pk + z.shuf + z.acc + z.slot_cw + z.slot_pos + z.slot_cpk + z.rbuf +
+           z.bsub + z.ssub + z.ul_idx + z.ul_pk + z.shuf + z.acc + z.slot_cw + z.slot_cpk + z.rbuf +
            z.ctl + z.cell + z.small + z.arrive + z.dirty + z.jump + z.xor32 + z.th + z.cst + z.cbs + z.cps + z.csr + z.dth + (s.N + 64) +
            ((s.N >> 5) + 2) * 4;
 }
@@ -612,7 +611,6 @@ int cov_create(int q, int n, int R, int M, cov_cell_t** out)
     AL(C.shuf, z.shuf);
     AL(C.acc, z.acc);
     AL(C.slot_cw, z.slot_cw);
-    AL(C.slot_pos, z.slot_pos);
     AL(C.slot_cpk, z.slot_cpk);
     AL(C.rbuf, z.rbuf);
     AL(C.ctl, z.ctl);
