@@ -1281,51 +1281,74 @@ __device__ __forceinline__ void wave_select(const Cell& C, Ctl* ctl, SelSh& sh, 
             if (s <= take) sh.rv[s] = pr.rv[k];
         }
         if (lane == 0 && take == SEL_WAVE) sh.rv[SEL_WAVE] = pr.rv_last;
+        // The scans below run on 32-bit values (a delta is a difference of
+        // sphere counts; T counts at most SEL_WAVE ties), one lane shuffle per
+        // step; the prefix and total maxima use the order of T instead.
         // exclusive prefix minimum of the valid deltas
-        int64_t x[SEL_PER], lmin = INT64_MAX;
+        int32_t x[SEL_PER], lmin = INT32_MAX;
 #pragma unroll
         for (int k = 0; k < SEL_PER; k++) {
-            x[k] = valid[k] ? cd[k] : INT64_MAX;
+            x[k] = valid[k] ? (int32_t)cd[k] : INT32_MAX;
             lmin = x[k] < lmin ? x[k] : lmin;
         }
-        int64_t pm = wave_excl(wave_incl_min(lmin), INT64_MAX);
+        int32_t im = lmin;
+#pragma unroll
+        for (int o = 1; o < 64; o <<= 1) {
+            const int32_t y = __shfl_up(im, o, 64);
+            if (lane >= o && y < im) im = y;
+        }
+        int32_t pm = __shfl_up(im, 1, 64);
+        if (lane == 0) pm = INT32_MAX;
         bool nb[SEL_PER], tie[SEL_PER];
-        int64_t nt = 0;
+        int32_t nt = 0;
 #pragma unroll
         for (int k = 0; k < SEL_PER; k++) {
-            nb[k] = valid[k] && cd[k] < pm;
-            tie[k] = valid[k] && cd[k] == pm;
+            nb[k] = valid[k] && x[k] < pm;
+            tie[k] = valid[k] && x[k] == pm;
             nt += tie[k] ? 1 : 0;
             pm = x[k] < pm ? x[k] : pm;
         }
         // T: ties up to each candidate; Tb: the last restart's T (at least 0)
-        const int64_t tincl = wave_incl_add(nt);
-        int64_t T = tincl - nt;
-        int64_t Tk[SEL_PER], mk = -1;
+        int32_t tincl = nt;
+#pragma unroll
+        for (int o = 1; o < 64; o <<= 1) {
+            const int32_t y = __shfl_up(tincl, o, 64);
+            if (lane >= o) tincl += y;
+        }
+        int32_t T = tincl - nt;
+        int32_t Tk[SEL_PER], mk = -1;
 #pragma unroll
         for (int k = 0; k < SEL_PER; k++) {
             T += tie[k] ? 1 : 0;
             Tk[k] = T;
             if (nb[k]) mk = T;
         }
-        int64_t tb = wave_excl(wave_incl_max(mk), -1);
+        // T never decreases in the candidates' order, so the lanes' mk (where
+        // set) do not either: the exclusive prefix maximum is the mk of the
+        // nearest lane below with one, the maximum that of the last such lane
+        const uint64_t mkm = __ballot(mk >= 0);
+        const uint64_t below = mkm & ((1ull << lane) - 1);
+        const int32_t mkb = __shfl(mk, below ? 63 - __clzll((long long)below) : 0, 64);
+        int32_t tb = below ? mkb : -1;
         // the draws written by other lanes of the wave are read below
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-        int64_t wl = -1;
+        int32_t wl = -1;
 #pragma unroll
         for (int k = 0; k < SEL_PER; k++) {
             if (nb[k]) tb = Tk[k];
-            const int64_t Tb = tb < 0 ? 0 : tb;
+            const int32_t Tb = tb < 0 ? 0 : tb;
             bool ev = nb[k];
             if (tie[k]) ev = mod_any(sh.rv[Tk[k] - 1], (uint64_t)(1 + Tk[k] - Tb)) == 0;
-            if (ev) wl = (int64_t)SEL_PER * lane + k;
+            if (ev) wl = SEL_PER * lane + k;
         }
-        const int64_t winner = wave_max64(wl);
-        const int64_t Tall = __shfl(tincl, 63, 64);
-        int64_t Tball = wave_max64(mk);
-        if (Tball < 0) Tball = 0;
+        // (the candidate indices grow with the lane: the largest is the last
+        // lane's that has one)
+        const uint64_t wlm = __ballot(wl >= 0);
+        const int64_t winner = wlm ? __builtin_amdgcn_readlane(wl, 63 - __clzll((long long)wlm)) : -1;
+        const int64_t Tall = __builtin_amdgcn_readlane(tincl, 63);
+        const int64_t Tball = mkm ? __builtin_amdgcn_readlane(mk, 63 - __clzll((long long)mkm)) : 0;
         const int64_t ssel = winner >= 0 ? winner : (int64_t)mod_any(sh.rv[0], (uint64_t)take);
 #pragma unroll
         for (int k = 0; k < SEL_PER; k++) {
