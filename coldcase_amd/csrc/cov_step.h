@@ -202,13 +202,12 @@ __device__ __forceinline__ int64_t wg_pick(const Cell& C, Ctl* ctl, NarrowSh& sh
     const int64_t wp = sh.v64[0];
     const int64_t start_all = sh.v64[1];
     __syncthreads();
+    // (the cursor is thread 0's: every caller reads it next after a barrier)
     if (fh >= 0) {
         if (tid == 0) ctl->ruse = pos + fh + 1;
-        __syncthreads();
         return wp;
     }
     if (tid == 0) ctl->ruse = pos + 65;
-    __syncthreads();
     if (ms != INT64_MAX) return ms;
     if (mb != INT64_MAX) {
         // stage 3: the first non-empty block after the start block
@@ -1388,6 +1387,12 @@ __device__ __forceinline__ void wave_select(const Cell& C, Ctl* ctl, SelSh& sh, 
 __device__ __forceinline__ void wg_select(const Cell& C, Ctl* ctl, SelSh& sh, const SelPre& pr)
 {
     const int tid = threadIdx.x;
+    // up to SEL_WAVE candidates: wave 0 alone, from the evaluator's staged
+    // entries and draws (begin reserved the draws; the ring is not read here)
+    if (ctl->take <= SEL_WAVE) {
+        wave_select(C, ctl, sh, pr);
+        return;
+    }
     if (tid == 0) ctl->rlow = ctl->ruse;
     __syncthreads();
     const bool short_ = ctl->rgen < ctl->ruse + ctl->take + 2;
@@ -1397,9 +1402,7 @@ __device__ __forceinline__ void wg_select(const Cell& C, Ctl* ctl, SelSh& sh, co
     const int64_t take = ctl->take, it = ctl->it, unc = ctl->unc, best = ctl->best_unc;
     const int64_t r0 = ctl->ruse;
     const int n = C.n, q = C.q;
-    if (take <= SEL_WAVE) {
-        wave_select(C, ctl, sh, pr);
-    } else {
+    {
     int64_t carry_pm = INT64_MAX, carry_T = 0, carry_Tb = 0, winner = -1;
     for (int64_t s0 = 0; s0 < take; s0 += WG) {
         const int64_t s = s0 + tid;
