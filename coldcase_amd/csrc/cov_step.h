@@ -46,6 +46,7 @@
 namespace cov {
 
 constexpr int SHUF_PAR = 1024;     // draws handled by the parallel shuffle
+constexpr int ATOM_DIRECT = 16;    // commit: lanes with transitions up to which a wave needs no runs
 constexpr int EMIT_LDS = 1024;     // begin: slots listed in LDS before their digits are read
 constexpr int SEL_PER = 4;         // candidates per lane in the one-wave selection
 constexpr int SEL_WAVE = 64 * SEL_PER;   // the most candidates it takes
@@ -1890,6 +1891,52 @@ __host__ __device__ __forceinline__ uint32_t sub_avoiding(uint32_t base, int R, 
     return out;
 }
 
+// Runs of contiguous lanes with equal keys (key < 0: no operation) combine
+// their values; the last lane of each run gets the run's total (segmented
+// inclusive scan).  Neighbouring patterns of a sphere walk are neighbouring
+// words, so a wave's plane-word and block updates come in long runs, and one
+// atomic per run replaces one per lane where a wave has many transitions.
+// All 64 lanes.
+__device__ __forceinline__ bool run_tail(int64_t key, bool* head)
+{
+    const int lane = threadIdx.x & 63;
+    const int64_t kp = __shfl_up(key, 1, 64);
+    const int64_t kn = __shfl_down(key, 1, 64);
+    *head = lane == 0 || kp != key;
+    return lane == 63 || kn != key;
+}
+template <int K>
+__device__ __forceinline__ void run_or(bool head, uint32_t (&v)[K])
+{
+    const int lane = threadIdx.x & 63;
+    int f = head ? 1 : 0;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        uint32_t y[K];
+#pragma unroll
+        for (int k = 0; k < K; k++) y[k] = __shfl_up(v[k], o, 64);
+        const int fy = __shfl_up(f, o, 64);
+        if (lane >= o && !f) {
+#pragma unroll
+            for (int k = 0; k < K; k++) v[k] |= y[k];
+        }
+        if (lane >= o) f |= fy;
+    }
+}
+__device__ __forceinline__ int run_add(bool head, int v)
+{
+    const int lane = threadIdx.x & 63;
+    int f = head ? 1 : 0;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const int y = __shfl_up(v, o, 64);
+        const int fy = __shfl_up(f, o, 64);
+        if (lane >= o && !f) v += y;
+        if (lane >= o) f |= fy;
+    }
+    return v;
+}
+
 // the commit units: every pattern of the subsets avoiding bp, flattened; a
 // wave takes 64 consecutive patterns per round (all lanes take part in every
 // round, those past the end without an operation).  sub0: sub_base of the
@@ -1951,18 +1998,64 @@ __device__ __forceinline__ int64_t commit_units(const Cell& C, const Ctl* ctl, i
                 ob_clr = mb;
             }
         }
-        // the planes and the uncovered-count blocks: an atomic per transition
-        // (each word is one unit's, so the bits are distinct and the order of
-        // the operations does not matter); most words cross no threshold, so
-        // most lanes issue none
-        if (za) atomicOr(&C.zb[a >> 5], za);
-        if (oa_set) atomicOr(&C.ob[a >> 5], oa_set);
-        if (oa_clr) atomicAnd(&C.ob[a >> 5], ~oa_clr);
-        if (zb_clr) atomicAnd(&C.zb[b >> 5], ~zb_clr);
-        if (ob_set) atomicOr(&C.ob[b >> 5], ob_set);
-        if (ob_clr) atomicAnd(&C.ob[b >> 5], ~ob_clr);
-        if (da) atomicAdd(&C.blk[a >> BLK_SHIFT], 1u);
-        if (db) atomicSub(&C.blk[b >> BLK_SHIFT], 1u);
+        // the planes and the uncovered-count blocks.  Each word is one unit's,
+        // so the bits are distinct and the operations commute: a wave with few
+        // transitions (most words cross no threshold on well covered cells)
+        // issues an atomic per transition; one with many (sparsely covered
+        // cells, where most words do) first combines runs of lanes on the same
+        // plane word or block, one atomic per run
+        const bool tr = (za | oa_set | oa_clr | zb_clr | ob_set | ob_clr) != 0;
+        if (__popcll(__ballot(tr)) <= ATOM_DIRECT) {
+            if (za) atomicOr(&C.zb[a >> 5], za);
+            if (oa_set) atomicOr(&C.ob[a >> 5], oa_set);
+            if (oa_clr) atomicAnd(&C.ob[a >> 5], ~oa_clr);
+            if (zb_clr) atomicAnd(&C.zb[b >> 5], ~zb_clr);
+            if (ob_set) atomicOr(&C.ob[b >> 5], ob_set);
+            if (ob_clr) atomicAnd(&C.ob[b >> 5], ~ob_clr);
+            if (da) atomicAdd(&C.blk[a >> BLK_SHIFT], 1u);
+            if (db) atomicSub(&C.blk[b >> BLK_SHIFT], 1u);
+            continue;
+        }
+        // the planes: one atomic per plane word and run
+        {
+            const int64_t key = (za | oa_set | oa_clr) ? (a >> 5) : -1;
+            bool head;
+            const bool tail = run_tail(key, &head);
+            uint32_t v[3] = {za, oa_set, oa_clr};
+            run_or<3>(head, v);
+            if (tail && key >= 0) {
+                if (v[0]) atomicOr(&C.zb[key], v[0]);
+                if (v[1]) atomicOr(&C.ob[key], v[1]);
+                if (v[2]) atomicAnd(&C.ob[key], ~v[2]);
+            }
+        }
+        {
+            const int64_t key = (zb_clr | ob_set | ob_clr) ? (b >> 5) : -1;
+            bool head;
+            const bool tail = run_tail(key, &head);
+            uint32_t v[3] = {zb_clr, ob_set, ob_clr};
+            run_or<3>(head, v);
+            if (tail && key >= 0) {
+                if (v[0]) atomicAnd(&C.zb[key], ~v[0]);
+                if (v[1]) atomicOr(&C.ob[key], v[1]);
+                if (v[2]) atomicAnd(&C.ob[key], ~v[2]);
+            }
+        }
+        // the uncovered-count blocks
+        {
+            const int64_t key = da ? (a >> BLK_SHIFT) : -1;
+            bool head;
+            const bool tail = run_tail(key, &head);
+            const int v = run_add(head, da);
+            if (tail && key >= 0 && v) atomicAdd(&C.blk[key], (uint32_t)v);
+        }
+        {
+            const int64_t key = db ? (b >> BLK_SHIFT) : -1;
+            bool head;
+            const bool tail = run_tail(key, &head);
+            const int v = run_add(head, -db);
+            if (tail && key >= 0 && v) atomicSub(&C.blk[key], (uint32_t)v);
+        }
     }
     return d;
 }
