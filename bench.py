@@ -631,7 +631,7 @@ def main():
     ap.add_argument('--groups', type=int, default=4, help='batched: concurrent batches, cells grouped by step cost')
     ap.add_argument('--requests', type=int, default=60, help='serve: requests in the trace')
     ap.add_argument('--serve-moves', type=int, default=100000)
-    ap.add_argument('--tick', type=int, default=240, help='serve: steps per batched tick at most')
+    ap.add_argument('--tick', type=int, default=480, help='serve: steps per batched tick at most')
     ap.add_argument('--budget-gb', type=float, default=64.0, help='serve: HBM budget')
     ap.add_argument('--lane-cap', type=int, default=16, help='serve: searches a lane advances at once (0: all)')
     a = ap.parse_args()

@@ -1485,7 +1485,8 @@ __device__ __forceinline__ void wg_select(const Cell& C, Ctl* ctl, SelSh& sh, co
 // accumulators start the next step at zero, and the distinct codewords among
 // the first `take` candidates (fewer than nq only when --cand cuts the
 // sample).  Here rather than in the selection, which every commit workgroup
-// computes from the accumulators.  All threads.
+// computes from the accumulators.  All threads; ends without a barrier (its
+// one control-block write is thread 0's).
 __device__ __forceinline__ void wg_sel_after(const Cell& C, Ctl* ctl, WgRed& red)
 {
     const int tid = threadIdx.x;
@@ -1509,7 +1510,6 @@ __device__ __forceinline__ void wg_sel_after(const Cell& C, Ctl* ctl, WgRed& red
         cntd = wg_sum(cntd, red);
         if (tid == 0) ctl->ncw = (int32_t)cntd;
     }
-    __syncthreads();
 }
 
 // Arrival of the nblk workgroups of one kernel, in two levels, so that the
@@ -2166,11 +2166,13 @@ __device__ __forceinline__ void post_body(const Cell& C)
     ctl_to_lds(&lc, C.ctl);
     Ctl* L = &lc;
     if (!L->begun || L->done || !L->p_ok) return;
+    // (no barrier between these: sel_apply writes only fields that
+    // wg_sel_after does not read, and thread 0, which wrote them, is the one
+    // that reads them first in wg_post)
     if (threadIdx.x == 0) {
         sel_apply(L);
         if (L->t_eve) L->tph[9] += tp0 - L->t_eve;
     }
-    __syncthreads();
     wg_sel_after(C, L, sh.red);
     wg_post(C, L, sh);
     wg_begin(C, L, sh, spw);
