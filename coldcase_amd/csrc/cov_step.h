@@ -928,7 +928,8 @@ __device__ __forceinline__ uint32_t wg_shuffle(const Cell& C, const Ctl* ctl, Sh
                 }
             }
         }
-        __syncthreads();
+        // (no barrier: the searches read j, off and item, not g or f; the
+        // scatter's last use of g as a cursor was before the barrier above)
 #pragma unroll
         for (int k = 0; k < SHUF_PAR / WG; k++) {
             const int s = k * WG + tid;
