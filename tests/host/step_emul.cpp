@@ -161,6 +161,22 @@ int main(int argc, char** argv)
         if (pack_index(C, w) != B.packed(B.digits(w))) FAIL("pack_index(%lld)", (long long)w);
     }
 
+    // slot_diff: the t-th position (increasing) where two words' digits
+    // differ, against a list of the differing positions made digit by digit
+    for (int i = 0; i < 2000; i++) {
+        const int64_t a = (int64_t)(sm64(rs) % (uint64_t)C.N);
+        const int64_t b = i == 0 ? a : (int64_t)(sm64(rs) % (uint64_t)C.N);
+        const std::vector<int> da = B.digits(a), db = B.digits(b);
+        std::vector<int> diff;
+        for (int j = 0; j < n; j++)
+            if (da[j] != db[j]) diff.push_back(j);
+        if (nib_dist(B.packed(da), B.packed(db)) != (int)diff.size()) FAIL("nib_dist of %lld, %lld", (long long)a, (long long)b);
+        for (int t = 0; t < (int)diff.size(); t++)
+            if (slot_diff(B.packed(da), B.packed(db), t) != diff[t])
+                FAIL("slot_diff(%lld, %lld, %d) = %d, not %d", (long long)a, (long long)b, t,
+                     slot_diff(B.packed(da), B.packed(db), t), diff[t]);
+    }
+
     // pattern t of subset `sub` around the centre: the brute-force word
     // differs from the centre exactly on the subset, there taking the k-th
     // value other than the centre's (the last position fastest)
