@@ -3,7 +3,8 @@ cell plan that cov_create builds (coldcase_amd/csrc/cov_plan.h), its subset
 tables in exact-size heap arrays under AddressSanitizer, and the index
 functions the evaluator, commit, ball and kept-count code use (pattern_word,
 the sphere-walk tables and walk_word, ball_word_at, pack_index, the division
-magics, commit_offset and commit_word) against brute-force digit arithmetic,
+magics, commit_offset and commit_word, the commit's subset rows derived from
+the row avoiding the last position, slot_diff) against brute-force digit arithmetic,
 for the workload cells and shapes at the edges of the plan: q = 2 (one
 pattern per subset), q = 16, R = 0, R = 8, walks without tables ((q-1)^ceil(R/2)
 above 128), and q^n above 2^32 (64-bit digit extraction)."""
