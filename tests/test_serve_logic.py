@@ -25,7 +25,14 @@ class _Cell:
         self.done = False
         self.fault = 0
         self.holders = 0
+        self.launch = (1024, 64)
         _Cell.live.append(self)
+
+    def get_launch(self):
+        return self.launch
+
+    def set_launch(self, eval_groups, commit_groups):
+        self.launch = (eval_groups, commit_groups)
 
     def search_params(self, **k):
         pass
