@@ -283,11 +283,9 @@ __host__ __device__ __forceinline__ int64_t ball_word_at(const Cell& C, int64_t 
 }
 
 // incremental single-ball update over the whole grid (k == 1 centre)
-__global__ void __launch_bounds__(WG) k_ball_one(const Cell C, int64_t base,
-                                                 uint64_t pk, int delta)
+__device__ inline void ball_one(const Cell& C, int64_t base, uint64_t pk, int delta, WgRed& red)
 {
     Ctl* ctl = C.ctl;
-    __shared__ WgRed red;
     const bool list_on = ctl->ul_valid != 0;
     int64_t d = 0;
     for (int64_t g = (int64_t)blockIdx.x * WG + threadIdx.x; g < C.V; g += (int64_t)gridDim.x * WG) {
@@ -298,6 +296,23 @@ __global__ void __launch_bounds__(WG) k_ball_one(const Cell C, int64_t base,
     const int64_t s = wg_sum(d, red);
     if (threadIdx.x == 0 && s) atomicAdd(reinterpret_cast<unsigned long long*>(&ctl->unc),
                                          (unsigned long long)s);
+}
+
+__global__ void __launch_bounds__(WG) k_ball_one(const Cell C, int64_t base,
+                                                 uint64_t pk, int delta)
+{
+    __shared__ WgRed red;
+    ball_one(C, base, pk, delta, red);
+}
+
+// the remove-and-repair descent's removal: the ball of the word that
+// k_rr_pick left in *sel (none when it is negative)
+__global__ void __launch_bounds__(WG) k_ball_drop(const Cell C, const int64_t* __restrict__ sel)
+{
+    __shared__ WgRed red;
+    const int64_t base = *sel;
+    if (base < 0) return;
+    ball_one(C, base, pack_index(C, base), -1, red);
 }
 
 // counts only, several centres (planes re-derived afterwards)

@@ -2345,19 +2345,49 @@ __global__ void __launch_bounds__(WG) k_ball_cw(const Cell C, int i, int delta)
     if (threadIdx.x == 0 && s) atomicAdd(reinterpret_cast<unsigned long long*>(&ctl->unc), (unsigned long long)s);
 }
 
-// words of each given centre's ball with cnt == 1 (remove-and-repair, :927-946)
+// words of each given centre's ball with cnt == 1 (remove-and-repair, :927-946);
+// a negative centre has left the list and scores none
 __global__ void __launch_bounds__(WG) k_unique_cover(const Cell C, const int64_t* centers,
                                                      int k, int64_t* out)
 {
     __shared__ WgRed red;
     for (int c = blockIdx.x; c < k; c += gridDim.x) {
         const int64_t base = centers[c];
+        if (base < 0) continue;
         const uint64_t pk = pack_index(C, base);
         int64_t u = 0;
         for (int64_t g = threadIdx.x; g < C.V; g += WG)
             u += bit_of(C.ob, ball_word_at(C, base, pk, g, nullptr));
         u = wg_sum(u, red);
         if (threadIdx.x == 0) out[c] = u;
+    }
+}
+
+// the descent's choice (:1177-1189): the first listed centre with the fewest
+// uniquely covered words leaves the list (its entry becomes -1) and its word
+// goes to sel[0] for k_ball_drop; -1 there when none is listed.  One
+// workgroup; the key (count << 32 | position) makes the first of equal
+// counts the smallest.
+__global__ void __launch_bounds__(WG) k_rr_pick(int64_t* __restrict__ centers,
+                                                const int64_t* __restrict__ uc, int k,
+                                                int64_t* __restrict__ sel)
+{
+    __shared__ WgRed red;
+    int64_t key = INT64_MAX;
+    for (int c = threadIdx.x; c < k; c += WG)
+        if (centers[c] >= 0) {
+            const int64_t kc = (uc[c] << 32) | c;
+            key = kc < key ? kc : key;
+        }
+    key = wg_min(key, red);
+    if (threadIdx.x == 0) {
+        if (key == INT64_MAX) {
+            sel[0] = -1;
+        } else {
+            const int c = (int)(key & 0xffffffff);
+            sel[0] = centers[c];
+            centers[c] = -1;
+        }
     }
 }
 

@@ -68,6 +68,10 @@ struct cov_cell {
     // count transform: (R+1) int32 layers of N words (allocated on first
     // use); layer 0 holds the map after a sum transform
     int32_t* ct = nullptr;
+    // remove-and-repair descent: centres, their unique counts and the chosen
+    // word (2 * cap + 1 int64), grown on demand
+    int64_t* rr = nullptr;
+    int64_t rr_cap = 0;
     int ct_lds = 0;                       // LDS bytes per transform tile (0: the default)
     int map_kind = -1;                    // source of the map in layer 0, -1: none
     // recorded on a batch's stream after each batched launch that includes
@@ -708,6 +712,7 @@ int cov_destroy(cov_cell_t* h)
     if (h->cwb) (void)hipFree(h->cwb);
     if (h->dtstat) (void)hipFree(h->dtstat);
     if (h->ct) (void)hipFree(h->ct);
+    if (h->rr) (void)hipFree(h->rr);
     for (const auto& p : h->pend) { (void)hipEventDestroy(p.a); (void)hipEventDestroy(p.b); }
     for (hipEvent_t e : h->evs) (void)hipEventDestroy(e);
     (void)hipStreamDestroy(h->stream);
@@ -1288,41 +1293,46 @@ int cov_search_init(cov_cell_t* h, uint64_t seed, const uint8_t* seed_code, int 
         return pk;
     };
     if (mseed > M) {
-        // remove-and-repair descent (:1156-1196): counts of all seed words, then
-        // repeatedly drop the first word with the fewest uniquely covered words
+        // remove-and-repair descent (:1156-1189): counts of all seed words, then
+        // repeatedly drop the first word with the fewest uniquely covered
+        // words.  The choices stay on the device (three launches a removal,
+        // no wait); the dropped words are read back once at the end.
         std::vector<uint8_t> sc(seed_code, seed_code + (size_t)mseed * n);
         std::vector<int64_t> si((size_t)mseed);
         for (int i = 0; i < mseed; i++) si[i] = index_of(&sc[(size_t)i * n]);
-        int64_t *dc = nullptr, *du = nullptr;
-        HCK(h, hipMalloc((void**)&dc, (size_t)mseed * 8));
-        HCK(h, hipMalloc((void**)&du, (size_t)mseed * 8));
+        if (h->rr_cap < mseed) {
+            if (h->rr) (void)hipFree(h->rr);
+            h->rr = nullptr;
+            h->rr_cap = 0;
+            HCK(h, hipMalloc((void**)&h->rr, (2 * (size_t)mseed + 1) * 8));
+            h->rr_cap = mseed;
+        }
+        int64_t* dc = h->rr;
+        int64_t* du = h->rr + mseed;
+        int64_t* dsel = h->rr + 2 * (int64_t)mseed;
         HCK(h, hipMemsetAsync(C.cnt32, 0, (size_t)((C.N + 1) / 2) * 4, h->stream));
         HCK(h, hipMemcpyAsync(dc, si.data(), (size_t)mseed * 8, hipMemcpyHostToDevice, h->stream));
         hipLaunchKernelGGL(k_ball_multi, dim3(grid_for((int64_t)mseed * C.V, WG, 1, 4096)), dim3(WG), 0, h->stream,
                            h->h, dc, mseed, 1);
         HCK(h, hipGetLastError());
         RC(do_derive(h));
-        std::vector<int64_t> uc((size_t)mseed);
-        while (mseed > M) {
-            HCK(h, hipMemcpyAsync(dc, si.data(), (size_t)mseed * 8, hipMemcpyHostToDevice, h->stream));
+        for (int left = mseed; left > M; left--) {
             hipLaunchKernelGGL(k_unique_cover, dim3(std::min(mseed, 4096)), dim3(WG), 0, h->stream, h->h, dc, mseed,
                                du);
+            hipLaunchKernelGGL(k_rr_pick, dim3(1), dim3(WG), 0, h->stream, dc, du, mseed, dsel);
+            hipLaunchKernelGGL(k_ball_drop, dim3(grid_for(C.V, WG, 1, 1024)), dim3(WG), 0, h->stream, h->h, dsel);
             HCK(h, hipGetLastError());
-            HCK(h, hipMemcpyAsync(uc.data(), du, (size_t)mseed * 8, hipMemcpyDeviceToHost, h->stream));
-            HCK(h, hipStreamSynchronize(h->stream));
-            int worst = 0;
-            for (int i = 1; i < mseed; i++)
-                if (uc[i] < uc[worst]) worst = i;
-            hipLaunchKernelGGL(k_ball_one, dim3(grid_for(C.V, WG, 1, 1024)), dim3(WG), 0, h->stream, h->h,
-                               si[worst], pack_of(&sc[(size_t)worst * n]), -1);
-            HCK(h, hipGetLastError());
-            sc.erase(sc.begin() + (size_t)worst * n, sc.begin() + (size_t)(worst + 1) * n);
-            si.erase(si.begin() + worst);
-            mseed--;
         }
+        HCK(h, hipMemcpyAsync(si.data(), dc, (size_t)mseed * 8, hipMemcpyDeviceToHost, h->stream));
         HCK(h, hipStreamSynchronize(h->stream));
-        (void)hipFree(dc);
-        (void)hipFree(du);
+        // keep the listed words in their order (:1185-1188)
+        int kept = 0;
+        for (int i = 0; i < mseed; i++)
+            if (si[i] >= 0) {
+                if (kept != i) std::memmove(&sc[(size_t)kept * n], &sc[(size_t)i * n], (size_t)n);
+                kept++;
+            }
+        if (kept != M) { h->err = "remove-and-repair kept the wrong number of words"; return COV_ESTATE; }
         RC(upload(h, C.code, sc.data(), (int64_t)M * n));
         hipLaunchKernelGGL(k_pack_code, dim3(grid_for(M, WG, 1, 256)), dim3(WG), 0, h->stream, h->h);
         HCK(h, hipGetLastError());

@@ -170,3 +170,33 @@ def test_wide_kicks_and_sampling():
 def test_wide_k6_9_4(leave):
     tr = compare(6, 9, 4, 660, 1, 400, leave=leave, wide=True, threads=8)
     assert tr[-1, 2] > 300
+
+
+def test_descent_many_removals():
+    """Remove-and-repair over many removals (cov/engine/covsearch2e.c:1177-1189),
+    run on the device with one wait at the end: K6(9,4)'s 661-word record code
+    down to 650 words, then random K5(7,2) seeds 20-120 words over M = 400 with
+    repeated words (ties at zero uniquely covered words, where the first goes),
+    one cell taking a larger, a smaller and a larger seed in turn so that its
+    descent buffer is reused and regrown."""
+    import os
+    from tests._util import DATA, read_code
+    compare(6, 9, 4, 650, 3, 200, seed_code=read_code(os.path.join(DATA, 'K6_9_4_M661.txt'), 9), threads=8)
+    rng = np.random.default_rng(5)
+    cell = _cell(5, 7, 2, 400)
+    cell.search_params(tenure=10, stall=5000, kick=2, cand=100000)
+    for extra in (70, 20, 120):
+        print('case descent K5(7,2) M=400 from %d words' % (400 + extra), flush=True)
+        sc = rng.integers(0, 5, size=(400 + extra, 7), dtype=np.uint8)
+        sc[7] = sc[2]
+        sc[9] = sc[2]
+        cell.search_init(3, seed_code=sc)
+        o = Oracle(5, 7, 2, 400, 3).init(sc)
+        assert cell.state()['uncovered'] == o.uncovered
+        assert (cell.code() == o.code()).all()
+        assert (cell.cnt() == o.cnt()).all()
+        gpu = cell.run(300, trace=True)
+        ref = o.run(300, tabu=10, stall=5000, cand=100000, trace=True)
+        assert (gpu == ref).all()
+        assert (cell.code() == o.code()).all()
+    cell.close()
