@@ -1311,7 +1311,7 @@ int cov_search_init(cov_cell_t* h, uint64_t seed, const uint8_t* seed_code, int 
         int64_t* du = h->rr + mseed;
         int64_t* dsel = h->rr + 2 * (int64_t)mseed;
         HCK(h, hipMemsetAsync(C.cnt32, 0, (size_t)((C.N + 1) / 2) * 4, h->stream));
-        HCK(h, hipMemcpyAsync(dc, si.data(), (size_t)mseed * 8, hipMemcpyHostToDevice, h->stream));
+        RC(upload(h, dc, si.data(), (int64_t)mseed * 8));
         hipLaunchKernelGGL(k_ball_multi, dim3(grid_for((int64_t)mseed * C.V, WG, 1, 4096)), dim3(WG), 0, h->stream,
                            h->h, dc, mseed, 1);
         HCK(h, hipGetLastError());
