@@ -2264,28 +2264,59 @@ __global__ void __launch_bounds__(WG) k_begin_run(const Cell C, int64_t nsteps)
 }
 
 // --------------------------------------------------------- init kernels
-// uniform random digits for codewords have..M-1, row-major (:1235-1238).
-// One wave.
-__global__ void __launch_bounds__(64) k_init_random(const Cell C, int have)
+// uniform random digits for codewords have..M-1, row-major (:1235-1238):
+// digit k is rmod(q) of the run's draw k.  The control block is fresh (no
+// draw generated or used), so the draws skip the ring: chunks of RNG_CHUNK
+// draws, wave w making the chunk's part w (RNG_PART draws) from the state
+// J_w s (as ring_fill_parts); the wave that makes the chunk's last draw keeps
+// the state after it.  Afterwards rgen = ruse = rlow = the draws used.  One
+// workgroup of RNG_SPLIT waves.
+static_assert(RNG_PART == 64, "k_init_random: lane k keeps draw k of a part");
+__global__ void __launch_bounds__(RNG_SPLIT * 64) k_init_random(const Cell C, int have)
 {
     Ctl* ctl = C.ctl;
-    const int lane = threadIdx.x & 63;
+    __shared__ uint64_t ss[4];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const int64_t need = (int64_t)(C.M - have) * C.n;
-    int64_t done = 0;
-    while (done < need) {
-        const int64_t r0 = ctl->ruse;
-        if (lane == 0 && ctl->rlow < r0) ctl->rlow = r0;
-        __threadfence_block();
-        if (ctl->rgen <= r0) wave_ring_fill(C, ctl, r0 + RB / 2, INT64_MAX);
-        __threadfence_block();
-        const int64_t avail = ctl->rgen - r0;
-        const int64_t k = (need - done < avail) ? need - done : avail;
-        for (int64_t x = lane; x < k; x += 64)
-            C.code[(int64_t)have * C.n + done + x] = (uint8_t)(ring_at(C, r0 + x) % (uint64_t)C.q);
-        __threadfence_block();
-        if (lane == 0) ctl->ruse = r0 + k;
-        __threadfence_block();
-        done += k;
+    uint8_t* out = C.code + (int64_t)have * C.n;
+    if (threadIdx.x < 4) ss[threadIdx.x] = ctl->rst[threadIdx.x];
+    __syncthreads();
+    for (int64_t done = 0; done < need; done += RNG_CHUNK) {
+        const uint64_t s0[4] = {uniform64(ss[0]), uniform64(ss[1]), uniform64(ss[2]), uniform64(ss[3])};
+        __syncthreads();
+        const int64_t gb = done + (int64_t)RNG_PART * w;
+        if (gb < need) {
+            uint64_t st[4] = {s0[0], s0[1], s0[2], s0[3]};
+            if (w > 0) {
+                const uint64_t* Mj = C.jump + (size_t)(w - 1) * 256 * 4;
+                uint64_t o[4];
+#pragma unroll
+                for (int k = 0; k < 4; k++) {
+                    const uint64_t* row = Mj + (size_t)(64 * k + lane) * 4;
+                    const uint64_t x = (row[0] & s0[0]) ^ (row[1] & s0[1]) ^ (row[2] & s0[2]) ^ (row[3] & s0[3]);
+                    o[k] = __ballot(__popcll(x) & 1);
+                }
+#pragma unroll
+                for (int k = 0; k < 4; k++) st[k] = uniform64(o[k]);
+            }
+            const int nb = (need - gb < RNG_PART) ? (int)(need - gb) : RNG_PART;
+            uint64_t mine = 0;
+            for (int k = 0; k < nb; k++) {
+                const uint64_t r = xo_next(st);
+                if (lane == k) mine = r;
+            }
+            if (lane < nb) out[gb + lane] = (uint8_t)(mine % (uint64_t)C.q);
+            if (lane == 0 && (gb + nb == need || w == RNG_SPLIT - 1)) {
+                ss[0] = st[0]; ss[1] = st[1]; ss[2] = st[2]; ss[3] = st[3];
+            }
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        ctl->rst[0] = ss[0]; ctl->rst[1] = ss[1]; ctl->rst[2] = ss[2]; ctl->rst[3] = ss[3];
+        ctl->rgen = need;
+        ctl->ruse = need;
+        ctl->rlow = need;
     }
 }
 

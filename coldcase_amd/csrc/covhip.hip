@@ -1356,7 +1356,7 @@ int cov_search_init(cov_cell_t* h, uint64_t seed, const uint8_t* seed_code, int 
                 HCK(h, hipGetLastError());
             }
         } else {
-            hipLaunchKernelGGL(k_init_random, dim3(1), dim3(64), 0, h->stream, h->h, have);
+            hipLaunchKernelGGL(k_init_random, dim3(1), dim3(RNG_SPLIT * 64), 0, h->stream, h->h, have);
             hipLaunchKernelGGL(k_pack_code, dim3(grid_for(M, WG, 1, 256)), dim3(WG), 0, h->stream, h->h);
             HCK(h, hipGetLastError());
             RC(do_recount(h));
