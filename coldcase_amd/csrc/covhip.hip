@@ -1297,6 +1297,8 @@ int cov_search_init(cov_cell_t* h, uint64_t seed, const uint8_t* seed_code, int 
         // repeatedly drop the first word with the fewest uniquely covered
         // words.  The choices stay on the device (three launches a removal,
         // no wait); the dropped words are read back once at the end.
+        // (k_rr_pick orders by count << 32 | position: counts below 2^31)
+        if (C.V >= ((int64_t)1 << 31)) { h->err = "remove-and-repair: ball of 2^31 words or more"; return COV_ELIMIT; }
         std::vector<uint8_t> sc(seed_code, seed_code + (size_t)mseed * n);
         std::vector<int64_t> si((size_t)mseed);
         for (int i = 0; i < mseed; i++) si[i] = index_of(&sc[(size_t)i * n]);

@@ -196,7 +196,8 @@ int     cov_load_words(cov_cell_t *h, const int64_t *words, int m);
  * remove-and-repair if mseed > M, else greedy (greedy=1) or uniform random
  * digits for the rest (:1156-1239).  The descent runs on the device with one
  * wait at its end; it keeps 16 bytes per seed word of device memory, allocated
- * by the first call with that many words and held until cov_destroy. */
+ * by the first call with that many words and held until cov_destroy
+ * (COV_ELIMIT for a descent when the ball has 2^31 words or more). */
 int     cov_search_init(cov_cell_t *h, uint64_t seed, const uint8_t *seed_code,
                         int mseed, int greedy);
 /* Search knobs: --tabu, --stall, --kick, --cand (:1030-1066). */
